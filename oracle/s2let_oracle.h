@@ -1,0 +1,129 @@
+/*
+ * s2let_oracle.h -- CPU FP64 restatement of S2LET's directional/spin wavelet
+ * transform on MW-sampled spheres.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it may be linked or
+ * loaded only by tests/, by __graft_entry__.smoke() and by bench.py's
+ * cpu_baseline leg.  The product (s2let_amd/) never calls it.
+ *
+ * It restates, in plain C99:
+ *   - the tiling generation of src/c/s2let_tiling.c and src/c/s2let_math.c
+ *     (bit-faithful: same operation order, 300-point trapezoid, NaN fallbacks);
+ *   - the harmonic multiplies of src/c/s2let_analysis.c:28-89,
+ *     src/c/s2let_synthesis.c:25-88, src/c/s2let_analysis_adjoint.c:25-88,
+ *     src/c/s2let_synthesis_adjoint.c:28-90;
+ *   - the drivers s2let_analysis_lm2wav (s2let_analysis.c:290-357),
+ *     s2let_synthesis_wav2lm (s2let_synthesis.c:303-372), the px variants and
+ *     the adjoints;
+ *   - the external SSHT / SO3 kernels they call (MW sampling, separation of
+ *     variables through Wigner Delta = d(pi/2) computed by Risbo's recursion),
+ *     restated from McEwen & Wiaux 2011 (MW sampling) and McEwen et al. 2015
+ *     (SO(3) sampling), because those libraries are not in the reference tree.
+ *
+ * Conventions (all checked by tests/test_oracle_*.py):
+ *   flm[l*l + l + m];  MW map f[t*(2L-1) + p], theta_t = pi(2t+1)/(2L-1),
+ *   phi_p = 2 pi p/(2L-1);  SO(3) samples f[g*L*(2L-1) + t*(2L-1) + a],
+ *   gamma_g = 2 pi g/(2N-1);  sY_lm = (-1)^s sqrt((2l+1)/4pi) D^{l*}_{m,-s}(phi,theta,0);
+ *   D^l_mn(a,b,g) = e^{-ima} d^l_mn(b) e^{-ing};
+ *   compact Wigner storage, negative n first:
+ *     idx(l,m,n) = sum_{n'=-N+1}^{n-1}(L^2-n'^2) + (l^2-n^2) + (l+m).
+ */
+#ifndef S2LET_ORACLE_H
+#define S2LET_ORACLE_H
+
+#include <complex.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same field order and types as s2let_parameters_t (include/s2let/s2let_types.h:56-137). */
+typedef struct {
+  int verbosity;
+  int reality;
+  int upsample;
+  double B;
+  int L;
+  int J_min;
+  int N;
+  int spin;
+  int original_spin;
+  int sampling_scheme; /* 0 = MW (only MW is restated) */
+  int dl_method;       /* 0 = Risbo */
+} oracle_params_t;
+
+typedef double _Complex ocplx;
+
+/* kernel type: 1 = S2DW (default), 2 = needlet, 3 = spline (s2let_tiling.c:24-39) */
+void oracle_switch_wavtype(int typenum);
+
+int oracle_j_max(const oracle_params_t *p);
+int oracle_bandlimit(int j, const oracle_params_t *p);
+int oracle_L0(int j, const oracle_params_t *p);
+unsigned long oracle_binomial_coefficient(int n, int k, int exact);
+
+void oracle_tiling_axisym(double *kappa, double *kappa0, const oracle_params_t *p);
+void oracle_tiling_direction(ocplx *s_elm, const oracle_params_t *p);
+void oracle_tiling_wavelet(ocplx *psi, double *phi, const oracle_params_t *p);
+
+/* sizes (s2let_helper.c) */
+int oracle_n_wav(const oracle_params_t *p);
+int oracle_n_scal(const oracle_params_t *p);
+int oracle_n_lmn_wav(const oracle_params_t *p);
+int oracle_n_lm_scal(const oracle_params_t *p);
+int oracle_flmn_size(int L, int N);
+int oracle_elmn2ind(int el, int m, int n, int L, int N);
+
+/* Wigner Delta^l_{m'm} = d^l_{m'm}(pi/2), full plane, out[(m'+l)*(2l+1) + (m+l)] for one l. */
+void oracle_delta_plane(double *out, int l);
+/* Wigner d^l_{mm'}(beta) full planes by Risbo for l < L: out[l] block (2l+1)^2 concatenated. */
+void oracle_dl_beta_full(double *out, int L, double beta);
+
+/* SSHT (MW) */
+void oracle_ssht_inverse(ocplx *f, const ocplx *flm, int L, int spin);
+void oracle_ssht_forward(ocplx *flm, const ocplx *f, int L, int spin);
+void oracle_ssht_adjoint_inverse(ocplx *flm, const ocplx *f, int L, int spin);
+void oracle_ssht_adjoint_forward(ocplx *f, const ocplx *flm, int L, int spin);
+
+/* SO3 (MW, compact, negative-n first); nmode 0 = all, 1 = even only, 2 = odd only */
+void oracle_so3_inverse(ocplx *f, const ocplx *flmn, int L, int N, int L0, int nmode);
+void oracle_so3_forward(ocplx *flmn, const ocplx *f, int L, int N, int L0, int nmode);
+void oracle_so3_adjoint_inverse(ocplx *flmn, const ocplx *f, int L, int N, int L0, int nmode);
+void oracle_so3_adjoint_forward(ocplx *f, const ocplx *flmn, int L, int N, int L0, int nmode);
+
+/* S2LET harmonic multiplies */
+void oracle_analysis_lm2lmn(ocplx *f_wav_lmn, ocplx *f_scal_lm, const ocplx *flm,
+                            const ocplx *wav_lm, const double *scal_l, const oracle_params_t *p);
+void oracle_synthesis_lmn2lm(ocplx *flm, const ocplx *f_wav_lmn, const ocplx *f_scal_lm,
+                             const ocplx *wav_lm, const double *scal_l, const oracle_params_t *p);
+void oracle_analysis_adjoint_lmn2lm(ocplx *flm, const ocplx *f_wav_lmn, const ocplx *f_scal_lm,
+                                    const ocplx *wav_lm, const double *scal_l,
+                                    const oracle_params_t *p);
+void oracle_synthesis_adjoint_lm2lmn(ocplx *f_wav_lmn, ocplx *f_scal_lm, const ocplx *flm,
+                                     const ocplx *wav_lm, const double *scal_l,
+                                     const oracle_params_t *p);
+
+/* S2LET drivers */
+void oracle_analysis_lm2wav(ocplx *f_wav, ocplx *f_scal, const ocplx *flm, const oracle_params_t *p);
+void oracle_synthesis_wav2lm(ocplx *flm, const ocplx *f_wav, const ocplx *f_scal, const oracle_params_t *p);
+void oracle_analysis_px2wav(ocplx *f_wav, ocplx *f_scal, const ocplx *f, const oracle_params_t *p);
+void oracle_synthesis_wav2px(ocplx *f, const ocplx *f_wav, const ocplx *f_scal, const oracle_params_t *p);
+void oracle_analysis_adjoint_wav2lm(ocplx *flm, const ocplx *f_wav, const ocplx *f_scal, const oracle_params_t *p);
+void oracle_analysis_adjoint_wav2px(ocplx *f, const ocplx *f_wav, const ocplx *f_scal, const oracle_params_t *p);
+void oracle_synthesis_adjoint_lm2wav(ocplx *f_wav, ocplx *f_scal, const ocplx *flm, const oracle_params_t *p);
+void oracle_synthesis_adjoint_px2wav(ocplx *f_wav, ocplx *f_scal, const ocplx *f, const oracle_params_t *p);
+
+/* Axisymmetric wavelet analysis in MW pixel space, multiresolution
+ * (s2let_transform_axisym_mw.c: s2let_transform_axisym_wav_analysis_mw_multires);
+ * used only to pin the directional transform at N=1. */
+void oracle_axisym_wav_analysis_mw_multires(ocplx *f_wav, ocplx *f_scal, const ocplx *f,
+                                            const oracle_params_t *p);
+
+/* Number of OpenMP threads the oracle uses (1 when built without OpenMP). */
+int oracle_num_threads(void);
+void oracle_set_num_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
