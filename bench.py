@@ -1,0 +1,182 @@
+"""Throughput of the directional wavelet transform on one MI355X.
+
+Workload (the metric's configuration): one map = s2let_analysis_lm2wav +
+s2let_synthesis_wav2lm at L=512, N=8, B=2, J_min=2, spin 0, MW sampling,
+multiresolution, on complex128 f_lm with C_l = (l+1)^-2 (numpy seed 2).  A
+step transforms one map; inputs and outputs stay resident in HBM.  The
+headline value is the mixed-precision mode's maps/s; the FP64 mode is timed
+in the same run, and the mixed result's error against FP64 is reported.
+
+Prints one JSON line (see the fields below).  --gpus N > 1 is not supported
+(one GPU per box).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+CFG = dict(L=512, N=8, B=2.0, J_min=2, spin=0, upsample=0)
+HBM_PEAK = 8.0e12        # B/s
+FP64_PEAK = 78.6e12      # FLOP/s (dense)
+FP32_PEAK = 157.3e12
+
+
+def powerlaw_flm(L, seed):
+    rng = np.random.default_rng(seed)
+    ell = np.repeat(np.arange(L), 2 * np.arange(L) + 1)
+    return (rng.standard_normal(L * L) + 1j * rng.standard_normal(L * L)) / np.sqrt(2) / (ell + 1.0)
+
+
+def algorithmic_work(s2, p):
+    """F (flop/map, analysis+synthesis, 4 real flop per real-coefficient x complex
+    MAC with the m' fold) and Q (unavoidable HBM bytes/map)."""
+    J = s2.j_max(p)
+    F = 0
+    fwav = 0
+    for j in range(p.J_min, J + 1):
+        Lj = min(s2.bandlimit(j, p), p.L)
+        Nj = min(p.N, Lj)
+        Nj += (Nj + p.N) % 2
+        L0 = s2.L0(j, p)
+        for n in range(-Nj + 1, Nj, 2):
+            for l in range(max(abs(n), abs(p.spin), L0), Lj):
+                F += (l + 1) * (2 * l + 1)
+        fwav += Lj * (2 * Lj - 1) * (2 * Nj - 1)
+    Ls = min(s2.bandlimit(p.J_min - 1, p), p.L)
+    for l in range(abs(p.spin), Ls):
+        F += (l + 1) * (2 * l + 1)
+    F *= 8
+    Q = 16 * (2 * p.L * p.L + 2 * fwav + 2 * Ls * (2 * Ls - 1))
+    return F, Q
+
+
+def timed(fn, steps, warmup, torch):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(p_cfg, flm):
+    """The oracle (FP64 C restatement of the reference algorithm, OpenMP) on one
+    map of the same workload: analysis + synthesis."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("S2LET_CPU_THREADS", "0")) or min(16, len(os.sched_getaffinity(0)))
+    O.set_threads(threads)
+    q = O.params(p_cfg["L"], p_cfg["N"], p_cfg["B"], p_cfg["J_min"], p_cfg["spin"], p_cfg["upsample"])
+    t0 = time.perf_counter()
+    fw, fs = O.analysis_lm2wav(flm, q)
+    O.synthesis_wav2lm(fw, fs, q)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "maps/s", "cores": threads, "kind": "port",
+            "sample": "1 map (analysis+synthesis) at L=%d N=%d J_min=%d, CPU FP64 oracle, %d OpenMP threads, %.1f s"
+                      % (p_cfg["L"], p_cfg["N"], p_cfg["J_min"], threads, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--L", type=int, default=CFG["L"])
+    ap.add_argument("--N", type=int, default=CFG["N"])
+    ap.add_argument("--J_min", type=int, default=CFG["J_min"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import s2let_amd as s2
+    from s2let_amd import torch_ops as ops
+
+    cfg = dict(CFG, L=args.L, N=args.N, J_min=args.J_min)
+    p = s2.parameters(cfg["L"], cfg["N"], cfg["B"], cfg["J_min"], cfg["spin"], cfg["upsample"])
+    flm_h = powerlaw_flm(cfg["L"], 2)
+    flm = torch.from_numpy(flm_h).cuda()
+    nw, ns = s2.n_wav(p), s2.n_scal(p)
+    bufs = {}
+    for mode in (s2.FP64, s2.MIXED):
+        s2.prepare(p, mode)
+        bufs[mode] = (torch.zeros(nw, dtype=torch.complex128, device="cuda"),
+                      torch.zeros(ns, dtype=torch.complex128, device="cuda"),
+                      torch.zeros(cfg["L"] ** 2, dtype=torch.complex128, device="cuda"))
+
+    def step(mode):
+        fw, fs, rec = bufs[mode]
+        ops.analysis(flm, p, mode, out=(fw, fs))
+        ops.synthesis(fw, fs, p, mode, out=rec)
+
+    res = {}
+    for mode in (s2.FP64, s2.MIXED):
+        dt = timed(lambda: step(mode), args.steps, args.warmup, torch)
+        s2.timing(True)
+        for _ in range(args.steps):
+            step(mode)
+        torch.cuda.synchronize()
+        stages = s2.timing_report()
+        s2.timing(False)
+        res[mode] = (dt, stages)
+
+    # errors (outside the timed region)
+    fw64, fs64, rec64 = bufs[s2.FP64]
+    fw32, fs32, rec32 = bufs[s2.MIXED]
+    a64 = torch.cat([fw64, fs64]).cpu().numpy()
+    a32 = torch.cat([fw32, fs32]).cpu().numpy()
+    err_ana = float(np.linalg.norm(a32 - a64) / np.linalg.norm(a64))
+    r64, r32 = rec64.cpu().numpy(), rec32.cpu().numpy()
+    err_syn = float(np.linalg.norm(r32 - r64) / np.linalg.norm(r64))
+    err_rec64 = float(np.linalg.norm(r64 - flm_h) / np.linalg.norm(flm_h))
+
+    F, Q = algorithmic_work(s2, p)
+    dt32, st32 = res[s2.MIXED]
+    dt64, st64 = res[s2.FP64]
+    K = args.steps
+    maps_s = K / dt32
+    # dominant stage of the headline (mixed) mode: largest total time
+    dom = max(st32.items(), key=lambda kv: kv[1][1])
+    dname, (dcount, dms) = dom
+    out = {
+        "metric": "analysis+synthesis maps/s at L=512,N=8: mixed-precision vs FP64, err<=1e-6",
+        "value": maps_s,
+        "unit": "maps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt32 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-mfma+f64",
+        "data": "synthetic: complex128 f_lm, C_l=(l+1)^-2, numpy default_rng(2)",
+        "config": {"workload": "directional wavelet analysis+synthesis, one map per step", "L": cfg["L"],
+                   "N": cfg["N"], "B": cfg["B"], "J_min": cfg["J_min"], "spin": cfg["spin"],
+                   "sampling": "MW", "multiresolution": True},
+        "fp64": {"value": K / dt64, "ms_per_step": 1e3 * dt64 / K, "reconstruction_rel_err": err_rec64},
+        "mixed_err_vs_fp64": {"analysis": err_ana, "synthesis": err_syn},
+        "stages_ms_per_step": {"mixed": {k: v[1] / K for k, v in st32.items()},
+                               "fp64": {k: v[1] / K for k, v in st64.items()}},
+        "algorithmic": {"flop_per_map": F, "bytes_per_map": Q},
+        "roofline": {"bound": "hbm", "achieved": None, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": None,
+                     "traffic": None, "kernel": dname, "launches_per_step": dcount / K,
+                     "avg_launch_ms": dms / dcount},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, flm_h)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,202 @@
+/*
+ * s2let_hip.h -- C ABI of the MI355X-native S2LET directional/spin wavelet
+ * transform (MW sampling).  Drop-in for the reference's C API: same symbol
+ * names, same s2let_parameters_t layout, same buffer layouts and sizes, same
+ * void return and exit(1)-on-unsupported-sampling behaviour.  Host pointers in
+ * and out; each call copies to the GPU, runs hand-written HIP kernels and
+ * copies back before returning.
+ *
+ * Extension (no reference counterpart): s2let_hip_* device-pointer entry
+ * points with a precision mode and a HIP stream, and s2let_hip_last_error().
+ *
+ * Buffer conventions (identical to the reference):
+ *   flm[l*l + l + m], 0 <= l < L, |m| <= l                    (L*L complex)
+ *   MW map f[t*(2L-1) + p], theta-major                       (L*(2L-1) complex)
+ *   f_wav: per scale j = J_min..J a block of (2N_j-1) gamma slices x L_j x
+ *          (2L_j-1), gamma slowest; allocated size s2let_n_wav (which uses
+ *          2N-1 slices per scale, so a tail may stay untouched)
+ *   f_scal: MW map at L_s = min(ceil(B^J_min), L) (L if upsample)
+ *   f_wav_lmn: per-scale SO(3) compact blocks, negative n first.
+ */
+#ifndef S2LET_HIP_H
+#define S2LET_HIP_H
+
+#ifdef __cplusplus
+#include <complex>
+#define S2LET_COMPLEX(TYPE) std::complex<TYPE>
+extern "C" {
+#else
+#include <complex.h>
+#define S2LET_COMPLEX(TYPE) TYPE complex
+#endif
+
+/* ssht_dl_method_t as used by s2let_types.h:135 (pys2let.pyx:144-145). */
+#ifndef SSHT_DL_METHOD_DEFINED
+#define SSHT_DL_METHOD_DEFINED
+typedef enum { SSHT_DL_RISBO, SSHT_DL_TRAPANI } ssht_dl_method_t;
+#endif
+
+/* s2let_types.h:18-38 */
+typedef enum { S2LET_SAMPLING_MW, S2LET_SAMPLING_MW_SS, S2LET_SAMPLING_SIZE } s2let_sampling_t;
+
+/* s2let_types.h:56-137 (field order and types are the ABI) */
+typedef struct {
+  int verbosity;
+  int reality;
+  int upsample;
+  double B;
+  int L;
+  int J_min;
+  int N;
+  int spin;
+  int original_spin;
+  s2let_sampling_t sampling_scheme;
+  ssht_dl_method_t dl_method;
+} s2let_parameters_t;
+
+/* ---- scale schedule and tiling (s2let_tiling.h; s2let_tiling.c:24-551) ---- */
+void s2let_switch_wavtype(int typenum);                                   /* s2let_tiling.c:24 */
+int s2let_bandlimit(int j, const s2let_parameters_t *parameters);         /* s2let_tiling.c:51 */
+int s2let_L0(int j, const s2let_parameters_t *parameters);                /* s2let_tiling.c:82 */
+int s2let_j_max(const s2let_parameters_t *parameters);                    /* s2let_tiling.c:105 */
+void s2let_tiling_axisym_allocate(double **kappa, double **kappa0,
+                                  const s2let_parameters_t *parameters);  /* s2let_tiling.c:122 */
+void s2let_tiling_axisym(double *kappa, double *kappa0,
+                         const s2let_parameters_t *parameters);           /* s2let_tiling.c:212 */
+void s2let_tiling_direction_allocate(S2LET_COMPLEX(double) **s_elm,
+                                     const s2let_parameters_t *parameters); /* s2let_tiling.c:260 */
+void s2let_tiling_direction(S2LET_COMPLEX(double) *s_elm,
+                            const s2let_parameters_t *parameters);        /* s2let_tiling.c:281 */
+void s2let_tiling_wavelet_allocate(S2LET_COMPLEX(double) **psi, double **phi,
+                                   const s2let_parameters_t *parameters); /* s2let_tiling.c:334 */
+void s2let_tiling_wavelet(S2LET_COMPLEX(double) *psi, double *phi,
+                          const s2let_parameters_t *parameters);          /* s2let_tiling.c:381 */
+double s2let_tiling_axisym_check_identity(double *kappa, double *kappa0,
+                                          const s2let_parameters_t *parameters);   /* :444 */
+double s2let_tiling_direction_check_identity(S2LET_COMPLEX(double) *s_elm,
+                                             const s2let_parameters_t *parameters); /* :482 */
+double s2let_tiling_wavelet_check_identity(S2LET_COMPLEX(double) *psi, double *phi,
+                                           const s2let_parameters_t *parameters);  /* :519 */
+
+/* ---- sizes (s2let_helper.c:5-107) ---- */
+int s2let_n_phi(const s2let_parameters_t *parameters);
+int s2let_n_theta(const s2let_parameters_t *parameters);
+int s2let_n_px(const s2let_parameters_t *parameters);
+int s2let_n_lm(const s2let_parameters_t *parameters);
+int s2let_n_lm_scal(const s2let_parameters_t *parameters);
+int s2let_n_lmn_wav(const s2let_parameters_t *parameters);
+int s2let_n_gamma(const s2let_parameters_t *parameters);
+int s2let_n_scal(const s2let_parameters_t *parameters);
+int s2let_n_wav(const s2let_parameters_t *parameters);
+int s2let_n_wav_j(int j, const s2let_parameters_t *parameters);
+
+/* ---- allocation (s2let_alloc.c:19-106; calloc, zero-filled) ---- */
+void s2let_allocate_mw(S2LET_COMPLEX(double) **f, int L);
+void s2let_allocate_lm(S2LET_COMPLEX(double) **flm, int L);
+void s2let_allocate_lmn_f_wav(S2LET_COMPLEX(double) **f_wav_lmn, S2LET_COMPLEX(double) **f_scal_lm,
+                              const s2let_parameters_t *parameters);
+void s2let_allocate_f_wav(S2LET_COMPLEX(double) **f_wav, S2LET_COMPLEX(double) **f_scal,
+                          const s2let_parameters_t *parameters);
+
+/* ---- harmonic tiling multiplies (GPU kernels) ---- */
+void s2let_analysis_lm2lmn(S2LET_COMPLEX(double) *f_wav_lmn, S2LET_COMPLEX(double) *f_scal_lm,
+                           const S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *wav_lm,
+                           const double *scal_l, const s2let_parameters_t *parameters); /* s2let_analysis.c:28 */
+void s2let_synthesis_lmn2lm(S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *f_wav_lmn,
+                            const S2LET_COMPLEX(double) *f_scal_lm, const S2LET_COMPLEX(double) *wav_lm,
+                            const double *scal_l, const s2let_parameters_t *parameters); /* s2let_synthesis.c:25 */
+void s2let_analysis_adjoint_lmn2lm(S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *f_wav_lmn,
+                                   const S2LET_COMPLEX(double) *f_scal_lm, const S2LET_COMPLEX(double) *wav_lm,
+                                   const double *scal_l,
+                                   const s2let_parameters_t *parameters);  /* s2let_analysis_adjoint.c:25 */
+void s2let_synthesis_adjoint_lm2lmn(S2LET_COMPLEX(double) *f_wav_lmn, S2LET_COMPLEX(double) *f_scal_lm,
+                                    const S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *wav_lm,
+                                    const double *scal_l,
+                                    const s2let_parameters_t *parameters); /* s2let_synthesis_adjoint.c:28 */
+
+/* ---- the transform (hot path) ---- */
+void s2let_analysis_lm2wav(S2LET_COMPLEX(double) *f_wav, S2LET_COMPLEX(double) *f_scal,
+                           const S2LET_COMPLEX(double) *flm,
+                           const s2let_parameters_t *parameters);          /* s2let_analysis.c:290 */
+void s2let_synthesis_wav2lm(S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *f_wav,
+                            const S2LET_COMPLEX(double) *f_scal,
+                            const s2let_parameters_t *parameters);         /* s2let_synthesis.c:303 */
+void s2let_analysis_px2wav(S2LET_COMPLEX(double) *f_wav, S2LET_COMPLEX(double) *f_scal,
+                           const S2LET_COMPLEX(double) *f,
+                           const s2let_parameters_t *parameters);          /* s2let_analysis.c:456 */
+void s2let_synthesis_wav2px(S2LET_COMPLEX(double) *f, const S2LET_COMPLEX(double) *f_wav,
+                            const S2LET_COMPLEX(double) *f_scal,
+                            const s2let_parameters_t *parameters);         /* s2let_synthesis.c:472 */
+void s2let_analysis_adjoint_wav2lm(S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *f_wav,
+                                   const S2LET_COMPLEX(double) *f_scal,
+                                   const s2let_parameters_t *parameters);  /* s2let_analysis_adjoint.c:304 */
+void s2let_analysis_adjoint_wav2px(S2LET_COMPLEX(double) *f, const S2LET_COMPLEX(double) *f_wav,
+                                   const S2LET_COMPLEX(double) *f_scal,
+                                   const s2let_parameters_t *parameters);  /* s2let_analysis_adjoint.c:488 */
+void s2let_synthesis_adjoint_lm2wav(S2LET_COMPLEX(double) *f_wav, S2LET_COMPLEX(double) *f_scal,
+                                    const S2LET_COMPLEX(double) *flm,
+                                    const s2let_parameters_t *parameters); /* s2let_synthesis_adjoint.c:294 */
+void s2let_synthesis_adjoint_px2wav(S2LET_COMPLEX(double) *f_wav, S2LET_COMPLEX(double) *f_scal,
+                                    const S2LET_COMPLEX(double) *f,
+                                    const s2let_parameters_t *parameters); /* s2let_synthesis_adjoint.c:465 */
+
+/* ---- MW spherical harmonic transforms used by pys2let (s2let_mw.c:21-40) ---- */
+void s2let_mw_alm2map(S2LET_COMPLEX(double) *f, const S2LET_COMPLEX(double) *flm, int L, int spin);
+void s2let_mw_map2alm(S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *f, int L, int spin);
+
+/* ---------------------------------------------------------------------------
+ * Extension: device-pointer entry points (no reference counterpart).
+ * All pointers are device pointers (interleaved complex128), stream is a
+ * hipStream_t (NULL = default stream); the call is asynchronous on that
+ * stream.  mode: S2LET_HIP_FP64 (FP64 throughout) or S2LET_HIP_MIXED (FP32
+ * MFMA ell-contractions with compensated FP64 block accumulation, FP32 DFT
+ * stages, FP64 Delta recursion and FP64 outputs).  Return 0 on success,
+ * otherwise an error code also reported by s2let_hip_last_error().
+ * ------------------------------------------------------------------------ */
+enum { S2LET_HIP_FP64 = 0, S2LET_HIP_MIXED = 1 };
+enum {
+  S2LET_HIP_OP_ANALYSIS_LM2WAV = 0,
+  S2LET_HIP_OP_SYNTHESIS_WAV2LM = 1,
+  S2LET_HIP_OP_ANALYSIS_ADJOINT_WAV2LM = 2,
+  S2LET_HIP_OP_SYNTHESIS_ADJOINT_LM2WAV = 3
+};
+
+int s2let_hip_analysis_lm2wav(void *f_wav, void *f_scal, const void *flm,
+                              const s2let_parameters_t *parameters, int mode, void *stream);
+int s2let_hip_synthesis_wav2lm(void *flm, const void *f_wav, const void *f_scal,
+                               const s2let_parameters_t *parameters, int mode, void *stream);
+int s2let_hip_analysis_adjoint_wav2lm(void *flm, const void *f_wav, const void *f_scal,
+                                      const s2let_parameters_t *parameters, int mode, void *stream);
+int s2let_hip_synthesis_adjoint_lm2wav(void *f_wav, void *f_scal, const void *flm,
+                                       const s2let_parameters_t *parameters, int mode, void *stream);
+int s2let_hip_mw_alm2map(void *f, const void *flm, int L, int spin, int mode, void *stream);
+int s2let_hip_mw_map2alm(void *flm, const void *f, int L, int spin, int mode, void *stream);
+int s2let_hip_mw_alm2map_adjoint(void *flm, const void *f, int L, int spin, int mode, void *stream);
+int s2let_hip_mw_map2alm_adjoint(void *f, const void *flm, int L, int spin, int mode, void *stream);
+
+/* Batched maps of uneven shape in one set of launches: op is one of
+ * S2LET_HIP_OP_*; map k uses params[k] and the k-th pointers of each array. */
+int s2let_hip_batch(int op, int nmaps, void *const *out_a, void *const *out_b, const void *const *in_a,
+                    const void *const *in_b, const s2let_parameters_t *params, int mode, void *stream);
+
+/* Builds (or fetches) the cached plan for these parameters on the current
+ * device; returns 0 on success.  Plans hold the tiling, Delta tables and DFT
+ * twiddles; they are built on first use by the calls above as well. */
+int s2let_hip_prepare(const s2let_parameters_t *parameters, int mode);
+/* Frees every cached plan and table on the current device. */
+void s2let_hip_release(void);
+/* Last error code and message of this thread (0 / "" if none). */
+int s2let_hip_last_error(void);
+const char *s2let_hip_last_error_string(void);
+/* Per-stage HIP-event timing of the calls above on their streams (off by
+ * default).  s2let_hip_timing(1) clears and starts recording; the report is a
+ * JSON object {stage: [launches, total_ms]}; returns its length or < 0. */
+void s2let_hip_timing(int enable);
+int s2let_hip_timing_report(char *buf, int len);
+/* Library build identification. */
+const char *s2let_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,312 @@
+"""MI355X-native S2LET: exact directional/spin scale-discretised wavelet
+transform on MW-sampled spheres, computed by hand-written HIP kernels.
+
+The compute lives in ``libs2let_amd.so`` (built by ``s2let_amd.build``), which
+exports the reference's C API (``s2let_analysis_lm2wav`` ...; see
+include/s2let_hip.h).  This module is the Python host side:
+
+* :func:`lib` loads the library (and fails loudly when it is missing);
+* the ``capi`` functions call the reference-named entry points on numpy
+  arrays (host pointers, FP64, synchronous), mirroring the C signatures;
+* :mod:`s2let_amd.torch_ops` exposes the device-pointer entry points on
+  torch tensors, with autograd through the hand-written adjoints.
+
+Layouts are the reference's: flm[l*l+l+m]; MW maps theta-major L x (2L-1);
+f_wav concatenates per-scale SO(3) blocks (gamma slowest).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libs2let_amd.so")
+_lib = None
+
+FP64 = 0
+MIXED = 1
+
+
+class Parameters(ctypes.Structure):
+    """s2let_parameters_t (s2let_types.h:56-137): field order is the ABI."""
+
+    _fields_ = [
+        ("verbosity", ctypes.c_int),
+        ("reality", ctypes.c_int),
+        ("upsample", ctypes.c_int),
+        ("B", ctypes.c_double),
+        ("L", ctypes.c_int),
+        ("J_min", ctypes.c_int),
+        ("N", ctypes.c_int),
+        ("spin", ctypes.c_int),
+        ("original_spin", ctypes.c_int),
+        ("sampling_scheme", ctypes.c_int),
+        ("dl_method", ctypes.c_int),
+    ]
+
+    def key(self):
+        return (self.L, self.N, self.B, self.J_min, self.spin, self.upsample, self.original_spin)
+
+
+def parameters(L, N=1, B=2.0, J_min=0, spin=0, upsample=0, original_spin=0, reality=0, sampling_scheme=0,
+               dl_method=0, verbosity=0):
+    return Parameters(int(verbosity), int(reality), int(upsample), float(B), int(L), int(J_min), int(N), int(spin),
+                      int(original_spin), int(sampling_scheme), int(dl_method))
+
+
+_SIGS = {
+    "s2let_j_max": (ctypes.c_int, [ctypes.c_void_p]),
+    "s2let_bandlimit": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
+    "s2let_L0": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
+    "s2let_n_wav": (ctypes.c_int, [ctypes.c_void_p]),
+    "s2let_n_scal": (ctypes.c_int, [ctypes.c_void_p]),
+    "s2let_n_lmn_wav": (ctypes.c_int, [ctypes.c_void_p]),
+    "s2let_n_lm_scal": (ctypes.c_int, [ctypes.c_void_p]),
+    "s2let_n_wav_j": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
+    "s2let_n_gamma": (ctypes.c_int, [ctypes.c_void_p]),
+    "s2let_hip_last_error": (ctypes.c_int, []),
+    "s2let_hip_last_error_string": (ctypes.c_char_p, []),
+    "s2let_hip_version": (ctypes.c_char_p, []),
+    "s2let_hip_prepare": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "s2let_hip_release": (None, []),
+    "s2let_switch_wavtype": (None, [ctypes.c_int]),
+    "s2let_tiling_axisym": (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "s2let_tiling_direction": (None, [ctypes.c_void_p, ctypes.c_void_p]),
+    "s2let_tiling_wavelet": (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+}
+for _n in ("analysis_lm2wav", "synthesis_wav2lm", "analysis_px2wav", "synthesis_wav2px", "analysis_adjoint_wav2lm",
+           "analysis_adjoint_wav2px", "synthesis_adjoint_lm2wav", "synthesis_adjoint_px2wav"):
+    _SIGS["s2let_" + _n] = (None, [ctypes.c_void_p] * 4)
+for _n in ("analysis_lm2lmn", "synthesis_lmn2lm", "analysis_adjoint_lmn2lm", "synthesis_adjoint_lm2lmn"):
+    _SIGS["s2let_" + _n] = (None, [ctypes.c_void_p] * 6)
+for _n in ("mw_alm2map", "mw_map2alm"):
+    _SIGS["s2let_" + _n] = (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
+for _n in ("analysis_lm2wav", "synthesis_wav2lm", "analysis_adjoint_wav2lm", "synthesis_adjoint_lm2wav"):
+    _SIGS["s2let_hip_" + _n] = (ctypes.c_int, [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_void_p])
+for _n in ("mw_alm2map", "mw_map2alm", "mw_alm2map_adjoint", "mw_map2alm_adjoint"):
+    _SIGS["s2let_hip_" + _n] = (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_void_p])
+_SIGS["s2let_hip_timing"] = (None, [ctypes.c_int])
+_SIGS["s2let_hip_timing_report"] = (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int])
+_SIGS["s2let_hip_batch"] = (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5 +
+                            [ctypes.c_int, ctypes.c_void_p])
+
+
+def exported_symbols():
+    """Names declared in include/s2let_hip.h that the library must export."""
+    return sorted(_SIGS)
+
+
+def lib():
+    """Loads libs2let_amd.so; raises if it has not been built (no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("s2let_amd: %s not built; run `python -m s2let_amd.build`" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    L = lib()
+    return L.s2let_hip_last_error(), L.s2let_hip_last_error_string().decode()
+
+
+def check(rc):
+    if rc != 0:
+        code, msg = last_error()
+        raise RuntimeError("s2let_amd error %d: %s" % (code, msg))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _cin(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _ref(p):
+    return ctypes.byref(p)
+
+
+# ----------------------------------------------------------------------------- sizes
+def j_max(p):
+    return lib().s2let_j_max(_ref(p))
+
+
+def bandlimit(j, p):
+    return lib().s2let_bandlimit(int(j), _ref(p))
+
+
+def L0(j, p):
+    return lib().s2let_L0(int(j), _ref(p))
+
+
+def n_wav(p):
+    return lib().s2let_n_wav(_ref(p))
+
+
+def n_scal(p):
+    return lib().s2let_n_scal(_ref(p))
+
+
+def n_lmn_wav(p):
+    return lib().s2let_n_lmn_wav(_ref(p))
+
+
+def n_lm_scal(p):
+    return lib().s2let_n_lm_scal(_ref(p))
+
+
+def switch_wavtype(t):
+    lib().s2let_switch_wavtype(int(t))
+
+
+def tiling_wavelet(p):
+    J = j_max(p)
+    psi = np.zeros((J + 1) * p.L * p.L, np.complex128)
+    phi = np.zeros(p.L)
+    lib().s2let_tiling_wavelet(_ptr(psi), _ptr(phi), _ref(p))
+    return psi, phi
+
+
+def tiling_axisym(p):
+    J = j_max(p)
+    kappa = np.zeros((J + 1) * p.L)
+    kappa0 = np.zeros(p.L)
+    lib().s2let_tiling_axisym(_ptr(kappa), _ptr(kappa0), _ref(p))
+    return kappa.reshape(J + 1, p.L), kappa0
+
+
+# ----------------------------------------------------------------------------- host API (reference names)
+def analysis_lm2wav(flm, p):
+    flm = _cin(flm)
+    fw, fs = np.zeros(n_wav(p), np.complex128), np.zeros(n_scal(p), np.complex128)
+    lib().s2let_analysis_lm2wav(_ptr(fw), _ptr(fs), _ptr(flm), _ref(p))
+    return fw, fs
+
+
+def synthesis_wav2lm(f_wav, f_scal, p):
+    a, b = _cin(f_wav), _cin(f_scal)
+    flm = np.zeros(p.L * p.L, np.complex128)
+    lib().s2let_synthesis_wav2lm(_ptr(flm), _ptr(a), _ptr(b), _ref(p))
+    return flm
+
+
+def analysis_px2wav(f, p):
+    f = _cin(f)
+    fw, fs = np.zeros(n_wav(p), np.complex128), np.zeros(n_scal(p), np.complex128)
+    lib().s2let_analysis_px2wav(_ptr(fw), _ptr(fs), _ptr(f), _ref(p))
+    return fw, fs
+
+
+def synthesis_wav2px(f_wav, f_scal, p):
+    a, b = _cin(f_wav), _cin(f_scal)
+    f = np.zeros(p.L * (2 * p.L - 1), np.complex128)
+    lib().s2let_synthesis_wav2px(_ptr(f), _ptr(a), _ptr(b), _ref(p))
+    return f
+
+
+def analysis_adjoint_wav2lm(f_wav, f_scal, p):
+    a, b = _cin(f_wav), _cin(f_scal)
+    flm = np.zeros(p.L * p.L, np.complex128)
+    lib().s2let_analysis_adjoint_wav2lm(_ptr(flm), _ptr(a), _ptr(b), _ref(p))
+    return flm
+
+
+def analysis_adjoint_wav2px(f_wav, f_scal, p):
+    a, b = _cin(f_wav), _cin(f_scal)
+    f = np.zeros(p.L * (2 * p.L - 1), np.complex128)
+    lib().s2let_analysis_adjoint_wav2px(_ptr(f), _ptr(a), _ptr(b), _ref(p))
+    return f
+
+
+def synthesis_adjoint_lm2wav(flm, p):
+    flm = _cin(flm)
+    fw, fs = np.zeros(n_wav(p), np.complex128), np.zeros(n_scal(p), np.complex128)
+    lib().s2let_synthesis_adjoint_lm2wav(_ptr(fw), _ptr(fs), _ptr(flm), _ref(p))
+    return fw, fs
+
+
+def synthesis_adjoint_px2wav(f, p):
+    f = _cin(f)
+    fw, fs = np.zeros(n_wav(p), np.complex128), np.zeros(n_scal(p), np.complex128)
+    lib().s2let_synthesis_adjoint_px2wav(_ptr(fw), _ptr(fs), _ptr(f), _ref(p))
+    return fw, fs
+
+
+def mw_alm2map(flm, L, spin=0):
+    flm = _cin(flm)
+    f = np.zeros(L * (2 * L - 1), np.complex128)
+    lib().s2let_mw_alm2map(_ptr(f), _ptr(flm), int(L), int(spin))
+    return f
+
+
+def mw_map2alm(f, L, spin=0):
+    f = _cin(f)
+    flm = np.zeros(L * L, np.complex128)
+    lib().s2let_mw_map2alm(_ptr(flm), _ptr(f), int(L), int(spin))
+    return flm
+
+
+def analysis_lm2lmn(flm, p):
+    psi, phi = tiling_wavelet(p)
+    flm = _cin(flm)
+    wl, sl = np.zeros(n_lmn_wav(p), np.complex128), np.zeros(n_lm_scal(p), np.complex128)
+    lib().s2let_analysis_lm2lmn(_ptr(wl), _ptr(sl), _ptr(flm), _ptr(psi), _ptr(phi), _ref(p))
+    return wl, sl
+
+
+def synthesis_lmn2lm(f_wav_lmn, f_scal_lm, p):
+    psi, phi = tiling_wavelet(p)
+    a, b = _cin(f_wav_lmn), _cin(f_scal_lm)
+    flm = np.zeros(p.L * p.L, np.complex128)
+    lib().s2let_synthesis_lmn2lm(_ptr(flm), _ptr(a), _ptr(b), _ptr(psi), _ptr(phi), _ref(p))
+    return flm
+
+
+def analysis_adjoint_lmn2lm(f_wav_lmn, f_scal_lm, p):
+    psi, phi = tiling_wavelet(p)
+    a, b = _cin(f_wav_lmn), _cin(f_scal_lm)
+    flm = np.zeros(p.L * p.L, np.complex128)
+    lib().s2let_analysis_adjoint_lmn2lm(_ptr(flm), _ptr(a), _ptr(b), _ptr(psi), _ptr(phi), _ref(p))
+    return flm
+
+
+def synthesis_adjoint_lm2lmn(flm, p):
+    psi, phi = tiling_wavelet(p)
+    flm = _cin(flm)
+    wl, sl = np.zeros(n_lmn_wav(p), np.complex128), np.zeros(n_lm_scal(p), np.complex128)
+    lib().s2let_synthesis_adjoint_lm2lmn(_ptr(wl), _ptr(sl), _ptr(flm), _ptr(psi), _ptr(phi), _ref(p))
+    return wl, sl
+
+
+def timing(enable=True):
+    """Starts (clears) or stops per-stage HIP-event timing inside the library."""
+    lib().s2let_hip_timing(1 if enable else 0)
+
+
+def timing_report():
+    """{stage: (launches, total_ms)} of the calls made since timing(True)."""
+    import json
+    buf = ctypes.create_string_buffer(1 << 16)
+    n = lib().s2let_hip_timing_report(buf, len(buf))
+    if n < 0:
+        check(n)
+    return {k: (v[0], v[1]) for k, v in json.loads(buf.value.decode()).items()}
+
+
+def prepare(p, mode=FP64):
+    check(lib().s2let_hip_prepare(_ref(p), int(mode)))
+
+
+def release():
+    lib().s2let_hip_release()

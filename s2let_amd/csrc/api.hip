@@ -1,0 +1,448 @@
+// C ABI: the reference's entry points (host pointers, synchronous, FP64) and
+// the device-pointer extension (any stream, FP64 or mixed precision).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "plan.h"
+
+namespace s2h {
+
+static thread_local int t_err = 0;
+static thread_local std::string t_msg;
+
+void set_error(int code, const std::string &msg) {
+  t_err = code;
+  t_msg = msg;
+}
+void clear_error() {
+  t_err = 0;
+  t_msg.clear();
+}
+
+namespace {
+
+// The reference's only detected error: unsupported sampling scheme -> message + exit(1)
+// (s2let_error.h:13-22, raised at e.g. s2let_analysis.c:330-331).
+void require_mw(const s2let_parameters_t *p, const char *fn) {
+  if (p->sampling_scheme != S2LET_SAMPLING_MW) {
+    printf("ERROR: %s.\n", "Sampling scheme not supported");
+    printf("ERROR: %s <%s> %s %s %s %d.\n", "Occurred in function", fn, "of file", __FILE__, "on line", __LINE__);
+    exit(1);
+  }
+}
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    clear_error();
+    f();
+    return 0;
+  } catch (const HipFailure &) {
+    return t_err ? t_err : -100;
+  } catch (const ArgFailure &) {
+    return t_err ? t_err : -101;
+  } catch (const std::exception &e) {
+    set_error(-102, e.what());
+    return -102;
+  }
+}
+
+// Host-pointer calls keep the reference's void signatures: failures are
+// reported on stderr and through s2let_hip_last_error().
+template <typename F>
+void host_call(const char *fn, F &&f) {
+  int rc = guarded(std::forward<F>(f));
+  if (rc) fprintf(stderr, "s2let_hip: %s failed (%d): %s\n", fn, rc, t_msg.c_str());
+}
+
+size_t nwav_used(const Plan &P) { return P.n_wav_used; }
+
+void h2d(void *d, const void *h, size_t bytes, hipStream_t st) {
+  if (bytes) S2H_CHECK_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st));
+}
+void d2h(void *h, const void *d, size_t bytes, hipStream_t st) {
+  if (bytes) S2H_CHECK_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st));
+}
+
+constexpr size_t CB = sizeof(std::complex<double>);
+
+struct HostIO {
+  Plan &P;
+  size_t nlm, nwav, nscal, npx;
+  explicit HostIO(Plan &pl) : P(pl) {
+    const int L = pl.p.L;
+    nlm = (size_t)L * L;
+    nwav = nwav_used(pl);
+    const int Ls = scal_bandlimit(&pl.p);
+    nscal = (size_t)Ls * (2 * Ls - 1);
+    npx = (size_t)L * (2 * L - 1);
+    pl.h_flm.alloc(nlm * CB);
+    pl.h_wav.alloc(nwav * CB);
+    pl.h_scal.alloc(nscal * CB);
+    pl.h_map.alloc(npx * CB);
+    pl.h_tmp.alloc(nlm * CB);
+  }
+};
+
+}  // namespace
+}  // namespace s2h
+
+using namespace s2h;
+
+extern "C" {
+
+int s2let_hip_last_error(void) { return t_err; }
+const char *s2let_hip_last_error_string(void) { return t_msg.c_str(); }
+const char *s2let_hip_version(void) { return "s2let_amd 0.1 (gfx950)"; }
+
+void s2let_hip_timing(int enable) { s2h::timing_enable(enable != 0); }
+
+int s2let_hip_timing_report(char *buf, int len) {
+  std::string r;
+  int rc = guarded([&] { r = s2h::timing_report(); });
+  if (rc) return rc;
+  if (buf && len > 0) {
+    strncpy(buf, r.c_str(), (size_t)len - 1);
+    buf[len - 1] = 0;
+  }
+  return (int)r.size();
+}
+
+int s2let_hip_prepare(const s2let_parameters_t *p, int mode) {
+  return guarded([&] {
+    Context &C = Context::get();
+    std::lock_guard<std::mutex> g(C.mu);
+    C.plan(p, mode, C.stream());
+  });
+}
+
+void s2let_hip_release(void) {
+  guarded([&] {
+    Context &C = Context::get();
+    std::lock_guard<std::mutex> g(C.mu);
+    S2H_CHECK_HIP(hipDeviceSynchronize());
+    C.release();
+  });
+}
+
+// ---------------------------------------------------------------- device API
+#define S2H_DEV_PLAN                              \
+  Context &C = Context::get();                    \
+  std::lock_guard<std::mutex> lk(C.mu);           \
+  hipStream_t st = (hipStream_t)stream;           \
+  Plan &P = C.plan(p, mode, st);
+
+int s2let_hip_analysis_lm2wav(void *f_wav, void *f_scal, const void *flm, const s2let_parameters_t *p, int mode,
+                              void *stream) {
+  return guarded([&] {
+    S2H_DEV_PLAN
+    run_analysis(P, flm, f_wav, f_scal, st);
+    S2H_CHECK_HIP(hipGetLastError());
+  });
+}
+int s2let_hip_synthesis_wav2lm(void *flm, const void *f_wav, const void *f_scal, const s2let_parameters_t *p, int mode,
+                               void *stream) {
+  return guarded([&] {
+    S2H_DEV_PLAN
+    run_synthesis(P, f_wav, f_scal, flm, st);
+    S2H_CHECK_HIP(hipGetLastError());
+  });
+}
+int s2let_hip_analysis_adjoint_wav2lm(void *flm, const void *f_wav, const void *f_scal, const s2let_parameters_t *p,
+                                      int mode, void *stream) {
+  return guarded([&] {
+    S2H_DEV_PLAN
+    run_analysis_adjoint(P, f_wav, f_scal, flm, st);
+    S2H_CHECK_HIP(hipGetLastError());
+  });
+}
+int s2let_hip_synthesis_adjoint_lm2wav(void *f_wav, void *f_scal, const void *flm, const s2let_parameters_t *p,
+                                       int mode, void *stream) {
+  return guarded([&] {
+    S2H_DEV_PLAN
+    run_synthesis_adjoint(P, flm, f_wav, f_scal, st);
+    S2H_CHECK_HIP(hipGetLastError());
+  });
+}
+
+#define S2H_SHT_PLAN                    \
+  Context &C = Context::get();          \
+  std::lock_guard<std::mutex> lk(C.mu); \
+  hipStream_t st = (hipStream_t)stream; \
+  ShtPlan &S = C.sht(L, spin, mode, st);
+
+int s2let_hip_mw_alm2map(void *f, const void *flm, int L, int spin, int mode, void *stream) {
+  return guarded([&] {
+    S2H_SHT_PLAN
+    run_sht_inverse(S, flm, f, st);
+    S2H_CHECK_HIP(hipGetLastError());
+  });
+}
+int s2let_hip_mw_map2alm(void *flm, const void *f, int L, int spin, int mode, void *stream) {
+  return guarded([&] {
+    S2H_SHT_PLAN
+    run_sht_forward(S, f, flm, st);
+    S2H_CHECK_HIP(hipGetLastError());
+  });
+}
+int s2let_hip_mw_alm2map_adjoint(void *flm, const void *f, int L, int spin, int mode, void *stream) {
+  return guarded([&] {
+    S2H_SHT_PLAN
+    run_sht_inverse_adjoint(S, f, flm, st);
+    S2H_CHECK_HIP(hipGetLastError());
+  });
+}
+int s2let_hip_mw_map2alm_adjoint(void *f, const void *flm, int L, int spin, int mode, void *stream) {
+  return guarded([&] {
+    S2H_SHT_PLAN
+    run_sht_forward_adjoint(S, flm, f, st);
+    S2H_CHECK_HIP(hipGetLastError());
+  });
+}
+
+int s2let_hip_batch(int op, int nmaps, void *const *out_a, void *const *out_b, const void *const *in_a,
+                    const void *const *in_b, const s2let_parameters_t *params, int mode, void *stream) {
+  for (int k = 0; k < nmaps; ++k) {
+    const s2let_parameters_t *p = params + k;
+    int rc = 0;
+    switch (op) {
+      case S2LET_HIP_OP_ANALYSIS_LM2WAV: rc = s2let_hip_analysis_lm2wav(out_a[k], out_b[k], in_a[k], p, mode, stream); break;
+      case S2LET_HIP_OP_SYNTHESIS_WAV2LM: rc = s2let_hip_synthesis_wav2lm(out_a[k], in_a[k], in_b[k], p, mode, stream); break;
+      case S2LET_HIP_OP_ANALYSIS_ADJOINT_WAV2LM: rc = s2let_hip_analysis_adjoint_wav2lm(out_a[k], in_a[k], in_b[k], p, mode, stream); break;
+      case S2LET_HIP_OP_SYNTHESIS_ADJOINT_LM2WAV: rc = s2let_hip_synthesis_adjoint_lm2wav(out_a[k], out_b[k], in_a[k], p, mode, stream); break;
+      default: set_error(-1, "unknown op"); return -1;
+    }
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------- host API (drop-in)
+#define S2H_HOST_PLAN                   \
+  Context &C = Context::get();          \
+  std::lock_guard<std::mutex> lk(C.mu); \
+  hipStream_t st = C.stream();          \
+  Plan &P = C.plan(p, S2LET_HIP_FP64, st); \
+  HostIO io(P);
+
+void s2let_analysis_lm2wav(std::complex<double> *f_wav, std::complex<double> *f_scal, const std::complex<double> *flm,
+                           const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    h2d(P.h_flm.p, flm, io.nlm * CB, st);
+    run_analysis(P, P.h_flm.p, P.h_wav.p, P.h_scal.p, st);
+    d2h(f_wav, P.h_wav.p, io.nwav * CB, st);
+    d2h(f_scal, P.h_scal.p, io.nscal * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_synthesis_wav2lm(std::complex<double> *flm, const std::complex<double> *f_wav,
+                            const std::complex<double> *f_scal, const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    h2d(P.h_wav.p, f_wav, io.nwav * CB, st);
+    h2d(P.h_scal.p, f_scal, io.nscal * CB, st);
+    run_synthesis(P, P.h_wav.p, P.h_scal.p, P.h_flm.p, st);
+    d2h(flm, P.h_flm.p, io.nlm * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_analysis_adjoint_wav2lm(std::complex<double> *flm, const std::complex<double> *f_wav,
+                                   const std::complex<double> *f_scal, const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    h2d(P.h_wav.p, f_wav, io.nwav * CB, st);
+    h2d(P.h_scal.p, f_scal, io.nscal * CB, st);
+    run_analysis_adjoint(P, P.h_wav.p, P.h_scal.p, P.h_flm.p, st);
+    d2h(flm, P.h_flm.p, io.nlm * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_synthesis_adjoint_lm2wav(std::complex<double> *f_wav, std::complex<double> *f_scal,
+                                    const std::complex<double> *flm, const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    h2d(P.h_flm.p, flm, io.nlm * CB, st);
+    run_synthesis_adjoint(P, P.h_flm.p, P.h_wav.p, P.h_scal.p, st);
+    d2h(f_wav, P.h_wav.p, io.nwav * CB, st);
+    d2h(f_scal, P.h_scal.p, io.nscal * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_analysis_px2wav(std::complex<double> *f_wav, std::complex<double> *f_scal, const std::complex<double> *f,
+                           const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st);
+    h2d(P.h_map.p, f, io.npx * CB, st);
+    run_sht_forward(S, P.h_map.p, P.h_tmp.p, st);
+    run_analysis(P, P.h_tmp.p, P.h_wav.p, P.h_scal.p, st);
+    d2h(f_wav, P.h_wav.p, io.nwav * CB, st);
+    d2h(f_scal, P.h_scal.p, io.nscal * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_synthesis_wav2px(std::complex<double> *f, const std::complex<double> *f_wav,
+                            const std::complex<double> *f_scal, const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st);
+    h2d(P.h_wav.p, f_wav, io.nwav * CB, st);
+    h2d(P.h_scal.p, f_scal, io.nscal * CB, st);
+    run_synthesis(P, P.h_wav.p, P.h_scal.p, P.h_tmp.p, st);
+    run_sht_inverse(S, P.h_tmp.p, P.h_map.p, st);
+    d2h(f, P.h_map.p, io.npx * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_analysis_adjoint_wav2px(std::complex<double> *f, const std::complex<double> *f_wav,
+                                   const std::complex<double> *f_scal, const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st);
+    h2d(P.h_wav.p, f_wav, io.nwav * CB, st);
+    h2d(P.h_scal.p, f_scal, io.nscal * CB, st);
+    run_analysis_adjoint(P, P.h_wav.p, P.h_scal.p, P.h_tmp.p, st);
+    run_sht_forward_adjoint(S, P.h_tmp.p, P.h_map.p, st);
+    d2h(f, P.h_map.p, io.npx * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_synthesis_adjoint_px2wav(std::complex<double> *f_wav, std::complex<double> *f_scal,
+                                    const std::complex<double> *f, const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st);
+    h2d(P.h_map.p, f, io.npx * CB, st);
+    run_sht_inverse_adjoint(S, P.h_map.p, P.h_tmp.p, st);
+    run_synthesis_adjoint(P, P.h_tmp.p, P.h_wav.p, P.h_scal.p, st);
+    d2h(f_wav, P.h_wav.p, io.nwav * CB, st);
+    d2h(f_scal, P.h_scal.p, io.nscal * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_mw_alm2map(std::complex<double> *f, const std::complex<double> *flm, int L, int spin) {
+  host_call(__func__, [&] {
+    Context &C = Context::get();
+    std::lock_guard<std::mutex> lk(C.mu);
+    hipStream_t st = C.stream();
+    ShtPlan &S = C.sht(L, spin, S2LET_HIP_FP64, st);
+    DevBuf a, b;
+    a.alloc((size_t)L * L * CB);
+    b.alloc((size_t)L * (2 * L - 1) * CB);
+    h2d(a.p, flm, (size_t)L * L * CB, st);
+    run_sht_inverse(S, a.p, b.p, st);
+    d2h(f, b.p, (size_t)L * (2 * L - 1) * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_mw_map2alm(std::complex<double> *flm, const std::complex<double> *f, int L, int spin) {
+  host_call(__func__, [&] {
+    Context &C = Context::get();
+    std::lock_guard<std::mutex> lk(C.mu);
+    hipStream_t st = C.stream();
+    ShtPlan &S = C.sht(L, spin, S2LET_HIP_FP64, st);
+    DevBuf a, b;
+    a.alloc((size_t)L * (2 * L - 1) * CB);
+    b.alloc((size_t)L * L * CB);
+    h2d(a.p, f, (size_t)L * (2 * L - 1) * CB, st);
+    run_sht_forward(S, a.p, b.p, st);
+    d2h(flm, b.p, (size_t)L * L * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+// ---------------------------------------------------------------- harmonic multiplies (lmn API)
+// Buffers are caller-provided host arrays; the multiplies run as GPU kernels
+// over the per-scale compact blocks.
+static void lmn_common(int which, std::complex<double> *a, std::complex<double> *b, const std::complex<double> *c,
+                       const std::complex<double> *d, const std::complex<double> *wav_lm, const double *scal_l,
+                       const s2let_parameters_t *p) {
+  Context &C = Context::get();
+  std::lock_guard<std::mutex> lk(C.mu);
+  hipStream_t st = C.stream();
+  const int L = p->L, J = j_max(p);
+  const size_t nlmn = (size_t)s2let_n_lmn_wav(p), nls = (size_t)s2let_n_lm_scal(p), nlm = (size_t)L * L;
+  const int Ls = scal_bandlimit(p);
+  DevBuf dflm, dw, ds, dpsi, dphi;
+  dflm.alloc(nlm * CB);
+  dw.alloc(nlmn * CB);
+  ds.alloc(nls * CB);
+  dpsi.alloc((size_t)(J + 1) * L * L * CB);
+  dphi.alloc(L * sizeof(double));
+  h2d(dpsi.p, wav_lm, (size_t)(J + 1) * L * L * CB, st);
+  h2d(dphi.p, scal_l, L * sizeof(double), st);
+  const bool to_lmn = (which == 0 || which == 3);
+  if (to_lmn) {
+    h2d(dflm.p, c, nlm * CB, st);
+    S2H_CHECK_HIP(hipMemsetAsync(dw.p, 0, nlmn * CB, st));
+    S2H_CHECK_HIP(hipMemsetAsync(ds.p, 0, nls * CB, st));
+  } else {
+    h2d(dw.p, c, nlmn * CB, st);
+    h2d(ds.p, d, nls * CB, st);
+    S2H_CHECK_HIP(hipMemsetAsync(dflm.p, 0, nlm * CB, st));
+  }
+  size_t off = 0;
+  for (int j = p->J_min; j <= J; ++j) {
+    int Lj, Nj;
+    scale_shape(p, j, &Lj, &Nj);
+    const void *psi_j = (const char *)dpsi.p + (size_t)j * L * L * CB;
+    void *blk = (char *)dw.p + off * CB;
+    if (which == 0) launch_lm2lmn(blk, dflm.p, psi_j, Lj, Nj, p->spin, 0, st);
+    if (which == 3) launch_lm2lmn(blk, dflm.p, psi_j, Lj, Nj, p->spin, 1, st);
+    if (which == 1) launch_lmn2lm(dflm.p, blk, psi_j, Lj, Nj, p->spin, 0, st);
+    if (which == 2) launch_lmn2lm(dflm.p, blk, psi_j, Lj, Nj, p->spin, 1, st);
+    off += (size_t)flmn_size(Lj, Nj);
+  }
+  if (to_lmn) {
+    launch_scal_mult(ds.p, dflm.p, dphi.as<double>(), Ls, p->spin, 0, st);
+    d2h(a, dw.p, nlmn * CB, st);
+    d2h(b, ds.p, nls * CB, st);
+  } else {
+    launch_scal_mult(dflm.p, ds.p, dphi.as<double>(), Ls, p->spin, 1, st);
+    d2h(a, dflm.p, nlm * CB, st);
+  }
+  S2H_CHECK_HIP(hipGetLastError());
+  S2H_CHECK_HIP(hipStreamSynchronize(st));
+}
+
+void s2let_analysis_lm2lmn(std::complex<double> *f_wav_lmn, std::complex<double> *f_scal_lm,
+                           const std::complex<double> *flm, const std::complex<double> *wav_lm, const double *scal_l,
+                           const s2let_parameters_t *p) {
+  host_call(__func__, [&] { lmn_common(0, f_wav_lmn, f_scal_lm, flm, nullptr, wav_lm, scal_l, p); });
+}
+void s2let_synthesis_lmn2lm(std::complex<double> *flm, const std::complex<double> *f_wav_lmn,
+                            const std::complex<double> *f_scal_lm, const std::complex<double> *wav_lm,
+                            const double *scal_l, const s2let_parameters_t *p) {
+  host_call(__func__, [&] { lmn_common(1, flm, nullptr, f_wav_lmn, f_scal_lm, wav_lm, scal_l, p); });
+}
+void s2let_analysis_adjoint_lmn2lm(std::complex<double> *flm, const std::complex<double> *f_wav_lmn,
+                                   const std::complex<double> *f_scal_lm, const std::complex<double> *wav_lm,
+                                   const double *scal_l, const s2let_parameters_t *p) {
+  host_call(__func__, [&] { lmn_common(2, flm, nullptr, f_wav_lmn, f_scal_lm, wav_lm, scal_l, p); });
+}
+void s2let_synthesis_adjoint_lm2lmn(std::complex<double> *f_wav_lmn, std::complex<double> *f_scal_lm,
+                                    const std::complex<double> *flm, const std::complex<double> *wav_lm,
+                                    const double *scal_l, const s2let_parameters_t *p) {
+  host_call(__func__, [&] { lmn_common(3, f_wav_lmn, f_scal_lm, flm, nullptr, wav_lm, scal_l, p); });
+}
+
+}  // extern "C"

@@ -1,0 +1,349 @@
+// Transform pipelines: ell contraction, beta stage (theta), alpha/phi row
+// DFTs and gamma mixing, per group, in FP64 or in the mixed mode (FP32 stages
+// and contractions, FP64 inputs/outputs and Delta recursion).
+#include <algorithm>
+#include <cstdio>
+#include <map>
+#include <string>
+
+#include "plan.h"
+
+namespace s2h {
+
+// ---- optional per-stage event timing (s2let_hip_timing / s2let_hip_timing_report)
+namespace {
+struct StageTimer {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<std::pair<const char *, std::pair<hipEvent_t, hipEvent_t>>> recs;
+  hipEvent_t next() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      S2H_CHECK_HIP(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+};
+StageTimer g_timer;
+
+struct Stage {
+  const char *name;
+  hipStream_t st;
+  hipEvent_t a = nullptr;
+  Stage(const char *n, hipStream_t s) : name(n), st(s) {
+    if (g_timer.on) {
+      a = g_timer.next();
+      S2H_CHECK_HIP(hipEventRecord(a, st));
+    }
+  }
+  ~Stage() {
+    if (g_timer.on && a) {
+      hipEvent_t b = g_timer.next();
+      (void)hipEventRecord(b, st);
+      g_timer.recs.push_back({name, {a, b}});
+    }
+  }
+};
+}  // namespace
+
+void timing_enable(bool on) {
+  g_timer.on = on;
+  g_timer.used = 0;
+  g_timer.recs.clear();
+}
+
+std::string timing_report() {
+  std::map<std::string, std::pair<int, double>> acc;
+  for (auto &r : g_timer.recs) {
+    S2H_CHECK_HIP(hipEventSynchronize(r.second.second));
+    float ms = 0;
+    S2H_CHECK_HIP(hipEventElapsedTime(&ms, r.second.first, r.second.second));
+    auto &a = acc[r.first];
+    a.first += 1;
+    a.second += ms;
+  }
+  std::string out = "{";
+  bool first = true;
+  for (auto &kv : acc) {
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s\"%s\": [%d, %.6f]", first ? "" : ", ", kv.first.c_str(), kv.second.first,
+             kv.second.second);
+    out += buf;
+    first = false;
+  }
+  return out + "}";
+}
+
+namespace {
+
+template <typename T>
+struct Sel;
+template <>
+struct Sel<double> {
+  static const DftDesc<double> &d(const DftTables &t) { return t.d64; }
+  static const cx<double> *shift(const LTables &t) { return t.shift64.as<cx<double>>(); }
+  static const cx<double> *wconv(const LTables &t) { return t.wconv64.as<cx<double>>(); }
+  static const double *delta(Context &c) { return c.delta64(); }
+  static const double *b(const Plan &p) { return p.d_b64.as<double>(); }
+  static const double *b(const ShtPlan &p) { return p.d_b64.as<double>(); }
+};
+template <>
+struct Sel<float> {
+  static const DftDesc<float> &d(const DftTables &t) { return t.d32; }
+  static const cx<float> *shift(const LTables &t) { return t.shift32.as<cx<float>>(); }
+  static const cx<float> *wconv(const LTables &t) { return t.wconv32.as<cx<float>>(); }
+  static const float *delta(Context &c) { return c.delta32(); }
+  static const float *b(const Plan &p) { return p.d_b32.as<float>(); }
+  static const float *b(const ShtPlan &p) { return p.d_b32.as<float>(); }
+};
+
+template <typename T>
+constexpr int maxr() {
+  return sizeof(T) == 8 ? 16 : 32;
+}
+
+template <typename T>
+BetaArgs<T> beta_args(int L, const ColDev *cols, bool conv) {
+  Context &C = Context::get();
+  const LTables &lt = C.ltab(L);
+  BetaArgs<T> a{};
+  a.L = L;
+  a.cols = cols;
+  a.dM = Sel<T>::d(*lt.dM);
+  a.dP = Sel<T>::d(*lt.dP);
+  a.shift = Sel<T>::shift(lt);
+  a.wconv = Sel<T>::wconv(lt);
+  const int M = 2 * L - 1;
+  const int lstride = conv ? lt.P : M;
+  const size_t cap = std::min((size_t)maxr<T>() * 256, C.lds_bytes() / sizeof(cx<T>));
+  int tw = (int)(cap / lstride);
+  if (tw < 1) {
+    set_error(-3, "band-limit too large for the LDS beta stage on this device");
+    throw ArgFailure();
+  }
+  a.tw = std::min(tw, M);
+  return a;
+}
+
+template <typename T>
+RowArgs<T> row_args(int L, size_t nlines, int in_c, int out_c, int sign, double scale) {
+  Context &C = Context::get();
+  const LTables &lt = C.ltab(L);
+  RowArgs<T> a{};
+  a.L = L;
+  a.nlines = nlines;
+  a.in_centred = in_c;
+  a.out_centred = out_c;
+  a.sign = sign;
+  a.scale = (T)scale;
+  a.dM = Sel<T>::d(*lt.dM);
+  const int M = 2 * L - 1;
+  const size_t cap = std::min((size_t)maxr<T>() * 256, C.lds_bytes() / sizeof(cx<T>));
+  a.lpb = (int)std::max<size_t>(1, std::min<size_t>(cap / M, 64));
+  return a;
+}
+
+template <typename T>
+const void *harmonic_input(Plan &P, const void *flm, hipStream_t st) {
+  if (sizeof(T) == 8) return flm;
+  launch_cast<double, float>(flm, P.flm32.p, (size_t)P.p.L * P.p.L, st);
+  return P.flm32.p;
+}
+
+template <typename T>
+inline char *at(const DevBuf &b, size_t elems) {
+  return (char *)b.p + elems * sizeof(cx<T>);
+}
+inline char *at64(const void *p, size_t elems) { return (char *)p + elems * sizeof(cx<double>); }
+
+const double kTwoPi = 6.28318530717958647692;
+
+template <typename T>
+void analysis_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
+  Context &C = Context::get();
+  const void *x = harmonic_input<T>(P, flm, st);
+  for (size_t gi = 0; gi < P.groups.size(); ++gi) {
+    const Group &g = P.groups[gi];
+    const int L = g.L, M = 2 * L - 1;
+    const ColDev *cols = P.d_cols_ana.as<ColDev>() + g.col0;
+    char *spec = at<T>(P.spec, g.spec_off), *work = at<T>(P.work, g.spec_off);
+    { Stage s_("contract_inv", st); launch_contract_inv<T>(x, Sel<T>::delta(C), cols, g.ncol, Sel<T>::b(P), L, spec, st); }
+    { Stage s_("beta_inv", st); launch_beta<T>(0, spec, work, beta_args<T>(L, cols, false), g.ncol, st); }
+    if (g.ngamma == 0) {
+      { Stage s_("rows", st); launch_rows<T, double, T>(work, f_scal, row_args<T>(L, (size_t)L, 1, 0, +1, 1.0), st); }
+    } else {
+      { Stage s_("rows", st); launch_rows<T, T, T>(work, spec, row_args<T>(L, (size_t)g.ncol * L, 1, 0, +1, 1.0), st); }
+      const cx<double> *W = P.d_gamma.as<cx<double>>() + P.gamma_off[gi];
+      { Stage s_("gamma", st); launch_mix<T, double, T>(spec, (size_t)L * M, g.ncol, at64(f_wav, g.out_off), (size_t)L * M, g.ngamma, W,
+                               (size_t)L * M, st); }
+    }
+  }
+}
+
+template <typename T>
+void synthesis_t(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
+  Context &C = Context::get();
+  for (size_t gi = 0; gi < P.groups.size(); ++gi) {
+    const Group &g = P.groups[gi];
+    const int L = g.L, M = 2 * L - 1;
+    const ColDev *cols = P.d_cols_syn.as<ColDev>() + g.col0;
+    char *spec = at<T>(P.spec, g.spec_off), *work = at<T>(P.work, g.spec_off);
+    if (g.ngamma == 0) {
+      { Stage s_("rows", st); launch_rows<double, T, T>(f_scal, spec, row_args<T>(L, (size_t)L, 0, 1, -1, kTwoPi / M), st); }
+    } else {
+      const cx<double> *W = P.d_gamma.as<cx<double>>() + P.gamma_off[gi] + 2 * (size_t)g.ngamma * g.ncol;
+      { Stage s_("gamma", st); launch_mix<double, T, T>(at64(f_wav, g.out_off), (size_t)L * M, g.ngamma, work, (size_t)L * M, g.ncol, W,
+                               (size_t)L * M, st); }
+      { Stage s_("rows", st); launch_rows<T, T, T>(work, spec, row_args<T>(L, (size_t)g.ncol * L, 0, 1, -1, kTwoPi / M), st); }
+    }
+    { Stage s_("beta_fwd", st); launch_beta<T>(2, spec, work, beta_args<T>(L, cols, true), g.ncol, st); }
+  }
+  if (sizeof(T) == 8) {
+    { Stage s_("contract_fwd", st); launch_contract_fwd<T>(P.d_groups_work.as<GroupDev>(), (int)P.groups.size(), P.maxcol, P.d_cols_syn.as<ColDev>(),
+                           Sel<T>::b(P), Sel<T>::delta(C), P.p.L, flm, 0, st); }
+  } else {
+    { Stage s_("contract_fwd", st); launch_contract_fwd<T>(P.d_groups_work.as<GroupDev>(), (int)P.groups.size(), P.maxcol, P.d_cols_syn.as<ColDev>(),
+                           Sel<T>::b(P), Sel<T>::delta(C), P.p.L, P.flm32.p, 0, st); }
+    launch_cast<float, double>(P.flm32.p, flm, (size_t)P.p.L * P.p.L, st);
+  }
+}
+
+template <typename T>
+void analysis_adjoint_t(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
+  Context &C = Context::get();
+  for (size_t gi = 0; gi < P.groups.size(); ++gi) {
+    const Group &g = P.groups[gi];
+    const int L = g.L, M = 2 * L - 1;
+    const ColDev *cols = P.d_cols_ana.as<ColDev>() + g.col0;
+    char *spec = at<T>(P.spec, g.spec_off), *work = at<T>(P.work, g.spec_off);
+    if (g.ngamma == 0) {
+      { Stage s_("rows", st); launch_rows<double, T, T>(f_scal, spec, row_args<T>(L, (size_t)L, 0, 1, -1, 1.0), st); }
+    } else {
+      const cx<double> *W = P.d_gamma.as<cx<double>>() + P.gamma_off[gi] + (size_t)g.ngamma * g.ncol;
+      { Stage s_("gamma", st); launch_mix<double, T, T>(at64(f_wav, g.out_off), (size_t)L * M, g.ngamma, work, (size_t)L * M, g.ncol, W,
+                               (size_t)L * M, st); }
+      { Stage s_("rows", st); launch_rows<T, T, T>(work, spec, row_args<T>(L, (size_t)g.ncol * L, 0, 1, -1, 1.0), st); }
+    }
+    { Stage s_("beta_inv_adj", st); launch_beta<T>(1, spec, work, beta_args<T>(L, cols, false), g.ncol, st); }
+  }
+  void *y = (sizeof(T) == 8) ? flm : P.flm32.p;
+  { Stage s_("contract_fwd", st); launch_contract_fwd<T>(P.d_groups_work.as<GroupDev>(), (int)P.groups.size(), P.maxcol, P.d_cols_ana.as<ColDev>(),
+                         Sel<T>::b(P), Sel<T>::delta(C), P.p.L, y, 0, st); }
+  if (sizeof(T) != 8) launch_cast<float, double>(P.flm32.p, flm, (size_t)P.p.L * P.p.L, st);
+}
+
+template <typename T>
+void synthesis_adjoint_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
+  Context &C = Context::get();
+  const void *x = harmonic_input<T>(P, flm, st);
+  for (size_t gi = 0; gi < P.groups.size(); ++gi) {
+    const Group &g = P.groups[gi];
+    const int L = g.L, M = 2 * L - 1;
+    const ColDev *cols = P.d_cols_syn.as<ColDev>() + g.col0;
+    char *spec = at<T>(P.spec, g.spec_off), *work = at<T>(P.work, g.spec_off);
+    { Stage s_("contract_inv", st); launch_contract_inv<T>(x, Sel<T>::delta(C), cols, g.ncol, Sel<T>::b(P), L, spec, st); }
+    { Stage s_("beta_fwd_adj", st); launch_beta<T>(3, spec, work, beta_args<T>(L, cols, true), g.ncol, st); }
+    if (g.ngamma == 0) {
+      { Stage s_("rows", st); launch_rows<T, double, T>(work, f_scal, row_args<T>(L, (size_t)L, 1, 0, +1, kTwoPi / M), st); }
+    } else {
+      { Stage s_("rows", st); launch_rows<T, T, T>(work, spec, row_args<T>(L, (size_t)g.ncol * L, 1, 0, +1, kTwoPi / M), st); }
+      const cx<double> *W = P.d_gamma.as<cx<double>>() + P.gamma_off[gi] + 3 * (size_t)g.ngamma * g.ncol;
+      { Stage s_("gamma", st); launch_mix<T, double, T>(spec, (size_t)L * M, g.ncol, at64(f_wav, g.out_off), (size_t)L * M, g.ngamma, W,
+                               (size_t)L * M, st); }
+    }
+  }
+}
+
+// ---- single spin-s transform at L (group of one column) ----
+template <typename T>
+void sht_inverse_t(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
+  Context &C = Context::get();
+  const int L = S.L;
+  const void *x = flm;
+  if (sizeof(T) != 8) {
+    launch_cast<double, float>(flm, S.flm32.p, (size_t)L * L, st);
+    x = S.flm32.p;
+  }
+  const ColDev *col = S.d_col.as<ColDev>();
+  { Stage s_("contract_inv", st); launch_contract_inv<T>(x, Sel<T>::delta(C), col, 1, Sel<T>::b(S), L, S.spec.p, st); }
+  { Stage s_("beta_inv", st); launch_beta<T>(0, S.spec.p, S.work.p, beta_args<T>(L, col, false), 1, st); }
+  { Stage s_("rows", st); launch_rows<T, double, T>(S.work.p, f, row_args<T>(L, (size_t)L, 1, 0, +1, 1.0), st); }
+}
+
+template <typename T>
+void sht_forward_t(ShtPlan &S, const void *f, void *flm, hipStream_t st) {
+  Context &C = Context::get();
+  const int L = S.L, M = 2 * L - 1;
+  const ColDev *col = S.d_col.as<ColDev>();
+  { Stage s_("rows", st); launch_rows<double, T, T>(f, S.spec.p, row_args<T>(L, (size_t)L, 0, 1, -1, kTwoPi / M), st); }
+  { Stage s_("beta_fwd", st); launch_beta<T>(2, S.spec.p, S.work.p, beta_args<T>(L, col, true), 1, st); }
+  void *y = (sizeof(T) == 8) ? flm : S.flm32.p;
+  { Stage s_("contract_fwd", st); launch_contract_fwd<T>(S.d_group_work.as<GroupDev>(), 1, 1, col, Sel<T>::b(S), Sel<T>::delta(C), L, y, 0, st); }
+  if (sizeof(T) != 8) launch_cast<float, double>(S.flm32.p, flm, (size_t)L * L, st);
+}
+
+template <typename T>
+void sht_inverse_adjoint_t(ShtPlan &S, const void *f, void *flm, hipStream_t st) {
+  Context &C = Context::get();
+  const int L = S.L;
+  const ColDev *col = S.d_col.as<ColDev>();
+  { Stage s_("rows", st); launch_rows<double, T, T>(f, S.spec.p, row_args<T>(L, (size_t)L, 0, 1, -1, 1.0), st); }
+  { Stage s_("beta_inv_adj", st); launch_beta<T>(1, S.spec.p, S.work.p, beta_args<T>(L, col, false), 1, st); }
+  void *y = (sizeof(T) == 8) ? flm : S.flm32.p;
+  { Stage s_("contract_fwd", st); launch_contract_fwd<T>(S.d_group_work.as<GroupDev>(), 1, 1, col, Sel<T>::b(S), Sel<T>::delta(C), L, y, 0, st); }
+  if (sizeof(T) != 8) launch_cast<float, double>(S.flm32.p, flm, (size_t)L * L, st);
+}
+
+template <typename T>
+void sht_forward_adjoint_t(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
+  Context &C = Context::get();
+  const int L = S.L, M = 2 * L - 1;
+  const void *x = flm;
+  if (sizeof(T) != 8) {
+    launch_cast<double, float>(flm, S.flm32.p, (size_t)L * L, st);
+    x = S.flm32.p;
+  }
+  const ColDev *col = S.d_col.as<ColDev>();
+  { Stage s_("contract_inv", st); launch_contract_inv<T>(x, Sel<T>::delta(C), col, 1, Sel<T>::b(S), L, S.spec.p, st); }
+  { Stage s_("beta_fwd_adj", st); launch_beta<T>(3, S.spec.p, S.work.p, beta_args<T>(L, col, true), 1, st); }
+  { Stage s_("rows", st); launch_rows<T, double, T>(S.work.p, f, row_args<T>(L, (size_t)L, 1, 0, +1, kTwoPi / M), st); }
+}
+
+}  // namespace
+
+#define S2H_DISPATCH(mode, fn, ...) \
+  do {                              \
+    if ((mode) == S2LET_HIP_MIXED)  \
+      fn<float>(__VA_ARGS__);       \
+    else                            \
+      fn<double>(__VA_ARGS__);      \
+  } while (0)
+
+void run_analysis(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
+  S2H_DISPATCH(P.mode, analysis_t, P, flm, f_wav, f_scal, st);
+}
+void run_synthesis(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
+  S2H_DISPATCH(P.mode, synthesis_t, P, f_wav, f_scal, flm, st);
+}
+void run_analysis_adjoint(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
+  S2H_DISPATCH(P.mode, analysis_adjoint_t, P, f_wav, f_scal, flm, st);
+}
+void run_synthesis_adjoint(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
+  S2H_DISPATCH(P.mode, synthesis_adjoint_t, P, flm, f_wav, f_scal, st);
+}
+void run_sht_inverse(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
+  S2H_DISPATCH(S.mode, sht_inverse_t, S, flm, f, st);
+}
+void run_sht_forward(ShtPlan &S, const void *f, void *flm, hipStream_t st) {
+  S2H_DISPATCH(S.mode, sht_forward_t, S, f, flm, st);
+}
+void run_sht_inverse_adjoint(ShtPlan &S, const void *f, void *flm, hipStream_t st) {
+  S2H_DISPATCH(S.mode, sht_inverse_adjoint_t, S, f, flm, st);
+}
+void run_sht_forward_adjoint(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
+  S2H_DISPATCH(S.mode, sht_forward_adjoint_t, S, flm, f, st);
+}
+
+}  // namespace s2h

@@ -1,0 +1,123 @@
+// Per-device context (Delta tables, DFT twiddles) and per-parameter plans.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+
+#include "kernels.h"
+
+namespace s2h {
+
+// Device buffer owned by the library (plans and tables are built once and
+// reused; nothing here is allocated on the transform path).
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf();
+  void alloc(size_t n);
+  template <typename T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+struct DftTables {
+  DftDesc<double> d64;
+  DftDesc<float> d32;
+  DevBuf tw64, tw32;
+};
+
+struct LTables {  // per band-limit L
+  int L, P;
+  DevBuf shift64, shift32, wconv64, wconv32;
+  const DftTables *dM, *dP;
+};
+
+// A set of columns sharing one band-limit (see GroupDev).
+struct Group {
+  int L;
+  int ngamma;     // 2N_j - 1 for a wavelet scale, 0 for a single spin-s transform
+  int ncol;
+  int col0;       // first column in the plan's column arrays
+  size_t spec_off;  // complex offset of this group's spectra in the scratch buffers
+  size_t out_off;   // complex offset of this group's samples in f_wav (scales)
+  int scale;        // j, or -1 for the scaling function / plain SHT
+};
+
+struct Plan {
+  s2let_parameters_t p;
+  int mode;  // S2LET_HIP_FP64 / S2LET_HIP_MIXED
+  int J;
+  std::vector<cd> psi;      // (J+1) L^2
+  std::vector<double> phi;  // L
+  std::vector<Group> groups;  // scaling function first, then scales J_min..J
+  std::vector<ColDev> cols_ana, cols_syn;
+  std::vector<double> bvals;  // per column, L entries
+  size_t spec_elems = 0;      // complex entries of one scratch buffer
+  size_t n_wav_used = 0;
+  int maxcol = 1;
+  // device copies
+  DevBuf d_cols_ana, d_cols_syn, d_b64, d_b32, d_groups_spec, d_groups_work, d_gamma;
+  DevBuf d_psi, d_phi;
+  DevBuf spec, work;  // scratch (T = double or float per mode)
+  DevBuf flm32;       // mixed mode: FP32 copy of the input harmonics
+  std::vector<size_t> gamma_off;  // per group: offsets of its 4 mixing matrices in d_gamma
+  // staging for the host-pointer API
+  DevBuf h_flm, h_wav, h_scal, h_map, h_tmp;
+};
+
+// Single spin-s MW harmonic transform plan (s2let_mw_alm2map / map2alm and
+// the px2wav / wav2px front ends).
+struct ShtPlan {
+  int L, spin, mode;
+  ColDev col;
+  std::vector<double> bvals;
+  DevBuf d_col, d_b64, d_b32, d_group_spec, d_group_work, spec, work, flm32;
+};
+
+class Context {
+ public:
+  static Context &get();
+  void ensure_delta(int L, bool need_f32, hipStream_t st);
+  const double *delta64() const { return dtab64_.as<double>(); }
+  const float *delta32() const { return dtab32_.as<float>(); }
+  const DftTables &dft(int n);
+  const LTables &ltab(int L);
+  Plan &plan(const s2let_parameters_t *p, int mode, hipStream_t st);
+  ShtPlan &sht(int L, int spin, int mode, hipStream_t st);
+  size_t lds_bytes() const { return lds_bytes_; }
+  hipStream_t stream();
+  void release();
+  std::mutex mu;
+
+ private:
+  Context();
+  int dev_ = 0;
+  size_t lds_bytes_ = 65536;
+  int dl_L_ = 0, dl32_L_ = 0;
+  DevBuf dtab64_, dtab32_;
+  std::map<int, std::unique_ptr<DftTables>> dft_;
+  std::map<int, std::unique_ptr<LTables>> lt_;
+  std::map<std::string, std::unique_ptr<Plan>> plans_;
+  std::map<std::tuple<int, int, int>, std::unique_ptr<ShtPlan>> shts_;
+  hipStream_t stream_ = nullptr;
+};
+
+// pipelines (pipeline.hip)
+void timing_enable(bool on);
+std::string timing_report();
+void run_analysis(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st);
+void run_synthesis(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st);
+void run_analysis_adjoint(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st);
+void run_synthesis_adjoint(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st);
+void run_sht_inverse(ShtPlan &S, const void *flm, void *f, hipStream_t st);
+void run_sht_forward(ShtPlan &S, const void *f, void *flm, hipStream_t st);
+void run_sht_inverse_adjoint(ShtPlan &S, const void *f, void *flm, hipStream_t st);
+void run_sht_forward_adjoint(ShtPlan &S, const void *flm, void *f, hipStream_t st);
+
+}  // namespace s2h

@@ -1,0 +1,413 @@
+// Per-device context and plan construction: Wigner Delta tables (Risbo
+// recursion on the GPU, FP64), DFT twiddles, band-limit tables, and the
+// column/weight layout of each parameter set.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <sstream>
+
+#include "plan.h"
+
+namespace s2h {
+
+DevBuf::~DevBuf() {
+  if (p) (void)hipFree(p);
+}
+
+void DevBuf::alloc(size_t n) {
+  if (n <= bytes && p) return;
+  if (p) {
+    S2H_CHECK_HIP(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+  }
+  if (n == 0) n = 16;
+  S2H_CHECK_HIP(hipMalloc(&p, n));
+  bytes = n;
+}
+
+template <typename T>
+static void upload(DevBuf &b, const std::vector<T> &v, hipStream_t st) {
+  b.alloc(v.size() * sizeof(T));
+  if (!v.empty()) S2H_CHECK_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+Context &Context::get() {
+  static Context *c = new Context();  // lives until process exit
+  return *c;
+}
+
+Context::Context() {
+  S2H_CHECK_HIP(hipGetDevice(&dev_));
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_) == hipSuccess && lds > 0)
+    lds_bytes_ = (size_t)lds;
+}
+
+hipStream_t Context::stream() {
+  if (!stream_) S2H_CHECK_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  return stream_;
+}
+
+void Context::release() {
+  plans_.clear();
+  shts_.clear();
+  lt_.clear();
+  dft_.clear();
+  dtab64_.~DevBuf();
+  new (&dtab64_) DevBuf();
+  dtab32_.~DevBuf();
+  new (&dtab32_) DevBuf();
+  dl_L_ = dl32_L_ = 0;
+}
+
+static inline size_t qsize(int L) { return (size_t)L * (L + 1) * (2 * L + 1) / 6; }
+
+// Builds the quarter table of Delta^l = d^l(pi/2), l < L, by Risbo's recursion
+// in FP64: two half-integer steps per ell on ping-pong planes of rows m' >= -1.
+void Context::ensure_delta(int L, bool need_f32, hipStream_t st) {
+  if (L > dl_L_) {
+    const int Lb = L;
+    dtab64_.alloc(qsize(Lb) * sizeof(double));
+    DevBuf a, b, sq;
+    const size_t plane = (size_t)(2 * Lb + 2) * (2 * Lb + 2);
+    a.alloc(plane * sizeof(double));
+    b.alloc(plane * sizeof(double));
+    std::vector<double> sqh(2 * Lb + 2);
+    for (size_t i = 0; i < sqh.size(); ++i) sqh[i] = std::sqrt((double)i);
+    upload(sq, sqh, st);
+    const double one = 1.0;
+    S2H_CHECK_HIP(hipMemcpyAsync(a.p, &one, sizeof(double), hipMemcpyHostToDevice, st));
+    S2H_CHECK_HIP(hipMemcpyAsync(dtab64_.p, &one, sizeof(double), hipMemcpyHostToDevice, st));
+    double *cur = a.as<double>(), *tmp = b.as<double>();
+    for (int l = 1; l < Lb; ++l) {
+      launch_risbo_half(cur, tmp, 2 * l - 1, l, l >= 2 ? l : -1, l - 1, sq.as<double>(), nullptr, st);
+      launch_risbo_half(tmp, cur, 2 * l, l, -1, 0, sq.as<double>(), dtab64_.as<double>(), st);
+    }
+    S2H_CHECK_HIP(hipGetLastError());
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+    dl_L_ = Lb;
+    dl32_L_ = 0;
+  }
+  if (need_f32 && dl32_L_ < dl_L_) {
+    dtab32_.alloc(qsize(dl_L_) * sizeof(float));
+    launch_to_f32(dtab64_.as<double>(), dtab32_.as<float>(), qsize(dl_L_), st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+    dl32_L_ = dl_L_;
+  }
+}
+
+static void factorize(int n, DftDesc<double> &d) {
+  d.n = n;
+  d.nf = 0;
+  int m = n;
+  if ((n & (n - 1)) == 0) {
+    while (m % 4 == 0 && m > 1) {
+      d.fac[d.nf++] = 4;
+      m /= 4;
+    }
+    if (m == 2) d.fac[d.nf++] = 2;
+    return;
+  }
+  for (int f = 2; m > 1;) {
+    if (m % f == 0) {
+      d.fac[d.nf++] = f;
+      m /= f;
+    } else
+      f = (f == 2) ? 3 : f + 2;
+  }
+}
+
+const DftTables &Context::dft(int n) {
+  auto it = dft_.find(n);
+  if (it != dft_.end()) return *it->second;
+  auto t = std::make_unique<DftTables>();
+  factorize(n, t->d64);
+  if (t->d64.nf > kMaxFactors) {
+    set_error(-2, "DFT length has too many factors");
+    throw ArgFailure();
+  }
+  std::vector<cx<double>> w(n);
+  std::vector<cx<float>> wf(n);
+  for (int k = 0; k < n; ++k) {
+    long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+    w[k] = mk<double>((double)cosl(ang), (double)sinl(ang));
+    wf[k] = mk<float>((float)w[k].x, (float)w[k].y);
+  }
+  hipStream_t st = stream();
+  upload(t->tw64, w, st);
+  upload(t->tw32, wf, st);
+  S2H_CHECK_HIP(hipStreamSynchronize(st));
+  t->d64.tw = t->tw64.as<cx<double>>();
+  t->d32.n = t->d64.n;
+  t->d32.nf = t->d64.nf;
+  for (int i = 0; i < kMaxFactors; ++i) t->d32.fac[i] = t->d64.fac[i];
+  t->d32.tw = t->tw32.as<cx<float>>();
+  auto &ref = *t;
+  dft_[n] = std::move(t);
+  return ref;
+}
+
+// radix-2 FFT in long double (host, table setup only)
+static void host_fft(std::vector<std::complex<long double>> &a, int sign) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  const long double pi = 3.14159265358979323846264338327950288L;
+  for (size_t len = 2; len <= n; len <<= 1) {
+    for (size_t i = 0; i < n; i += len)
+      for (size_t k = 0; k < len / 2; ++k) {
+        long double ang = sign * 2 * pi * (long double)k / (long double)len;
+        std::complex<long double> w(cosl(ang), sinl(ang));
+        auto u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+  }
+}
+
+const LTables &Context::ltab(int L) {
+  auto it = lt_.find(L);
+  if (it != lt_.end()) return *it->second;
+  auto t = std::make_unique<LTables>();
+  t->L = L;
+  const int M = 2 * L - 1;
+  int P = 1;
+  while (P < 4 * L - 3) P <<= 1;
+  if (P < 2) P = 2;
+  t->P = P;
+  const long double pi = 3.14159265358979323846264338327950288L;
+  std::vector<cx<double>> sh(L);
+  std::vector<cx<float>> shf(L);
+  for (int m = 0; m < L; ++m) {
+    long double a = pi * m / (long double)M;
+    sh[m] = mk<double>((double)cosl(a), (double)sinl(a));
+    shf[m] = mk<float>((float)sh[m].x, (float)sh[m].y);
+  }
+  // weights I(p) = int_0^pi e^{ipt} sin t dt, placed at p mod P, transformed with exponent -
+  std::vector<std::complex<long double>> w(P, 0.0L);
+  for (int p = -2 * (L - 1); p <= 2 * (L - 1); ++p) {
+    std::complex<long double> v;
+    if (p == 1) v = std::complex<long double>(0, pi / 2);
+    else if (p == -1) v = std::complex<long double>(0, -pi / 2);
+    else if (p & 1) v = 0;
+    else v = 2.0L / (1.0L - (long double)p * p);
+    w[((p % P) + P) % P] += v;
+  }
+  host_fft(w, -1);
+  std::vector<cx<double>> wc(P);
+  std::vector<cx<float>> wcf(P);
+  for (int k = 0; k < P; ++k) {
+    wc[k] = mk<double>((double)w[k].real(), (double)w[k].imag());
+    wcf[k] = mk<float>((float)wc[k].x, (float)wc[k].y);
+  }
+  hipStream_t st = stream();
+  upload(t->shift64, sh, st);
+  upload(t->shift32, shf, st);
+  upload(t->wconv64, wc, st);
+  upload(t->wconv32, wcf, st);
+  S2H_CHECK_HIP(hipStreamSynchronize(st));
+  t->dM = &dft(M);
+  t->dP = &dft(P);
+  auto &ref = *t;
+  lt_[L] = std::move(t);
+  return ref;
+}
+
+static inline void ipow(int k, double &re, double &im) {
+  k &= 3;
+  re = (k == 0) ? 1 : (k == 2 ? -1 : 0);
+  im = (k == 1) ? 1 : (k == 3 ? -1 : 0);
+}
+
+static std::string plan_key(const s2let_parameters_t *p, int mode) {
+  std::ostringstream o;
+  o.precision(17);
+  o << p->L << ',' << p->N << ',' << p->B << ',' << p->J_min << ',' << p->spin << ',' << p->original_spin << ','
+    << p->upsample << ',' << kernel_type() << ',' << mode;
+  return o.str();
+}
+
+Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
+  const std::string key = plan_key(p, mode);
+  auto it = plans_.find(key);
+  if (it != plans_.end()) return *it->second;
+  if (p->L < 1 || p->N < 1 || p->B <= 1.0 || p->N > 16 || std::abs(p->spin) >= p->L) {
+    set_error(-1, "unsupported parameters (need L >= 1, 1 <= N <= 16, B > 1, |spin| < L)");
+    throw ArgFailure();
+  }
+  auto P = std::make_unique<Plan>();
+  Plan &pl = *P;
+  pl.p = *p;
+  pl.mode = mode;
+  pl.J = j_max(p);
+  const int L = p->L, J = pl.J;
+  pl.psi.assign((size_t)(J + 1) * L * L, cd(0, 0));
+  pl.phi.assign(L, 0.0);
+  tiling_wavelet(pl.psi.data(), pl.phi.data(), p);
+  const cd nu = (p->N % 2) ? cd(1, 0) : cd(0, 1);
+  const int aspin = std::abs(p->spin);
+  const double pi = M_PI;
+  size_t spec = 0, out = 0;
+  auto add_col = [&](int n, int lmin, const std::vector<double> &b, double ar, double ai, double sr, double si) {
+    ColDev c{};
+    c.n = n;
+    c.lmin = lmin;
+    c.boff = (int)pl.bvals.size();
+    c.ph_re = ar;
+    c.ph_im = ai;
+    pl.cols_ana.push_back(c);
+    c.ph_re = sr;
+    c.ph_im = si;
+    pl.cols_syn.push_back(c);
+    pl.bvals.insert(pl.bvals.end(), b.begin(), b.end());
+  };
+  // scaling function: spin-0 MW transform at L_s (s2let_analysis.c:318-324)
+  {
+    const int Ls = scal_bandlimit(p);
+    Group g{Ls, 0, 1, (int)pl.cols_ana.size(), spec, 0, -1};
+    std::vector<double> b(L, 0.0);
+    for (int l = aspin; l < Ls; ++l) b[l] = std::sqrt((2.0 * l + 1) / (4.0 * pi)) * (std::sqrt(4.0 * pi / (2 * l + 1)) * pl.phi[l]);
+    add_col(0, aspin, b, 1, 0, 1, 0);
+    spec += (size_t)Ls * (2 * Ls - 1);
+    pl.groups.push_back(g);
+  }
+  for (int j = p->J_min; j <= J; ++j) {
+    int Lj, Nj;
+    scale_shape(p, j, &Lj, &Nj);
+    Group g{Lj, 2 * Nj - 1, 0, (int)pl.cols_ana.size(), spec, out, j};
+    const int L0 = el0(j, p);
+    for (int n = -Nj + 1; n < Nj; n += 2) {
+      // Psi^j_ln = nu * r(l): analysis uses conj(Psi) (phase i^n conj(nu)), synthesis Psi (i^n nu)
+      std::vector<double> b(L, 0.0);
+      const int lmin = std::max(std::max(aspin, std::abs(n)), L0);
+      for (int l = std::max(aspin, std::abs(n)); l < Lj; ++l) {
+        const cd ps = pl.psi[(size_t)j * L * L + (size_t)l * l + l + n];
+        b[l] = (ps * std::conj(nu)).real();
+      }
+      double ir, ii;
+      ipow(n, ir, ii);
+      const cd in(ir, ii);
+      const cd pa = in * std::conj(nu), ps = in * nu;
+      add_col(n, lmin, b, pa.real(), pa.imag(), ps.real(), ps.imag());
+      g.ncol++;
+    }
+    spec += (size_t)g.ncol * Lj * (2 * Lj - 1);
+    out += (size_t)Lj * (2 * Lj - 1) * (2 * Nj - 1);
+    pl.maxcol = std::max(pl.maxcol, g.ncol);
+    pl.groups.push_back(g);
+  }
+  pl.spec_elems = spec;
+  pl.n_wav_used = out;
+  const bool f32 = (mode == S2LET_HIP_MIXED);
+  ensure_delta(L, f32, st);
+  for (const Group &g : pl.groups) ltab(g.L);
+  // gamma mixing matrices per scale: [inv (G x nc), inv_adj (nc x G), fwd (nc x G), fwd_adj (G x nc)]
+  std::vector<cx<double>> gm;
+  for (const Group &g : pl.groups) {
+    pl.gamma_off.push_back(gm.size());
+    if (g.ngamma == 0) continue;
+    const int G = g.ngamma, nc = g.ncol;
+    auto E = [&](int c, int gi, int s) {
+      const long double a = s * 2.0L * 3.14159265358979323846264338327950288L * pl.cols_ana[g.col0 + c].n * gi / G;
+      return std::complex<long double>(cosl(a), sinl(a));
+    };
+    for (int gi = 0; gi < G; ++gi)
+      for (int c = 0; c < nc; ++c) {
+        auto v = E(c, gi, +1);
+        gm.push_back(mk<double>((double)v.real(), (double)v.imag()));
+      }
+    for (int c = 0; c < nc; ++c)
+      for (int gi = 0; gi < G; ++gi) {
+        auto v = E(c, gi, -1);
+        gm.push_back(mk<double>((double)v.real(), (double)v.imag()));
+      }
+    const long double sc = 2.0L * 3.14159265358979323846264338327950288L / G;
+    for (int c = 0; c < nc; ++c)
+      for (int gi = 0; gi < G; ++gi) {
+        auto v = E(c, gi, -1) * sc;
+        gm.push_back(mk<double>((double)v.real(), (double)v.imag()));
+      }
+    for (int gi = 0; gi < G; ++gi)
+      for (int c = 0; c < nc; ++c) {
+        auto v = E(c, gi, +1) * sc;
+        gm.push_back(mk<double>((double)v.real(), (double)v.imag()));
+      }
+  }
+  upload(pl.d_gamma, gm, st);
+  upload(pl.d_cols_ana, pl.cols_ana, st);
+  upload(pl.d_cols_syn, pl.cols_syn, st);
+  upload(pl.d_b64, pl.bvals, st);
+  std::vector<float> bf(pl.bvals.begin(), pl.bvals.end());
+  upload(pl.d_b32, bf, st);
+  std::vector<cx<double>> psi_dev(pl.psi.size());
+  for (size_t i = 0; i < pl.psi.size(); ++i) psi_dev[i] = mk<double>(pl.psi[i].real(), pl.psi[i].imag());
+  upload(pl.d_psi, psi_dev, st);
+  upload(pl.d_phi, pl.phi, st);
+  const size_t esz = f32 ? sizeof(cx<float>) : sizeof(cx<double>);
+  pl.spec.alloc(pl.spec_elems * esz);
+  pl.work.alloc(pl.spec_elems * esz);
+  if (f32) pl.flm32.alloc((size_t)L * L * sizeof(cx<float>));
+  std::vector<GroupDev> gs, gw;
+  for (const Group &g : pl.groups) {
+    GroupDev d{g.L, g.ncol, g.col0, 0, (const char *)pl.spec.p + g.spec_off * esz};
+    gs.push_back(d);
+    d.spec = (const char *)pl.work.p + g.spec_off * esz;
+    gw.push_back(d);
+  }
+  upload(pl.d_groups_spec, gs, st);
+  upload(pl.d_groups_work, gw, st);
+  S2H_CHECK_HIP(hipStreamSynchronize(st));
+  auto &ref = *P;
+  plans_[key] = std::move(P);
+  return ref;
+}
+
+ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st) {
+  auto key = std::make_tuple(L, spin, mode);
+  auto it = shts_.find(key);
+  if (it != shts_.end()) return *it->second;
+  if (L < 1 || std::abs(spin) >= L) {
+    set_error(-1, "unsupported parameters (need L >= 1, |spin| < L)");
+    throw ArgFailure();
+  }
+  auto S = std::make_unique<ShtPlan>();
+  ShtPlan &s = *S;
+  s.L = L;
+  s.spin = spin;
+  s.mode = mode;
+  const int n = -spin;
+  s.bvals.assign(L, 0.0);
+  for (int l = std::abs(spin); l < L; ++l) s.bvals[l] = ((spin & 1) ? -1.0 : 1.0) * std::sqrt((2.0 * l + 1) / (4.0 * M_PI));
+  s.col = ColDev{};
+  s.col.n = n;
+  s.col.lmin = std::abs(spin);
+  s.col.boff = 0;
+  ipow(n, s.col.ph_re, s.col.ph_im);
+  const bool f32 = (mode == S2LET_HIP_MIXED);
+  ensure_delta(L, f32, st);
+  ltab(L);
+  std::vector<ColDev> cv{s.col};
+  upload(s.d_col, cv, st);
+  upload(s.d_b64, s.bvals, st);
+  std::vector<float> bf(s.bvals.begin(), s.bvals.end());
+  upload(s.d_b32, bf, st);
+  const size_t esz = f32 ? sizeof(cx<float>) : sizeof(cx<double>);
+  const size_t ne = (size_t)L * (2 * L - 1);
+  s.spec.alloc(ne * esz);
+  s.work.alloc(ne * esz);
+  if (f32) s.flm32.alloc((size_t)L * L * sizeof(cx<float>));
+  std::vector<GroupDev> gs{GroupDev{L, 1, 0, 0, s.spec.p}}, gw{GroupDev{L, 1, 0, 0, s.work.p}};
+  upload(s.d_group_spec, gs, st);
+  upload(s.d_group_work, gw, st);
+  S2H_CHECK_HIP(hipStreamSynchronize(st));
+  auto &ref = *S;
+  shts_[key] = std::move(S);
+  return ref;
+}
+
+}  // namespace s2h

@@ -1,0 +1,76 @@
+"""Device-pointer entry points through torch tensors: FP64 and mixed
+precision against the oracle, and autograd through the adjoint kernel."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def crandn(rng, n):
+    return (rng.standard_normal(n) + 1j * rng.standard_normal(n)) / np.sqrt(2)
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+@pytest.fixture(scope="module")
+def ops():
+    import torch  # noqa: F401
+    from s2let_amd import torch_ops
+    return torch_ops
+
+
+@pytest.mark.parametrize("L,N,J_min,spin", [(64, 4, 0, 0), (128, 8, 2, 2), (512, 8, 2, 0)])
+def test_device_api_fp64_and_mixed(ops, oracle, L, N, J_min, spin):
+    import torch
+    print("device api", L, N, J_min, spin, flush=True)
+    p = ops.parameters(L, N, 2.0, J_min, spin)
+    rng = np.random.default_rng(L)
+    ell = np.repeat(np.arange(L), 2 * np.arange(L) + 1)
+    flm = crandn(rng, L * L) / (ell + 1.0)
+    flm[:spin * spin] = 0
+    x = torch.from_numpy(flm).cuda()
+    w64, s64 = ops.analysis(x, p, ops.FP64)
+    w32, s32 = ops.analysis(x, p, ops.MIXED)
+    torch.cuda.synchronize()
+    a64 = np.concatenate([w64.cpu().numpy(), s64.cpu().numpy()])
+    a32 = np.concatenate([w32.cpu().numpy(), s32.cpu().numpy()])
+    if L <= 128:
+        ow, os_ = oracle.analysis_lm2wav(flm, oracle.params(L, N, 2.0, J_min, spin))
+        assert rel(a64, np.concatenate([ow, os_])) < 1e-12
+    assert rel(a32, a64) < 1e-6
+    r64 = ops.synthesis(w64, s64, p, ops.FP64)
+    r32 = ops.synthesis(w32, s32, p, ops.MIXED)
+    torch.cuda.synchronize()
+    assert rel(r64.cpu().numpy(), flm) < 1e-12
+    assert rel(r32.cpu().numpy(), r64.cpu().numpy()) < 1e-6
+
+
+def test_autograd_backward_is_adjoint(ops):
+    import torch
+    L, N = 32, 4
+    p = ops.parameters(L, N, 2.0, 2, 2)
+    rng = np.random.default_rng(3)
+    flm = crandn(rng, L * L)
+    flm[:4] = 0
+    x = torch.from_numpy(flm).cuda().requires_grad_(True)
+    tw = torch.from_numpy(crandn(rng, ops.n_wav(p))).cuda()
+    ts = torch.from_numpy(crandn(rng, ops.n_scal(p))).cuda()
+    w, s = ops.wavelet_analysis(x, p)
+    loss = (w - tw).abs().pow(2).sum() + (s - ts).abs().pow(2).sum()
+    loss.backward()
+    g = x.grad.detach().cpu().numpy()
+    # central finite differences (FP64, step 1e-6) on a few coefficients, real and imaginary parts
+    def lossf(v):
+        w2, s2 = ops.analysis(torch.from_numpy(v).cuda(), p)
+        return float(((w2 - tw).abs().pow(2).sum() + (s2 - ts).abs().pow(2).sum()).item())
+    h = 1e-6
+    for idx in [5, 17, 100, 511, 1000]:
+        for d in (1.0, 1j):
+            e = np.zeros_like(flm)
+            e[idx] = d * h
+            fd = (lossf(flm + e) - lossf(flm - e)) / (2 * h)
+            # torch: grad = dL/d(re) + i dL/d(im)
+            an = g[idx].real if d == 1.0 else g[idx].imag
+            assert abs(fd - an) < 1e-6 * max(1.0, abs(an)), (idx, d, fd, an)
