@@ -105,6 +105,12 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("s2let_amd: %s not built; run `python -m s2let_amd.build`" % LIB_PATH)
+        # One HIP runtime per process: when PyTorch-ROCm is installed, load it first so
+        # that the library binds to the libamdhip64 that torch uses (same SONAME).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)
