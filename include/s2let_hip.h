@@ -193,6 +193,9 @@ const char *s2let_hip_last_error_string(void);
  * JSON object {stage: [launches, total_ms]}; returns its length or < 0. */
 void s2let_hip_timing(int enable);
 int s2let_hip_timing_report(char *buf, int len);
+/* Device self-test of the matrix-core operand layouts the kernels rely on:
+ * 0 = ok, bit 0 = FP64 16x16x4 mismatch, bit 1 = FP32 16x16x4 mismatch, < 0 = HIP error. */
+int s2let_hip_selftest(void);
 /* Library build identification. */
 const char *s2let_hip_version(void);
 

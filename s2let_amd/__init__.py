@@ -88,6 +88,7 @@ for _n in ("analysis_lm2wav", "synthesis_wav2lm", "analysis_adjoint_wav2lm", "sy
 for _n in ("mw_alm2map", "mw_map2alm", "mw_alm2map_adjoint", "mw_map2alm_adjoint"):
     _SIGS["s2let_hip_" + _n] = (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int, ctypes.c_void_p])
+_SIGS["s2let_hip_selftest"] = (ctypes.c_int, [])
 _SIGS["s2let_hip_timing"] = (None, [ctypes.c_int])
 _SIGS["s2let_hip_timing_report"] = (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int])
 _SIGS["s2let_hip_batch"] = (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5 +

@@ -96,6 +96,12 @@ int s2let_hip_last_error(void) { return t_err; }
 const char *s2let_hip_last_error_string(void) { return t_msg.c_str(); }
 const char *s2let_hip_version(void) { return "s2let_amd 0.1 (gfx950)"; }
 
+int s2let_hip_selftest(void) {
+  int bad = 0;
+  int rc = guarded([&] { bad = s2h::mfma_selftest(); });
+  return rc ? rc : bad;
+}
+
 void s2let_hip_timing(int enable) { s2h::timing_enable(enable != 0); }
 
 int s2let_hip_timing_report(char *buf, int len) {

@@ -15,6 +15,16 @@ struct ColDev {
   int boff;
   int pad;
   double ph_re, ph_im;
+  long long out_off;  // complex offset of this column's spectrum plane in the scratch buffers
+};
+
+// Consecutive columns sharing one band-limit (several wavelet scales may share
+// L_j), at most 16: one MFMA column tile.
+struct SuperDev {
+  int L;
+  int ncol;
+  int col0;
+  int pad;
 };
 
 // A group of columns sharing a band-limit (one wavelet scale, the scaling
@@ -61,6 +71,15 @@ void launch_contract_inv(const void *x, const T *dtab, const ColDev *cols, int n
 template <typename T>
 void launch_contract_fwd(const GroupDev *groups, int ngroups, int maxcol, const ColDev *cols, const T *bvals,
                          const T *dtab, int L, void *y, int accumulate, hipStream_t st);
+
+// matrix-core contractions (contract_mfma.hip)
+template <typename T>
+void launch_inv_mfma(const void *x, const T *dtab, const ColDev *cols, int ncol, const T *bvals, int L, void *Fbase,
+                     hipStream_t st);
+template <typename T>
+void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *bvals, const T *dtab, int L,
+                     const void *Jbase, void *y, hipStream_t st);
+int mfma_selftest();
 
 // kind: 0 inverse, 1 inverse adjoint, 2 forward, 3 forward adjoint
 template <typename T>

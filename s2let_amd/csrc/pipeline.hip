@@ -161,15 +161,31 @@ inline char *at64(const void *p, size_t elems) { return (char *)p + elems * size
 const double kTwoPi = 6.28318530717958647692;
 
 template <typename T>
-void analysis_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
+void contract_inv_all(Plan &P, const void *x, const ColDev *cols, void *Fbase, hipStream_t st) {
   Context &C = Context::get();
+  for (const SuperDev &S : P.supers) {
+    Stage s_("contract_inv", st);
+    launch_inv_mfma<T>(x, Sel<T>::delta(C), cols + S.col0, S.ncol, Sel<T>::b(P), S.L, Fbase, st);
+  }
+}
+
+template <typename T>
+void contract_fwd_all(Plan &P, const ColDev *cols, const void *Jbase, void *flm, hipStream_t st) {
+  Context &C = Context::get();
+  Stage s_("contract_fwd", st);
+  launch_fwd_mfma<T>(P.d_supers.as<SuperDev>(), (int)P.supers.size(), cols, Sel<T>::b(P), Sel<T>::delta(C), P.p.L,
+                     Jbase, flm, st);
+}
+
+template <typename T>
+void analysis_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
   const void *x = harmonic_input<T>(P, flm, st);
+  contract_inv_all<T>(P, x, P.d_cols_ana.as<ColDev>(), P.spec.p, st);
   for (size_t gi = 0; gi < P.groups.size(); ++gi) {
     const Group &g = P.groups[gi];
     const int L = g.L, M = 2 * L - 1;
     const ColDev *cols = P.d_cols_ana.as<ColDev>() + g.col0;
     char *spec = at<T>(P.spec, g.spec_off), *work = at<T>(P.work, g.spec_off);
-    { Stage s_("contract_inv", st); launch_contract_inv<T>(x, Sel<T>::delta(C), cols, g.ncol, Sel<T>::b(P), L, spec, st); }
     { Stage s_("beta_inv", st); launch_beta<T>(0, spec, work, beta_args<T>(L, cols, false), g.ncol, st); }
     if (g.ngamma == 0) {
       { Stage s_("rows", st); launch_rows<T, double, T>(work, f_scal, row_args<T>(L, (size_t)L, 1, 0, +1, 1.0), st); }
@@ -184,7 +200,6 @@ void analysis_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t
 
 template <typename T>
 void synthesis_t(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
-  Context &C = Context::get();
   for (size_t gi = 0; gi < P.groups.size(); ++gi) {
     const Group &g = P.groups[gi];
     const int L = g.L, M = 2 * L - 1;
@@ -200,19 +215,11 @@ void synthesis_t(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipS
     }
     { Stage s_("beta_fwd", st); launch_beta<T>(2, spec, work, beta_args<T>(L, cols, true), g.ncol, st); }
   }
-  if (sizeof(T) == 8) {
-    { Stage s_("contract_fwd", st); launch_contract_fwd<T>(P.d_groups_work.as<GroupDev>(), (int)P.groups.size(), P.maxcol, P.d_cols_syn.as<ColDev>(),
-                           Sel<T>::b(P), Sel<T>::delta(C), P.p.L, flm, 0, st); }
-  } else {
-    { Stage s_("contract_fwd", st); launch_contract_fwd<T>(P.d_groups_work.as<GroupDev>(), (int)P.groups.size(), P.maxcol, P.d_cols_syn.as<ColDev>(),
-                           Sel<T>::b(P), Sel<T>::delta(C), P.p.L, P.flm32.p, 0, st); }
-    launch_cast<float, double>(P.flm32.p, flm, (size_t)P.p.L * P.p.L, st);
-  }
+  contract_fwd_all<T>(P, P.d_cols_syn.as<ColDev>(), P.work.p, flm, st);
 }
 
 template <typename T>
 void analysis_adjoint_t(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
-  Context &C = Context::get();
   for (size_t gi = 0; gi < P.groups.size(); ++gi) {
     const Group &g = P.groups[gi];
     const int L = g.L, M = 2 * L - 1;
@@ -228,22 +235,18 @@ void analysis_adjoint_t(Plan &P, const void *f_wav, const void *f_scal, void *fl
     }
     { Stage s_("beta_inv_adj", st); launch_beta<T>(1, spec, work, beta_args<T>(L, cols, false), g.ncol, st); }
   }
-  void *y = (sizeof(T) == 8) ? flm : P.flm32.p;
-  { Stage s_("contract_fwd", st); launch_contract_fwd<T>(P.d_groups_work.as<GroupDev>(), (int)P.groups.size(), P.maxcol, P.d_cols_ana.as<ColDev>(),
-                         Sel<T>::b(P), Sel<T>::delta(C), P.p.L, y, 0, st); }
-  if (sizeof(T) != 8) launch_cast<float, double>(P.flm32.p, flm, (size_t)P.p.L * P.p.L, st);
+  contract_fwd_all<T>(P, P.d_cols_ana.as<ColDev>(), P.work.p, flm, st);
 }
 
 template <typename T>
 void synthesis_adjoint_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
-  Context &C = Context::get();
   const void *x = harmonic_input<T>(P, flm, st);
+  contract_inv_all<T>(P, x, P.d_cols_syn.as<ColDev>(), P.spec.p, st);
   for (size_t gi = 0; gi < P.groups.size(); ++gi) {
     const Group &g = P.groups[gi];
     const int L = g.L, M = 2 * L - 1;
     const ColDev *cols = P.d_cols_syn.as<ColDev>() + g.col0;
     char *spec = at<T>(P.spec, g.spec_off), *work = at<T>(P.work, g.spec_off);
-    { Stage s_("contract_inv", st); launch_contract_inv<T>(x, Sel<T>::delta(C), cols, g.ncol, Sel<T>::b(P), L, spec, st); }
     { Stage s_("beta_fwd_adj", st); launch_beta<T>(3, spec, work, beta_args<T>(L, cols, true), g.ncol, st); }
     if (g.ngamma == 0) {
       { Stage s_("rows", st); launch_rows<T, double, T>(work, f_scal, row_args<T>(L, (size_t)L, 1, 0, +1, kTwoPi / M), st); }
@@ -256,18 +259,20 @@ void synthesis_adjoint_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hi
   }
 }
 
-// ---- single spin-s transform at L (group of one column) ----
+// ---- single spin-s transform at L (one column)
+template <typename T>
+const void *sht_input(ShtPlan &S, const void *flm, hipStream_t st) {
+  if (sizeof(T) == 8) return flm;
+  launch_cast<double, float>(flm, S.flm32.p, (size_t)S.L * S.L, st);
+  return S.flm32.p;
+}
+
 template <typename T>
 void sht_inverse_t(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
   Context &C = Context::get();
   const int L = S.L;
-  const void *x = flm;
-  if (sizeof(T) != 8) {
-    launch_cast<double, float>(flm, S.flm32.p, (size_t)L * L, st);
-    x = S.flm32.p;
-  }
   const ColDev *col = S.d_col.as<ColDev>();
-  { Stage s_("contract_inv", st); launch_contract_inv<T>(x, Sel<T>::delta(C), col, 1, Sel<T>::b(S), L, S.spec.p, st); }
+  { Stage s_("contract_inv", st); launch_inv_mfma<T>(sht_input<T>(S, flm, st), Sel<T>::delta(C), col, 1, Sel<T>::b(S), L, S.spec.p, st); }
   { Stage s_("beta_inv", st); launch_beta<T>(0, S.spec.p, S.work.p, beta_args<T>(L, col, false), 1, st); }
   { Stage s_("rows", st); launch_rows<T, double, T>(S.work.p, f, row_args<T>(L, (size_t)L, 1, 0, +1, 1.0), st); }
 }
@@ -279,9 +284,7 @@ void sht_forward_t(ShtPlan &S, const void *f, void *flm, hipStream_t st) {
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("rows", st); launch_rows<double, T, T>(f, S.spec.p, row_args<T>(L, (size_t)L, 0, 1, -1, kTwoPi / M), st); }
   { Stage s_("beta_fwd", st); launch_beta<T>(2, S.spec.p, S.work.p, beta_args<T>(L, col, true), 1, st); }
-  void *y = (sizeof(T) == 8) ? flm : S.flm32.p;
-  { Stage s_("contract_fwd", st); launch_contract_fwd<T>(S.d_group_work.as<GroupDev>(), 1, 1, col, Sel<T>::b(S), Sel<T>::delta(C), L, y, 0, st); }
-  if (sizeof(T) != 8) launch_cast<float, double>(S.flm32.p, flm, (size_t)L * L, st);
+  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, Sel<T>::b(S), Sel<T>::delta(C), L, S.work.p, flm, st); }
 }
 
 template <typename T>
@@ -291,22 +294,15 @@ void sht_inverse_adjoint_t(ShtPlan &S, const void *f, void *flm, hipStream_t st)
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("rows", st); launch_rows<double, T, T>(f, S.spec.p, row_args<T>(L, (size_t)L, 0, 1, -1, 1.0), st); }
   { Stage s_("beta_inv_adj", st); launch_beta<T>(1, S.spec.p, S.work.p, beta_args<T>(L, col, false), 1, st); }
-  void *y = (sizeof(T) == 8) ? flm : S.flm32.p;
-  { Stage s_("contract_fwd", st); launch_contract_fwd<T>(S.d_group_work.as<GroupDev>(), 1, 1, col, Sel<T>::b(S), Sel<T>::delta(C), L, y, 0, st); }
-  if (sizeof(T) != 8) launch_cast<float, double>(S.flm32.p, flm, (size_t)L * L, st);
+  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, Sel<T>::b(S), Sel<T>::delta(C), L, S.work.p, flm, st); }
 }
 
 template <typename T>
 void sht_forward_adjoint_t(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
   Context &C = Context::get();
   const int L = S.L, M = 2 * L - 1;
-  const void *x = flm;
-  if (sizeof(T) != 8) {
-    launch_cast<double, float>(flm, S.flm32.p, (size_t)L * L, st);
-    x = S.flm32.p;
-  }
   const ColDev *col = S.d_col.as<ColDev>();
-  { Stage s_("contract_inv", st); launch_contract_inv<T>(x, Sel<T>::delta(C), col, 1, Sel<T>::b(S), L, S.spec.p, st); }
+  { Stage s_("contract_inv", st); launch_inv_mfma<T>(sht_input<T>(S, flm, st), Sel<T>::delta(C), col, 1, Sel<T>::b(S), L, S.spec.p, st); }
   { Stage s_("beta_fwd_adj", st); launch_beta<T>(3, S.spec.p, S.work.p, beta_args<T>(L, col, true), 1, st); }
   { Stage s_("rows", st); launch_rows<T, double, T>(S.work.p, f, row_args<T>(L, (size_t)L, 1, 0, +1, kTwoPi / M), st); }
 }

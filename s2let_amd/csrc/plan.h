@@ -61,6 +61,8 @@ struct Plan {
   size_t spec_elems = 0;      // complex entries of one scratch buffer
   size_t n_wav_used = 0;
   int maxcol = 1;
+  std::vector<SuperDev> supers;  // equal-L column runs (<= 16 columns each)
+  DevBuf d_supers;
   // device copies
   DevBuf d_cols_ana, d_cols_syn, d_b64, d_b32, d_groups_spec, d_groups_work, d_gamma;
   DevBuf d_psi, d_phi;
@@ -77,7 +79,7 @@ struct ShtPlan {
   int L, spin, mode;
   ColDev col;
   std::vector<double> bvals;
-  DevBuf d_col, d_b64, d_b32, d_group_spec, d_group_work, spec, work, flm32;
+  DevBuf d_col, d_b64, d_b32, d_group_spec, d_group_work, spec, work, flm32, d_super;
 };
 
 class Context {

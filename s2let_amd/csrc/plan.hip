@@ -303,6 +303,26 @@ Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
   }
   pl.spec_elems = spec;
   pl.n_wav_used = out;
+  // spectrum plane of every column, and the equal-L runs of columns (MFMA column tiles)
+  for (const Group &g : pl.groups)
+    for (int k = 0; k < g.ncol; ++k) {
+      const long long off = (long long)g.spec_off + (long long)k * g.L * (2 * g.L - 1);
+      pl.cols_ana[g.col0 + k].out_off = off;
+      pl.cols_syn[g.col0 + k].out_off = off;
+    }
+  for (const Group &g : pl.groups) {
+    if (!pl.supers.empty() && pl.supers.back().L == g.L && pl.supers.back().ncol + g.ncol <= 16) {
+      pl.supers.back().ncol += g.ncol;
+    } else {
+      int c0 = g.col0, left = g.ncol;
+      while (left > 0) {
+        const int take = std::min(left, 16);
+        pl.supers.push_back(SuperDev{g.L, take, c0, 0});
+        c0 += take;
+        left -= take;
+      }
+    }
+  }
   const bool f32 = (mode == S2LET_HIP_MIXED);
   ensure_delta(L, f32, st);
   for (const Group &g : pl.groups) ltab(g.L);
@@ -339,6 +359,7 @@ Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
       }
   }
   upload(pl.d_gamma, gm, st);
+  upload(pl.d_supers, pl.supers, st);
   upload(pl.d_cols_ana, pl.cols_ana, st);
   upload(pl.d_cols_syn, pl.cols_syn, st);
   upload(pl.d_b64, pl.bvals, st);
@@ -391,8 +412,11 @@ ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st) {
   const bool f32 = (mode == S2LET_HIP_MIXED);
   ensure_delta(L, f32, st);
   ltab(L);
+  s.col.out_off = 0;
   std::vector<ColDev> cv{s.col};
   upload(s.d_col, cv, st);
+  std::vector<SuperDev> sv{SuperDev{L, 1, 0, 0}};
+  upload(s.d_super, sv, st);
   upload(s.d_b64, s.bvals, st);
   std::vector<float> bf(s.bvals.begin(), s.bvals.end());
   upload(s.d_b32, bf, st);

@@ -45,6 +45,11 @@ def s2():
     return s2let_amd
 
 
+def test_mfma_operand_layout(s2):
+    # the matrix-core lane layouts the contraction kernels assume (FP64 and FP32 16x16x4)
+    assert s2.lib().s2let_hip_selftest() == 0
+
+
 CASES = [(16, 4, 2, s, u) for s in (0, 2) for u in (0, 1)] + [
     (64, 4, 0, 0, 0),    # L=64, N=4, J_min=0 (zero tail in f_wav)
     (32, 3, 1, 1, 0),    # odd N, spin 1
