@@ -85,30 +85,44 @@ __global__ __launch_bounds__(256) void k_inv_mfma(const cx<T> *__restrict__ x, c
   acc_t arp = Mfma<T>::zero(), aip = Mfma<T>::zero(), arm = Mfma<T>::zero(), aim = Mfma<T>::zero();
   double hrp[4] = {0, 0, 0, 0}, hip[4] = {0, 0, 0, 0}, hrm[4] = {0, 0, 0, 0}, him[4] = {0, 0, 0, 0};
   int since = 0;
-  for (int l0 = lstart; l0 < L; l0 += 4) {
-    const int l = l0 + k4;
-    T a_rp = 0, a_ip = 0, a_rm = 0, a_im = 0, b = 0;
-    if (l < L && l >= mp) {
-      if (m <= l) {
-        const T d = dtab[qo(l) + (size_t)mp * (l + 1) + m];
-        const cx<T> xp = x[(size_t)l * l + l + m], xm = x[(size_t)l * l + l - m];
-        const T dm = ((l + mp) & 1) ? -d : d;
-        a_rp = d * xp.x;
-        a_ip = d * xp.y;
-        a_rm = dm * xm.x;
-        a_im = dm * xm.y;
+  // K steps in groups of kU (all loads of a group issued before its MFMAs)
+  constexpr int kU = 4;
+  for (int lg = lstart; lg < L; lg += 4 * kU) {
+    T va_rp[kU], va_ip[kU], va_rm[kU], va_im[kU], vb[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int l = lg + 4 * u + k4;
+      T a_rp = 0, a_ip = 0, a_rm = 0, a_im = 0, b = 0;
+      if (l < L && l >= mp) {
+        if (m <= l) {
+          const T d = dtab[qo(l) + (size_t)mp * (l + 1) + m];
+          const cx<T> xp = x[(size_t)l * l + l + m], xm = x[(size_t)l * l + l - m];
+          const T dm = ((l + mp) & 1) ? -d : d;
+          a_rp = d * xp.x;
+          a_ip = d * xp.y;
+          a_rm = dm * xm.x;
+          a_im = dm * xm.y;
+        }
+        if (c < ncol && l >= clmin && can <= l) {
+          T e = dtab[qo(l) + (size_t)mp * (l + 1) + can];
+          if (cn < 0 && ((l + mp) & 1)) e = -e;
+          b = bvals[cboff + l] * e;
+        }
       }
-      if (c < ncol && l >= clmin && can <= l) {
-        T e = dtab[qo(l) + (size_t)mp * (l + 1) + can];
-        if (cn < 0 && ((l + mp) & 1)) e = -e;
-        b = bvals[cboff + l] * e;
-      }
+      va_rp[u] = a_rp;
+      va_ip[u] = a_ip;
+      va_rm[u] = a_rm;
+      va_im[u] = a_im;
+      vb[u] = b;
     }
-    arp = Mfma<T>::mma(a_rp, b, arp);
-    aip = Mfma<T>::mma(a_ip, b, aip);
-    arm = Mfma<T>::mma(a_rm, b, arm);
-    aim = Mfma<T>::mma(a_im, b, aim);
-    if (sizeof(T) == 4 && ++since == 8) {  // FP32: flush the block sum into FP64
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      arp = Mfma<T>::mma(va_rp[u], vb[u], arp);
+      aip = Mfma<T>::mma(va_ip[u], vb[u], aip);
+      arm = Mfma<T>::mma(va_rm[u], vb[u], arm);
+      aim = Mfma<T>::mma(va_im[u], vb[u], aim);
+    }
+    if (sizeof(T) == 4 && ++since == 2) {  // FP32: flush the 32-ell block sum into FP64
       since = 0;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -145,7 +159,10 @@ template <typename T>
 __global__ __launch_bounds__(512) void k_fwd_mfma(const SuperDev *__restrict__ sgs, int nsg,
                                                   const ColDev *__restrict__ cols, const T *__restrict__ bvals,
                                                   const T *__restrict__ dtab, int L, const cx<T> *__restrict__ Jbase,
-                                                  cx<double> *__restrict__ y) {
+                                                  cx<double> *__restrict__ ypart, int mpchunk) {
+  // split s of the m' range: partial sums go to ypart + s * L^2 (reduced by k_sum_parts)
+  cx<double> *__restrict__ y = ypart + (size_t)blockIdx.z * L * L;
+  const int mps = blockIdx.z * mpchunk;
   typedef typename Mfma<T>::acc acc_t;
   __shared__ T sJ[kFwdMC][16][16];  // [m'][c][(m,+-,re/im)]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -173,7 +190,8 @@ __global__ __launch_bounds__(512) void k_fwd_mfma(const SuperDev *__restrict__ s
     // m' range needed by this block: up to the largest l of any of its waves
     int lmaxb = min(L, Lg) - 1;
     const int mpend = lmaxb;  // m' <= l < Lg
-    for (int mp0 = 0; mp0 <= mpend; mp0 += kFwdMC) {
+    const int mpe = min(mpend, mps + mpchunk - 1);
+    for (int mp0 = mps; mp0 <= mpe; mp0 += kFwdMC) {
       __syncthreads();
       for (int e = threadIdx.x; e < kFwdMC * 16 * 16; e += blockDim.x) {
         const int mc = e >> 8, cc = (e >> 4) & 15, j = e & 15;
@@ -193,7 +211,7 @@ __global__ __launch_bounds__(512) void k_fwd_mfma(const SuperDev *__restrict__ s
         const int la = l0 + r16;                 // A-operand row l
         for (int mc = 0; mc < kFwdMC; ++mc) {
           const int mpp = mp0 + mc;
-          if (mpp > l0 + 15 || mpp >= Lg) break;
+          if (mpp > l0 + 15 || mpp >= Lg || mpp > mpe) break;
           typename Mfma<T>::acc h = Mfma<T>::zero();
           for (int c0 = 0; c0 < nc; c0 += 4) {
             const int cc = c0 + k4;
@@ -235,6 +253,18 @@ __global__ __launch_bounds__(512) void k_fwd_mfma(const SuperDev *__restrict__ s
   }
 }
 
+__global__ void k_sum_parts(const cx<double> *__restrict__ part, int nsplit, size_t n, cx<double> *__restrict__ y) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double a = 0, b = 0;
+    for (int s = 0; s < nsplit; ++s) {
+      const cx<double> v = part[(size_t)s * n + i];
+      a += v.x;
+      b += v.y;
+    }
+    y[i] = mk<double>(a, b);
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 template <typename T>
 void launch_inv_mfma(const void *x, const T *dtab, const ColDev *cols, int ncol, const T *bvals, int L, void *Fbase,
@@ -246,17 +276,28 @@ void launch_inv_mfma(const void *x, const T *dtab, const ColDev *cols, int ncol,
 template void launch_inv_mfma<double>(const void *, const double *, const ColDev *, int, const double *, int, void *, hipStream_t);
 template void launch_inv_mfma<float>(const void *, const float *, const ColDev *, int, const float *, int, void *, hipStream_t);
 
+int fwd_splits(int L) { return std::max(1, std::min(8, L / 64)); }
+
 template <typename T>
 void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *bvals, const T *dtab, int L,
-                     const void *Jbase, void *y, hipStream_t st) {
+                     const void *Jbase, void *y, void *parts, hipStream_t st) {
   const int nb = (L + 15) / 16;
   const int npairs = (nb + 1) / 2;
-  dim3 grid((L + 3) / 4, (npairs + kFwdWaves - 1) / kFwdWaves);
+  const int S = fwd_splits(L);
+  int chunk = (L + S - 1) / S;
+  chunk = (chunk + kFwdMC - 1) / kFwdMC * kFwdMC;
+  dim3 grid((L + 3) / 4, (npairs + kFwdWaves - 1) / kFwdWaves, S);
+  cx<double> *out = (S == 1) ? (cx<double> *)y : (cx<double> *)parts;
   hipLaunchKernelGGL((k_fwd_mfma<T>), grid, dim3(64 * kFwdWaves), 0, st, sgs, nsg, cols, bvals, dtab, L,
-                     (const cx<T> *)Jbase, (cx<double> *)y);
+                     (const cx<T> *)Jbase, out, chunk);
+  if (S > 1) {
+    const size_t n = (size_t)L * L;
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0, st,
+                       (const cx<double> *)parts, S, n, (cx<double> *)y);
+  }
 }
-template void launch_fwd_mfma<double>(const SuperDev *, int, const ColDev *, const double *, const double *, int, const void *, void *, hipStream_t);
-template void launch_fwd_mfma<float>(const SuperDev *, int, const ColDev *, const float *, const float *, int, const void *, void *, hipStream_t);
+template void launch_fwd_mfma<double>(const SuperDev *, int, const ColDev *, const double *, const double *, int, const void *, void *, void *, hipStream_t);
+template void launch_fwd_mfma<float>(const SuperDev *, int, const ColDev *, const float *, const float *, int, const void *, void *, void *, hipStream_t);
 
 int mfma_selftest() {
   double *d64 = nullptr;

@@ -42,6 +42,7 @@ template <typename T>
 struct BetaArgs {
   int L;
   int tw;  // lines (m columns) per block
+  int ncol;
   const ColDev *cols;
   DftDesc<T> dM;          // length 2L-1
   DftDesc<T> dP;          // convolution length P >= 4L-3
@@ -78,7 +79,8 @@ void launch_inv_mfma(const void *x, const T *dtab, const ColDev *cols, int ncol,
                      hipStream_t st);
 template <typename T>
 void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *bvals, const T *dtab, int L,
-                     const void *Jbase, void *y, hipStream_t st);
+                     const void *Jbase, void *y, void *parts, hipStream_t st);
+int fwd_splits(int L);  // m' splits of the forward contraction (partials: fwd_splits(L) * L^2 complex128)
 int mfma_selftest();
 
 // kind: 0 inverse, 1 inverse adjoint, 2 forward, 3 forward adjoint

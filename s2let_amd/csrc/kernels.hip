@@ -12,6 +12,7 @@
 // Delta^l_{m'm}, m', m >= 0, and
 //     Delta^l_{m',-m} = (-1)^{l+m'} Delta^l_{m'm}.
 #include "kernels.h"
+#include "fft.h"
 
 namespace s2h {
 
@@ -225,15 +226,34 @@ __global__ __launch_bounds__(256) void k_contract_fwd(const GroupDev *__restrict
 }
 
 // ------------------------------------------------------------------ beta stages
-// Lines of one m-tile of one column are staged in LDS, tw lines of stride ls.
+// Lines of one m-tile of one column are staged in LDS (two buffers of tw lines
+// of stride ls for the ping-pong FFT).  Blocks are mapped so that each XCD
+// (blocks are dispatched round-robin over 8) works on a contiguous range of
+// (column, m-tile) items: neighbouring m-tiles share L2 lines.
+struct BlockItem {
+  int c, mi0, tw;
+};
+__device__ inline BlockItem beta_item(int ntiles, int ncol, int twmax, int M) {
+  const int B = ntiles * ncol;
+  const int lin = blockIdx.x;
+  const int x = lin & 7, k = lin >> 3;
+  const int q = B >> 3, r = B & 7;
+  const int w = x * q + min(x, r) + k;
+  BlockItem it;
+  it.c = w / ntiles;
+  it.mi0 = (w - it.c * ntiles) * twmax;
+  it.tw = min(twmax, M - it.mi0);
+  return it;
+}
 
 // F[m'][m] -> T[t][m], t < L (inverse direction, exponent +)
-template <typename T, int MAXR>
+template <typename T>
 __global__ __launch_bounds__(256) void k_beta_inv(const cx<T> *__restrict__ F, cx<T> *__restrict__ Tt, BetaArgs<T> a) {
   extern __shared__ char smem_raw[];
-  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  const int L = a.L, M = 2 * L - 1, c = blockIdx.y;
-  const int mi0 = blockIdx.x * a.tw, tw = min(a.tw, M - mi0);
+  const int L = a.L, M = 2 * L - 1;
+  cx<T> *A = reinterpret_cast<cx<T> *>(smem_raw), *Bf = A + (size_t)a.tw * M;
+  const BlockItem it = beta_item((M + a.tw - 1) / a.tw, a.ncol, a.tw, M);
+  const int c = it.c, mi0 = it.mi0, tw = it.tw;
   const ColDev col = a.cols[c];
   const cx<T> ph = mk<T>((T)col.ph_re, (T)col.ph_im);
   const cx<T> *Fc = F + (size_t)c * L * M;
@@ -241,45 +261,46 @@ __global__ __launch_bounds__(256) void k_beta_inv(const cx<T> *__restrict__ F, c
     const int mp = e / tw, k = e - mp * tw, mi = mi0 + k, m = mi - (L - 1);
     const cx<T> base = ipow_mul(-m, cmul(ph, Fc[(size_t)mp * M + mi]));
     const cx<T> sh = a.shift[mp];
-    buf[k * M + mp] = cmul(base, sh);
+    A[k * M + mp] = cmul(base, sh);
     if (mp > 0) {
       cx<T> v = cmulc(base, sh);
       if ((m + col.n) & 1) v = mk<T>(-v.x, -v.y);
-      buf[k * M + (M - mp)] = v;
+      A[k * M + (M - mp)] = v;
     }
   }
   __syncthreads();
-  lds_dft<T, MAXR>(buf, tw, M, a.dM, +1, L);
+  const cx<T> *R = lds_fft(A, Bf, tw, M, a.dM, +1);
   cx<T> *Tc = Tt + (size_t)c * L * M;
   for (int e = threadIdx.x; e < L * tw; e += blockDim.x) {
     const int t = e / tw, k = e - t * tw;
-    Tc[(size_t)t * M + mi0 + k] = buf[k * M + t];
+    Tc[(size_t)t * M + mi0 + k] = R[k * M + t];
   }
 }
 
 // adjoint of k_beta_inv: T~[t][m] (t < L) -> F~[m'][m]
-template <typename T, int MAXR>
+template <typename T>
 __global__ __launch_bounds__(256) void k_beta_inv_adj(const cx<T> *__restrict__ Tt, cx<T> *__restrict__ F, BetaArgs<T> a) {
   extern __shared__ char smem_raw[];
-  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  const int L = a.L, M = 2 * L - 1, c = blockIdx.y;
-  const int mi0 = blockIdx.x * a.tw, tw = min(a.tw, M - mi0);
+  const int L = a.L, M = 2 * L - 1;
+  cx<T> *A = reinterpret_cast<cx<T> *>(smem_raw), *Bf = A + (size_t)a.tw * M;
+  const BlockItem it = beta_item((M + a.tw - 1) / a.tw, a.ncol, a.tw, M);
+  const int c = it.c, mi0 = it.mi0, tw = it.tw;
   const ColDev col = a.cols[c];
   const cx<T> ph = mk<T>((T)col.ph_re, (T)col.ph_im);
   const cx<T> *Tc = Tt + (size_t)c * L * M;
   for (int e = threadIdx.x; e < M * tw; e += blockDim.x) {
     const int t = e / tw, k = e - t * tw;
-    buf[k * M + t] = (t < L) ? Tc[(size_t)t * M + mi0 + k] : mk<T>(0, 0);
+    A[k * M + t] = (t < L) ? Tc[(size_t)t * M + mi0 + k] : mk<T>(0, 0);
   }
   __syncthreads();
-  lds_dft<T, MAXR>(buf, tw, M, a.dM, -1, M);
+  const cx<T> *R = lds_fft(A, Bf, tw, M, a.dM, -1);
   cx<T> *Fc = F + (size_t)c * L * M;
   for (int e = threadIdx.x; e < L * tw; e += blockDim.x) {
     const int mp = e / tw, k = e - mp * tw, mi = mi0 + k, m = mi - (L - 1);
     const cx<T> sh = a.shift[mp];
-    cx<T> acc = cmulc(buf[k * M + mp], sh);  // * e^{-i m' pi/M}
+    cx<T> acc = cmulc(R[k * M + mp], sh);  // * e^{-i m' pi/M}
     if (mp > 0) {
-      cx<T> v = cmul(buf[k * M + (M - mp)], sh);  // * e^{+i m' pi/M}
+      cx<T> v = cmul(R[k * M + (M - mp)], sh);  // * e^{+i m' pi/M}
       if ((m + col.n) & 1) v = mk<T>(-v.x, -v.y);
       acc = cadd(acc, v);
     }
@@ -288,12 +309,13 @@ __global__ __launch_bounds__(256) void k_beta_inv_adj(const cx<T> *__restrict__ 
 }
 
 // G[t][m] (t < L) -> Jf[m'][m]: extension, theta DFT (-), weights convolution, fold
-template <typename T, int MAXR>
+template <typename T>
 __global__ __launch_bounds__(256) void k_beta_fwd(const cx<T> *__restrict__ G, cx<T> *__restrict__ Jf, BetaArgs<T> a) {
   extern __shared__ char smem_raw[];
-  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  const int L = a.L, M = 2 * L - 1, P = a.dP.n, c = blockIdx.y;
-  const int mi0 = blockIdx.x * a.tw, tw = min(a.tw, M - mi0);
+  const int L = a.L, M = 2 * L - 1, P = a.dP.n;
+  cx<T> *A = reinterpret_cast<cx<T> *>(smem_raw), *Bf = A + (size_t)a.tw * P;
+  const BlockItem it = beta_item((M + a.tw - 1) / a.tw, a.ncol, a.tw, M);
+  const int c = it.c, mi0 = it.mi0, tw = it.tw;
   const ColDev col = a.cols[c];
   const cx<T> ph = mk<T>((T)col.ph_re, (T)col.ph_im);
   const cx<T> *Gc = G + (size_t)c * L * M;
@@ -305,54 +327,35 @@ __global__ __launch_bounds__(256) void k_beta_fwd(const cx<T> *__restrict__ G, c
       v = Gc[(size_t)(M - 1 - t) * M + mi];
       if ((m + col.n) & 1) v = mk<T>(-v.x, -v.y);
     }
-    buf[k * P + t] = v;
+    A[k * P + t] = v;
   }
   __syncthreads();
-  lds_dft<T, MAXR>(buf, tw, P, a.dM, -1, M);
-  // reorder into the convolution line: col[(-m'') mod P] = s[m''] e^{-i m'' pi/M} / M
-  {
-    cx<T> res[MAXR];
-    const T invM = (T)1 / (T)M;
-#pragma unroll
-    for (int r = 0; r < MAXR; ++r) {
-      const int e = threadIdx.x + r * blockDim.x;
-      if (e < tw * M) {
-        const int k = e / M, q = e - k * M;       // q = m'' mod M
-        const int mpp = (q < L) ? q : q - M;      // m''
-        cx<T> s = buf[k * P + q];
-        const cx<T> sh = a.shift[mpp < 0 ? -mpp : mpp];
-        s = (mpp >= 0) ? cmulc(s, sh) : cmul(s, sh);
-        res[r] = cscale(s, invM);
-      }
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < tw * P; e += blockDim.x) buf[e] = mk<T>(0, 0);
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < MAXR; ++r) {
-      const int e = threadIdx.x + r * blockDim.x;
-      if (e < tw * M) {
-        const int k = e / M, q = e - k * M;
-        const int mpp = (q < L) ? q : q - M;
-        buf[k * P + ((-mpp + P) % P)] = res[r];
-      }
-    }
-    __syncthreads();
-  }
-  lds_dft<T, MAXR>(buf, tw, P, a.dP, -1, P);
-  for (int e = threadIdx.x; e < tw * P; e += blockDim.x) {
-    const int q = e % P;
-    buf[e] = cmul(buf[e], a.wconv[q]);
+  cx<T> *R1 = lds_fft(A, Bf, tw, P, a.dM, -1);
+  cx<T> *O1 = (R1 == A) ? Bf : A;
+  for (int e = threadIdx.x; e < tw * P; e += blockDim.x) O1[e] = mk<T>(0, 0);
+  __syncthreads();
+  // convolution line: col[(-m'') mod P] = s[m''] e^{-i m'' pi/M} / M
+  const T invM = (T)1 / (T)M;
+  for (int e = threadIdx.x; e < tw * M; e += blockDim.x) {
+    const int k = e / M, q = e - k * M;
+    const int mpp = (q < L) ? q : q - M;
+    cx<T> s = R1[k * P + q];
+    const cx<T> sh = a.shift[mpp < 0 ? -mpp : mpp];
+    s = (mpp >= 0) ? cmulc(s, sh) : cmul(s, sh);
+    O1[k * P + ((-mpp + P) % P)] = cscale(s, invM);
   }
   __syncthreads();
-  lds_dft<T, MAXR>(buf, tw, P, a.dP, +1, P);
+  cx<T> *R2 = lds_fft(O1, R1, tw, P, a.dP, -1);
+  for (int e = threadIdx.x; e < tw * P; e += blockDim.x) R2[e] = cmul(R2[e], a.wconv[e % P]);
+  __syncthreads();
+  cx<T> *R3 = lds_fft(R2, (R2 == A) ? Bf : A, tw, P, a.dP, +1);
   cx<T> *Jc = Jf + (size_t)c * L * M;
   const T invP = (T)1 / (T)P;
   for (int e = threadIdx.x; e < L * tw; e += blockDim.x) {
     const int mp = e / tw, k = e - mp * tw, mi = mi0 + k, m = mi - (L - 1);
-    cx<T> v = buf[k * P + mp];
+    cx<T> v = R3[k * P + mp];
     if (mp > 0) {
-      cx<T> u = buf[k * P + (P - mp)];
+      cx<T> u = R3[k * P + (P - mp)];
       if ((m + col.n) & 1) u = mk<T>(-u.x, -u.y);
       v = cadd(v, u);
     }
@@ -361,64 +364,47 @@ __global__ __launch_bounds__(256) void k_beta_fwd(const cx<T> *__restrict__ G, c
 }
 
 // adjoint of k_beta_fwd: Jf~[m'][m] -> G~[t][m] (t < L)
-template <typename T, int MAXR>
+template <typename T>
 __global__ __launch_bounds__(256) void k_beta_fwd_adj(const cx<T> *__restrict__ Jf, cx<T> *__restrict__ G, BetaArgs<T> a) {
   extern __shared__ char smem_raw[];
-  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  const int L = a.L, M = 2 * L - 1, P = a.dP.n, c = blockIdx.y;
-  const int mi0 = blockIdx.x * a.tw, tw = min(a.tw, M - mi0);
+  const int L = a.L, M = 2 * L - 1, P = a.dP.n;
+  cx<T> *A = reinterpret_cast<cx<T> *>(smem_raw), *Bf = A + (size_t)a.tw * P;
+  const BlockItem it = beta_item((M + a.tw - 1) / a.tw, a.ncol, a.tw, M);
+  const int c = it.c, mi0 = it.mi0, tw = it.tw;
   const ColDev col = a.cols[c];
   const cx<T> ph = mk<T>((T)col.ph_re, (T)col.ph_im);
   const cx<T> *Jc = Jf + (size_t)c * L * M;
-  for (int e = threadIdx.x; e < tw * P; e += blockDim.x) buf[e] = mk<T>(0, 0);
+  for (int e = threadIdx.x; e < tw * P; e += blockDim.x) A[e] = mk<T>(0, 0);
   __syncthreads();
   for (int e = threadIdx.x; e < L * tw; e += blockDim.x) {
     const int mp = e / tw, k = e - mp * tw, mi = mi0 + k, m = mi - (L - 1);
     const cx<T> v = ipow_mul(m, cmulc(Jc[(size_t)mp * M + mi], ph));
-    buf[k * P + mp] = v;  // rows m' and P - m' are distinct for m' in [1, L)
-    if (mp > 0) buf[k * P + (P - mp)] = ((m + col.n) & 1) ? mk<T>(-v.x, -v.y) : v;
+    A[k * P + mp] = v;  // rows m' and P - m' are distinct for m' in [1, L)
+    if (mp > 0) A[k * P + (P - mp)] = ((m + col.n) & 1) ? mk<T>(-v.x, -v.y) : v;
   }
   __syncthreads();
-  lds_dft<T, MAXR>(buf, tw, P, a.dP, -1, P);
-  for (int e = threadIdx.x; e < tw * P; e += blockDim.x) {
-    const int q = e % P;
-    buf[e] = cmulc(buf[e], a.wconv[q]);
+  cx<T> *R1 = lds_fft(A, Bf, tw, P, a.dP, -1);
+  for (int e = threadIdx.x; e < tw * P; e += blockDim.x) R1[e] = cmulc(R1[e], a.wconv[e % P]);
+  __syncthreads();
+  cx<T> *R2 = lds_fft(R1, (R1 == A) ? Bf : A, tw, P, a.dP, +1);
+  cx<T> *O2 = (R2 == A) ? Bf : A;
+  const T scl = (T)1 / ((T)P * (T)M);
+  for (int e = threadIdx.x; e < tw * M; e += blockDim.x) {
+    const int k = e / M, q = e - k * M;
+    const int mpp = (q < L) ? q : q - M;
+    cx<T> s = R2[k * P + ((-mpp + P) % P)];
+    const cx<T> sh = a.shift[mpp < 0 ? -mpp : mpp];
+    s = (mpp >= 0) ? cmul(s, sh) : cmulc(s, sh);  // * e^{+i m'' pi/M}
+    O2[k * P + q] = cscale(s, scl);
   }
   __syncthreads();
-  lds_dft<T, MAXR>(buf, tw, P, a.dP, +1, P);
-  {
-    cx<T> res[MAXR];
-    const T scl = (T)1 / ((T)P * (T)M);
-#pragma unroll
-    for (int r = 0; r < MAXR; ++r) {
-      const int e = threadIdx.x + r * blockDim.x;
-      if (e < tw * M) {
-        const int k = e / M, q = e - k * M;
-        const int mpp = (q < L) ? q : q - M;
-        cx<T> s = buf[k * P + ((-mpp + P) % P)];
-        const cx<T> sh = a.shift[mpp < 0 ? -mpp : mpp];
-        s = (mpp >= 0) ? cmul(s, sh) : cmulc(s, sh);  // * e^{+i m'' pi/M}
-        res[r] = cscale(s, scl);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < MAXR; ++r) {
-      const int e = threadIdx.x + r * blockDim.x;
-      if (e < tw * M) {
-        const int k = e / M, q = e - k * M;
-        buf[k * P + q] = res[r];
-      }
-    }
-    __syncthreads();
-  }
-  lds_dft<T, MAXR>(buf, tw, P, a.dM, +1, M);
+  const cx<T> *R3 = lds_fft(O2, R2, tw, P, a.dM, +1);
   cx<T> *Gc = G + (size_t)c * L * M;
   for (int e = threadIdx.x; e < L * tw; e += blockDim.x) {
     const int t = e / tw, k = e - t * tw, mi = mi0 + k, m = mi - (L - 1);
-    cx<T> v = buf[k * P + t];
+    cx<T> v = R3[k * P + t];
     if (t < L - 1) {
-      cx<T> u = buf[k * P + (M - 1 - t)];
+      cx<T> u = R3[k * P + (M - 1 - t)];
       if ((m + col.n) & 1) u = mk<T>(-u.x, -u.y);
       v = cadd(v, u);
     }
@@ -429,11 +415,11 @@ __global__ __launch_bounds__(256) void k_beta_fwd_adj(const cx<T> *__restrict__ 
 // ------------------------------------------------------------------ row DFTs (alpha / phi)
 // nlines contiguous lines of length M (stride M).  Input/output either in
 // natural DFT order or centred (index m + L - 1 for m in [-(L-1), L-1]).
-template <typename TI, typename TO, typename T, int MAXR>
+template <typename TI, typename TO, typename T>
 __global__ __launch_bounds__(256) void k_rows(const cx<TI> *__restrict__ in, cx<TO> *__restrict__ out, RowArgs<T> a) {
   extern __shared__ char smem_raw[];
-  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
   const int L = a.L, M = 2 * L - 1;
+  cx<T> *A = reinterpret_cast<cx<T> *>(smem_raw), *Bf = A + (size_t)a.lpb * M;
   const size_t line0 = (size_t)blockIdx.x * a.lpb;
   const int nl = (int)min((size_t)a.lpb, a.nlines - line0);
   for (int e = threadIdx.x; e < nl * M; e += blockDim.x) {
@@ -441,15 +427,15 @@ __global__ __launch_bounds__(256) void k_rows(const cx<TI> *__restrict__ in, cx<
     const cx<TI> v = in[(line0 + k) * M + i];
     int q = i;
     if (a.in_centred) q = (i - (L - 1) + M) % M;
-    buf[k * M + q] = mk<T>((T)v.x, (T)v.y);
+    A[k * M + q] = mk<T>((T)v.x, (T)v.y);
   }
   __syncthreads();
-  lds_dft<T, MAXR>(buf, nl, M, a.dM, a.sign, M);
+  const cx<T> *R = lds_fft(A, Bf, nl, M, a.dM, a.sign);
   for (int e = threadIdx.x; e < nl * M; e += blockDim.x) {
     const int k = e / M, i = e - k * M;
     int q = i;
     if (a.out_centred) q = (i - (L - 1) + M) % M;
-    const cx<T> v = buf[k * M + q];
+    const cx<T> v = R[k * M + q];
     out[(line0 + k) * M + i] = mk<TO>((TO)(v.x * a.scale), (TO)(v.y * a.scale));
   }
 }
@@ -538,17 +524,18 @@ static constexpr int beta_maxr() { return sizeof(T) == 8 ? 16 : 32; }
 template <typename T>
 void launch_beta(int kind, const void *in, void *out, const BetaArgs<T> &a, int ncol, hipStream_t st) {
   const int M = 2 * a.L - 1;
-  dim3 grid((M + a.tw - 1) / a.tw, ncol);
+  BetaArgs<T> b = a;
+  b.ncol = ncol;
+  dim3 grid(((M + a.tw - 1) / a.tw) * ncol);
   const int lstride = (kind == 0 || kind == 1) ? M : a.dP.n;
-  const size_t shm = (size_t)a.tw * lstride * sizeof(cx<T>);
-  constexpr int R = beta_maxr<T>();
+  const size_t shm = 2 * (size_t)a.tw * lstride * sizeof(cx<T>);
   const cx<T> *i = (const cx<T> *)in;
   cx<T> *o = (cx<T> *)out;
   switch (kind) {
-    case 0: hipLaunchKernelGGL((k_beta_inv<T, R>), grid, dim3(256), shm, st, i, o, a); break;
-    case 1: hipLaunchKernelGGL((k_beta_inv_adj<T, R>), grid, dim3(256), shm, st, i, o, a); break;
-    case 2: hipLaunchKernelGGL((k_beta_fwd<T, R>), grid, dim3(256), shm, st, i, o, a); break;
-    case 3: hipLaunchKernelGGL((k_beta_fwd_adj<T, R>), grid, dim3(256), shm, st, i, o, a); break;
+    case 0: hipLaunchKernelGGL((k_beta_inv<T>), grid, dim3(256), shm, st, i, o, b); break;
+    case 1: hipLaunchKernelGGL((k_beta_inv_adj<T>), grid, dim3(256), shm, st, i, o, b); break;
+    case 2: hipLaunchKernelGGL((k_beta_fwd<T>), grid, dim3(256), shm, st, i, o, b); break;
+    case 3: hipLaunchKernelGGL((k_beta_fwd_adj<T>), grid, dim3(256), shm, st, i, o, b); break;
   }
 }
 template void launch_beta<double>(int, const void *, void *, const BetaArgs<double> &, int, hipStream_t);
@@ -558,9 +545,8 @@ template <typename TI, typename TO, typename T>
 void launch_rows(const void *in, void *out, const RowArgs<T> &a, hipStream_t st) {
   const int M = 2 * a.L - 1;
   const size_t nb = (a.nlines + a.lpb - 1) / a.lpb;
-  const size_t shm = (size_t)a.lpb * M * sizeof(cx<T>);
-  constexpr int R = beta_maxr<T>();
-  hipLaunchKernelGGL((k_rows<TI, TO, T, R>), dim3((unsigned)nb), dim3(256), shm, st, (const cx<TI> *)in, (cx<TO> *)out, a);
+  const size_t shm = 2 * (size_t)a.lpb * M * sizeof(cx<T>);
+  hipLaunchKernelGGL((k_rows<TI, TO, T>), dim3((unsigned)nb), dim3(256), shm, st, (const cx<TI> *)in, (cx<TO> *)out, a);
 }
 template void launch_rows<double, double, double>(const void *, void *, const RowArgs<double> &, hipStream_t);
 template void launch_rows<float, float, float>(const void *, void *, const RowArgs<float> &, hipStream_t);

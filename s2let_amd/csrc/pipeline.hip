@@ -2,6 +2,7 @@
 // DFTs and gamma mixing, per group, in FP64 or in the mixed mode (FP32 stages
 // and contractions, FP64 inputs/outputs and Delta recursion).
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <map>
 #include <string>
@@ -104,6 +105,18 @@ constexpr int maxr() {
   return sizeof(T) == 8 ? 16 : 32;
 }
 
+// LDS bytes one block of a DFT stage may use: half the device's per-block
+// maximum (two blocks per CU), overridable by S2LET_LDS_BUDGET.
+size_t lds_budget(Context &C) {
+  static size_t v = 0;
+  if (!v) {
+    const char *e = getenv("S2LET_LDS_BUDGET");
+    v = e ? (size_t)atol(e) : C.lds_bytes() / 2;
+    v = std::min(v, C.lds_bytes());
+  }
+  return v;
+}
+
 template <typename T>
 BetaArgs<T> beta_args(int L, const ColDev *cols, bool conv) {
   Context &C = Context::get();
@@ -117,7 +130,7 @@ BetaArgs<T> beta_args(int L, const ColDev *cols, bool conv) {
   a.wconv = Sel<T>::wconv(lt);
   const int M = 2 * L - 1;
   const int lstride = conv ? lt.P : M;
-  const size_t cap = std::min((size_t)maxr<T>() * 256, C.lds_bytes() / sizeof(cx<T>));
+  const size_t cap = lds_budget(C) / (2 * sizeof(cx<T>));  // two ping-pong buffers
   int tw = (int)(cap / lstride);
   if (tw < 1) {
     set_error(-3, "band-limit too large for the LDS beta stage on this device");
@@ -140,7 +153,11 @@ RowArgs<T> row_args(int L, size_t nlines, int in_c, int out_c, int sign, double 
   a.scale = (T)scale;
   a.dM = Sel<T>::d(*lt.dM);
   const int M = 2 * L - 1;
-  const size_t cap = std::min((size_t)maxr<T>() * 256, C.lds_bytes() / sizeof(cx<T>));
+  const size_t cap = lds_budget(C) / (2 * sizeof(cx<T>));
+  if ((size_t)M > cap) {
+    set_error(-3, "band-limit too large for the LDS row stage on this device");
+    throw ArgFailure();
+  }
   a.lpb = (int)std::max<size_t>(1, std::min<size_t>(cap / M, 64));
   return a;
 }
@@ -174,7 +191,7 @@ void contract_fwd_all(Plan &P, const ColDev *cols, const void *Jbase, void *flm,
   Context &C = Context::get();
   Stage s_("contract_fwd", st);
   launch_fwd_mfma<T>(P.d_supers.as<SuperDev>(), (int)P.supers.size(), cols, Sel<T>::b(P), Sel<T>::delta(C), P.p.L,
-                     Jbase, flm, st);
+                     Jbase, flm, P.parts.p, st);
 }
 
 template <typename T>
@@ -284,7 +301,7 @@ void sht_forward_t(ShtPlan &S, const void *f, void *flm, hipStream_t st) {
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("rows", st); launch_rows<double, T, T>(f, S.spec.p, row_args<T>(L, (size_t)L, 0, 1, -1, kTwoPi / M), st); }
   { Stage s_("beta_fwd", st); launch_beta<T>(2, S.spec.p, S.work.p, beta_args<T>(L, col, true), 1, st); }
-  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, Sel<T>::b(S), Sel<T>::delta(C), L, S.work.p, flm, st); }
+  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, Sel<T>::b(S), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, st); }
 }
 
 template <typename T>
@@ -294,7 +311,7 @@ void sht_inverse_adjoint_t(ShtPlan &S, const void *f, void *flm, hipStream_t st)
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("rows", st); launch_rows<double, T, T>(f, S.spec.p, row_args<T>(L, (size_t)L, 0, 1, -1, 1.0), st); }
   { Stage s_("beta_inv_adj", st); launch_beta<T>(1, S.spec.p, S.work.p, beta_args<T>(L, col, false), 1, st); }
-  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, Sel<T>::b(S), Sel<T>::delta(C), L, S.work.p, flm, st); }
+  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, Sel<T>::b(S), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, st); }
 }
 
 template <typename T>

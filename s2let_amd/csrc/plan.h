@@ -68,6 +68,7 @@ struct Plan {
   DevBuf d_psi, d_phi;
   DevBuf spec, work;  // scratch (T = double or float per mode)
   DevBuf flm32;       // mixed mode: FP32 copy of the input harmonics
+  DevBuf parts;       // forward-contraction partial sums over m' splits
   std::vector<size_t> gamma_off;  // per group: offsets of its 4 mixing matrices in d_gamma
   // staging for the host-pointer API
   DevBuf h_flm, h_wav, h_scal, h_map, h_tmp;
@@ -79,7 +80,7 @@ struct ShtPlan {
   int L, spin, mode;
   ColDev col;
   std::vector<double> bvals;
-  DevBuf d_col, d_b64, d_b32, d_group_spec, d_group_work, spec, work, flm32, d_super;
+  DevBuf d_col, d_b64, d_b32, d_group_spec, d_group_work, spec, work, flm32, d_super, parts;
 };
 
 class Context {

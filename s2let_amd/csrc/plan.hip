@@ -373,6 +373,7 @@ Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
   pl.spec.alloc(pl.spec_elems * esz);
   pl.work.alloc(pl.spec_elems * esz);
   if (f32) pl.flm32.alloc((size_t)L * L * sizeof(cx<float>));
+  pl.parts.alloc((size_t)fwd_splits(L) * L * L * sizeof(cx<double>));
   std::vector<GroupDev> gs, gw;
   for (const Group &g : pl.groups) {
     GroupDev d{g.L, g.ncol, g.col0, 0, (const char *)pl.spec.p + g.spec_off * esz};
@@ -425,6 +426,7 @@ ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st) {
   s.spec.alloc(ne * esz);
   s.work.alloc(ne * esz);
   if (f32) s.flm32.alloc((size_t)L * L * sizeof(cx<float>));
+  s.parts.alloc((size_t)fwd_splits(L) * L * L * sizeof(cx<double>));
   std::vector<GroupDev> gs{GroupDev{L, 1, 0, 0, s.spec.p}}, gw{GroupDev{L, 1, 0, 0, s.work.p}};
   upload(s.d_group_spec, gs, st);
   upload(s.d_group_work, gw, st);
