@@ -171,6 +171,16 @@ __device__ inline void fft_stage_generic(const cx<T> *__restrict__ in, cx<T> *__
   }
 }
 
+// Copies the twiddle table of d into LDS at dst (n entries) and returns a
+// descriptor that reads it there.  The caller issues the barrier.
+template <typename T>
+__device__ inline DftDesc<T> lds_twiddles(const DftDesc<T> &d, cx<T> *dst) {
+  for (int i = threadIdx.x; i < d.n; i += blockDim.x) dst[i] = d.tw[i];
+  DftDesc<T> r = d;
+  r.tw = dst;
+  return r;
+}
+
 // Transforms nlines lines (stride ls) held in A; B is scratch of the same
 // shape.  Returns the buffer that holds the result.  Ends with a barrier.
 template <typename T>

@@ -252,6 +252,7 @@ __global__ __launch_bounds__(256) void k_beta_inv(const cx<T> *__restrict__ F, c
   extern __shared__ char smem_raw[];
   const int L = a.L, M = 2 * L - 1;
   cx<T> *A = reinterpret_cast<cx<T> *>(smem_raw), *Bf = A + (size_t)a.tw * M;
+  const DftDesc<T> dM = lds_twiddles(a.dM, Bf + (size_t)a.tw * M);
   const BlockItem it = beta_item((M + a.tw - 1) / a.tw, a.ncol, a.tw, M);
   const int c = it.c, mi0 = it.mi0, tw = it.tw;
   const ColDev col = a.cols[c];
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(256) void k_beta_inv(const cx<T> *__restrict__ F, c
     }
   }
   __syncthreads();
-  const cx<T> *R = lds_fft(A, Bf, tw, M, a.dM, +1);
+  const cx<T> *R = lds_fft(A, Bf, tw, M, dM, +1);
   cx<T> *Tc = Tt + (size_t)c * L * M;
   for (int e = threadIdx.x; e < L * tw; e += blockDim.x) {
     const int t = e / tw, k = e - t * tw;
@@ -283,6 +284,7 @@ __global__ __launch_bounds__(256) void k_beta_inv_adj(const cx<T> *__restrict__ 
   extern __shared__ char smem_raw[];
   const int L = a.L, M = 2 * L - 1;
   cx<T> *A = reinterpret_cast<cx<T> *>(smem_raw), *Bf = A + (size_t)a.tw * M;
+  const DftDesc<T> dM = lds_twiddles(a.dM, Bf + (size_t)a.tw * M);
   const BlockItem it = beta_item((M + a.tw - 1) / a.tw, a.ncol, a.tw, M);
   const int c = it.c, mi0 = it.mi0, tw = it.tw;
   const ColDev col = a.cols[c];
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(256) void k_beta_inv_adj(const cx<T> *__restrict__ 
     A[k * M + t] = (t < L) ? Tc[(size_t)t * M + mi0 + k] : mk<T>(0, 0);
   }
   __syncthreads();
-  const cx<T> *R = lds_fft(A, Bf, tw, M, a.dM, -1);
+  const cx<T> *R = lds_fft(A, Bf, tw, M, dM, -1);
   cx<T> *Fc = F + (size_t)c * L * M;
   for (int e = threadIdx.x; e < L * tw; e += blockDim.x) {
     const int mp = e / tw, k = e - mp * tw, mi = mi0 + k, m = mi - (L - 1);
@@ -314,6 +316,7 @@ __global__ __launch_bounds__(256) void k_beta_fwd(const cx<T> *__restrict__ G, c
   extern __shared__ char smem_raw[];
   const int L = a.L, M = 2 * L - 1, P = a.dP.n;
   cx<T> *A = reinterpret_cast<cx<T> *>(smem_raw), *Bf = A + (size_t)a.tw * P;
+  const DftDesc<T> &dM = a.dM, &dP = a.dP;  // twiddles stay in global memory (L1/L2): LDS holds the lines
   const BlockItem it = beta_item((M + a.tw - 1) / a.tw, a.ncol, a.tw, M);
   const int c = it.c, mi0 = it.mi0, tw = it.tw;
   const ColDev col = a.cols[c];
@@ -330,7 +333,7 @@ __global__ __launch_bounds__(256) void k_beta_fwd(const cx<T> *__restrict__ G, c
     A[k * P + t] = v;
   }
   __syncthreads();
-  cx<T> *R1 = lds_fft(A, Bf, tw, P, a.dM, -1);
+  cx<T> *R1 = lds_fft(A, Bf, tw, P, dM, -1);
   cx<T> *O1 = (R1 == A) ? Bf : A;
   for (int e = threadIdx.x; e < tw * P; e += blockDim.x) O1[e] = mk<T>(0, 0);
   __syncthreads();
@@ -345,10 +348,10 @@ __global__ __launch_bounds__(256) void k_beta_fwd(const cx<T> *__restrict__ G, c
     O1[k * P + ((-mpp + P) % P)] = cscale(s, invM);
   }
   __syncthreads();
-  cx<T> *R2 = lds_fft(O1, R1, tw, P, a.dP, -1);
+  cx<T> *R2 = lds_fft(O1, R1, tw, P, dP, -1);
   for (int e = threadIdx.x; e < tw * P; e += blockDim.x) R2[e] = cmul(R2[e], a.wconv[e % P]);
   __syncthreads();
-  cx<T> *R3 = lds_fft(R2, (R2 == A) ? Bf : A, tw, P, a.dP, +1);
+  cx<T> *R3 = lds_fft(R2, (R2 == A) ? Bf : A, tw, P, dP, +1);
   cx<T> *Jc = Jf + (size_t)c * L * M;
   const T invP = (T)1 / (T)P;
   for (int e = threadIdx.x; e < L * tw; e += blockDim.x) {
@@ -369,6 +372,7 @@ __global__ __launch_bounds__(256) void k_beta_fwd_adj(const cx<T> *__restrict__ 
   extern __shared__ char smem_raw[];
   const int L = a.L, M = 2 * L - 1, P = a.dP.n;
   cx<T> *A = reinterpret_cast<cx<T> *>(smem_raw), *Bf = A + (size_t)a.tw * P;
+  const DftDesc<T> &dM = a.dM, &dP = a.dP;  // twiddles stay in global memory (L1/L2): LDS holds the lines
   const BlockItem it = beta_item((M + a.tw - 1) / a.tw, a.ncol, a.tw, M);
   const int c = it.c, mi0 = it.mi0, tw = it.tw;
   const ColDev col = a.cols[c];
@@ -383,10 +387,10 @@ __global__ __launch_bounds__(256) void k_beta_fwd_adj(const cx<T> *__restrict__ 
     if (mp > 0) A[k * P + (P - mp)] = ((m + col.n) & 1) ? mk<T>(-v.x, -v.y) : v;
   }
   __syncthreads();
-  cx<T> *R1 = lds_fft(A, Bf, tw, P, a.dP, -1);
+  cx<T> *R1 = lds_fft(A, Bf, tw, P, dP, -1);
   for (int e = threadIdx.x; e < tw * P; e += blockDim.x) R1[e] = cmulc(R1[e], a.wconv[e % P]);
   __syncthreads();
-  cx<T> *R2 = lds_fft(R1, (R1 == A) ? Bf : A, tw, P, a.dP, +1);
+  cx<T> *R2 = lds_fft(R1, (R1 == A) ? Bf : A, tw, P, dP, +1);
   cx<T> *O2 = (R2 == A) ? Bf : A;
   const T scl = (T)1 / ((T)P * (T)M);
   for (int e = threadIdx.x; e < tw * M; e += blockDim.x) {
@@ -398,7 +402,7 @@ __global__ __launch_bounds__(256) void k_beta_fwd_adj(const cx<T> *__restrict__ 
     O2[k * P + q] = cscale(s, scl);
   }
   __syncthreads();
-  const cx<T> *R3 = lds_fft(O2, R2, tw, P, a.dM, +1);
+  const cx<T> *R3 = lds_fft(O2, R2, tw, P, dM, +1);
   cx<T> *Gc = G + (size_t)c * L * M;
   for (int e = threadIdx.x; e < L * tw; e += blockDim.x) {
     const int t = e / tw, k = e - t * tw, mi = mi0 + k, m = mi - (L - 1);
@@ -420,6 +424,7 @@ __global__ __launch_bounds__(256) void k_rows(const cx<TI> *__restrict__ in, cx<
   extern __shared__ char smem_raw[];
   const int L = a.L, M = 2 * L - 1;
   cx<T> *A = reinterpret_cast<cx<T> *>(smem_raw), *Bf = A + (size_t)a.lpb * M;
+  const DftDesc<T> dM = lds_twiddles(a.dM, Bf + (size_t)a.lpb * M);
   const size_t line0 = (size_t)blockIdx.x * a.lpb;
   const int nl = (int)min((size_t)a.lpb, a.nlines - line0);
   for (int e = threadIdx.x; e < nl * M; e += blockDim.x) {
@@ -430,7 +435,7 @@ __global__ __launch_bounds__(256) void k_rows(const cx<TI> *__restrict__ in, cx<
     A[k * M + q] = mk<T>((T)v.x, (T)v.y);
   }
   __syncthreads();
-  const cx<T> *R = lds_fft(A, Bf, nl, M, a.dM, a.sign);
+  const cx<T> *R = lds_fft(A, Bf, nl, M, dM, a.sign);
   for (int e = threadIdx.x; e < nl * M; e += blockDim.x) {
     const int k = e / M, i = e - k * M;
     int q = i;
@@ -445,6 +450,9 @@ __global__ __launch_bounds__(256) void k_rows(const cx<TI> *__restrict__ in, cx<
 template <typename TI, typename TO, typename T>
 __global__ void k_mix(const cx<TI> *__restrict__ in, size_t in_stride, int nin, cx<TO> *__restrict__ out,
                       size_t out_stride, int nout, const cx<double> *__restrict__ W, size_t npts) {
+  __shared__ cx<T> sW[32 * 32];
+  for (int i = threadIdx.x; i < nin * nout; i += blockDim.x) sW[i] = mk<T>((T)W[i].x, (T)W[i].y);
+  __syncthreads();
   for (size_t pt = blockIdx.x * (size_t)blockDim.x + threadIdx.x; pt < npts; pt += (size_t)gridDim.x * blockDim.x) {
     cx<T> v[32];
     for (int j = 0; j < nin; ++j) {
@@ -454,9 +462,9 @@ __global__ void k_mix(const cx<TI> *__restrict__ in, size_t in_stride, int nin, 
     for (int k = 0; k < nout; ++k) {
       T ax = 0, ay = 0;
       for (int j = 0; j < nin; ++j) {
-        const cx<double> w = W[k * nin + j];
-        ax += v[j].x * (T)w.x - v[j].y * (T)w.y;
-        ay += v[j].x * (T)w.y + v[j].y * (T)w.x;
+        const cx<T> w = sW[k * nin + j];
+        ax += v[j].x * w.x - v[j].y * w.y;
+        ay += v[j].x * w.y + v[j].y * w.x;
       }
       out[k * out_stride + pt] = mk<TO>((TO)ax, (TO)ay);
     }
@@ -528,7 +536,7 @@ void launch_beta(int kind, const void *in, void *out, const BetaArgs<T> &a, int 
   b.ncol = ncol;
   dim3 grid(((M + a.tw - 1) / a.tw) * ncol);
   const int lstride = (kind == 0 || kind == 1) ? M : a.dP.n;
-  const size_t shm = 2 * (size_t)a.tw * lstride * sizeof(cx<T>);
+  const size_t shm = (2 * (size_t)a.tw * lstride + ((kind >= 2) ? 0 : M)) * sizeof(cx<T>);
   const cx<T> *i = (const cx<T> *)in;
   cx<T> *o = (cx<T> *)out;
   switch (kind) {
@@ -545,7 +553,7 @@ template <typename TI, typename TO, typename T>
 void launch_rows(const void *in, void *out, const RowArgs<T> &a, hipStream_t st) {
   const int M = 2 * a.L - 1;
   const size_t nb = (a.nlines + a.lpb - 1) / a.lpb;
-  const size_t shm = 2 * (size_t)a.lpb * M * sizeof(cx<T>);
+  const size_t shm = (2 * (size_t)a.lpb * M + M) * sizeof(cx<T>);
   hipLaunchKernelGGL((k_rows<TI, TO, T>), dim3((unsigned)nb), dim3(256), shm, st, (const cx<TI> *)in, (cx<TO> *)out, a);
 }
 template void launch_rows<double, double, double>(const void *, void *, const RowArgs<double> &, hipStream_t);

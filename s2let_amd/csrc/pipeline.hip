@@ -130,8 +130,13 @@ BetaArgs<T> beta_args(int L, const ColDev *cols, bool conv) {
   a.wconv = Sel<T>::wconv(lt);
   const int M = 2 * L - 1;
   const int lstride = conv ? lt.P : M;
-  const size_t cap = lds_budget(C) / (2 * sizeof(cx<T>));  // two ping-pong buffers
+  const size_t twb = conv ? 0 : (size_t)M * sizeof(cx<T>);  // beta_inv kernels keep their twiddles in LDS
+  const size_t bud = lds_budget(C) > twb ? lds_budget(C) - twb : 0;
+  const size_t cap = bud / (2 * sizeof(cx<T>));  // two ping-pong buffers
   int tw = (int)(cap / lstride);
+  if (tw < 1 && C.lds_bytes() > twb) {  // one line needs more than the default share: use the whole LDS
+    tw = (int)((C.lds_bytes() - twb) / (2 * sizeof(cx<T>)) / lstride);
+  }
   if (tw < 1) {
     set_error(-3, "band-limit too large for the LDS beta stage on this device");
     throw ArgFailure();
@@ -153,7 +158,8 @@ RowArgs<T> row_args(int L, size_t nlines, int in_c, int out_c, int sign, double 
   a.scale = (T)scale;
   a.dM = Sel<T>::d(*lt.dM);
   const int M = 2 * L - 1;
-  const size_t cap = lds_budget(C) / (2 * sizeof(cx<T>));
+  const size_t twb = (size_t)M * sizeof(cx<T>);
+  const size_t cap = (lds_budget(C) > twb ? lds_budget(C) - twb : 0) / (2 * sizeof(cx<T>));
   if ((size_t)M > cap) {
     set_error(-3, "band-limit too large for the LDS row stage on this device");
     throw ArgFailure();
