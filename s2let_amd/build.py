@@ -29,6 +29,7 @@ SOURCES = [
     ("multiply.hip", [HIPCC] + HIP_FLAGS + ["-ffp-contract=off"]),
     ("kernels.hip", [HIPCC] + HIP_FLAGS),
     ("contract_mfma.hip", [HIPCC] + HIP_FLAGS),
+    ("theta_mfma.hip", [HIPCC] + HIP_FLAGS),
     ("plan.hip", [HIPCC] + HIP_FLAGS),
     ("pipeline.hip", [HIPCC] + HIP_FLAGS),
     ("api.hip", [HIPCC] + HIP_FLAGS),

@@ -27,27 +27,19 @@ struct SuperDev {
   int pad;
 };
 
-// A group of columns sharing a band-limit (one wavelet scale, the scaling
-// function, or one spin-s harmonic transform) and its folded spectra buffer
-// spec[c][m'][m] (c < ncol, m' < L, m + L - 1 < 2L - 1).
-struct GroupDev {
-  int L;
-  int ncol;
-  int col0;
-  int pad;
-  const void *spec;
-};
-
+// Theta stage of one group (one launch): per column c and parity class cls of
+// m + n_c, out_c = phase(c, m) i^{z_e} D_out A_cls D_in in_c, with A_cls real
+// L x L (stored transposed, [k][i]) and D_in / D_out multiplying one row by
+// i^{in_e} / i^{out_e} (row -1: none).  phase = i^{-m} ph_c, or i^{m} conj(ph_c)
+// when conj_ph is set (adjoint kinds).
 template <typename T>
-struct BetaArgs {
+struct ThetaArgs {
   int L;
-  int tw;  // lines (m columns) per block
   int ncol;
+  int conj_ph;
   const ColDev *cols;
-  DftDesc<T> dM;          // length 2L-1
-  DftDesc<T> dP;          // convolution length P >= 4L-3
-  const cx<T> *shift;     // e^{i m' pi/(2L-1)}, m' in [0, L)
-  const cx<T> *wconv;     // DFT_P (exponent -) of I(p) = int_0^pi e^{ip t} sin t dt placed circularly
+  const T *AT[2];
+  int z_e[2], in_row[2], in_e[2], out_row[2], out_e[2];
 };
 
 template <typename T>
@@ -66,13 +58,6 @@ void launch_to_f32(const double *a, float *b, size_t n, hipStream_t st);
 template <typename A, typename B>
 void launch_cast(const void *a, void *b, size_t n, hipStream_t st);
 
-template <typename T>
-void launch_contract_inv(const void *x, const T *dtab, const ColDev *cols, int ncol, const T *bvals, int L, void *F,
-                         hipStream_t st);
-template <typename T>
-void launch_contract_fwd(const GroupDev *groups, int ngroups, int maxcol, const ColDev *cols, const T *bvals,
-                         const T *dtab, int L, void *y, int accumulate, hipStream_t st);
-
 // matrix-core contractions (contract_mfma.hip)
 template <typename T>
 void launch_inv_mfma(const void *x, const T *dtab, const ColDev *cols, int ncol, const T *bvals, int L, void *Fbase,
@@ -83,9 +68,11 @@ void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *
 int fwd_splits(int L);  // m' splits of the forward contraction (partials: fwd_splits(L) * L^2 complex128)
 int mfma_selftest();
 
-// kind: 0 inverse, 1 inverse adjoint, 2 forward, 3 forward adjoint
+// theta stage (theta_mfma.hip).  Tables: 8 L x L matrices [kind][class],
+// kind 0 inverse, 1 inverse adjoint, 2 forward, 3 forward adjoint.
+void build_theta_tables(int L, const double *cs, const double *sn, double *th, hipStream_t st);
 template <typename T>
-void launch_beta(int kind, const void *in, void *out, const BetaArgs<T> &a, int ncol, hipStream_t st);
+void launch_theta(const void *in, void *out, const ThetaArgs<T> &a, hipStream_t st);
 template <typename TI, typename TO, typename T>
 void launch_rows(const void *in, void *out, const RowArgs<T> &a, hipStream_t st);
 template <typename TI, typename TO, typename T>

@@ -84,8 +84,7 @@ struct Sel;
 template <>
 struct Sel<double> {
   static const DftDesc<double> &d(const DftTables &t) { return t.d64; }
-  static const cx<double> *shift(const LTables &t) { return t.shift64.as<cx<double>>(); }
-  static const cx<double> *wconv(const LTables &t) { return t.wconv64.as<cx<double>>(); }
+  static const double *theta(const LTables &t) { return t.th64.as<double>(); }
   static const double *delta(Context &c) { return c.delta64(); }
   static const double *b(const Plan &p) { return p.d_b64.as<double>(); }
   static const double *b(const ShtPlan &p) { return p.d_b64.as<double>(); }
@@ -93,17 +92,11 @@ struct Sel<double> {
 template <>
 struct Sel<float> {
   static const DftDesc<float> &d(const DftTables &t) { return t.d32; }
-  static const cx<float> *shift(const LTables &t) { return t.shift32.as<cx<float>>(); }
-  static const cx<float> *wconv(const LTables &t) { return t.wconv32.as<cx<float>>(); }
+  static const float *theta(const LTables &t) { return t.th32.as<float>(); }
   static const float *delta(Context &c) { return c.delta32(); }
   static const float *b(const Plan &p) { return p.d_b32.as<float>(); }
   static const float *b(const ShtPlan &p) { return p.d_b32.as<float>(); }
 };
-
-template <typename T>
-constexpr int maxr() {
-  return sizeof(T) == 8 ? 16 : 32;
-}
 
 // LDS bytes one block of a DFT stage may use: half the device's per-block
 // maximum (two blocks per CU), overridable by S2LET_LDS_BUDGET.
@@ -117,31 +110,30 @@ size_t lds_budget(Context &C) {
   return v;
 }
 
+// kind 0 inverse, 1 inverse adjoint, 2 forward, 3 forward adjoint.  Class 1
+// (m + n odd): K_inv = i A diag(-i, 1, ...), K_fwd = -i diag(i, 1, ...) A; the
+// adjoints conjugate and transpose these.
 template <typename T>
-BetaArgs<T> beta_args(int L, const ColDev *cols, bool conv) {
-  Context &C = Context::get();
-  const LTables &lt = C.ltab(L);
-  BetaArgs<T> a{};
+ThetaArgs<T> theta_args(int L, const ColDev *cols, int ncol, int kind) {
+  const LTables &lt = Context::get().ltab(L);
+  ThetaArgs<T> a{};
   a.L = L;
+  a.ncol = ncol;
   a.cols = cols;
-  a.dM = Sel<T>::d(*lt.dM);
-  a.dP = Sel<T>::d(*lt.dP);
-  a.shift = Sel<T>::shift(lt);
-  a.wconv = Sel<T>::wconv(lt);
-  const int M = 2 * L - 1;
-  const int lstride = conv ? lt.P : M;
-  const size_t twb = conv ? 0 : (size_t)M * sizeof(cx<T>);  // beta_inv kernels keep their twiddles in LDS
-  const size_t bud = lds_budget(C) > twb ? lds_budget(C) - twb : 0;
-  const size_t cap = bud / (2 * sizeof(cx<T>));  // two ping-pong buffers
-  int tw = (int)(cap / lstride);
-  if (tw < 1 && C.lds_bytes() > twb) {  // one line needs more than the default share: use the whole LDS
-    tw = (int)((C.lds_bytes() - twb) / (2 * sizeof(cx<T>)) / lstride);
+  a.conj_ph = kind & 1;
+  const T *th = Sel<T>::theta(lt);
+  for (int c = 0; c < 2; ++c) {
+    a.AT[c] = th + (size_t)(kind * 2 + c) * L * L;
+    a.z_e[c] = 0;
+    a.in_row[c] = a.out_row[c] = -1;
+    a.in_e[c] = a.out_e[c] = 0;
   }
-  if (tw < 1) {
-    set_error(-3, "band-limit too large for the LDS beta stage on this device");
-    throw ArgFailure();
+  switch (kind) {
+    case 0: a.z_e[1] = 1; a.in_row[1] = 0; a.in_e[1] = 3; break;
+    case 1: a.z_e[1] = 3; a.out_row[1] = 0; a.out_e[1] = 1; break;
+    case 2: a.z_e[1] = 3; a.out_row[1] = 0; a.out_e[1] = 1; break;
+    default: a.z_e[1] = 1; a.in_row[1] = 0; a.in_e[1] = 3; break;
   }
-  a.tw = std::min(tw, M);
   return a;
 }
 
@@ -209,7 +201,7 @@ void analysis_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t
     const int L = g.L, M = 2 * L - 1;
     const ColDev *cols = P.d_cols_ana.as<ColDev>() + g.col0;
     char *spec = at<T>(P.spec, g.spec_off), *work = at<T>(P.work, g.spec_off);
-    { Stage s_("beta_inv", st); launch_beta<T>(0, spec, work, beta_args<T>(L, cols, false), g.ncol, st); }
+    { Stage s_("beta_inv", st); launch_theta<T>(spec, work, theta_args<T>(L, cols, g.ncol, 0), st); }
     if (g.ngamma == 0) {
       { Stage s_("rows", st); launch_rows<T, double, T>(work, f_scal, row_args<T>(L, (size_t)L, 1, 0, +1, 1.0), st); }
     } else {
@@ -236,7 +228,7 @@ void synthesis_t(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipS
                                (size_t)L * M, st); }
       { Stage s_("rows", st); launch_rows<T, T, T>(work, spec, row_args<T>(L, (size_t)g.ncol * L, 0, 1, -1, kTwoPi / M), st); }
     }
-    { Stage s_("beta_fwd", st); launch_beta<T>(2, spec, work, beta_args<T>(L, cols, true), g.ncol, st); }
+    { Stage s_("beta_fwd", st); launch_theta<T>(spec, work, theta_args<T>(L, cols, g.ncol, 2), st); }
   }
   contract_fwd_all<T>(P, P.d_cols_syn.as<ColDev>(), P.work.p, flm, st);
 }
@@ -256,7 +248,7 @@ void analysis_adjoint_t(Plan &P, const void *f_wav, const void *f_scal, void *fl
                                (size_t)L * M, st); }
       { Stage s_("rows", st); launch_rows<T, T, T>(work, spec, row_args<T>(L, (size_t)g.ncol * L, 0, 1, -1, 1.0), st); }
     }
-    { Stage s_("beta_inv_adj", st); launch_beta<T>(1, spec, work, beta_args<T>(L, cols, false), g.ncol, st); }
+    { Stage s_("beta_inv_adj", st); launch_theta<T>(spec, work, theta_args<T>(L, cols, g.ncol, 1), st); }
   }
   contract_fwd_all<T>(P, P.d_cols_ana.as<ColDev>(), P.work.p, flm, st);
 }
@@ -270,7 +262,7 @@ void synthesis_adjoint_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hi
     const int L = g.L, M = 2 * L - 1;
     const ColDev *cols = P.d_cols_syn.as<ColDev>() + g.col0;
     char *spec = at<T>(P.spec, g.spec_off), *work = at<T>(P.work, g.spec_off);
-    { Stage s_("beta_fwd_adj", st); launch_beta<T>(3, spec, work, beta_args<T>(L, cols, true), g.ncol, st); }
+    { Stage s_("beta_fwd_adj", st); launch_theta<T>(spec, work, theta_args<T>(L, cols, g.ncol, 3), st); }
     if (g.ngamma == 0) {
       { Stage s_("rows", st); launch_rows<T, double, T>(work, f_scal, row_args<T>(L, (size_t)L, 1, 0, +1, kTwoPi / M), st); }
     } else {
@@ -296,7 +288,7 @@ void sht_inverse_t(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
   const int L = S.L;
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("contract_inv", st); launch_inv_mfma<T>(sht_input<T>(S, flm, st), Sel<T>::delta(C), col, 1, Sel<T>::b(S), L, S.spec.p, st); }
-  { Stage s_("beta_inv", st); launch_beta<T>(0, S.spec.p, S.work.p, beta_args<T>(L, col, false), 1, st); }
+  { Stage s_("beta_inv", st); launch_theta<T>(S.spec.p, S.work.p, theta_args<T>(L, col, 1, 0), st); }
   { Stage s_("rows", st); launch_rows<T, double, T>(S.work.p, f, row_args<T>(L, (size_t)L, 1, 0, +1, 1.0), st); }
 }
 
@@ -306,7 +298,7 @@ void sht_forward_t(ShtPlan &S, const void *f, void *flm, hipStream_t st) {
   const int L = S.L, M = 2 * L - 1;
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("rows", st); launch_rows<double, T, T>(f, S.spec.p, row_args<T>(L, (size_t)L, 0, 1, -1, kTwoPi / M), st); }
-  { Stage s_("beta_fwd", st); launch_beta<T>(2, S.spec.p, S.work.p, beta_args<T>(L, col, true), 1, st); }
+  { Stage s_("beta_fwd", st); launch_theta<T>(S.spec.p, S.work.p, theta_args<T>(L, col, 1, 2), st); }
   { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, Sel<T>::b(S), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, st); }
 }
 
@@ -316,7 +308,7 @@ void sht_inverse_adjoint_t(ShtPlan &S, const void *f, void *flm, hipStream_t st)
   const int L = S.L;
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("rows", st); launch_rows<double, T, T>(f, S.spec.p, row_args<T>(L, (size_t)L, 0, 1, -1, 1.0), st); }
-  { Stage s_("beta_inv_adj", st); launch_beta<T>(1, S.spec.p, S.work.p, beta_args<T>(L, col, false), 1, st); }
+  { Stage s_("beta_inv_adj", st); launch_theta<T>(S.spec.p, S.work.p, theta_args<T>(L, col, 1, 1), st); }
   { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, Sel<T>::b(S), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, st); }
 }
 
@@ -326,7 +318,7 @@ void sht_forward_adjoint_t(ShtPlan &S, const void *flm, void *f, hipStream_t st)
   const int L = S.L, M = 2 * L - 1;
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("contract_inv", st); launch_inv_mfma<T>(sht_input<T>(S, flm, st), Sel<T>::delta(C), col, 1, Sel<T>::b(S), L, S.spec.p, st); }
-  { Stage s_("beta_fwd_adj", st); launch_beta<T>(3, S.spec.p, S.work.p, beta_args<T>(L, col, true), 1, st); }
+  { Stage s_("beta_fwd_adj", st); launch_theta<T>(S.spec.p, S.work.p, theta_args<T>(L, col, 1, 3), st); }
   { Stage s_("rows", st); launch_rows<T, double, T>(S.work.p, f, row_args<T>(L, (size_t)L, 1, 0, +1, kTwoPi / M), st); }
 }
 

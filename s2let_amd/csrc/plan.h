@@ -33,9 +33,9 @@ struct DftTables {
 };
 
 struct LTables {  // per band-limit L
-  int L, P;
-  DevBuf shift64, shift32, wconv64, wconv32;
-  const DftTables *dM, *dP;
+  int L;
+  DevBuf th64, th32;  // theta-stage matrices, 8 x L x L (see build_theta_tables)
+  const DftTables *dM;
 };
 
 // A set of columns sharing one band-limit (see GroupDev).
@@ -64,7 +64,7 @@ struct Plan {
   std::vector<SuperDev> supers;  // equal-L column runs (<= 16 columns each)
   DevBuf d_supers;
   // device copies
-  DevBuf d_cols_ana, d_cols_syn, d_b64, d_b32, d_groups_spec, d_groups_work, d_gamma;
+  DevBuf d_cols_ana, d_cols_syn, d_b64, d_b32, d_gamma;
   DevBuf d_psi, d_phi;
   DevBuf spec, work;  // scratch (T = double or float per mode)
   DevBuf flm32;       // mixed mode: FP32 copy of the input harmonics
@@ -80,7 +80,7 @@ struct ShtPlan {
   int L, spin, mode;
   ColDev col;
   std::vector<double> bvals;
-  DevBuf d_col, d_b64, d_b32, d_group_spec, d_group_work, spec, work, flm32, d_super, parts;
+  DevBuf d_col, d_b64, d_b32, spec, work, flm32, d_super, parts;
 };
 
 class Context {

@@ -148,71 +148,32 @@ const DftTables &Context::dft(int n) {
   return ref;
 }
 
-// radix-2 FFT in long double (host, table setup only)
-static void host_fft(std::vector<std::complex<long double>> &a, int sign) {
-  const size_t n = a.size();
-  for (size_t i = 1, j = 0; i < n; ++i) {
-    size_t bit = n >> 1;
-    for (; j & bit; bit >>= 1) j ^= bit;
-    j ^= bit;
-    if (i < j) std::swap(a[i], a[j]);
-  }
-  const long double pi = 3.14159265358979323846264338327950288L;
-  for (size_t len = 2; len <= n; len <<= 1) {
-    for (size_t i = 0; i < n; i += len)
-      for (size_t k = 0; k < len / 2; ++k) {
-        long double ang = sign * 2 * pi * (long double)k / (long double)len;
-        std::complex<long double> w(cosl(ang), sinl(ang));
-        auto u = a[i + k], v = a[i + k + len / 2] * w;
-        a[i + k] = u + v;
-        a[i + k + len / 2] = u - v;
-      }
-  }
-}
-
 const LTables &Context::ltab(int L) {
   auto it = lt_.find(L);
   if (it != lt_.end()) return *it->second;
   auto t = std::make_unique<LTables>();
   t->L = L;
   const int M = 2 * L - 1;
-  int P = 1;
-  while (P < 4 * L - 3) P <<= 1;
-  if (P < 2) P = 2;
-  t->P = P;
   const long double pi = 3.14159265358979323846264338327950288L;
-  std::vector<cx<double>> sh(L);
-  std::vector<cx<float>> shf(L);
-  for (int m = 0; m < L; ++m) {
-    long double a = pi * m / (long double)M;
-    sh[m] = mk<double>((double)cosl(a), (double)sinl(a));
-    shf[m] = mk<float>((float)sh[m].x, (float)sh[m].y);
-  }
-  // weights I(p) = int_0^pi e^{ipt} sin t dt, placed at p mod P, transformed with exponent -
-  std::vector<std::complex<long double>> w(P, 0.0L);
-  for (int p = -2 * (L - 1); p <= 2 * (L - 1); ++p) {
-    std::complex<long double> v;
-    if (p == 1) v = std::complex<long double>(0, pi / 2);
-    else if (p == -1) v = std::complex<long double>(0, -pi / 2);
-    else if (p & 1) v = 0;
-    else v = 2.0L / (1.0L - (long double)p * p);
-    w[((p % P) + P) % P] += v;
-  }
-  host_fft(w, -1);
-  std::vector<cx<double>> wc(P);
-  std::vector<cx<float>> wcf(P);
-  for (int k = 0; k < P; ++k) {
-    wc[k] = mk<double>((double)w[k].real(), (double)w[k].imag());
-    wcf[k] = mk<float>((float)wc[k].x, (float)wc[k].y);
+  // cos / sin (pi j / M), j < 2M: every theta_t multiple m theta_t = pi m (2t+1) / M is one of these
+  std::vector<double> cs(2 * M), sn(2 * M);
+  for (int j = 0; j < 2 * M; ++j) {
+    const long double a = pi * (long double)j / (long double)M;
+    cs[j] = (double)cosl(a);
+    sn[j] = (double)sinl(a);
   }
   hipStream_t st = stream();
-  upload(t->shift64, sh, st);
-  upload(t->shift32, shf, st);
-  upload(t->wconv64, wc, st);
-  upload(t->wconv32, wcf, st);
+  DevBuf dcs, dsn;
+  upload(dcs, cs, st);
+  upload(dsn, sn, st);
+  const size_t n = 8 * (size_t)L * L;
+  t->th64.alloc(n * sizeof(double));
+  t->th32.alloc(n * sizeof(float));
+  build_theta_tables(L, dcs.as<double>(), dsn.as<double>(), t->th64.as<double>(), st);
+  launch_to_f32(t->th64.as<double>(), t->th32.as<float>(), n, st);
+  S2H_CHECK_HIP(hipGetLastError());
   S2H_CHECK_HIP(hipStreamSynchronize(st));
   t->dM = &dft(M);
-  t->dP = &dft(P);
   auto &ref = *t;
   lt_[L] = std::move(t);
   return ref;
@@ -374,15 +335,6 @@ Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
   pl.work.alloc(pl.spec_elems * esz);
   if (f32) pl.flm32.alloc((size_t)L * L * sizeof(cx<float>));
   pl.parts.alloc((size_t)fwd_splits(L) * L * L * sizeof(cx<double>));
-  std::vector<GroupDev> gs, gw;
-  for (const Group &g : pl.groups) {
-    GroupDev d{g.L, g.ncol, g.col0, 0, (const char *)pl.spec.p + g.spec_off * esz};
-    gs.push_back(d);
-    d.spec = (const char *)pl.work.p + g.spec_off * esz;
-    gw.push_back(d);
-  }
-  upload(pl.d_groups_spec, gs, st);
-  upload(pl.d_groups_work, gw, st);
   S2H_CHECK_HIP(hipStreamSynchronize(st));
   auto &ref = *P;
   plans_[key] = std::move(P);
@@ -427,9 +379,6 @@ ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st) {
   s.work.alloc(ne * esz);
   if (f32) s.flm32.alloc((size_t)L * L * sizeof(cx<float>));
   s.parts.alloc((size_t)fwd_splits(L) * L * L * sizeof(cx<double>));
-  std::vector<GroupDev> gs{GroupDev{L, 1, 0, 0, s.spec.p}}, gw{GroupDev{L, 1, 0, 0, s.work.p}};
-  upload(s.d_group_spec, gs, st);
-  upload(s.d_group_work, gw, st);
   S2H_CHECK_HIP(hipStreamSynchronize(st));
   auto &ref = *S;
   shts_[key] = std::move(S);

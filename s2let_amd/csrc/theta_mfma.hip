@@ -1,0 +1,251 @@
+// Theta (beta) stage as real matrix products on the matrix cores.
+//
+// For one column (direction n) and one m, the theta stage of the MW inverse
+// (reference: ssht core, used by s2let_synthesis_wav2lm via so3_core_inverse)
+//     T[t] = sum_{m'=-(L-1)}^{L-1} S(m') e^{i m' theta_t},  theta_t = pi (2t+1)/M,
+// with the folded spectrum S(-m') = sigma S(m'), sigma = (-1)^{m+n}, is
+//     T[t] = sum_{m' >= 0} K_inv[t][m'] S(m'),
+//     K_inv = [1, 2 cos(m' theta_t)]  (sigma = +1),  [1, 2i sin(m' theta_t)]  (sigma = -1).
+// The theta stage of the MW forward transform (extension of the samples to
+// the full circle with parity sigma, theta DFT, convolution with the weights
+// I(p) = int_0^pi e^{i p theta} sin theta d theta, fold onto m' >= 0) is
+//     Jf[m'] = sum_{t < L} K_fwd[m'][t] G[t],
+// K_fwd[m'][t] = J_t[m'] + sigma J_t[-m'] (m' > 0), J_t[0] (m' = 0),
+// J_t[j] = (1/M) sum_{m''} c_t(m'') I(j + m''),
+// c_t(m'') = e^{-i m'' theta_t} + sigma e^{i m'' theta_t} (t < L-1), (-1)^{m''} (t = L-1).
+// For sigma = +1 both matrices are real.  For sigma = -1, K_inv = i A D with
+// D = diag(-i, 1, 1, ...) and A real; K_fwd = -i D' A with D' = diag(i, 1, ...)
+// and A real (row 0 of K_fwd is real, the other rows are imaginary).  The
+// adjoints are the conjugate transposes.  So every theta stage is, per column
+// and parity class of m, a real L x L matrix times the complex samples, a
+// unit twist of one input or output row, and a per-(column, m) phase.
+//
+// Tables (per L, FP64 and FP32 copies): 8 matrices [kind][class], stored
+// transposed (reduction index major) so that the A-operand fragments load
+// coalesced: th[((kind * 2 + cls) * L + k) * L + i] = A[i][k].
+#include "kernels.h"
+#include "mfma.h"
+
+namespace s2h {
+
+// ------------------------------------------------------------------ tables
+// cs[j] = cos(pi j / M), sn[j] = sin(pi j / M), j < 2M (exact to double rounding).
+__device__ inline double Ire(int p) { return (p & 1) ? 0.0 : 2.0 / (1.0 - (double)p * (double)p); }
+
+__global__ void k_theta_tables(int L, const double *__restrict__ cs, const double *__restrict__ sn,
+                               double *__restrict__ th) {
+  const int M = 2 * L - 1, twoM = 2 * M;
+  const size_t LL = (size_t)L * L;
+  const size_t total = 2 * LL;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int cls = (int)(e / LL);
+    const int rem = (int)(e - (size_t)cls * LL);
+    const int mp = rem / L, t = rem - mp * L;  // (m', t)
+    const double sg = cls ? -1.0 : 1.0;
+    // ---- inverse: A[t][m']
+    double ainv;
+    if (mp == 0) ainv = 1.0;
+    else {
+      const int j = (int)(((long long)mp * (2 * t + 1)) % twoM);
+      ainv = cls ? 2.0 * sn[j] : 2.0 * cs[j];
+    }
+    th[((size_t)(0 * 2 + cls) * L + mp) * L + t] = ainv;  // kind 0: A[i=t][k=m']
+    th[((size_t)(1 * 2 + cls) * L + t) * L + mp] = ainv;  // kind 1: A[i=m'][k=t] = K_inv[t][m']
+    // ---- forward: K[m'][t] = J[m'] + sg J[-m'] (m' > 0), J[0]
+    auto ct = [&](int mpp, double &re, double &im) {  // c_t(m'')
+      if (t == L - 1) {
+        re = (mpp & 1) ? -1.0 : 1.0;
+        im = 0.0;
+        return;
+      }
+      long long q = ((long long)mpp * (2 * t + 1)) % twoM;
+      if (q < 0) q += twoM;
+      const double c = cs[q], s = sn[q];
+      // e^{-i a} + sg e^{i a}
+      re = c + sg * c;
+      im = -s + sg * s;
+    };
+    auto Jsum = [&](int j, double &re, double &im) {  // M J_t[j] = sum_{m''} c_t(m'') I(j + m'')
+      re = 0.0;
+      im = 0.0;
+      for (int mpp = -(L - 1); mpp <= L - 1; ++mpp) {
+        const int p = j + mpp;
+        double cr, ci;
+        if (p == 1 || p == -1) {
+          ct(mpp, cr, ci);
+          const double ii = (p == 1) ? 1.5707963267948966192 : -1.5707963267948966192;  // I = i ii
+          re += -ci * ii;
+          im += cr * ii;
+        } else if (!(p & 1)) {
+          ct(mpp, cr, ci);
+          const double w = Ire(p);
+          re += cr * w;
+          im += ci * w;
+        }
+      }
+    };
+    double kr, ki;
+    Jsum(mp, kr, ki);
+    if (mp > 0) {
+      double r2, i2;
+      Jsum(-mp, r2, i2);
+      kr += sg * r2;
+      ki += sg * i2;
+    }
+    kr /= M;
+    ki /= M;
+    const double afwd = (cls == 0 || mp == 0) ? kr : -ki;
+    th[((size_t)(2 * 2 + cls) * L + t) * L + mp] = afwd;  // kind 2: A[i=m'][k=t]
+    th[((size_t)(3 * 2 + cls) * L + mp) * L + t] = afwd;  // kind 3: A[i=t][k=m']
+  }
+}
+
+void build_theta_tables(int L, const double *cs, const double *sn, double *th, hipStream_t st) {
+  const size_t total = 2 * (size_t)L * L;
+  const unsigned g = (unsigned)std::min<size_t>((total + 255) / 256, 65535 * 4);
+  hipLaunchKernelGGL(k_theta_tables, dim3(g), dim3(256), 0, st, L, cs, sn, th);
+}
+
+// ------------------------------------------------------------------ GEMM
+// out_c[i][mi] = phase(c, m) z_cls D_out[i] sum_k A_cls[i][k] D_in[k] in_c[k][mi]
+// for every column c, rows i < L, k < L, mi < M (m = mi - (L-1)), cls = parity
+// of m + n_c.  Block tile: 64 rows x 64 consecutive mi of one column; waves
+// 0,1 take the 32 mi of the block's first class (rows 0-31, 32-63), waves 2,3
+// the other class.  Per wave: 2 row tiles x {re, im} x 2 column tiles of 16.
+// B (the samples) is staged in LDS split by class and re/im; A fragments are
+// read from the (L2-resident) table.  FP32: products accumulate in FP32 per
+// K chunk of 32 and the chunk sums are added in FP64.
+template <typename T>
+struct ThetaTile {
+  static constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+  static constexpr int SJ = 32 + (sizeof(T) == 8 ? 8 : 16);  // padded LDS row
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_theta(const cx<T> *__restrict__ in, cx<T> *__restrict__ out, ThetaArgs<T> a) {
+  constexpr int BK = ThetaTile<T>::BK, SJ = ThetaTile<T>::SJ;
+  __shared__ T sB[2][2][BK][SJ];  // [class][re/im][k][j]
+  typedef typename Mfma<T>::acc acc_t;
+  const int L = a.L, M = 2 * L - 1;
+  const int nti = (L + 63) / 64, ntm = (M + 63) / 64;
+  // XCD-aware block order: blocks are dispatched round-robin over 8 XCDs; give
+  // each XCD a contiguous range of (column, m-tile, row-tile) items so that the
+  // row tiles of one m-tile share that XCD's L2.
+  const int nb = nti * ntm * a.ncol;
+  const int lin = blockIdx.x, xx = lin & 7, kq = lin >> 3, q8 = nb >> 3, r8 = nb & 7;
+  const int w = xx * q8 + min(xx, r8) + kq;
+  const int ti = w % nti, rest = w / nti, tm = rest % ntm, c = rest / ntm;
+  const int i0 = ti * 64, mi0 = tm * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int cls_lo = (mi0 - (L - 1)) & 1;  // parity of m at mi0 (class of ee = 0 is cls_lo ^ (n & 1))
+  const ColDev col = a.cols[c];
+  const int par0 = (cls_lo + col.n) & 1;
+  const int cls = wave >> 1;
+  const int i0w = i0 + 32 * (wave & 1);
+  const int r16 = lane & 15, k4 = lane >> 4;
+  const cx<T> *__restrict__ inc = in + (size_t)c * L * M;
+  const T *__restrict__ AT = a.AT[cls];
+  const int in_row0 = a.in_row[0], in_row1 = a.in_row[1], in_e0 = a.in_e[0], in_e1 = a.in_e[1];
+
+  acc_t acc[2][4];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) acc[r][qq] = Mfma<T>::zero();
+  double hs[2][4][4];
+  if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) hs[r][qq][v] = 0.0;
+  }
+
+  for (int k0 = 0; k0 < L; k0 += BK) {
+#pragma unroll
+    for (int u = 0; u < BK * 64 / 256; ++u) {
+      const int e = tid + 256 * u, kk = e >> 6, ee = e & 63;
+      const int k = k0 + kk, mi = mi0 + ee;
+      cx<T> v = mk<T>(0, 0);
+      if (k < L && mi < M) v = inc[(size_t)k * M + mi];
+      const int cl = (ee & 1) ^ par0;
+      if (k == (cl ? in_row1 : in_row0)) v = ipow_mul(cl ? in_e1 : in_e0, v);
+      sB[cl][0][kk][ee >> 1] = v.x;
+      sB[cl][1][kk][ee >> 1] = v.y;
+    }
+    __syncthreads();
+    T av[BK / 4][2];
+#pragma unroll
+    for (int s = 0; s < BK / 4; ++s) {
+      const int k = k0 + 4 * s + k4;
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int i = i0w + 16 * r + r16;
+        av[s][r] = (k < L && i < L) ? AT[(size_t)k * L + i] : (T)0;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < BK / 4; ++s) {
+      const int kk = 4 * s + k4;
+      T bv[4];
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) bv[qq] = sB[cls][qq >> 1][kk][16 * (qq & 1) + r16];
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) acc[r][qq] = Mfma<T>::mma(av[s][r], bv[qq], acc[r][qq]);
+    }
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) hs[r][qq][v] += (double)acc[r][qq][v];
+          acc[r][qq] = Mfma<T>::zero();
+        }
+    }
+    __syncthreads();
+  }
+
+  const cx<double> ph = mk<double>(col.ph_re, col.ph_im);
+  const int zc = a.z_e[cls], orow = a.out_row[cls], oe = a.out_e[cls];
+  cx<T> *__restrict__ outc = out + (size_t)c * L * M;
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int qq = 0; qq < 2; ++qq) {
+      const int j = 16 * qq + r16, ee = 2 * j + (cls ^ par0), mi = mi0 + ee;
+      const int m = mi - (L - 1);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0w + 16 * r + Mfma<T>::row(k4, v);
+        if (i >= L || mi >= M) continue;
+        double re, im;
+        if constexpr (sizeof(T) == 4) {
+          re = hs[r][qq][v];
+          im = hs[r][qq + 2][v];
+        } else {
+          re = (double)acc[r][qq][v];
+          im = (double)acc[r][qq + 2][v];
+        }
+        cx<double> val = mk<double>(re, im);
+        if (i == orow) val = ipow_mul(oe, val);
+        val = ipow_mul(zc, val);
+        val = a.conj_ph ? ipow_mul(m, cmulc(val, ph)) : ipow_mul(-m, cmul(ph, val));
+        outc[(size_t)i * M + mi] = mk<T>((T)val.x, (T)val.y);
+      }
+    }
+}
+
+template <typename T>
+void launch_theta(const void *in, void *out, const ThetaArgs<T> &a, hipStream_t st) {
+  const int M = 2 * a.L - 1;
+  const int nb = ((a.L + 63) / 64) * ((M + 63) / 64) * a.ncol;
+  hipLaunchKernelGGL((k_theta<T>), dim3(nb), dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out, a);
+}
+template void launch_theta<double>(const void *, void *, const ThetaArgs<double> &, hipStream_t);
+template void launch_theta<float>(const void *, void *, const ThetaArgs<float> &, hipStream_t);
+
+}  // namespace s2h
