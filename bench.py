@@ -56,6 +56,66 @@ def algorithmic_work(s2, p):
     return F, Q
 
 
+# one kernel function per timed stage of the library (s2let_hip_timing)
+STAGE_KERNEL = {
+    "contract_inv": "k_inv3", "contract_fwd": "k_fwd3", "contract_fwd_sum": "k_sum_parts",
+    "beta_inv": "k_theta", "beta_fwd": "k_theta", "beta_inv_adj": "k_theta", "beta_fwd_adj": "k_theta",
+    "rows": "k_rows", "gamma": "k_mix", "cast": "k_cast_cx",
+}
+
+
+def column_groups(s2, p):
+    """(L, columns, gamma samples) of the scaling function and of every scale."""
+    out = [(min(s2.bandlimit(p.J_min - 1, p), p.L), 1, 0)]
+    for j in range(p.J_min, s2.j_max(p) + 1):
+        Lj = min(s2.bandlimit(j, p), p.L)
+        Nj = min(p.N, Lj)
+        Nj += (Nj + p.N) % 2
+        out.append((Lj, Nj, 2 * Nj - 1))
+    return out
+
+
+def kernel_work(s2, p, F, e):
+    """Algorithmic work per step (one analysis + one synthesis) of each kernel
+    function, with its roof: ("mfma", flop) or ("hbm", bytes).  e = bytes of one
+    complex intermediate (8 mixed, 16 FP64); f_wav, f_scal and flm are complex128.
+      k_inv3 / k_fwd3: the analysis / synthesis ell-sums, F/2 each (the path's
+        algorithmic flop count);
+      k_theta: the theta stage as real matrix products, 2 (directions) x 4 L^2 M ncol;
+      k_rows:  each phi/alpha line read and written once;
+      k_mix:   the column spectra and the f_wav slices read or written once;
+      k_sum_parts: the m' split partials read once and flm written once;
+      k_cast_cx: flm read (FP64) and written (FP32) once."""
+    g = column_groups(s2, p)
+    L = p.L
+    theta = sum(2 * 4 * Lg * Lg * (2 * Lg - 1) * nc for Lg, nc, ng in g)
+    rows = 0
+    mix = 0
+    for Lg, nc, ng in g:
+        M = 2 * Lg - 1
+        if ng == 0:
+            rows += 2 * Lg * M * (e + 16)
+        else:
+            rows += 2 * nc * Lg * M * (2 * e)
+            mix += 2 * Lg * M * (nc * e + ng * 16)
+    S = max(1, min(8, L // 64))
+    work = {"k_inv3": ("mfma", F / 2), "k_fwd3": ("mfma", F / 2), "k_theta": ("mfma", theta),
+            "k_rows": ("hbm", rows), "k_mix": ("hbm", mix), "k_sum_parts": ("hbm", (S + 1) * L * L * 16)}
+    if e == 8:
+        work["k_cast_cx"] = ("hbm", L * L * 24)
+    return work
+
+
+def pmc_traffic(path):
+    """HBM bytes per launch of each kernel function (mixed mode) from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary, or {} if absent."""
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch_mixed", {})
+    except (OSError, ValueError):
+        return {}
+
+
 def timed(fn, steps, warmup, torch):
     for _ in range(warmup):
         fn()
@@ -92,6 +152,8 @@ def main():
     ap.add_argument("--L", type=int, default=CFG["L"])
     ap.add_argument("--N", type=int, default=CFG["N"])
     ap.add_argument("--J_min", type=int, default=CFG["J_min"])
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device 3 times (reference kernel for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -101,6 +163,14 @@ def main():
 
     import s2let_amd as s2
     from s2let_amd import torch_ops as ops
+
+    if args.copy_calib > 0:
+        src = torch.ones(args.copy_calib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     cfg = dict(CFG, L=args.L, N=args.N, J_min=args.J_min)
     p = s2.parameters(cfg["L"], cfg["N"], cfg["B"], cfg["J_min"], cfg["spin"], cfg["upsample"])
@@ -145,9 +215,37 @@ def main():
     dt64, st64 = res[s2.FP64]
     K = args.steps
     maps_s = K / dt32
-    # dominant stage of the headline (mixed) mode: largest total time
-    dom = max(st32.items(), key=lambda kv: kv[1][1])
-    dname, (dcount, dms) = dom
+    # dominant kernel of the headline (mixed) mode: the kernel function with the
+    # largest time per step, from the per-stage HIP events recorded on the
+    # library's stream (each stage is one kernel function)
+    per_k = {}
+    for name, (cnt, ms) in st32.items():
+        kf = STAGE_KERNEL.get(name, name)
+        c0, m0 = per_k.get(kf, (0, 0.0))
+        per_k[kf] = (c0 + cnt, m0 + ms)
+    dk, (dcount, dms) = max(per_k.items(), key=lambda kv: kv[1][1])
+    work = kernel_work(s2, p, F, 8)
+    bound, w_step = work.get(dk, ("hbm", None))
+    t_step = dms / K / 1e3  # s
+    if bound == "mfma":
+        peak, unit = FP32_PEAK / 1e12, "TFLOP/s"
+        achieved = None if w_step is None else w_step / t_step / 1e12
+    else:
+        peak, unit = HBM_PEAK / 1e9, "GB/s"
+        achieved = None if w_step is None else w_step / t_step / 1e9
+    traffic = pmc_traffic(os.path.join(ROOT, "profiles", "pmc_traffic_L512_N8.json")).get(dk)
+    roof = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": None if achieved is None else achieved / peak, "traffic": traffic,
+            "kernel": dk, "launches_per_step": dcount / K, "avg_launch_ms": dms / dcount,
+            "work_per_launch": None if w_step is None else w_step * K / dcount}
+    kernels = {}
+    for kf, (cnt, ms) in per_k.items():
+        b, w = work.get(kf, ("hbm", None))
+        t = ms / K / 1e3
+        rate = None if w is None else (w / t / 1e12 if b == "mfma" else w / t / 1e9)
+        kernels[kf] = {"ms_per_step": ms / K, "launches_per_step": cnt / K, "bound": b,
+                       "achieved": rate, "frac": None if rate is None else
+                       rate / (FP32_PEAK / 1e12 if b == "mfma" else HBM_PEAK / 1e9)}
     out = {
         "metric": "analysis+synthesis maps/s at L=512,N=8: mixed-precision vs FP64, err<=1e-6",
         "value": maps_s,
@@ -169,9 +267,8 @@ def main():
         "stages_ms_per_step": {"mixed": {k: v[1] / K for k, v in st32.items()},
                                "fp64": {k: v[1] / K for k, v in st64.items()}},
         "algorithmic": {"flop_per_map": F, "bytes_per_map": Q},
-        "roofline": {"bound": "hbm", "achieved": None, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": None,
-                     "traffic": None, "kernel": dname, "launches_per_step": dcount / K,
-                     "avg_launch_ms": dms / dcount},
+        "roofline": roof,
+        "kernels_mixed": kernels,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, flm_h)

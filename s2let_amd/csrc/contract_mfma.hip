@@ -1,19 +1,64 @@
 // ell-contractions on the CDNA matrix cores (v_mfma_*_16x16x4).
 //
-// inverse (analysis, synthesis adjoint), one wave per (m', 16 |m|):
-//   F_c[m'][+-m] = sum_l A[(m,+-,re/im), l] B[l, c],
-//   A = Delta^l_{m'|m|} (sign (-1)^{l+m'} for -m) x[l][+-m],  B = b_c(l) Delta^l_{m' n_c}
-// forward (synthesis, analysis adjoint), one wave per (2 l-blocks, 4 |m|):
-//   H[l, (m,+-,re/im)] = sum_c E[l, c] J_c[m'][+-m],  E = b_c(l) Delta^l_{m' n_c},
-//   y[l][+-m] += sum_{m'} Delta^l_{m'|m|} (sign) H.
-// In FP32 the matrix products accumulate in FP32 over blocks of ell (or per m')
-// and the block sums are added to FP64 accumulators (compensated summation).
+// With the column weights E_c(l, m') = b_c(l) Delta^l_{m' |n_c|} s, s = (-1)^{l+m'}
+// for n_c < 0 (else 1), precomputed per plan:
+//
+// inverse (analysis, synthesis adjoint): for every m' and |m|,
+//   F_c[m'][+-m] = sum_l E_c(l, m') Delta^l_{m',+-m} x[l][+-m],
+//   MFMA product C[c][m] = sum_l E[c][l] B[l][m] per (m', 16 |m|), B the
+//   Delta-weighted harmonics in four real variants (+m / -m, re / im);
+// forward (synthesis, analysis adjoint): for every l and |m|,
+//   y[l][+-m] = sum_{m'} Delta^l_{m',+-m} H_{m'}[l][+-m],
+//   H_{m'}[l][m] = sum_c E_c(l, m') J_c[m'][m] (MFMA, K = column), then one FP64
+//   FMA per (l, m', m, variant) on the vector units.
+// Delta^l_{m',-m} = (-1)^{l+m'} Delta^l_{m'm}.
+//
+// Delta tiles.  The contractions read Delta^l_{m'm}, m', m >= 0, from a tiled
+// copy of the table: for every 16 x 16 tile (l-block lb, m-block mb <= lb) and
+// every m' <= 16 lb + 15 one 256-entry record in which lane (k4, r16) of a wave
+// (lane = 16 k4 + r16) finds its four values v = 0..3 at [4 lane + v]:
+//     Delta^l_{m'm},  l = 16 lb + k4 + 4 v,  m = 16 mb + r16   (zero for m' > l, m > l).
+// The forward kernel streams the records of one tile over m' (contiguous); the
+// inverse kernel reads one record per (m', l-block).  Both read each record
+// with one 16-byte (FP32) or two 16-byte (FP64) loads per lane.
+//
+// In FP32 the inverse products accumulate in FP32 over 32 ell and the block sums
+// are added in FP64; the forward H (16 columns) is converted to FP64 per m'.
 #include "kernels.h"
 #include "mfma.h"
 
 namespace s2h {
 
 __device__ inline size_t qo(int l) { return (size_t)l * (l + 1) * (2 * l + 1) / 6; }
+
+// first record of l-block lb; records of tile (lb, mb) start at
+// dtile_base(lb) + mb * 16 (lb + 1) * 256
+__host__ __device__ inline size_t dtile_base(int lb) {
+  return (size_t)4096 * ((size_t)lb * (lb + 1) * (2 * lb + 1) / 6);
+}
+size_t dtile_elems(int L) { return dtile_base((L + 15) / 16); }
+
+// A-operand row i of the forward product holds l = l0 + lrow(i), so that the C
+// element (k4, v) of a lane is l = l0 + k4 + 4 v for both precisions.
+template <typename T>
+__host__ __device__ inline int lrow_of(int i);
+template <>
+__host__ __device__ inline int lrow_of<float>(int i) { return (i >> 2) + 4 * (i & 3); }
+template <>
+__host__ __device__ inline int lrow_of<double>(int i) { return i; }
+
+template <typename T>
+struct V4 {
+  T v[4];
+};
+__device__ inline V4<float> ld4(const float *p) {
+  const float4 a = *reinterpret_cast<const float4 *>(p);
+  return V4<float>{{a.x, a.y, a.z, a.w}};
+}
+__device__ inline V4<double> ld4(const double *p) {
+  const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+  return V4<double>{{a.x, a.y, b.x, b.y}};
+}
 
 // ------------------------------------------------------------------ selftest
 // C = A B for A[16][4] = a_ik, B[4][16] = b_kj with small integers; out[i*16+j]
@@ -28,201 +73,253 @@ __global__ void k_mfma_probe(T *out) {
   for (int v = 0; v < 4; ++v) out[Mfma<T>::row(i / 16, v) * 16 + (i % 16)] = c[v];
 }
 
-// ------------------------------------------------------------------ inverse
-// Grid: x = m-block group (4 waves x 16 |m|), y = m'.  Columns: up to 16 of one
-// band-limit L; F_c = Fbase + cols[c].out_off (complex entries), layout [m'][m + L - 1].
+// ------------------------------------------------------------------ tables
+// Delta tiles of l-block lb = blockIdx.y from the FP64 quarter table (l < L).
 template <typename T>
-__global__ __launch_bounds__(256) void k_inv_mfma(const cx<T> *__restrict__ x, const T *__restrict__ dtab,
-                                                  const ColDev *__restrict__ cols, int ncol,
-                                                  const T *__restrict__ bvals, int L, cx<T> *__restrict__ Fbase) {
+__global__ void k_dtile(const double *__restrict__ q, int L, T *__restrict__ out) {
+  const int lb = blockIdx.y, nmp = 16 * (lb + 1);
+  const size_t n = (size_t)(lb + 1) * nmp * 256;
+  T *base = out + dtile_base(lb);
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int v = (int)(e & 3), lane = (int)((e >> 2) & 63);
+    const size_t r = e >> 8;
+    const int mp = (int)(r % nmp), mb = (int)(r / nmp);
+    const int l = 16 * lb + (lane >> 4) + 4 * v, m = 16 * mb + (lane & 15);
+    double val = 0.0;
+    if (l < L && m <= l && mp <= l) val = q[qo(l) + (size_t)mp * (l + 1) + m];
+    base[e] = (T)val;
+  }
+}
+
+// Column weights of one super-group (band-limit Ls, nc <= 16 columns, nlb = ceil(Ls/16)):
+//   Ef[((lb Ls + m') 64 + lane) 4 + s] = E_{4s+k4}(16 lb + lrow(r16), m')   (forward A operand)
+//   Ei[((m' nlb + lb) 64 + lane) 4 + s] = E_{r16}(16 lb + 4s + k4, m')     (inverse A operand)
+__device__ inline double eval_e(const double *dtab, const ColDev *cols, int nc, const double *bvals, int Ls, int c,
+                                int l, int mp) {
+  if (c >= nc || l >= Ls || mp > l) return 0.0;
+  const ColDev cd = cols[c];
+  const int an = cd.n < 0 ? -cd.n : cd.n;
+  if (l < cd.lmin || an > l) return 0.0;
+  double v = bvals[cd.boff + l] * dtab[qo(l) + (size_t)mp * (l + 1) + an];
+  if (cd.n < 0 && ((l + mp) & 1)) v = -v;
+  return v;
+}
+
+template <typename T>
+__global__ void k_etab(const double *__restrict__ dtab, const ColDev *__restrict__ cols, int nc,
+                       const double *__restrict__ bvals, int Ls, T *__restrict__ Ei, T *__restrict__ Ef) {
+  const int nlb = (Ls + 15) / 16;
+  const size_t total = (size_t)nlb * Ls * 256;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int s = (int)(e & 3), lane = (int)((e >> 2) & 63);
+    const int k4 = lane >> 4, r16 = lane & 15;
+    const size_t r = e >> 8;
+    {  // forward layout: r = lb * Ls + m'
+      const int mp = (int)(r % Ls), lb = (int)(r / Ls);
+      Ef[e] = (T)eval_e(dtab, cols, nc, bvals, Ls, 4 * s + k4, 16 * lb + lrow_of<T>(r16), mp);
+    }
+    {  // inverse layout: r = m' * nlb + lb
+      const int lb = (int)(r % nlb), mp = (int)(r / nlb);
+      Ei[e] = (T)eval_e(dtab, cols, nc, bvals, Ls, r16, 16 * lb + 4 * s + k4, mp);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ inverse
+// Block: 4 waves x 16 |m| (one m-block each), MP consecutive m'.  Grid: x = m-group
+// of 64 |m|, y = m' group.  F_c = Fbase + cols[c].out_off, layout [m'][m + L - 1].
+// x has Lx^2 entries (Lx >= L).
+template <typename T, int MP>
+__global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ x, int Lx, const T *__restrict__ dt,
+                                              const T *__restrict__ Ei, const ColDev *__restrict__ cols, int nc,
+                                              int lmin, int L, cx<T> *__restrict__ Fbase) {
   typedef typename Mfma<T>::acc acc_t;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int mp = blockIdx.y;
-  const int mb = (blockIdx.x * 4 + wave) * 16;  // first |m| of this wave
-  if (mb >= L) return;
-  const int r16 = lane & 15, k4 = lane >> 4;  // lane % 16, lane / 16
-  // column of this lane (B operand)
-  const int c = r16;
-  int cn = 0, clmin = L, cboff = 0;
-  if (c < ncol) {
-    cn = cols[c].n;
-    clmin = cols[c].lmin;
-    cboff = cols[c].boff;
-  }
-  const int can = cn < 0 ? -cn : cn;
-  const int m = mb + r16;  // A-operand row |m| of this lane
-  int lstart = max(mp, mb);
-  int lminc = L;
-  for (int q = 0; q < ncol; ++q) lminc = min(lminc, cols[q].lmin);
-  lstart = max(lstart, lminc);
-  lstart &= ~3;  // align K steps (rows below max(m', |m|) are predicated to zero)
-  acc_t arp = Mfma<T>::zero(), aip = Mfma<T>::zero(), arm = Mfma<T>::zero(), aim = Mfma<T>::zero();
-  double hrp[4] = {0, 0, 0, 0}, hip[4] = {0, 0, 0, 0}, hrm[4] = {0, 0, 0, 0}, him[4] = {0, 0, 0, 0};
-  int since = 0;
-  // K steps in groups of kU (all loads of a group issued before its MFMAs)
-  constexpr int kU = 4;
-  for (int lg = lstart; lg < L; lg += 4 * kU) {
-    T va_rp[kU], va_ip[kU], va_rm[kU], va_im[kU], vb[kU];
+  const int r16 = lane & 15, k4 = lane >> 4;
+  const int mp0 = blockIdx.y * MP;
+  const int mbi = blockIdx.x * 4 + wave, mb = 16 * mbi;
+  if (mb >= L || mp0 >= L) return;
+  const int nlb = (L + 15) / 16;
+  const int m = mb + r16;
+  const int lb0 = max(max(mp0, mb), lmin) >> 4;
+  acc_t acc[MP][4];
+  double hs[MP][4][4];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int l = lg + 4 * u + k4;
-      T a_rp = 0, a_ip = 0, a_rm = 0, a_im = 0, b = 0;
-      if (l < L && l >= mp) {
-        if (m <= l) {
-          const T d = dtab[qo(l) + (size_t)mp * (l + 1) + m];
-          const cx<T> xp = x[(size_t)l * l + l + m], xm = x[(size_t)l * l + l - m];
-          const T dm = ((l + mp) & 1) ? -d : d;
-          a_rp = d * xp.x;
-          a_ip = d * xp.y;
-          a_rm = dm * xm.x;
-          a_im = dm * xm.y;
-        }
-        if (c < ncol && l >= clmin && can <= l) {
-          T e = dtab[qo(l) + (size_t)mp * (l + 1) + can];
-          if (cn < 0 && ((l + mp) & 1)) e = -e;
-          b = bvals[cboff + l] * e;
-        }
+  for (int j = 0; j < MP; ++j)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      acc[j][q] = Mfma<T>::zero();
+#pragma unroll
+      for (int v = 0; v < 4; ++v) hs[j][q][v] = 0.0;
+    }
+  for (int lb = lb0; lb < nlb; ++lb) {
+    const int l0 = 16 * lb, nmp = 16 * (lb + 1);
+    V4<T> d[MP], e[MP];
+#pragma unroll
+    for (int j = 0; j < MP; ++j) {
+      const int mp = mp0 + j;
+      if (mp < nmp && mp < L) {
+        d[j] = ld4(dt + dtile_base(lb) + ((size_t)mbi * nmp + mp) * 256 + 4 * lane);
+        e[j] = ld4(Ei + (((size_t)mp * nlb + lb) * 64 + lane) * 4);
+      } else {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) d[j].v[s] = e[j].v[s] = (T)0;
       }
-      va_rp[u] = a_rp;
-      va_ip[u] = a_ip;
-      va_rm[u] = a_rm;
-      va_im[u] = a_im;
-      vb[u] = b;
     }
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      arp = Mfma<T>::mma(va_rp[u], vb[u], arp);
-      aip = Mfma<T>::mma(va_ip[u], vb[u], aip);
-      arm = Mfma<T>::mma(va_rm[u], vb[u], arm);
-      aim = Mfma<T>::mma(va_im[u], vb[u], aim);
-    }
-    if (sizeof(T) == 4 && ++since == 2) {  // FP32: flush the 32-ell block sum into FP64
-      since = 0;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        hrp[v] += (double)arp[v];
-        hip[v] += (double)aip[v];
-        hrm[v] += (double)arm[v];
-        him[v] += (double)aim[v];
+    for (int s = 0; s < 4; ++s) {
+      const int l = l0 + 4 * s + k4;
+      cx<T> xp = mk<T>(0, 0), xm = mk<T>(0, 0);
+      if (l < L && m <= l) {
+        xp = x[(size_t)l * l + l + m];
+        xm = x[(size_t)l * l + l - m];
       }
-      arp = aip = arm = aim = Mfma<T>::zero();
+#pragma unroll
+      for (int j = 0; j < MP; ++j) {
+        const T dp = d[j].v[s], dm = ((l + mp0 + j) & 1) ? -dp : dp;
+        acc[j][0] = Mfma<T>::mma(e[j].v[s], dp * xp.x, acc[j][0]);
+        acc[j][1] = Mfma<T>::mma(e[j].v[s], dp * xp.y, acc[j][1]);
+        acc[j][2] = Mfma<T>::mma(e[j].v[s], dm * xm.x, acc[j][2]);
+        acc[j][3] = Mfma<T>::mma(e[j].v[s], dm * xm.y, acc[j][3]);
+      }
+    }
+    if constexpr (sizeof(T) == 4) {
+      if ((lb - lb0) & 1) {  // flush the 32-ell block sums into FP64
+#pragma unroll
+        for (int j = 0; j < MP; ++j)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) hs[j][q][v] += (double)acc[j][q][v];
+            acc[j][q] = Mfma<T>::zero();
+          }
+      }
     }
   }
-  if (c >= ncol) return;
-  cx<T> *F = Fbase + cols[c].out_off;
+  (void)Lx;
+  if (m >= L) return;
   const int M = 2 * L - 1;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const int mm = mb + Mfma<T>::row(k4, v);
-    if (mm >= L) continue;
-    double rp = hrp[v] + (double)arp[v], ip = hip[v] + (double)aip[v];
-    double rm = hrm[v] + (double)arm[v], im = him[v] + (double)aim[v];
-    cx<T> *row = F + (size_t)mp * M + (L - 1);
-    row[mm] = mk<T>((T)rp, (T)ip);
-    if (mm > 0) row[-mm] = mk<T>((T)rm, (T)im);
+    const int c = Mfma<T>::row(k4, v);
+    if (c >= nc) continue;
+    cx<T> *F = Fbase + cols[c].out_off + (L - 1);
+#pragma unroll
+    for (int j = 0; j < MP; ++j) {
+      const int mp = mp0 + j;
+      if (mp >= L) continue;
+      const double rp = hs[j][0][v] + (double)acc[j][0][v], ip = hs[j][1][v] + (double)acc[j][1][v];
+      const double rm = hs[j][2][v] + (double)acc[j][2][v], im = hs[j][3][v] + (double)acc[j][3][v];
+      cx<T> *row = F + (size_t)mp * M;
+      row[m] = mk<T>((T)rp, (T)ip);
+      if (m > 0) row[-m] = mk<T>((T)rm, (T)im);
+    }
   }
 }
 
 // ------------------------------------------------------------------ forward
-// Wave: l-blocks (k, nb-1-k) x 4 |m|; block of 8 waves; grid x = m-block, y = pair group.
-// Output y (FP64) = sum over super-groups; J_c = Jbase + cols[col0 + c].out_off.
-constexpr int kFwdMC = 8;    // m' per LDS stage
-constexpr int kFwdWaves = 8;
+// Block: 4 waves; wave w takes the l-block pair (p, nb-1-p), p = 4 blockIdx.y + w,
+// for the |m| block 16 blockIdx.x, over the m' range of split blockIdx.z.  The
+// J tiles of MC consecutive m' (16 columns x 16 |m| x {+m re, +m im, -m re,
+// -m im}) are staged in LDS.  Partial sums over the m' split go to
+// ypart + z L^2 (reduced by k_sum_parts).
+template <typename T>
+struct FwdTile {
+  static constexpr int MC = sizeof(T) == 8 ? 4 : 8;
+};
 
 template <typename T>
-__global__ __launch_bounds__(512) void k_fwd_mfma(const SuperDev *__restrict__ sgs, int nsg,
-                                                  const ColDev *__restrict__ cols, const T *__restrict__ bvals,
-                                                  const T *__restrict__ dtab, int L, const cx<T> *__restrict__ Jbase,
-                                                  cx<double> *__restrict__ ypart, int mpchunk) {
-  // split s of the m' range: partial sums go to ypart + s * L^2 (reduced by k_sum_parts)
+__global__ __launch_bounds__(256) void k_fwd3(const SuperDev *__restrict__ sgs, int nsg, const ColDev *__restrict__ cols,
+                                              const T *const *__restrict__ Efs, const T *__restrict__ dt, int L,
+                                              const cx<T> *__restrict__ Jbase, cx<double> *__restrict__ ypart,
+                                              int mpchunk) {
+  constexpr int MC = FwdTile<T>::MC;
+  __shared__ T sJ[MC][16][16][4];  // [m'][column][|m|][variant]
+  typedef typename Mfma<T>::acc acc_t;
   cx<double> *__restrict__ y = ypart + (size_t)blockIdx.z * L * L;
   const int mps = blockIdx.z * mpchunk;
-  typedef typename Mfma<T>::acc acc_t;
-  __shared__ T sJ[kFwdMC][16][16];  // [m'][c][(m,+-,re/im)]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r16 = lane & 15, k4 = lane >> 4;
-  const int mb = blockIdx.x * 4;
+  const int mbi = blockIdx.x, mb = 16 * mbi;
   const int nb = (L + 15) / 16;
   const int npairs = (nb + 1) / 2;
-  const int pair = blockIdx.y * kFwdWaves + wave;
+  const int pair = blockIdx.y * 4 + wave;
   const bool active = pair < npairs;
   int lb[2];
-  lb[0] = pair * 16;
-  lb[1] = (nb - 1 - pair) * 16;
-  const int nblk = (active && lb[1] != lb[0]) ? 2 : (active ? 1 : 0);
-  // B-operand column (m, sign, re/im) of this lane
-  const int bm = mb + (r16 >> 2), bneg = (r16 >> 1) & 1, bim = r16 & 1;
-  double acc[2][4];
+  lb[0] = pair;
+  lb[1] = nb - 1 - pair;
+  const int nblk = active ? ((lb[1] != lb[0]) ? 2 : 1) : 0;
+  double acc[2][4][4];
 #pragma unroll
   for (int q = 0; q < 2; ++q)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[q][v] = 0;
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[q][u][v] = 0.0;
   for (int g = 0; g < nsg; ++g) {
     const SuperDev S = sgs[g];
     const int Lg = S.L, Mg = 2 * Lg - 1, nc = S.ncol;
+    if (mb >= Lg) continue;  // uniform over the block
     const ColDev *gc = cols + S.col0;
-    // m' range needed by this block: up to the largest l of any of its waves
-    int lmaxb = min(L, Lg) - 1;
-    const int mpend = lmaxb;  // m' <= l < Lg
-    const int mpe = min(mpend, mps + mpchunk - 1);
-    for (int mp0 = mps; mp0 <= mpe; mp0 += kFwdMC) {
+    const T *__restrict__ Ef = Efs[g];
+    const int mpe = min(Lg - 1, mps + mpchunk - 1);
+    for (int mp0 = mps; mp0 <= mpe; mp0 += MC) {
       __syncthreads();
-      for (int e = threadIdx.x; e < kFwdMC * 16 * 16; e += blockDim.x) {
-        const int mc = e >> 8, cc = (e >> 4) & 15, j = e & 15;
-        const int mpp = mp0 + mc;
-        const int jm = mb + (j >> 2), jneg = (j >> 1) & 1, jim = j & 1;
-        T v = 0;
-        if (cc < nc && mpp < Lg && jm < Lg) {
-          const cx<T> z = Jbase[gc[cc].out_off + (size_t)mpp * Mg + (Lg - 1) + (jneg ? -jm : jm)];
-          v = jim ? z.y : z.x;
-        }
-        sJ[mc][cc][j] = v;
+      for (int e = threadIdx.x; e < MC * 16 * 2 * 16; e += 256) {
+        const int ml = e & 15, pm = (e >> 4) & 1, cc = (e >> 5) & 15, mc = e >> 9;
+        const int mpp = mp0 + mc, mm = mb + ml;
+        cx<T> z = mk<T>(0, 0);
+        if (cc < nc && mpp < Lg && mm < Lg)
+          z = Jbase[gc[cc].out_off + (size_t)mpp * Mg + (Lg - 1) + (pm ? -mm : mm)];
+        sJ[mc][cc][ml][2 * pm] = z.x;
+        sJ[mc][cc][ml][2 * pm + 1] = z.y;
       }
       __syncthreads();
-      for (int q = 0; q < nblk; ++q) {
-        const int l0 = lb[q];
-        if (l0 >= Lg || l0 + 15 < mb) continue;  // no l < Lg, or all |m| > l
-        const int la = l0 + r16;                 // A-operand row l
-        for (int mc = 0; mc < kFwdMC; ++mc) {
-          const int mpp = mp0 + mc;
-          if (mpp > l0 + 15 || mpp >= Lg || mpp > mpe) break;
-          typename Mfma<T>::acc h = Mfma<T>::zero();
-          for (int c0 = 0; c0 < nc; c0 += 4) {
-            const int cc = c0 + k4;
-            T a = 0;
-            if (cc < nc && la < Lg && mpp <= la) {
-              const ColDev cd = gc[cc];
-              const int an = cd.n < 0 ? -cd.n : cd.n;
-              if (la >= cd.lmin && an <= la) {
-                T dd = dtab[qo(la) + (size_t)mpp * (la + 1) + an];
-                if (cd.n < 0 && ((la + mpp) & 1)) dd = -dd;
-                a = bvals[cd.boff + la] * dd;
-              }
-            }
-            h = Mfma<T>::mma(a, sJ[mc][cc & 15][r16], h);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int lbi = lb[q], l0 = 16 * lbi;
+        if (q >= nblk || l0 >= Lg || lbi < mbi) continue;  // no l < Lg, or all |m| > l
+        const int nmp = 16 * (lbi + 1);
+        const int n = min(MC, min(l0 + 16, mpe + 1) - mp0);  // m' <= l0 + 15
+        const T *dq = dt + dtile_base(lbi) + ((size_t)mbi * nmp + mp0) * 256 + 4 * lane;
+        const T *eq = Ef + ((size_t)lbi * Lg + mp0) * 256 + 4 * lane;
+        const int par = (l0 + k4 + mp0) & 1;
+        for (int mc = 0; mc < n; ++mc) {
+          const V4<T> dv = ld4(dq + (size_t)mc * 256), ev = ld4(eq + (size_t)mc * 256);
+          acc_t h[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) h[u] = Mfma<T>::zero();
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const T *bj = &sJ[mc][4 * s + k4][r16][0];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) h[u] = Mfma<T>::mma(ev.v[s], bj[u], h[u]);
           }
+          const bool odd = (par + mc) & 1;
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const int l = l0 + Mfma<T>::row(k4, v);
-            if (l < Lg && mpp <= l && bm <= l) {
-              T d = dtab[qo(l) + (size_t)mpp * (l + 1) + bm];
-              if (bneg && ((l + mpp) & 1)) d = -d;
-              acc[q][v] += (double)d * (double)h[v];
-            }
+            const double d = (double)dv.v[v], dn = odd ? -d : d;
+            acc[q][0][v] += d * (double)h[0][v];
+            acc[q][1][v] += d * (double)h[1][v];
+            acc[q][2][v] += dn * (double)h[2][v];
+            acc[q][3][v] += dn * (double)h[3][v];
           }
         }
       }
     }
   }
-  // write: lane holds rows lo+v, column (bm, bneg, bim)
-  for (int q = 0; q < nblk; ++q) {
+  // write: lane holds l = 16 lb[q] + k4 + 4 v, |m| = mb + r16
+  const int mm = mb + r16;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    if (q >= nblk) continue;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int l = lb[q] + Mfma<T>::row(k4, v);
-      if (l < L && bm <= l && !(bneg && bm == 0)) {
-        double *p = reinterpret_cast<double *>(y + (size_t)l * l + l + (bneg ? -bm : bm));
-        p[bim] = acc[q][v];
-      }
+      const int l = 16 * lb[q] + k4 + 4 * v;
+      if (l >= L || mm > l) continue;
+      cx<double> *yl = y + (size_t)l * l + l;
+      yl[mm] = mk<double>(acc[q][0][v], acc[q][1][v]);
+      if (mm > 0) yl[-mm] = mk<double>(acc[q][2][v], acc[q][3][v]);
     }
   }
 }
@@ -241,37 +338,62 @@ __global__ void k_sum_parts(const cx<double> *__restrict__ part, int nsplit, siz
 
 // ------------------------------------------------------------------ launchers
 template <typename T>
-void launch_inv_mfma(const void *x, const T *dtab, const ColDev *cols, int ncol, const T *bvals, int L, void *Fbase,
-                     hipStream_t st) {
-  dim3 grid((L + 63) / 64, L);
-  hipLaunchKernelGGL((k_inv_mfma<T>), grid, dim3(256), 0, st, (const cx<T> *)x, dtab, cols, ncol, bvals, L,
+void build_dtile(const double *q64, int L, T *out, hipStream_t st) {
+  const int nlb = (L + 15) / 16;
+  hipLaunchKernelGGL((k_dtile<T>), dim3(1024, nlb), dim3(256), 0, st, q64, L, out);
+}
+template void build_dtile<double>(const double *, int, double *, hipStream_t);
+template void build_dtile<float>(const double *, int, float *, hipStream_t);
+
+size_t etab_elems(int Ls) { return (size_t)((Ls + 15) / 16) * Ls * 256; }
+
+template <typename T>
+void build_etab(const double *dtab64, const ColDev *cols, int nc, const double *bvals, int Ls, T *Ei, T *Ef,
+                hipStream_t st) {
+  const size_t total = etab_elems(Ls);
+  const unsigned g = (unsigned)std::min<size_t>((total + 255) / 256, 65535 * 4);
+  hipLaunchKernelGGL((k_etab<T>), dim3(g), dim3(256), 0, st, dtab64, cols, nc, bvals, Ls, Ei, Ef);
+}
+template void build_etab<double>(const double *, const ColDev *, int, const double *, int, double *, double *, hipStream_t);
+template void build_etab<float>(const double *, const ColDev *, int, const double *, int, float *, float *, hipStream_t);
+
+constexpr int kInvMP = 2;
+
+template <typename T>
+void launch_inv_mfma(const void *x, int Lx, const T *dt, const T *Ei, const ColDev *cols, int ncol, int lmin, int L,
+                     void *Fbase, hipStream_t st) {
+  dim3 grid((L + 63) / 64, (L + kInvMP - 1) / kInvMP);
+  hipLaunchKernelGGL((k_inv3<T, kInvMP>), grid, dim3(256), 0, st, (const cx<T> *)x, Lx, dt, Ei, cols, ncol, lmin, L,
                      (cx<T> *)Fbase);
 }
-template void launch_inv_mfma<double>(const void *, const double *, const ColDev *, int, const double *, int, void *, hipStream_t);
-template void launch_inv_mfma<float>(const void *, const float *, const ColDev *, int, const float *, int, void *, hipStream_t);
+template void launch_inv_mfma<double>(const void *, int, const double *, const double *, const ColDev *, int, int, int, void *, hipStream_t);
+template void launch_inv_mfma<float>(const void *, int, const float *, const float *, const ColDev *, int, int, int, void *, hipStream_t);
 
 int fwd_splits(int L) { return std::max(1, std::min(8, L / 64)); }
 
 template <typename T>
-void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *bvals, const T *dtab, int L,
+void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *const *Efs, const T *dt, int L,
                      const void *Jbase, void *y, void *parts, hipStream_t st) {
   const int nb = (L + 15) / 16;
   const int npairs = (nb + 1) / 2;
   const int S = fwd_splits(L);
+  constexpr int MC = FwdTile<T>::MC;
   int chunk = (L + S - 1) / S;
-  chunk = (chunk + kFwdMC - 1) / kFwdMC * kFwdMC;
-  dim3 grid((L + 3) / 4, (npairs + kFwdWaves - 1) / kFwdWaves, S);
+  chunk = (chunk + MC - 1) / MC * MC;
+  dim3 grid(nb, (npairs + 3) / 4, S);
   cx<double> *out = (S == 1) ? (cx<double> *)y : (cx<double> *)parts;
-  hipLaunchKernelGGL((k_fwd_mfma<T>), grid, dim3(64 * kFwdWaves), 0, st, sgs, nsg, cols, bvals, dtab, L,
-                     (const cx<T> *)Jbase, out, chunk);
-  if (S > 1) {
-    const size_t n = (size_t)L * L;
-    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0, st,
-                       (const cx<double> *)parts, S, n, (cx<double> *)y);
-  }
+  hipLaunchKernelGGL((k_fwd3<T>), grid, dim3(256), 0, st, sgs, nsg, cols, Efs, dt, L, (const cx<T> *)Jbase, out, chunk);
 }
-template void launch_fwd_mfma<double>(const SuperDev *, int, const ColDev *, const double *, const double *, int, const void *, void *, void *, hipStream_t);
-template void launch_fwd_mfma<float>(const SuperDev *, int, const ColDev *, const float *, const float *, int, const void *, void *, void *, hipStream_t);
+template void launch_fwd_mfma<double>(const SuperDev *, int, const ColDev *, const double *const *, const double *, int, const void *, void *, void *, hipStream_t);
+template void launch_fwd_mfma<float>(const SuperDev *, int, const ColDev *, const float *const *, const float *, int, const void *, void *, void *, hipStream_t);
+
+void launch_sum_parts(int L, const void *parts, void *y, hipStream_t st) {
+  const int S = fwd_splits(L);
+  if (S == 1) return;
+  const size_t n = (size_t)L * L;
+  hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0, st,
+                     (const cx<double> *)parts, S, n, (cx<double> *)y);
+}
 
 int mfma_selftest() {
   double *d64 = nullptr;

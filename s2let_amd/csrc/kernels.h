@@ -58,14 +58,26 @@ void launch_to_f32(const double *a, float *b, size_t n, hipStream_t st);
 template <typename A, typename B>
 void launch_cast(const void *a, void *b, size_t n, hipStream_t st);
 
-// matrix-core contractions (contract_mfma.hip)
+// matrix-core contractions (contract_mfma.hip).
+// Delta tiles (16 x 16 tiles of Delta^l_{m'm}, one 256-entry record per m'):
+size_t dtile_elems(int L);
 template <typename T>
-void launch_inv_mfma(const void *x, const T *dtab, const ColDev *cols, int ncol, const T *bvals, int L, void *Fbase,
-                     hipStream_t st);
+void build_dtile(const double *q64, int L, T *out, hipStream_t st);
+// Column weight tables of one super-group (band-limit Ls, nc <= 16 columns),
+// E_c(l, m') = b_c(l) Delta^l_{m' |n_c|} (-1)^{(l+m')[n_c<0]}, in the A-operand
+// layouts of the forward (Ef) and inverse (Ei) products; etab_elems(Ls) each.
+size_t etab_elems(int Ls);
 template <typename T>
-void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *bvals, const T *dtab, int L,
+void build_etab(const double *dtab64, const ColDev *cols, int nc, const double *bvals, int Ls, T *Ei, T *Ef,
+                hipStream_t st);
+template <typename T>
+void launch_inv_mfma(const void *x, int Lx, const T *dt, const T *Ei, const ColDev *cols, int ncol, int lmin, int L,
+                     void *Fbase, hipStream_t st);
+template <typename T>
+void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *const *Efs, const T *dt, int L,
                      const void *Jbase, void *y, void *parts, hipStream_t st);
 int fwd_splits(int L);  // m' splits of the forward contraction (partials: fwd_splits(L) * L^2 complex128)
+void launch_sum_parts(int L, const void *parts, void *y, hipStream_t st);  // y = sum of the split partials
 int mfma_selftest();
 
 // theta stage (theta_mfma.hip).  Tables: 8 L x L matrices [kind][class],

@@ -85,7 +85,7 @@ template <>
 struct Sel<double> {
   static const DftDesc<double> &d(const DftTables &t) { return t.d64; }
   static const double *theta(const LTables &t) { return t.th64.as<double>(); }
-  static const double *delta(Context &c) { return c.delta64(); }
+  static const double *delta(Context &c) { return c.tile64(); }
   static const double *b(const Plan &p) { return p.d_b64.as<double>(); }
   static const double *b(const ShtPlan &p) { return p.d_b64.as<double>(); }
 };
@@ -93,7 +93,7 @@ template <>
 struct Sel<float> {
   static const DftDesc<float> &d(const DftTables &t) { return t.d32; }
   static const float *theta(const LTables &t) { return t.th32.as<float>(); }
-  static const float *delta(Context &c) { return c.delta32(); }
+  static const float *delta(Context &c) { return c.tile32(); }
   static const float *b(const Plan &p) { return p.d_b32.as<float>(); }
   static const float *b(const ShtPlan &p) { return p.d_b32.as<float>(); }
 };
@@ -163,6 +163,7 @@ RowArgs<T> row_args(int L, size_t nlines, int in_c, int out_c, int sign, double 
 template <typename T>
 const void *harmonic_input(Plan &P, const void *flm, hipStream_t st) {
   if (sizeof(T) == 8) return flm;
+  Stage s_("cast", st);
   launch_cast<double, float>(flm, P.flm32.p, (size_t)P.p.L * P.p.L, st);
   return P.flm32.p;
 }
@@ -178,18 +179,25 @@ const double kTwoPi = 6.28318530717958647692;
 template <typename T>
 void contract_inv_all(Plan &P, const void *x, const ColDev *cols, void *Fbase, hipStream_t st) {
   Context &C = Context::get();
-  for (const SuperDev &S : P.supers) {
+  for (size_t k = 0; k < P.supers.size(); ++k) {
+    const SuperDev &S = P.supers[k];
     Stage s_("contract_inv", st);
-    launch_inv_mfma<T>(x, Sel<T>::delta(C), cols + S.col0, S.ncol, Sel<T>::b(P), S.L, Fbase, st);
+    launch_inv_mfma<T>(x, P.p.L, Sel<T>::delta(C), (const T *)P.ei[k], cols + S.col0, S.ncol, P.super_lmin[k], S.L, Fbase, st);
   }
 }
 
 template <typename T>
 void contract_fwd_all(Plan &P, const ColDev *cols, const void *Jbase, void *flm, hipStream_t st) {
   Context &C = Context::get();
-  Stage s_("contract_fwd", st);
-  launch_fwd_mfma<T>(P.d_supers.as<SuperDev>(), (int)P.supers.size(), cols, Sel<T>::b(P), Sel<T>::delta(C), P.p.L,
-                     Jbase, flm, P.parts.p, st);
+  {
+    Stage s_("contract_fwd", st);
+    launch_fwd_mfma<T>(P.d_supers.as<SuperDev>(), (int)P.supers.size(), cols, P.d_efs.as<const T *>(),
+                       Sel<T>::delta(C), P.p.L, Jbase, flm, P.parts.p, st);
+  }
+  if (fwd_splits(P.p.L) > 1) {
+    Stage s_("contract_fwd_sum", st);
+    launch_sum_parts(P.p.L, P.parts.p, flm, st);
+  }
 }
 
 template <typename T>
@@ -287,7 +295,7 @@ void sht_inverse_t(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
   Context &C = Context::get();
   const int L = S.L;
   const ColDev *col = S.d_col.as<ColDev>();
-  { Stage s_("contract_inv", st); launch_inv_mfma<T>(sht_input<T>(S, flm, st), Sel<T>::delta(C), col, 1, Sel<T>::b(S), L, S.spec.p, st); }
+  { Stage s_("contract_inv", st); launch_inv_mfma<T>(sht_input<T>(S, flm, st), L, Sel<T>::delta(C), (const T *)S.ei, col, 1, std::abs(S.spin), L, S.spec.p, st); }
   { Stage s_("beta_inv", st); launch_theta<T>(S.spec.p, S.work.p, theta_args<T>(L, col, 1, 0), st); }
   { Stage s_("rows", st); launch_rows<T, double, T>(S.work.p, f, row_args<T>(L, (size_t)L, 1, 0, +1, 1.0), st); }
 }
@@ -299,7 +307,8 @@ void sht_forward_t(ShtPlan &S, const void *f, void *flm, hipStream_t st) {
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("rows", st); launch_rows<double, T, T>(f, S.spec.p, row_args<T>(L, (size_t)L, 0, 1, -1, kTwoPi / M), st); }
   { Stage s_("beta_fwd", st); launch_theta<T>(S.spec.p, S.work.p, theta_args<T>(L, col, 1, 2), st); }
-  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, Sel<T>::b(S), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, st); }
+  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, st); }
+  { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, S.parts.p, flm, st); }
 }
 
 template <typename T>
@@ -309,7 +318,8 @@ void sht_inverse_adjoint_t(ShtPlan &S, const void *f, void *flm, hipStream_t st)
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("rows", st); launch_rows<double, T, T>(f, S.spec.p, row_args<T>(L, (size_t)L, 0, 1, -1, 1.0), st); }
   { Stage s_("beta_inv_adj", st); launch_theta<T>(S.spec.p, S.work.p, theta_args<T>(L, col, 1, 1), st); }
-  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, Sel<T>::b(S), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, st); }
+  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, st); }
+  { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, S.parts.p, flm, st); }
 }
 
 template <typename T>
@@ -317,7 +327,7 @@ void sht_forward_adjoint_t(ShtPlan &S, const void *flm, void *f, hipStream_t st)
   Context &C = Context::get();
   const int L = S.L, M = 2 * L - 1;
   const ColDev *col = S.d_col.as<ColDev>();
-  { Stage s_("contract_inv", st); launch_inv_mfma<T>(sht_input<T>(S, flm, st), Sel<T>::delta(C), col, 1, Sel<T>::b(S), L, S.spec.p, st); }
+  { Stage s_("contract_inv", st); launch_inv_mfma<T>(sht_input<T>(S, flm, st), L, Sel<T>::delta(C), (const T *)S.ei, col, 1, std::abs(S.spin), L, S.spec.p, st); }
   { Stage s_("beta_fwd_adj", st); launch_theta<T>(S.spec.p, S.work.p, theta_args<T>(L, col, 1, 3), st); }
   { Stage s_("rows", st); launch_rows<T, double, T>(S.work.p, f, row_args<T>(L, (size_t)L, 1, 0, +1, kTwoPi / M), st); }
 }

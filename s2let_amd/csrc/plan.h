@@ -62,7 +62,11 @@ struct Plan {
   size_t n_wav_used = 0;
   int maxcol = 1;
   std::vector<SuperDev> supers;  // equal-L column runs (<= 16 columns each)
+  std::vector<int> super_lmin;   // smallest first ell of each super-group's columns
   DevBuf d_supers;
+  // column weight tables per super-group (precision of the mode): Ei, Ef pointers
+  DevBuf etab, d_efs;
+  std::vector<const void *> ei, ef;
   // device copies
   DevBuf d_cols_ana, d_cols_syn, d_b64, d_b32, d_gamma;
   DevBuf d_psi, d_phi;
@@ -81,14 +85,17 @@ struct ShtPlan {
   ColDev col;
   std::vector<double> bvals;
   DevBuf d_col, d_b64, d_b32, spec, work, flm32, d_super, parts;
+  DevBuf etab, d_efs;  // column weight tables (Ei, Ef) of the single column
+  const void *ei = nullptr;
 };
 
 class Context {
  public:
   static Context &get();
   void ensure_delta(int L, bool need_f32, hipStream_t st);
-  const double *delta64() const { return dtab64_.as<double>(); }
-  const float *delta32() const { return dtab32_.as<float>(); }
+  const double *delta64() const { return dtab64_.as<double>(); }  // quarter table
+  const double *tile64() const { return tile64_.as<double>(); }    // tiled copies
+  const float *tile32() const { return tile32_.as<float>(); }
   const DftTables &dft(int n);
   const LTables &ltab(int L);
   Plan &plan(const s2let_parameters_t *p, int mode, hipStream_t st);
@@ -102,8 +109,8 @@ class Context {
   Context();
   int dev_ = 0;
   size_t lds_bytes_ = 65536;
-  int dl_L_ = 0, dl32_L_ = 0;
-  DevBuf dtab64_, dtab32_;
+  int dl_L_ = 0, t64_L_ = 0, t32_L_ = 0;
+  DevBuf dtab64_, tile64_, tile32_;
   std::map<int, std::unique_ptr<DftTables>> dft_;
   std::map<int, std::unique_ptr<LTables>> lt_;
   std::map<std::string, std::unique_ptr<Plan>> plans_;

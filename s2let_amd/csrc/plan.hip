@@ -56,15 +56,18 @@ void Context::release() {
   dft_.clear();
   dtab64_.~DevBuf();
   new (&dtab64_) DevBuf();
-  dtab32_.~DevBuf();
-  new (&dtab32_) DevBuf();
-  dl_L_ = dl32_L_ = 0;
+  tile64_.~DevBuf();
+  new (&tile64_) DevBuf();
+  tile32_.~DevBuf();
+  new (&tile32_) DevBuf();
+  dl_L_ = t64_L_ = t32_L_ = 0;
 }
 
 static inline size_t qsize(int L) { return (size_t)L * (L + 1) * (2 * L + 1) / 6; }
 
 // Builds the quarter table of Delta^l = d^l(pi/2), l < L, by Risbo's recursion
-// in FP64: two half-integer steps per ell on ping-pong planes of rows m' >= -1.
+// in FP64 (two half-integer steps per ell on ping-pong planes of rows m' >= -1),
+// and its tiled copy in the precision the contractions of the mode read.
 void Context::ensure_delta(int L, bool need_f32, hipStream_t st) {
   if (L > dl_L_) {
     const int Lb = L;
@@ -87,13 +90,16 @@ void Context::ensure_delta(int L, bool need_f32, hipStream_t st) {
     S2H_CHECK_HIP(hipGetLastError());
     S2H_CHECK_HIP(hipStreamSynchronize(st));
     dl_L_ = Lb;
-    dl32_L_ = 0;
   }
-  if (need_f32 && dl32_L_ < dl_L_) {
-    dtab32_.alloc(qsize(dl_L_) * sizeof(float));
-    launch_to_f32(dtab64_.as<double>(), dtab32_.as<float>(), qsize(dl_L_), st);
+  int &tl = need_f32 ? t32_L_ : t64_L_;
+  if (tl < dl_L_) {
+    DevBuf &t = need_f32 ? tile32_ : tile64_;
+    t.alloc(dtile_elems(dl_L_) * (need_f32 ? sizeof(float) : sizeof(double)));
+    if (need_f32) build_dtile<float>(dtab64_.as<double>(), dl_L_, t.as<float>(), st);
+    else build_dtile<double>(dtab64_.as<double>(), dl_L_, t.as<double>(), st);
+    S2H_CHECK_HIP(hipGetLastError());
     S2H_CHECK_HIP(hipStreamSynchronize(st));
-    dl32_L_ = dl_L_;
+    tl = dl_L_;
   }
 }
 
@@ -177,6 +183,30 @@ const LTables &Context::ltab(int L) {
   auto &ref = *t;
   lt_[L] = std::move(t);
   return ref;
+}
+
+// Column weight tables (Ei, Ef) of each super-group, in the precision of the mode.
+static void build_weight_tables(const std::vector<SuperDev> &supers, const ColDev *d_cols, const double *d_b64,
+                                const double *dtab64, bool f32, DevBuf &etab, DevBuf &d_efs,
+                                std::vector<const void *> &ei, std::vector<const void *> &ef, hipStream_t st) {
+  const size_t esz = f32 ? sizeof(float) : sizeof(double);
+  size_t total = 0;
+  for (const SuperDev &S : supers) total += 2 * etab_elems(S.L);
+  etab.alloc(total * esz);
+  ei.clear();
+  ef.clear();
+  size_t off = 0;
+  for (const SuperDev &S : supers) {
+    const size_t n = etab_elems(S.L);
+    char *pi = (char *)etab.p + off * esz, *pf = pi + n * esz;
+    if (f32) build_etab<float>(dtab64, d_cols + S.col0, S.ncol, d_b64, S.L, (float *)pi, (float *)pf, st);
+    else build_etab<double>(dtab64, d_cols + S.col0, S.ncol, d_b64, S.L, (double *)pi, (double *)pf, st);
+    S2H_CHECK_HIP(hipGetLastError());
+    ei.push_back(pi);
+    ef.push_back(pf);
+    off += 2 * n;
+  }
+  upload(d_efs, ef, st);
 }
 
 static inline void ipow(int k, double &re, double &im) {
@@ -326,6 +356,13 @@ Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
   upload(pl.d_b64, pl.bvals, st);
   std::vector<float> bf(pl.bvals.begin(), pl.bvals.end());
   upload(pl.d_b32, bf, st);
+  for (const SuperDev &S : pl.supers) {
+    int lm = L;
+    for (int k = 0; k < S.ncol; ++k) lm = std::min(lm, pl.cols_ana[S.col0 + k].lmin);
+    pl.super_lmin.push_back(lm);
+  }
+  build_weight_tables(pl.supers, pl.d_cols_ana.as<ColDev>(), pl.d_b64.as<double>(), delta64(), f32, pl.etab, pl.d_efs,
+                      pl.ei, pl.ef, st);
   std::vector<cx<double>> psi_dev(pl.psi.size());
   for (size_t i = 0; i < pl.psi.size(); ++i) psi_dev[i] = mk<double>(pl.psi[i].real(), pl.psi[i].imag());
   upload(pl.d_psi, psi_dev, st);
@@ -373,6 +410,11 @@ ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st) {
   upload(s.d_b64, s.bvals, st);
   std::vector<float> bf(s.bvals.begin(), s.bvals.end());
   upload(s.d_b32, bf, st);
+  {
+    std::vector<const void *> ei, ef;
+    build_weight_tables(sv, s.d_col.as<ColDev>(), s.d_b64.as<double>(), delta64(), f32, s.etab, s.d_efs, ei, ef, st);
+    s.ei = ei[0];
+  }
   const size_t esz = f32 ? sizeof(cx<float>) : sizeof(cx<double>);
   const size_t ne = (size_t)L * (2 * L - 1);
   s.spec.alloc(ne * esz);
