@@ -284,8 +284,17 @@ __global__ __launch_bounds__(256) void k_fwd3(const SuperDev *__restrict__ sgs, 
         const T *dq = dt + dtile_base(lbi) + ((size_t)mbi * nmp + mp0) * 256 + 4 * lane;
         const T *eq = Ef + ((size_t)lbi * Lg + mp0) * 256 + 4 * lane;
         const int par = (l0 + k4 + mp0) & 1;
+        V4<T> dn{}, en{};
+        if (n > 0) {
+          dn = ld4(dq);
+          en = ld4(eq);
+        }
         for (int mc = 0; mc < n; ++mc) {
-          const V4<T> dv = ld4(dq + (size_t)mc * 256), ev = ld4(eq + (size_t)mc * 256);
+          const V4<T> dv = dn, ev = en;  // records of this m'; the next m' is loaded meanwhile
+          if (mc + 1 < n) {
+            dn = ld4(dq + (size_t)(mc + 1) * 256);
+            en = ld4(eq + (size_t)(mc + 1) * 256);
+          }
           acc_t h[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) h[u] = Mfma<T>::zero();

@@ -2,7 +2,6 @@
 #pragma once
 
 #include "common.h"
-#include "dft.h"
 
 namespace s2h {
 
@@ -42,6 +41,18 @@ struct ThetaArgs {
   int z_e[2], in_row[2], in_e[2], out_row[2], out_e[2];
 };
 
+// Two-level DFT of length M = M1 M2 on the matrix cores (rows_mfma.hip): real
+// embeddings of the M1- and M2-point DFT matrices, R x K padded (R % 16 == 0,
+// K % 4 == 0), stored transposed [k][r], for exponent sign -1 (f) and +1 (b);
+// tw[k] = e^{-2 pi i k / M}.
+template <typename T>
+struct Dft2 {
+  int M, M1, M2;
+  int R1, K1, R2, K2;
+  const T *G1f, *G1b, *G2f, *G2b;
+  const cx<T> *tw;
+};
+
 template <typename T>
 struct RowArgs {
   int L;
@@ -49,7 +60,7 @@ struct RowArgs {
   size_t nlines;
   int in_centred, out_centred, sign;
   T scale;
-  DftDesc<T> dM;
+  Dft2<T> d;
 };
 
 void launch_risbo_half(const double *in, double *out, int n, int rmax, int symrow, int lp, const double *sq,

@@ -12,7 +12,6 @@
 // Delta^l_{m'm}, m', m >= 0, and
 //     Delta^l_{m',-m} = (-1)^{l+m'} Delta^l_{m'm}.
 #include "kernels.h"
-#include "fft.h"
 
 namespace s2h {
 
@@ -62,35 +61,6 @@ template <typename A, typename B>
 __global__ void k_cast_cx(const cx<A> *__restrict__ a, cx<B> *__restrict__ b, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     b[i] = mk<B>((B)a[i].x, (B)a[i].y);
-}
-
-// ------------------------------------------------------------------ row DFTs (alpha / phi)
-// nlines contiguous lines of length M (stride M).  Input/output either in
-// natural DFT order or centred (index m + L - 1 for m in [-(L-1), L-1]).
-template <typename TI, typename TO, typename T>
-__global__ __launch_bounds__(256) void k_rows(const cx<TI> *__restrict__ in, cx<TO> *__restrict__ out, RowArgs<T> a) {
-  extern __shared__ char smem_raw[];
-  const int L = a.L, M = 2 * L - 1;
-  cx<T> *A = reinterpret_cast<cx<T> *>(smem_raw), *Bf = A + (size_t)a.lpb * M;
-  const DftDesc<T> dM = lds_twiddles(a.dM, Bf + (size_t)a.lpb * M);
-  const size_t line0 = (size_t)blockIdx.x * a.lpb;
-  const int nl = (int)min((size_t)a.lpb, a.nlines - line0);
-  for (int e = threadIdx.x; e < nl * M; e += blockDim.x) {
-    const int k = e / M, i = e - k * M;
-    const cx<TI> v = in[(line0 + k) * M + i];
-    int q = i;
-    if (a.in_centred) q = (i - (L - 1) + M) % M;
-    A[k * M + q] = mk<T>((T)v.x, (T)v.y);
-  }
-  __syncthreads();
-  const cx<T> *R = lds_fft(A, Bf, nl, M, dM, a.sign);
-  for (int e = threadIdx.x; e < nl * M; e += blockDim.x) {
-    const int k = e / M, i = e - k * M;
-    int q = i;
-    if (a.out_centred) q = (i - (L - 1) + M) % M;
-    const cx<T> v = R[k * M + q];
-    out[(line0 + k) * M + i] = mk<TO>((TO)(v.x * a.scale), (TO)(v.y * a.scale));
-  }
 }
 
 // ------------------------------------------------------------------ gamma mixing
@@ -151,18 +121,6 @@ void launch_cast(const void *a, void *b, size_t n, hipStream_t st) {
 }
 template void launch_cast<double, float>(const void *, void *, size_t, hipStream_t);
 template void launch_cast<float, double>(const void *, void *, size_t, hipStream_t);
-
-template <typename TI, typename TO, typename T>
-void launch_rows(const void *in, void *out, const RowArgs<T> &a, hipStream_t st) {
-  const int M = 2 * a.L - 1;
-  const size_t nb = (a.nlines + a.lpb - 1) / a.lpb;
-  const size_t shm = (2 * (size_t)a.lpb * M + M) * sizeof(cx<T>);
-  hipLaunchKernelGGL((k_rows<TI, TO, T>), dim3((unsigned)nb), dim3(256), shm, st, (const cx<TI> *)in, (cx<TO> *)out, a);
-}
-template void launch_rows<double, double, double>(const void *, void *, const RowArgs<double> &, hipStream_t);
-template void launch_rows<float, float, float>(const void *, void *, const RowArgs<float> &, hipStream_t);
-template void launch_rows<float, double, float>(const void *, void *, const RowArgs<float> &, hipStream_t);
-template void launch_rows<double, float, float>(const void *, void *, const RowArgs<float> &, hipStream_t);
 
 template <typename TI, typename TO, typename T>
 void launch_mix(const void *in, size_t in_stride, int nin, void *out, size_t out_stride, int nout, const void *W,

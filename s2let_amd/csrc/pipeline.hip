@@ -83,7 +83,7 @@ template <typename T>
 struct Sel;
 template <>
 struct Sel<double> {
-  static const DftDesc<double> &d(const DftTables &t) { return t.d64; }
+  static const Dft2<double> &d(const DftTables &t) { return t.d64; }
   static const double *theta(const LTables &t) { return t.th64.as<double>(); }
   static const double *delta(Context &c) { return c.tile64(); }
   static const double *b(const Plan &p) { return p.d_b64.as<double>(); }
@@ -91,7 +91,7 @@ struct Sel<double> {
 };
 template <>
 struct Sel<float> {
-  static const DftDesc<float> &d(const DftTables &t) { return t.d32; }
+  static const Dft2<float> &d(const DftTables &t) { return t.d32; }
   static const float *theta(const LTables &t) { return t.th32.as<float>(); }
   static const float *delta(Context &c) { return c.tile32(); }
   static const float *b(const Plan &p) { return p.d_b32.as<float>(); }
@@ -148,15 +148,16 @@ RowArgs<T> row_args(int L, size_t nlines, int in_c, int out_c, int sign, double 
   a.out_centred = out_c;
   a.sign = sign;
   a.scale = (T)scale;
-  a.dM = Sel<T>::d(*lt.dM);
+  a.d = Sel<T>::d(*lt.dM);
   const int M = 2 * L - 1;
-  const size_t twb = (size_t)M * sizeof(cx<T>);
-  const size_t cap = (lds_budget(C) > twb ? lds_budget(C) - twb : 0) / (2 * sizeof(cx<T>));
-  if ((size_t)M > cap) {
+  const size_t per_line = 2 * (size_t)M * sizeof(cx<T>);  // input and intermediate
+  size_t cap = lds_budget(C) / per_line;
+  if (cap < 1) cap = C.lds_bytes() / per_line;  // one line needs more than the default share
+  if (cap < 1) {
     set_error(-3, "band-limit too large for the LDS row stage on this device");
     throw ArgFailure();
   }
-  a.lpb = (int)std::max<size_t>(1, std::min<size_t>(cap / M, 64));
+  a.lpb = (int)std::min<size_t>(cap, 64);
   return a;
 }
 

@@ -26,10 +26,10 @@ struct DevBuf {
   }
 };
 
-struct DftTables {
-  DftDesc<double> d64;
-  DftDesc<float> d32;
-  DevBuf tw64, tw32;
+struct DftTables {  // two-level DFT of one length M (see Dft2)
+  Dft2<double> d64;
+  Dft2<float> d32;
+  DevBuf g64, g32, tw64, tw32;
 };
 
 struct LTables {  // per band-limit L

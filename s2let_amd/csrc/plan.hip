@@ -103,35 +103,53 @@ void Context::ensure_delta(int L, bool need_f32, hipStream_t st) {
   }
 }
 
-static void factorize(int n, DftDesc<double> &d) {
-  d.n = n;
-  d.nf = 0;
-  int m = n;
-  if ((n & (n - 1)) == 0) {
-    while (m % 4 == 0 && m > 1) {
-      d.fac[d.nf++] = 4;
-      m /= 4;
+// Real embedding of the P-point DFT matrix (exponent sign), R x K padded, stored
+// transposed [k][r]; row r = 16 mt + rl holds output (k, re/im) with
+// k = 8 mt + 2 k4 + (v >> 1), re/im = v & 1, where (k4, v) is the accumulator
+// element of a lane that holds row rl (FP32: rl = 4 k4 + v, FP64: rl = k4 + 4 v).
+template <typename T>
+static void dft_embed(int P, int R, int K, int sign, bool f32layout, std::vector<T> &G) {
+  const long double pi = 3.14159265358979323846264338327950288L;
+  G.assign((size_t)K * R, (T)0);
+  for (int r = 0; r < R; ++r) {
+    const int mt = r / 16, rl = r % 16;
+    const int k4 = f32layout ? (rl >> 2) : (rl & 3), v = f32layout ? (rl & 3) : (rl >> 2);
+    const int k = 8 * mt + 2 * k4 + (v >> 1), ri = v & 1;
+    if (k >= P) continue;
+    for (int n = 0; n < P; ++n) {
+      const long double ang = sign * 2.0L * pi * (long double)(((long long)n * k) % P) / (long double)P;
+      const long double c = cosl(ang), sn = sinl(ang);
+      G[(size_t)(2 * n) * R + r] = (T)(ri ? sn : c);
+      G[(size_t)(2 * n + 1) * R + r] = (T)(ri ? c : -sn);
     }
-    if (m == 2) d.fac[d.nf++] = 2;
-    return;
-  }
-  for (int f = 2; m > 1;) {
-    if (m % f == 0) {
-      d.fac[d.nf++] = f;
-      m /= f;
-    } else
-      f = (f == 2) ? 3 : f + 2;
   }
 }
+
+static inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
 const DftTables &Context::dft(int n) {
   auto it = dft_.find(n);
   if (it != dft_.end()) return *it->second;
-  auto t = std::make_unique<DftTables>();
-  factorize(n, t->d64);
-  if (t->d64.nf > kMaxFactors) {
-    set_error(-2, "DFT length has too many factors");
+  int m1 = 1;
+  for (int f = 1; (long long)f * f <= n; ++f)
+    if (n % f == 0) m1 = f;
+  const int m2 = n / m1;
+  if (m1 == 1 && n > 1024) {
+    set_error(-2, "DFT length 2L-1 is a prime above 1024 (unsupported band-limit)");
     throw ArgFailure();
+  }
+  auto t = std::make_unique<DftTables>();
+  const int R1 = round_up(2 * m1, 16), K1 = round_up(2 * m1, 4), R2 = round_up(2 * m2, 16), K2 = round_up(2 * m2, 4);
+  const size_t s1 = (size_t)R1 * K1, s2 = (size_t)R2 * K2;
+  std::vector<double> g64, part64;
+  std::vector<float> g32, part32;
+  for (int which = 0; which < 4; ++which) {  // G1f, G1b, G2f, G2b
+    const int P = which < 2 ? m1 : m2, R = which < 2 ? R1 : R2, K = which < 2 ? K1 : K2;
+    const int sign = (which & 1) ? +1 : -1;
+    dft_embed<double>(P, R, K, sign, false, part64);
+    dft_embed<float>(P, R, K, sign, true, part32);
+    g64.insert(g64.end(), part64.begin(), part64.end());
+    g32.insert(g32.end(), part32.begin(), part32.end());
   }
   std::vector<cx<double>> w(n);
   std::vector<cx<float>> wf(n);
@@ -141,14 +159,27 @@ const DftTables &Context::dft(int n) {
     wf[k] = mk<float>((float)w[k].x, (float)w[k].y);
   }
   hipStream_t st = stream();
+  upload(t->g64, g64, st);
+  upload(t->g32, g32, st);
   upload(t->tw64, w, st);
   upload(t->tw32, wf, st);
   S2H_CHECK_HIP(hipStreamSynchronize(st));
-  t->d64.tw = t->tw64.as<cx<double>>();
-  t->d32.n = t->d64.n;
-  t->d32.nf = t->d64.nf;
-  for (int i = 0; i < kMaxFactors; ++i) t->d32.fac[i] = t->d64.fac[i];
-  t->d32.tw = t->tw32.as<cx<float>>();
+  auto fill = [&](auto &d, auto *g, auto *tw) {
+    d.M = n;
+    d.M1 = m1;
+    d.M2 = m2;
+    d.R1 = R1;
+    d.K1 = K1;
+    d.R2 = R2;
+    d.K2 = K2;
+    d.G1f = g;
+    d.G1b = g + s1;
+    d.G2f = g + 2 * s1;
+    d.G2b = g + 2 * s1 + s2;
+    d.tw = tw;
+  };
+  fill(t->d64, t->g64.as<double>(), t->tw64.as<cx<double>>());
+  fill(t->d32, t->g32.as<float>(), t->tw32.as<cx<float>>());
   auto &ref = *t;
   dft_[n] = std::move(t);
   return ref;

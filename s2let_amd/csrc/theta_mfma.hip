@@ -124,7 +124,7 @@ struct ThetaTile {
 template <typename T>
 __global__ __launch_bounds__(256) void k_theta(const cx<T> *__restrict__ in, cx<T> *__restrict__ out, ThetaArgs<T> a) {
   constexpr int BK = ThetaTile<T>::BK, SJ = ThetaTile<T>::SJ;
-  __shared__ T sB[2][2][BK][SJ];  // [class][re/im][k][j]
+  __shared__ T sB[2][2][2][BK][SJ];  // [buffer][class][re/im][k][j]
   typedef typename Mfma<T>::acc acc_t;
   const int L = a.L, M = 2 * L - 1;
   const int nti = (L + 63) / 64, ntm = (M + 63) / 64;
@@ -162,39 +162,59 @@ __global__ __launch_bounds__(256) void k_theta(const cx<T> *__restrict__ in, cx<
         for (int v = 0; v < 4; ++v) hs[r][qq][v] = 0.0;
   }
 
-  for (int k0 = 0; k0 < L; k0 += BK) {
+  // software pipeline over K chunks: the next chunk's samples (HBM) and A
+  // fragments (L2) are loaded while the current chunk's MFMAs run; the samples
+  // go through a double-buffered LDS tile (one barrier per chunk).
+  constexpr int UPT = BK * 64 / 256;  // complex samples staged per thread per chunk
+  const int nch = (L + BK - 1) / BK;
+  cx<T> breg[UPT];
+  T anx[BK / 4][2], acur[BK / 4][2];
+  auto load_chunk = [&](int k0) {
 #pragma unroll
-    for (int u = 0; u < BK * 64 / 256; ++u) {
+    for (int u = 0; u < UPT; ++u) {
       const int e = tid + 256 * u, kk = e >> 6, ee = e & 63;
       const int k = k0 + kk, mi = mi0 + ee;
-      cx<T> v = mk<T>(0, 0);
-      if (k < L && mi < M) v = inc[(size_t)k * M + mi];
-      const int cl = (ee & 1) ^ par0;
-      if (k == (cl ? in_row1 : in_row0)) v = ipow_mul(cl ? in_e1 : in_e0, v);
-      sB[cl][0][kk][ee >> 1] = v.x;
-      sB[cl][1][kk][ee >> 1] = v.y;
+      breg[u] = (k < L && mi < M) ? inc[(size_t)k * M + mi] : mk<T>(0, 0);
     }
-    __syncthreads();
-    T av[BK / 4][2];
 #pragma unroll
     for (int s = 0; s < BK / 4; ++s) {
       const int k = k0 + 4 * s + k4;
 #pragma unroll
       for (int r = 0; r < 2; ++r) {
         const int i = i0w + 16 * r + r16;
-        av[s][r] = (k < L && i < L) ? AT[(size_t)k * L + i] : (T)0;
+        anx[s][r] = (k < L && i < L) ? AT[(size_t)k * L + i] : (T)0;
       }
     }
+  };
+  load_chunk(0);
+  for (int ch = 0; ch < nch; ++ch) {
+    const int k0 = ch * BK, buf = ch & 1;
+#pragma unroll
+    for (int u = 0; u < UPT; ++u) {
+      const int e = tid + 256 * u, kk = e >> 6, ee = e & 63;
+      const int cl = (ee & 1) ^ par0;
+      cx<T> v = breg[u];
+      if (k0 + kk == (cl ? in_row1 : in_row0)) v = ipow_mul(cl ? in_e1 : in_e0, v);
+      sB[buf][cl][0][kk][ee >> 1] = v.x;
+      sB[buf][cl][1][kk][ee >> 1] = v.y;
+    }
+#pragma unroll
+    for (int s = 0; s < BK / 4; ++s) {
+      acur[s][0] = anx[s][0];
+      acur[s][1] = anx[s][1];
+    }
+    __syncthreads();
+    if (ch + 1 < nch) load_chunk(k0 + BK);
 #pragma unroll
     for (int s = 0; s < BK / 4; ++s) {
       const int kk = 4 * s + k4;
       T bv[4];
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq) bv[qq] = sB[cls][qq >> 1][kk][16 * (qq & 1) + r16];
+      for (int qq = 0; qq < 4; ++qq) bv[qq] = sB[buf][cls][qq >> 1][kk][16 * (qq & 1) + r16];
 #pragma unroll
       for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq) acc[r][qq] = Mfma<T>::mma(av[s][r], bv[qq], acc[r][qq]);
+        for (int qq = 0; qq < 4; ++qq) acc[r][qq] = Mfma<T>::mma(acur[s][r], bv[qq], acc[r][qq]);
     }
     if constexpr (sizeof(T) == 4) {
 #pragma unroll
@@ -206,7 +226,6 @@ __global__ __launch_bounds__(256) void k_theta(const cx<T> *__restrict__ in, cx<
           acc[r][qq] = Mfma<T>::zero();
         }
     }
-    __syncthreads();
   }
 
   const cx<double> ph = mk<double>(col.ph_re, col.ph_im);
