@@ -42,15 +42,20 @@ struct ThetaArgs {
 };
 
 // Two-level DFT of length M = M1 M2 on the matrix cores (rows_mfma.hip): real
-// embeddings of the M1- and M2-point DFT matrices, R x K padded (R % 16 == 0,
-// K % 4 == 0), stored transposed [k][r], for exponent sign -1 (f) and +1 (b);
-// tw[k] = e^{-2 pi i k / M}.
+// embeddings of the M1- and M2-point DFT matrices, R x K (R, K multiples of 16),
+// for exponent sign -1 (f) and +1 (b), stored as MFMA A fragments:
+// F[((mt K/4) + s) 64 + lane] = G[16 mt + lane % 16][4 s + lane / 16].
+// The lines of a block are staged in LDS as columns of the two products, with
+// the reduction index fastest: column stride CSA = K1 + 4 (first product) and
+// CSC = K2 + 4 (second).  perm[q] is the LDS offset, within a line, of the real
+// part of input sample q.  tw[k] = e^{-2 pi i k / M}.
 template <typename T>
 struct Dft2 {
   int M, M1, M2;
-  int R1, K1, R2, K2;
-  const T *G1f, *G1b, *G2f, *G2b;
+  int R1, K1, R2, K2, CSA, CSC;
+  const T *F1f, *F1b, *F2f, *F2b;
   const cx<T> *tw;
+  const int *perm;
 };
 
 template <typename T>
@@ -91,16 +96,61 @@ int fwd_splits(int L);  // m' splits of the forward contraction (partials: fwd_s
 void launch_sum_parts(int L, const void *parts, void *y, hipStream_t st);  // y = sum of the split partials
 int mfma_selftest();
 
+// Grouped launches: one launch runs the jobs of several column groups (each job
+// its own arguments and block range [blk0, blk0 + its blocks)); the job list is
+// a kernel argument.
+constexpr int kMaxJobs = 16;
+template <typename T>
+struct ThetaJob {
+  ThetaArgs<T> a;
+  const void *in;
+  void *out;
+  int blk0;
+};
+template <typename T>
+struct ThetaJobs {
+  int n, nblk;
+  ThetaJob<T> j[kMaxJobs];
+};
+template <typename T>
+struct RowJob {
+  RowArgs<T> a;
+  const void *in;
+  void *out;
+  int blk0;
+};
+template <typename T>
+struct RowJobs {
+  int n, nblk;
+  RowJob<T> j[kMaxJobs];
+};
+struct MixJob {
+  const void *in;
+  void *out;
+  const void *W;  // complex128 nout x nin
+  size_t in_stride, out_stride, npts;
+  int nin, nout, blk0;
+};
+struct MixJobs {
+  int n, nblk;
+  MixJob j[kMaxJobs];
+};
+
 // theta stage (theta_mfma.hip).  Tables: 8 L x L matrices [kind][class],
 // kind 0 inverse, 1 inverse adjoint, 2 forward, 3 forward adjoint.
 void build_theta_tables(int L, const double *cs, const double *sn, double *th, hipStream_t st);
 template <typename T>
-void launch_theta(const void *in, void *out, const ThetaArgs<T> &a, hipStream_t st);
+int theta_blocks(const ThetaArgs<T> &a);
+template <typename T>
+void launch_theta(const ThetaJobs<T> &jobs, hipStream_t st);
 template <typename TI, typename TO, typename T>
-void launch_rows(const void *in, void *out, const RowArgs<T> &a, hipStream_t st);
+void launch_rows(const RowJobs<T> &jobs, hipStream_t st);
+constexpr int kRowsItems = 8;  // first-product items a wave keeps in registers
+int rows_items_per_wave(int M1, int M2, int R1, int lpb);
+size_t rows_lds_bytes(int M1, int M2, int CSA, int CSC, int lpb, size_t esz);
+constexpr int kMixBlocks = 256;  // blocks per mixing job
 template <typename TI, typename TO, typename T>
-void launch_mix(const void *in, size_t in_stride, int nin, void *out, size_t out_stride, int nout, const void *W,
-                size_t npts, hipStream_t st);
+void launch_mix(const MixJobs &jobs, hipStream_t st);
 
 void launch_lm2lmn(void *wlmn, const void *flm, const void *psi_j, int Lj, int Nj, int spin, int mode, hipStream_t st);
 void launch_lmn2lm(void *flm, const void *wlmn, const void *psi_j, int Lj, int Nj, int spin, int mode, hipStream_t st);

@@ -64,21 +64,28 @@ __global__ void k_cast_cx(const cx<A> *__restrict__ a, cx<B> *__restrict__ b, si
 }
 
 // ------------------------------------------------------------------ gamma mixing
-// out_k[pt] = sum_j W[k][j] in_j[pt] (complex, nout x nin matrix W, nin <= NMAX)
+// per job: out_k[pt] = sum_j W[k][j] in_j[pt] (complex, nout x nin matrix W, nin <= NMAX),
+// kMixBlocks blocks per job
 template <typename TI, typename TO, typename T, int NMAX>
-__global__ __launch_bounds__(256) void k_mix(const cx<TI> *__restrict__ in, size_t in_stride, int nin,
-                                             cx<TO> *__restrict__ out, size_t out_stride, int nout,
-                                             const cx<double> *__restrict__ W, size_t npts) {
+__global__ __launch_bounds__(256) void k_mix(MixJobs jobs) {
   __shared__ cx<T> sW[32 * 32];
+  int jb = 0;
+  while (jb + 1 < jobs.n && (int)blockIdx.x >= jobs.j[jb + 1].blk0) ++jb;
+  const MixJob &J = jobs.j[jb];
+  const cx<TI> *__restrict__ in = static_cast<const cx<TI> *>(J.in);
+  cx<TO> *__restrict__ out = static_cast<cx<TO> *>(J.out);
+  const cx<double> *__restrict__ W = static_cast<const cx<double> *>(J.W);
+  const int nin = J.nin, nout = J.nout;
   for (int i = threadIdx.x; i < nin * nout; i += blockDim.x) sW[i] = mk<T>((T)W[i].x, (T)W[i].y);
   __syncthreads();
-  for (size_t pt = blockIdx.x * (size_t)blockDim.x + threadIdx.x; pt < npts; pt += (size_t)gridDim.x * blockDim.x) {
+  const size_t b0 = blockIdx.x - J.blk0, nbj = (jb + 1 < jobs.n ? jobs.j[jb + 1].blk0 : jobs.nblk) - J.blk0;
+  for (size_t pt = b0 * blockDim.x + threadIdx.x; pt < J.npts; pt += nbj * blockDim.x) {
     cx<T> v[NMAX];
 #pragma unroll
     for (int j = 0; j < NMAX; ++j) {
       v[j] = mk<T>(0, 0);
       if (j < nin) {
-        const cx<TI> u = in[j * in_stride + pt];
+        const cx<TI> u = in[j * J.in_stride + pt];
         v[j] = mk<T>((T)u.x, (T)u.y);
       }
     }
@@ -92,7 +99,7 @@ __global__ __launch_bounds__(256) void k_mix(const cx<TI> *__restrict__ in, size
           ay += v[j].x * w.y + v[j].y * w.x;
         }
       }
-      out[k * out_stride + pt] = mk<TO>((TO)ax, (TO)ay);
+      out[k * J.out_stride + pt] = mk<TO>((TO)ax, (TO)ay);
     }
   }
 }
@@ -123,19 +130,17 @@ template void launch_cast<double, float>(const void *, void *, size_t, hipStream
 template void launch_cast<float, double>(const void *, void *, size_t, hipStream_t);
 
 template <typename TI, typename TO, typename T>
-void launch_mix(const void *in, size_t in_stride, int nin, void *out, size_t out_stride, int nout, const void *W,
-                size_t npts, hipStream_t st) {
-  const unsigned g = (unsigned)std::min<size_t>((npts + 255) / 256, 8192);
-  const cx<TI> *i = (const cx<TI> *)in;
-  cx<TO> *o = (cx<TO> *)out;
-  const cx<double> *w = (const cx<double> *)W;
-  if (nin <= 8) hipLaunchKernelGGL((k_mix<TI, TO, T, 8>), dim3(g), dim3(256), 0, st, i, in_stride, nin, o, out_stride, nout, w, npts);
-  else if (nin <= 16) hipLaunchKernelGGL((k_mix<TI, TO, T, 16>), dim3(g), dim3(256), 0, st, i, in_stride, nin, o, out_stride, nout, w, npts);
-  else hipLaunchKernelGGL((k_mix<TI, TO, T, 32>), dim3(g), dim3(256), 0, st, i, in_stride, nin, o, out_stride, nout, w, npts);
+void launch_mix(const MixJobs &jobs, hipStream_t st) {
+  if (jobs.n == 0 || jobs.nblk == 0) return;
+  int nmax = 0;
+  for (int k = 0; k < jobs.n; ++k) nmax = std::max(nmax, jobs.j[k].nin);
+  if (nmax <= 8) hipLaunchKernelGGL((k_mix<TI, TO, T, 8>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
+  else if (nmax <= 16) hipLaunchKernelGGL((k_mix<TI, TO, T, 16>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
+  else hipLaunchKernelGGL((k_mix<TI, TO, T, 32>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
 }
-template void launch_mix<double, double, double>(const void *, size_t, int, void *, size_t, int, const void *, size_t, hipStream_t);
-template void launch_mix<float, double, float>(const void *, size_t, int, void *, size_t, int, const void *, size_t, hipStream_t);
-template void launch_mix<double, float, float>(const void *, size_t, int, void *, size_t, int, const void *, size_t, hipStream_t);
-template void launch_mix<float, float, float>(const void *, size_t, int, void *, size_t, int, const void *, size_t, hipStream_t);
+template void launch_mix<double, double, double>(const MixJobs &, hipStream_t);
+template void launch_mix<float, double, float>(const MixJobs &, hipStream_t);
+template void launch_mix<double, float, float>(const MixJobs &, hipStream_t);
+template void launch_mix<float, float, float>(const MixJobs &, hipStream_t);
 
 }  // namespace s2h

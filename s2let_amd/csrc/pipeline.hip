@@ -149,15 +149,23 @@ RowArgs<T> row_args(int L, size_t nlines, int in_c, int out_c, int sign, double 
   a.sign = sign;
   a.scale = (T)scale;
   a.d = Sel<T>::d(*lt.dM);
-  const int M = 2 * L - 1;
-  const size_t per_line = 2 * (size_t)M * sizeof(cx<T>);  // input and intermediate
-  size_t cap = lds_budget(C) / per_line;
-  if (cap < 1) cap = C.lds_bytes() / per_line;  // one line needs more than the default share
-  if (cap < 1) {
+  const Dft2<T> &d = a.d;
+  auto fits = [&](int lpb, size_t cap) {
+    return rows_lds_bytes(d.M1, d.M2, d.CSA, d.CSC, lpb, sizeof(T)) <= cap &&
+           rows_items_per_wave(d.M1, d.M2, d.R1, lpb) <= kRowsItems;
+  };
+  // a quarter of the LDS per block (several blocks per CU overlap their staging
+  // with the matrix products of the others), more only if one line needs it
+  size_t cap = std::min(lds_budget(C), C.lds_bytes() / 4);
+  if (!fits(1, cap)) cap = lds_budget(C);
+  if (!fits(1, cap)) cap = C.lds_bytes();
+  if (!fits(1, cap)) {
     set_error(-3, "band-limit too large for the LDS row stage on this device");
     throw ArgFailure();
   }
-  a.lpb = (int)std::min<size_t>(cap, 64);
+  int lpb = 1;
+  while (lpb < 64 && fits(lpb + 1, cap)) ++lpb;
+  a.lpb = lpb;
   return a;
 }
 
@@ -201,89 +209,175 @@ void contract_fwd_all(Plan &P, const ColDev *cols, const void *Jbase, void *flm,
   }
 }
 
+// ---- grouped launches over the plan's column groups
+// theta stage of every group: in/out = plan buffers at each group's spectra offset
+template <typename T>
+void theta_groups(Plan &P, const ColDev *cols, int kind, const DevBuf &in, DevBuf &out, const char *name,
+                  hipStream_t st) {
+  ThetaJobs<T> J{};
+  auto flush = [&]() {
+    Stage s_(name, st);
+    launch_theta<T>(J, st);
+    J.n = 0;
+    J.nblk = 0;
+  };
+  for (const Group &g : P.groups) {
+    ThetaJob<T> &j = J.j[J.n++];
+    j.a = theta_args<T>(g.L, cols + g.col0, g.ncol, kind);
+    j.in = at<T>(in, g.spec_off);
+    j.out = at<T>(out, g.spec_off);
+    j.blk0 = J.nblk;
+    J.nblk += theta_blocks<T>(j.a);
+    if (J.n == kMaxJobs) flush();
+  }
+  if (J.n) flush();
+}
+
+// row DFTs of the wavelet groups (all lines of a group are contiguous, M each)
+template <typename T>
+void rows_groups(Plan &P, const DevBuf &in, DevBuf &out, int in_c, int out_c, int sign, bool scaled, hipStream_t st) {
+  RowJobs<T> J{};
+  auto flush = [&]() {
+    Stage s_("rows", st);
+    launch_rows<T, T, T>(J, st);
+    J.n = 0;
+    J.nblk = 0;
+  };
+  for (const Group &g : P.groups) {
+    if (g.ngamma == 0) continue;
+    const int M = 2 * g.L - 1;
+    RowJob<T> &j = J.j[J.n++];
+    j.a = row_args<T>(g.L, (size_t)g.ncol * g.L, in_c, out_c, sign, scaled ? kTwoPi / M : 1.0);
+    j.in = at<T>(in, g.spec_off);
+    j.out = at<T>(out, g.spec_off);
+    j.blk0 = J.nblk;
+    J.nblk += (int)((j.a.nlines + j.a.lpb - 1) / j.a.lpb);
+    if (J.n == kMaxJobs) flush();
+  }
+  if (J.n) flush();
+}
+
+// row DFT of the scaling-function group (one spin-0 map, FP64 on the caller's side)
+template <typename TI, typename TO, typename T>
+void rows_scal(Plan &P, const void *in, void *out, int in_c, int out_c, int sign, bool scaled, hipStream_t st) {
+  const Group &g = P.groups[0];
+  const int M = 2 * g.L - 1;
+  RowJobs<T> J{};
+  J.n = 1;
+  J.j[0].a = row_args<T>(g.L, (size_t)g.L, in_c, out_c, sign, scaled ? kTwoPi / M : 1.0);
+  J.j[0].in = in;
+  J.j[0].out = out;
+  J.j[0].blk0 = 0;
+  J.nblk = (int)((J.j[0].a.nlines + J.j[0].a.lpb - 1) / J.j[0].a.lpb);
+  Stage s_("rows", st);
+  launch_rows<TI, TO, T>(J, st);
+}
+
+// gamma mixing of every wavelet group between the plan's column spectra (cols)
+// and the samples of f_wav; which = 0 inverse, 1 inverse adjoint, 2 forward,
+// 3 forward adjoint (the mixing matrices of the plan)
+template <typename TI, typename TO, typename T>
+void mix_groups(Plan &P, int which, bool to_wav, const void *src_wav, void *dst_wav, const DevBuf &cols,
+                DevBuf &cols_out, hipStream_t st) {
+  MixJobs J{};
+  auto flush = [&]() {
+    Stage s_("gamma", st);
+    launch_mix<TI, TO, T>(J, st);
+    J.n = 0;
+    J.nblk = 0;
+  };
+  for (size_t gi = 0; gi < P.groups.size(); ++gi) {
+    const Group &g = P.groups[gi];
+    if (g.ngamma == 0) continue;
+    const size_t LM = (size_t)g.L * (2 * g.L - 1);
+    MixJob &j = J.j[J.n++];
+    j.W = P.d_gamma.as<cx<double>>() + P.gamma_off[gi] + (size_t)which * g.ngamma * g.ncol;
+    if (to_wav) {
+      j.in = at<T>(cols, g.spec_off);
+      j.nin = g.ncol;
+      j.out = at64(dst_wav, g.out_off);
+      j.nout = g.ngamma;
+    } else {
+      j.in = at64(src_wav, g.out_off);
+      j.nin = g.ngamma;
+      j.out = at<T>(cols_out, g.spec_off);
+      j.nout = g.ncol;
+    }
+    j.in_stride = j.out_stride = j.npts = LM;
+    j.blk0 = J.nblk;
+    J.nblk += kMixBlocks;
+    if (J.n == kMaxJobs) flush();
+  }
+  if (J.n) flush();
+}
+
 template <typename T>
 void analysis_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
   const void *x = harmonic_input<T>(P, flm, st);
-  contract_inv_all<T>(P, x, P.d_cols_ana.as<ColDev>(), P.spec.p, st);
-  for (size_t gi = 0; gi < P.groups.size(); ++gi) {
-    const Group &g = P.groups[gi];
-    const int L = g.L, M = 2 * L - 1;
-    const ColDev *cols = P.d_cols_ana.as<ColDev>() + g.col0;
-    char *spec = at<T>(P.spec, g.spec_off), *work = at<T>(P.work, g.spec_off);
-    { Stage s_("beta_inv", st); launch_theta<T>(spec, work, theta_args<T>(L, cols, g.ncol, 0), st); }
-    if (g.ngamma == 0) {
-      { Stage s_("rows", st); launch_rows<T, double, T>(work, f_scal, row_args<T>(L, (size_t)L, 1, 0, +1, 1.0), st); }
-    } else {
-      { Stage s_("rows", st); launch_rows<T, T, T>(work, spec, row_args<T>(L, (size_t)g.ncol * L, 1, 0, +1, 1.0), st); }
-      const cx<double> *W = P.d_gamma.as<cx<double>>() + P.gamma_off[gi];
-      { Stage s_("gamma", st); launch_mix<T, double, T>(spec, (size_t)L * M, g.ncol, at64(f_wav, g.out_off), (size_t)L * M, g.ngamma, W,
-                               (size_t)L * M, st); }
-    }
-  }
+  const ColDev *cols = P.d_cols_ana.as<ColDev>();
+  contract_inv_all<T>(P, x, cols, P.spec.p, st);
+  theta_groups<T>(P, cols, 0, P.spec, P.work, "beta_inv", st);
+  rows_scal<T, double, T>(P, at<T>(P.work, P.groups[0].spec_off), f_scal, 1, 0, +1, false, st);
+  rows_groups<T>(P, P.work, P.spec, 1, 0, +1, false, st);
+  mix_groups<T, double, T>(P, 0, true, nullptr, f_wav, P.spec, P.spec, st);
 }
 
 template <typename T>
 void synthesis_t(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
-  for (size_t gi = 0; gi < P.groups.size(); ++gi) {
-    const Group &g = P.groups[gi];
-    const int L = g.L, M = 2 * L - 1;
-    const ColDev *cols = P.d_cols_syn.as<ColDev>() + g.col0;
-    char *spec = at<T>(P.spec, g.spec_off), *work = at<T>(P.work, g.spec_off);
-    if (g.ngamma == 0) {
-      { Stage s_("rows", st); launch_rows<double, T, T>(f_scal, spec, row_args<T>(L, (size_t)L, 0, 1, -1, kTwoPi / M), st); }
-    } else {
-      const cx<double> *W = P.d_gamma.as<cx<double>>() + P.gamma_off[gi] + 2 * (size_t)g.ngamma * g.ncol;
-      { Stage s_("gamma", st); launch_mix<double, T, T>(at64(f_wav, g.out_off), (size_t)L * M, g.ngamma, work, (size_t)L * M, g.ncol, W,
-                               (size_t)L * M, st); }
-      { Stage s_("rows", st); launch_rows<T, T, T>(work, spec, row_args<T>(L, (size_t)g.ncol * L, 0, 1, -1, kTwoPi / M), st); }
-    }
-    { Stage s_("beta_fwd", st); launch_theta<T>(spec, work, theta_args<T>(L, cols, g.ncol, 2), st); }
-  }
-  contract_fwd_all<T>(P, P.d_cols_syn.as<ColDev>(), P.work.p, flm, st);
+  const ColDev *cols = P.d_cols_syn.as<ColDev>();
+  mix_groups<double, T, T>(P, 2, false, f_wav, nullptr, P.work, P.work, st);
+  rows_scal<double, T, T>(P, f_scal, at<T>(P.spec, P.groups[0].spec_off), 0, 1, -1, true, st);
+  rows_groups<T>(P, P.work, P.spec, 0, 1, -1, true, st);
+  theta_groups<T>(P, cols, 2, P.spec, P.work, "beta_fwd", st);
+  contract_fwd_all<T>(P, cols, P.work.p, flm, st);
 }
 
 template <typename T>
 void analysis_adjoint_t(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
-  for (size_t gi = 0; gi < P.groups.size(); ++gi) {
-    const Group &g = P.groups[gi];
-    const int L = g.L, M = 2 * L - 1;
-    const ColDev *cols = P.d_cols_ana.as<ColDev>() + g.col0;
-    char *spec = at<T>(P.spec, g.spec_off), *work = at<T>(P.work, g.spec_off);
-    if (g.ngamma == 0) {
-      { Stage s_("rows", st); launch_rows<double, T, T>(f_scal, spec, row_args<T>(L, (size_t)L, 0, 1, -1, 1.0), st); }
-    } else {
-      const cx<double> *W = P.d_gamma.as<cx<double>>() + P.gamma_off[gi] + (size_t)g.ngamma * g.ncol;
-      { Stage s_("gamma", st); launch_mix<double, T, T>(at64(f_wav, g.out_off), (size_t)L * M, g.ngamma, work, (size_t)L * M, g.ncol, W,
-                               (size_t)L * M, st); }
-      { Stage s_("rows", st); launch_rows<T, T, T>(work, spec, row_args<T>(L, (size_t)g.ncol * L, 0, 1, -1, 1.0), st); }
-    }
-    { Stage s_("beta_inv_adj", st); launch_theta<T>(spec, work, theta_args<T>(L, cols, g.ncol, 1), st); }
-  }
-  contract_fwd_all<T>(P, P.d_cols_ana.as<ColDev>(), P.work.p, flm, st);
+  const ColDev *cols = P.d_cols_ana.as<ColDev>();
+  mix_groups<double, T, T>(P, 1, false, f_wav, nullptr, P.work, P.work, st);
+  rows_scal<double, T, T>(P, f_scal, at<T>(P.spec, P.groups[0].spec_off), 0, 1, -1, false, st);
+  rows_groups<T>(P, P.work, P.spec, 0, 1, -1, false, st);
+  theta_groups<T>(P, cols, 1, P.spec, P.work, "beta_inv_adj", st);
+  contract_fwd_all<T>(P, cols, P.work.p, flm, st);
 }
 
 template <typename T>
 void synthesis_adjoint_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
   const void *x = harmonic_input<T>(P, flm, st);
-  contract_inv_all<T>(P, x, P.d_cols_syn.as<ColDev>(), P.spec.p, st);
-  for (size_t gi = 0; gi < P.groups.size(); ++gi) {
-    const Group &g = P.groups[gi];
-    const int L = g.L, M = 2 * L - 1;
-    const ColDev *cols = P.d_cols_syn.as<ColDev>() + g.col0;
-    char *spec = at<T>(P.spec, g.spec_off), *work = at<T>(P.work, g.spec_off);
-    { Stage s_("beta_fwd_adj", st); launch_theta<T>(spec, work, theta_args<T>(L, cols, g.ncol, 3), st); }
-    if (g.ngamma == 0) {
-      { Stage s_("rows", st); launch_rows<T, double, T>(work, f_scal, row_args<T>(L, (size_t)L, 1, 0, +1, kTwoPi / M), st); }
-    } else {
-      { Stage s_("rows", st); launch_rows<T, T, T>(work, spec, row_args<T>(L, (size_t)g.ncol * L, 1, 0, +1, kTwoPi / M), st); }
-      const cx<double> *W = P.d_gamma.as<cx<double>>() + P.gamma_off[gi] + 3 * (size_t)g.ngamma * g.ncol;
-      { Stage s_("gamma", st); launch_mix<T, double, T>(spec, (size_t)L * M, g.ncol, at64(f_wav, g.out_off), (size_t)L * M, g.ngamma, W,
-                               (size_t)L * M, st); }
-    }
-  }
+  const ColDev *cols = P.d_cols_syn.as<ColDev>();
+  contract_inv_all<T>(P, x, cols, P.spec.p, st);
+  theta_groups<T>(P, cols, 3, P.spec, P.work, "beta_fwd_adj", st);
+  rows_scal<T, double, T>(P, at<T>(P.work, P.groups[0].spec_off), f_scal, 1, 0, +1, true, st);
+  rows_groups<T>(P, P.work, P.spec, 1, 0, +1, true, st);
+  mix_groups<T, double, T>(P, 3, true, nullptr, f_wav, P.spec, P.spec, st);
 }
 
 // ---- single spin-s transform at L (one column)
+template <typename T>
+void theta_one(int L, const ColDev *col, int kind, const void *in, void *out, hipStream_t st) {
+  ThetaJobs<T> J{};
+  J.n = 1;
+  J.j[0].a = theta_args<T>(L, col, 1, kind);
+  J.j[0].in = in;
+  J.j[0].out = out;
+  J.j[0].blk0 = 0;
+  J.nblk = theta_blocks<T>(J.j[0].a);
+  launch_theta<T>(J, st);
+}
+
+template <typename TI, typename TO, typename T>
+void rows_one(int L, const void *in, void *out, int in_c, int out_c, int sign, double scale, hipStream_t st) {
+  RowJobs<T> J{};
+  J.n = 1;
+  J.j[0].a = row_args<T>(L, (size_t)L, in_c, out_c, sign, scale);
+  J.j[0].in = in;
+  J.j[0].out = out;
+  J.j[0].blk0 = 0;
+  J.nblk = (int)((J.j[0].a.nlines + J.j[0].a.lpb - 1) / J.j[0].a.lpb);
+  launch_rows<TI, TO, T>(J, st);
+}
+
 template <typename T>
 const void *sht_input(ShtPlan &S, const void *flm, hipStream_t st) {
   if (sizeof(T) == 8) return flm;
@@ -297,8 +391,8 @@ void sht_inverse_t(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
   const int L = S.L;
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("contract_inv", st); launch_inv_mfma<T>(sht_input<T>(S, flm, st), L, Sel<T>::delta(C), (const T *)S.ei, col, 1, std::abs(S.spin), L, S.spec.p, st); }
-  { Stage s_("beta_inv", st); launch_theta<T>(S.spec.p, S.work.p, theta_args<T>(L, col, 1, 0), st); }
-  { Stage s_("rows", st); launch_rows<T, double, T>(S.work.p, f, row_args<T>(L, (size_t)L, 1, 0, +1, 1.0), st); }
+  { Stage s_("beta_inv", st); theta_one<T>(L, col, 0, S.spec.p, S.work.p, st); }
+  { Stage s_("rows", st); rows_one<T, double, T>(L, S.work.p, f, 1, 0, +1, 1.0, st); }
 }
 
 template <typename T>
@@ -306,8 +400,8 @@ void sht_forward_t(ShtPlan &S, const void *f, void *flm, hipStream_t st) {
   Context &C = Context::get();
   const int L = S.L, M = 2 * L - 1;
   const ColDev *col = S.d_col.as<ColDev>();
-  { Stage s_("rows", st); launch_rows<double, T, T>(f, S.spec.p, row_args<T>(L, (size_t)L, 0, 1, -1, kTwoPi / M), st); }
-  { Stage s_("beta_fwd", st); launch_theta<T>(S.spec.p, S.work.p, theta_args<T>(L, col, 1, 2), st); }
+  { Stage s_("rows", st); rows_one<double, T, T>(L, f, S.spec.p, 0, 1, -1, kTwoPi / M, st); }
+  { Stage s_("beta_fwd", st); theta_one<T>(L, col, 2, S.spec.p, S.work.p, st); }
   { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, st); }
   { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, S.parts.p, flm, st); }
 }
@@ -317,8 +411,8 @@ void sht_inverse_adjoint_t(ShtPlan &S, const void *f, void *flm, hipStream_t st)
   Context &C = Context::get();
   const int L = S.L;
   const ColDev *col = S.d_col.as<ColDev>();
-  { Stage s_("rows", st); launch_rows<double, T, T>(f, S.spec.p, row_args<T>(L, (size_t)L, 0, 1, -1, 1.0), st); }
-  { Stage s_("beta_inv_adj", st); launch_theta<T>(S.spec.p, S.work.p, theta_args<T>(L, col, 1, 1), st); }
+  { Stage s_("rows", st); rows_one<double, T, T>(L, f, S.spec.p, 0, 1, -1, 1.0, st); }
+  { Stage s_("beta_inv_adj", st); theta_one<T>(L, col, 1, S.spec.p, S.work.p, st); }
   { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, st); }
   { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, S.parts.p, flm, st); }
 }
@@ -329,8 +423,8 @@ void sht_forward_adjoint_t(ShtPlan &S, const void *flm, void *f, hipStream_t st)
   const int L = S.L, M = 2 * L - 1;
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("contract_inv", st); launch_inv_mfma<T>(sht_input<T>(S, flm, st), L, Sel<T>::delta(C), (const T *)S.ei, col, 1, std::abs(S.spin), L, S.spec.p, st); }
-  { Stage s_("beta_fwd_adj", st); launch_theta<T>(S.spec.p, S.work.p, theta_args<T>(L, col, 1, 3), st); }
-  { Stage s_("rows", st); launch_rows<T, double, T>(S.work.p, f, row_args<T>(L, (size_t)L, 1, 0, +1, kTwoPi / M), st); }
+  { Stage s_("beta_fwd_adj", st); theta_one<T>(L, col, 3, S.spec.p, S.work.p, st); }
+  { Stage s_("rows", st); rows_one<T, double, T>(L, S.work.p, f, 1, 0, +1, kTwoPi / M, st); }
 }
 
 }  // namespace

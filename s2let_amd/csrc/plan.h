@@ -29,7 +29,7 @@ struct DevBuf {
 struct DftTables {  // two-level DFT of one length M (see Dft2)
   Dft2<double> d64;
   Dft2<float> d32;
-  DevBuf g64, g32, tw64, tw32;
+  DevBuf g64, g32, tw64, tw32, perm;
 };
 
 struct LTables {  // per band-limit L

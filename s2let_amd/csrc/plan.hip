@@ -139,17 +139,27 @@ const DftTables &Context::dft(int n) {
     throw ArgFailure();
   }
   auto t = std::make_unique<DftTables>();
-  const int R1 = round_up(2 * m1, 16), K1 = round_up(2 * m1, 4), R2 = round_up(2 * m2, 16), K2 = round_up(2 * m2, 4);
+  const int R1 = round_up(2 * m1, 16), K1 = round_up(2 * m1, 16), R2 = round_up(2 * m2, 16), K2 = round_up(2 * m2, 16);
+  const int CSA = K1 + 4, CSC = K2 + 4;
   const size_t s1 = (size_t)R1 * K1, s2 = (size_t)R2 * K2;
   std::vector<double> g64, part64;
   std::vector<float> g32, part32;
-  for (int which = 0; which < 4; ++which) {  // G1f, G1b, G2f, G2b
+  // [k][r] embedding -> A-fragment order [mt][s][lane]
+  auto frag = [](const auto &G, int R, int K, auto &out) {
+    const size_t base = out.size();
+    out.resize(base + (size_t)R * K);
+    for (int mt = 0; mt < R / 16; ++mt)
+      for (int s = 0; s < K / 4; ++s)
+        for (int ln = 0; ln < 64; ++ln)
+          out[base + ((size_t)mt * (K / 4) + s) * 64 + ln] = G[(size_t)(4 * s + ln / 16) * R + 16 * mt + ln % 16];
+  };
+  for (int which = 0; which < 4; ++which) {  // F1f, F1b, F2f, F2b
     const int P = which < 2 ? m1 : m2, R = which < 2 ? R1 : R2, K = which < 2 ? K1 : K2;
     const int sign = (which & 1) ? +1 : -1;
     dft_embed<double>(P, R, K, sign, false, part64);
     dft_embed<float>(P, R, K, sign, true, part32);
-    g64.insert(g64.end(), part64.begin(), part64.end());
-    g32.insert(g32.end(), part32.begin(), part32.end());
+    frag(part64, R, K, g64);
+    frag(part32, R, K, g32);
   }
   std::vector<cx<double>> w(n);
   std::vector<cx<float>> wf(n);
@@ -158,11 +168,16 @@ const DftTables &Context::dft(int n) {
     w[k] = mk<double>((double)cosl(ang), (double)sinl(ang));
     wf[k] = mk<float>((float)w[k].x, (float)w[k].y);
   }
+  // LDS offset of sample q = M2 n1 + n2 within its line: column n2, reduction row 2 n1
+  // (first product), or column 0, reduction row 2 q when M1 = 1 (second product only)
+  std::vector<int> perm(n);
+  for (int q = 0; q < n; ++q) perm[q] = (m1 > 1) ? (q % m2) * CSA + 2 * (q / m2) : 2 * q;
   hipStream_t st = stream();
   upload(t->g64, g64, st);
   upload(t->g32, g32, st);
   upload(t->tw64, w, st);
   upload(t->tw32, wf, st);
+  upload(t->perm, perm, st);
   S2H_CHECK_HIP(hipStreamSynchronize(st));
   auto fill = [&](auto &d, auto *g, auto *tw) {
     d.M = n;
@@ -172,11 +187,14 @@ const DftTables &Context::dft(int n) {
     d.K1 = K1;
     d.R2 = R2;
     d.K2 = K2;
-    d.G1f = g;
-    d.G1b = g + s1;
-    d.G2f = g + 2 * s1;
-    d.G2b = g + 2 * s1 + s2;
+    d.CSA = CSA;
+    d.CSC = CSC;
+    d.F1f = g;
+    d.F1b = g + s1;
+    d.F2f = g + 2 * s1;
+    d.F2b = g + 2 * s1 + s2;
     d.tw = tw;
+    d.perm = t->perm.as<int>();
   };
   fill(t->d64, t->g64.as<double>(), t->tw64.as<cx<double>>());
   fill(t->d32, t->g32.as<float>(), t->tw32.as<cx<float>>());

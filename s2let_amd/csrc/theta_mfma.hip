@@ -122,18 +122,24 @@ struct ThetaTile {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_theta(const cx<T> *__restrict__ in, cx<T> *__restrict__ out, ThetaArgs<T> a) {
+__global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
   constexpr int BK = ThetaTile<T>::BK, SJ = ThetaTile<T>::SJ;
   __shared__ T sB[2][2][2][BK][SJ];  // [buffer][class][re/im][k][j]
   typedef typename Mfma<T>::acc acc_t;
+  // XCD-aware block order: blocks are dispatched round-robin over 8 XCDs; give
+  // each XCD a contiguous range of (job, column, m-tile, row-tile) items so that
+  // the row tiles of one m-tile share that XCD's L2.
+  const int nb = jobs.nblk;
+  const int lin = blockIdx.x, xx = lin & 7, kq = lin >> 3, q8 = nb >> 3, r8 = nb & 7;
+  int w = xx * q8 + min(xx, r8) + kq;
+  int jb = 0;
+  while (jb + 1 < jobs.n && w >= jobs.j[jb + 1].blk0) ++jb;
+  const ThetaArgs<T> &a = jobs.j[jb].a;
+  const cx<T> *__restrict__ in = static_cast<const cx<T> *>(jobs.j[jb].in);
+  cx<T> *__restrict__ out = static_cast<cx<T> *>(jobs.j[jb].out);
+  w -= jobs.j[jb].blk0;
   const int L = a.L, M = 2 * L - 1;
   const int nti = (L + 63) / 64, ntm = (M + 63) / 64;
-  // XCD-aware block order: blocks are dispatched round-robin over 8 XCDs; give
-  // each XCD a contiguous range of (column, m-tile, row-tile) items so that the
-  // row tiles of one m-tile share that XCD's L2.
-  const int nb = nti * ntm * a.ncol;
-  const int lin = blockIdx.x, xx = lin & 7, kq = lin >> 3, q8 = nb >> 3, r8 = nb & 7;
-  const int w = xx * q8 + min(xx, r8) + kq;
   const int ti = w % nti, rest = w / nti, tm = rest % ntm, c = rest / ntm;
   const int i0 = ti * 64, mi0 = tm * 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -259,12 +265,19 @@ __global__ __launch_bounds__(256) void k_theta(const cx<T> *__restrict__ in, cx<
 }
 
 template <typename T>
-void launch_theta(const void *in, void *out, const ThetaArgs<T> &a, hipStream_t st) {
+int theta_blocks(const ThetaArgs<T> &a) {
   const int M = 2 * a.L - 1;
-  const int nb = ((a.L + 63) / 64) * ((M + 63) / 64) * a.ncol;
-  hipLaunchKernelGGL((k_theta<T>), dim3(nb), dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out, a);
+  return ((a.L + 63) / 64) * ((M + 63) / 64) * a.ncol;
 }
-template void launch_theta<double>(const void *, void *, const ThetaArgs<double> &, hipStream_t);
-template void launch_theta<float>(const void *, void *, const ThetaArgs<float> &, hipStream_t);
+template int theta_blocks<double>(const ThetaArgs<double> &);
+template int theta_blocks<float>(const ThetaArgs<float> &);
+
+template <typename T>
+void launch_theta(const ThetaJobs<T> &jobs, hipStream_t st) {
+  if (jobs.n == 0 || jobs.nblk == 0) return;
+  hipLaunchKernelGGL((k_theta<T>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
+}
+template void launch_theta<double>(const ThetaJobs<double> &, hipStream_t);
+template void launch_theta<float>(const ThetaJobs<float> &, hipStream_t);
 
 }  // namespace s2h
