@@ -130,14 +130,20 @@ __global__ void k_etab(const double *__restrict__ dtab, const ColDev *__restrict
 // of 64 |m|, y = m' group.  F_c = Fbase + cols[c].out_off, layout [m'][m + L - 1].
 // x has Lx^2 entries (Lx >= L).
 template <typename T, int MP>
-__global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ x, int Lx, const T *__restrict__ dt,
-                                              const T *__restrict__ Ei, const ColDev *__restrict__ cols, int nc,
-                                              int lmin, int L, cx<T> *__restrict__ Fbase) {
+__global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ x, const T *__restrict__ dt,
+                                              cx<T> *__restrict__ Fbase, InvJobs<T> jobs) {
   typedef typename Mfma<T>::acc acc_t;
+  int jb = 0;
+  while (jb + 1 < jobs.n && (int)blockIdx.x >= jobs.j[jb + 1].blk0) ++jb;
+  const InvJob<T> &J = jobs.j[jb];
+  const T *__restrict__ Ei = J.Ei;
+  const ColDev *__restrict__ cols = J.cols;
+  const int nc = J.nc, lmin = J.lmin, L = J.L;
+  const int b = blockIdx.x - J.blk0, bx = b % J.gx, by = b / J.gx;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r16 = lane & 15, k4 = lane >> 4;
-  const int mp0 = blockIdx.y * MP;
-  const int mbi = blockIdx.x * 4 + wave, mb = 16 * mbi;
+  const int mp0 = by * MP;
+  const int mbi = bx * 4 + wave, mb = 16 * mbi;
   if (mb >= L || mp0 >= L) return;
   const int nlb = (L + 15) / 16;
   const int m = mb + r16;
@@ -196,7 +202,6 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ x, int L
       }
     }
   }
-  (void)Lx;
   if (m >= L) return;
   const int M = 2 * L - 1;
 #pragma unroll
@@ -263,18 +268,36 @@ __global__ __launch_bounds__(256) void k_fwd3(const SuperDev *__restrict__ sgs, 
     const ColDev *gc = cols + S.col0;
     const T *__restrict__ Ef = Efs[g];
     const int mpe = min(Lg - 1, mps + mpchunk - 1);
+    // J staging: thread tid moves elements e = tid + 256 u of a chunk (|m| = mb + e % 16,
+    // sign (e / 16) % 2, column (e / 32) % 16 = tid / 32 + 8 (u % 2), m' = mp0 + e / 512)
+    constexpr int NU = MC * 2;
+    const int ml = threadIdx.x & 15, pm = (threadIdx.x >> 4) & 1, mm = mb + ml;
+    long long coff[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int cc = (threadIdx.x >> 5) + 8 * h;
+      coff[h] = (cc < nc && mm < Lg) ? gc[cc].out_off + (Lg - 1) + (pm ? -mm : mm) : -1;
+    }
+    cx<T> jr[NU];
+    auto fetch = [&](int mp0) {
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const int mpp = mp0 + (u >> 1);
+        const long long o = coff[u & 1];
+        jr[u] = (o >= 0 && mpp < Lg) ? Jbase[o + (long long)mpp * Mg] : mk<T>(0, 0);
+      }
+    };
+    fetch(mps);
     for (int mp0 = mps; mp0 <= mpe; mp0 += MC) {
       __syncthreads();
-      for (int e = threadIdx.x; e < MC * 16 * 2 * 16; e += 256) {
-        const int ml = e & 15, pm = (e >> 4) & 1, cc = (e >> 5) & 15, mc = e >> 9;
-        const int mpp = mp0 + mc, mm = mb + ml;
-        cx<T> z = mk<T>(0, 0);
-        if (cc < nc && mpp < Lg && mm < Lg)
-          z = Jbase[gc[cc].out_off + (size_t)mpp * Mg + (Lg - 1) + (pm ? -mm : mm)];
-        sJ[mc][cc][ml][2 * pm] = z.x;
-        sJ[mc][cc][ml][2 * pm + 1] = z.y;
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const int cc = (threadIdx.x >> 5) + 8 * (u & 1), mc = u >> 1;
+        sJ[mc][cc][ml][2 * pm] = jr[u].x;
+        sJ[mc][cc][ml][2 * pm + 1] = jr[u].y;
       }
       __syncthreads();
+      if (mp0 + MC <= mpe) fetch(mp0 + MC);  // in flight during this chunk's products
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         const int lbi = lb[q], l0 = 16 * lbi;
@@ -366,17 +389,14 @@ void build_etab(const double *dtab64, const ColDev *cols, int nc, const double *
 template void build_etab<double>(const double *, const ColDev *, int, const double *, int, double *, double *, hipStream_t);
 template void build_etab<float>(const double *, const ColDev *, int, const double *, int, float *, float *, hipStream_t);
 
-constexpr int kInvMP = 2;
-
 template <typename T>
-void launch_inv_mfma(const void *x, int Lx, const T *dt, const T *Ei, const ColDev *cols, int ncol, int lmin, int L,
-                     void *Fbase, hipStream_t st) {
-  dim3 grid((L + 63) / 64, (L + kInvMP - 1) / kInvMP);
-  hipLaunchKernelGGL((k_inv3<T, kInvMP>), grid, dim3(256), 0, st, (const cx<T> *)x, Lx, dt, Ei, cols, ncol, lmin, L,
-                     (cx<T> *)Fbase);
+void launch_inv_mfma(const void *x, const T *dt, void *Fbase, const InvJobs<T> &jobs, hipStream_t st) {
+  if (jobs.n == 0 || jobs.nblk == 0) return;
+  hipLaunchKernelGGL((k_inv3<T, kInvMP>), dim3(jobs.nblk), dim3(256), 0, st, (const cx<T> *)x, dt, (cx<T> *)Fbase,
+                     jobs);
 }
-template void launch_inv_mfma<double>(const void *, int, const double *, const double *, const ColDev *, int, int, int, void *, hipStream_t);
-template void launch_inv_mfma<float>(const void *, int, const float *, const float *, const ColDev *, int, int, int, void *, hipStream_t);
+template void launch_inv_mfma<double>(const void *, const double *, void *, const InvJobs<double> &, hipStream_t);
+template void launch_inv_mfma<float>(const void *, const float *, void *, const InvJobs<float> &, hipStream_t);
 
 int fwd_splits(int L) { return std::max(1, std::min(8, L / 64)); }
 

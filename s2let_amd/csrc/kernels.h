@@ -87,8 +87,19 @@ template <typename T>
 void build_etab(const double *dtab64, const ColDev *cols, int nc, const double *bvals, int Ls, T *Ei, T *Ef,
                 hipStream_t st);
 template <typename T>
-void launch_inv_mfma(const void *x, int Lx, const T *dt, const T *Ei, const ColDev *cols, int ncol, int lmin, int L,
-                     void *Fbase, hipStream_t st);
+struct InvJob {  // one super-group of the inverse contraction
+  const T *Ei;
+  const ColDev *cols;
+  int nc, lmin, L, gx, blk0;
+};
+template <typename T>
+struct InvJobs {
+  int n, nblk;
+  InvJob<T> j[16];
+};
+constexpr int kInvMP = 2;  // m' per block of the inverse contraction
+template <typename T>
+void launch_inv_mfma(const void *x, const T *dt, void *Fbase, const InvJobs<T> &jobs, hipStream_t st);
 template <typename T>
 void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *const *Efs, const T *dt, int L,
                      const void *Jbase, void *y, void *parts, hipStream_t st);

@@ -188,11 +188,27 @@ const double kTwoPi = 6.28318530717958647692;
 template <typename T>
 void contract_inv_all(Plan &P, const void *x, const ColDev *cols, void *Fbase, hipStream_t st) {
   Context &C = Context::get();
-  for (size_t k = 0; k < P.supers.size(); ++k) {
-    const SuperDev &S = P.supers[k];
+  InvJobs<T> J{};
+  auto flush = [&]() {
     Stage s_("contract_inv", st);
-    launch_inv_mfma<T>(x, P.p.L, Sel<T>::delta(C), (const T *)P.ei[k], cols + S.col0, S.ncol, P.super_lmin[k], S.L, Fbase, st);
+    launch_inv_mfma<T>(x, Sel<T>::delta(C), Fbase, J, st);
+    J.n = 0;
+    J.nblk = 0;
+  };
+  for (size_t k = P.supers.size(); k-- > 0;) {  // largest band-limit first
+    const SuperDev &S = P.supers[k];
+    InvJob<T> &j = J.j[J.n++];
+    j.Ei = (const T *)P.ei[k];
+    j.cols = cols + S.col0;
+    j.nc = S.ncol;
+    j.lmin = P.super_lmin[k];
+    j.L = S.L;
+    j.gx = (S.L + 63) / 64;
+    j.blk0 = J.nblk;
+    J.nblk += j.gx * ((S.L + kInvMP - 1) / kInvMP);
+    if (J.n == 16) flush();
   }
+  if (J.n) flush();
 }
 
 template <typename T>
@@ -221,7 +237,8 @@ void theta_groups(Plan &P, const ColDev *cols, int kind, const DevBuf &in, DevBu
     J.n = 0;
     J.nblk = 0;
   };
-  for (const Group &g : P.groups) {
+  for (size_t gi = P.groups.size(); gi-- > 0;) {  // largest band-limit first
+    const Group &g = P.groups[gi];
     ThetaJob<T> &j = J.j[J.n++];
     j.a = theta_args<T>(g.L, cols + g.col0, g.ncol, kind);
     j.in = at<T>(in, g.spec_off);
@@ -243,7 +260,7 @@ void rows_groups(Plan &P, const DevBuf &in, DevBuf &out, int in_c, int out_c, in
     J.n = 0;
     J.nblk = 0;
   };
-  for (const Group &g : P.groups) {
+  for (const Group &g : P.groups) {  // small band-limits (long single-level DFT blocks) first
     if (g.ngamma == 0) continue;
     const int M = 2 * g.L - 1;
     RowJob<T> &j = J.j[J.n++];
@@ -286,7 +303,7 @@ void mix_groups(Plan &P, int which, bool to_wav, const void *src_wav, void *dst_
     J.n = 0;
     J.nblk = 0;
   };
-  for (size_t gi = 0; gi < P.groups.size(); ++gi) {
+  for (size_t gi = P.groups.size(); gi-- > 0;) {  // largest band-limit first
     const Group &g = P.groups[gi];
     if (g.ngamma == 0) continue;
     const size_t LM = (size_t)g.L * (2 * g.L - 1);
@@ -390,7 +407,15 @@ void sht_inverse_t(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
   Context &C = Context::get();
   const int L = S.L;
   const ColDev *col = S.d_col.as<ColDev>();
-  { Stage s_("contract_inv", st); launch_inv_mfma<T>(sht_input<T>(S, flm, st), L, Sel<T>::delta(C), (const T *)S.ei, col, 1, std::abs(S.spin), L, S.spec.p, st); }
+  {
+    const void *x = sht_input<T>(S, flm, st);
+    InvJobs<T> J{};
+    J.n = 1;
+    J.j[0] = InvJob<T>{(const T *)S.ei, col, 1, std::abs(S.spin), L, (L + 63) / 64, 0};
+    J.nblk = J.j[0].gx * ((L + kInvMP - 1) / kInvMP);
+    Stage s_("contract_inv", st);
+    launch_inv_mfma<T>(x, Sel<T>::delta(C), S.spec.p, J, st);
+  }
   { Stage s_("beta_inv", st); theta_one<T>(L, col, 0, S.spec.p, S.work.p, st); }
   { Stage s_("rows", st); rows_one<T, double, T>(L, S.work.p, f, 1, 0, +1, 1.0, st); }
 }
@@ -422,7 +447,15 @@ void sht_forward_adjoint_t(ShtPlan &S, const void *flm, void *f, hipStream_t st)
   Context &C = Context::get();
   const int L = S.L, M = 2 * L - 1;
   const ColDev *col = S.d_col.as<ColDev>();
-  { Stage s_("contract_inv", st); launch_inv_mfma<T>(sht_input<T>(S, flm, st), L, Sel<T>::delta(C), (const T *)S.ei, col, 1, std::abs(S.spin), L, S.spec.p, st); }
+  {
+    const void *x = sht_input<T>(S, flm, st);
+    InvJobs<T> J{};
+    J.n = 1;
+    J.j[0] = InvJob<T>{(const T *)S.ei, col, 1, std::abs(S.spin), L, (L + 63) / 64, 0};
+    J.nblk = J.j[0].gx * ((L + kInvMP - 1) / kInvMP);
+    Stage s_("contract_inv", st);
+    launch_inv_mfma<T>(x, Sel<T>::delta(C), S.spec.p, J, st);
+  }
   { Stage s_("beta_fwd_adj", st); theta_one<T>(L, col, 3, S.spec.p, S.work.p, st); }
   { Stage s_("rows", st); rows_one<T, double, T>(L, S.work.p, f, 1, 0, +1, kTwoPi / M, st); }
 }
