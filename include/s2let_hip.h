@@ -179,6 +179,22 @@ int s2let_hip_mw_map2alm_adjoint(void *f, const void *flm, int L, int spin, int 
 int s2let_hip_batch(int op, int nmaps, void *const *out_a, void *const *out_b, const void *const *in_a,
                     const void *const *in_b, const s2let_parameters_t *params, int mode, void *stream);
 
+/* The same batch replayed as one HIP graph.  On the first call with a given
+ * (op, mode, params, pointers) set, the per-map launches are captured on
+ * parallel streams (one branch per distinct parameter set, the maps of a
+ * branch in order) and instantiated; later calls with the same set replay the
+ * graph on `stream`.  Arguments and results as s2let_hip_batch.  Graphs are
+ * cached until s2let_hip_release. */
+int s2let_hip_batch_graph(int op, int nmaps, void *const *out_a, void *const *out_b, const void *const *in_a,
+                          const void *const *in_b, const s2let_parameters_t *params, int mode, void *stream);
+
+/* Autotuner: times candidate launch shapes for the band-limits of this plan
+ * (forward-contraction m' splits; LDS share of the row-DFT blocks) on
+ * `stream` and keeps the fastest for every later call on those band-limits
+ * (results are unchanged: only the partition of the work differs).  Returns
+ * 0 on success. */
+int s2let_hip_autotune(const s2let_parameters_t *parameters, int mode, void *stream);
+
 /* Builds (or fetches) the cached plan for these parameters on the current
  * device; returns 0 on success.  Plans hold the tiling, Delta tables and DFT
  * twiddles; they are built on first use by the calls above as well. */

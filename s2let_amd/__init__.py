@@ -93,6 +93,9 @@ _SIGS["s2let_hip_timing"] = (None, [ctypes.c_int])
 _SIGS["s2let_hip_timing_report"] = (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int])
 _SIGS["s2let_hip_batch"] = (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5 +
                             [ctypes.c_int, ctypes.c_void_p])
+_SIGS["s2let_hip_batch_graph"] = _SIGS["s2let_hip_batch"]
+_SIGS["s2let_hip_autotune"] = (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p])
+OP_ANALYSIS_LM2WAV, OP_SYNTHESIS_WAV2LM, OP_ANALYSIS_ADJOINT_WAV2LM, OP_SYNTHESIS_ADJOINT_LM2WAV = 0, 1, 2, 3
 
 
 def exported_symbols():

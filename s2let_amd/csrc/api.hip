@@ -3,6 +3,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <map>
+#include <sstream>
 
 #include "plan.h"
 
@@ -85,6 +88,54 @@ struct HostIO {
   }
 };
 
+// ---- batch as a HIP graph
+
+void run_op(Plan &P, int op, void *oa, void *ob, const void *ia, const void *ib, hipStream_t st) {
+  switch (op) {
+    case S2LET_HIP_OP_ANALYSIS_LM2WAV: run_analysis(P, ia, oa, ob, st); break;
+    case S2LET_HIP_OP_SYNTHESIS_WAV2LM: run_synthesis(P, ia, ib, oa, st); break;
+    case S2LET_HIP_OP_ANALYSIS_ADJOINT_WAV2LM: run_analysis_adjoint(P, ia, ib, oa, st); break;
+    case S2LET_HIP_OP_SYNTHESIS_ADJOINT_LM2WAV: run_synthesis_adjoint(P, ia, oa, ob, st); break;
+    default: set_error(-1, "unknown op"); throw ArgFailure();
+  }
+}
+
+struct GraphCache {
+  std::map<std::string, hipGraphExec_t> exec;
+  std::vector<hipStream_t> streams;  // [0]: capture / launch stream, then one per branch
+  std::vector<hipEvent_t> events;    // [0], [1]: fences with the caller's stream, then fork / joins
+  hipStream_t stream(size_t i) {
+    while (streams.size() <= i) {
+      hipStream_t s;
+      S2H_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      streams.push_back(s);
+    }
+    return streams[i];
+  }
+  hipEvent_t event(size_t i) {
+    while (events.size() <= i) {
+      hipEvent_t e;
+      S2H_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      events.push_back(e);
+    }
+    return events[i];
+  }
+  void clear() {
+    for (auto &kv : exec) (void)hipGraphExecDestroy(kv.second);
+    exec.clear();
+  }
+};
+GraphCache &graphs() {
+  static GraphCache *g = new GraphCache();  // lives until process exit
+  return *g;
+}
+
+std::string param_key(const s2let_parameters_t *p) {
+  char b[160];
+  snprintf(b, sizeof b, "%d,%d,%.17g,%d,%d,%d,%d", p->L, p->N, p->B, p->J_min, p->spin, p->original_spin, p->upsample);
+  return b;
+}
+
 }  // namespace
 }  // namespace s2h
 
@@ -128,6 +179,7 @@ void s2let_hip_release(void) {
     Context &C = Context::get();
     std::lock_guard<std::mutex> g(C.mu);
     S2H_CHECK_HIP(hipDeviceSynchronize());
+    graphs().clear();
     C.release();
   });
 }
@@ -204,6 +256,154 @@ int s2let_hip_mw_map2alm_adjoint(void *f, const void *flm, int L, int spin, int 
     S2H_SHT_PLAN
     run_sht_forward_adjoint(S, flm, f, st);
     S2H_CHECK_HIP(hipGetLastError());
+  });
+}
+
+int s2let_hip_batch_graph(int op, int nmaps, void *const *out_a, void *const *out_b, const void *const *in_a,
+                          const void *const *in_b, const s2let_parameters_t *params, int mode, void *stream) {
+  return guarded([&] {
+    if (op < 0 || op > 3) {
+      set_error(-1, "unknown op");
+      throw ArgFailure();
+    }
+    Context &C = Context::get();
+    std::lock_guard<std::mutex> lk(C.mu);
+    // graphs are captured on and launched from a library stream fenced with the
+    // caller's stream (which may be the null stream, where capture is not allowed)
+    hipStream_t user = (hipStream_t)stream;
+    std::ostringstream key;
+    key << op << ';' << mode << ';' << nmaps;
+    std::map<std::string, std::vector<int>> branch;  // parameter set -> maps, in order
+    std::vector<std::string> order;
+    for (int k = 0; k < nmaps; ++k) {
+      require_mw(params + k, "s2let_hip_batch_graph");
+      const std::string pk = param_key(params + k);
+      key << ';' << pk << '@' << out_a[k] << ',' << (out_b ? out_b[k] : nullptr) << ',' << in_a[k] << ','
+          << (in_b ? in_b[k] : nullptr);
+      if (!branch.count(pk)) order.push_back(pk);
+      branch[pk].push_back(k);
+    }
+    GraphCache &G = graphs();
+    hipStream_t origin = G.stream(0);
+    auto it = G.exec.find(key.str());
+    if (it == G.exec.end()) {
+      std::vector<Plan *> plans(nmaps);
+      for (int k = 0; k < nmaps; ++k) plans[k] = &C.plan(params + k, mode, C.stream());  // built outside the capture
+      const bool was = timing_pause();
+      hipGraph_t graph = nullptr;
+      try {
+        S2H_CHECK_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeRelaxed));
+        hipEvent_t fork = G.event(2);
+        S2H_CHECK_HIP(hipEventRecord(fork, origin));
+        for (size_t b = 0; b < order.size(); ++b) {
+          hipStream_t sb = G.stream(b + 1);
+          S2H_CHECK_HIP(hipStreamWaitEvent(sb, fork, 0));
+          for (int k : branch[order[b]])
+            run_op(*plans[k], op, out_a[k], out_b ? out_b[k] : nullptr, in_a[k], in_b ? in_b[k] : nullptr, sb);
+          hipEvent_t join = G.event(b + 3);
+          S2H_CHECK_HIP(hipEventRecord(join, sb));
+          S2H_CHECK_HIP(hipStreamWaitEvent(origin, join, 0));
+        }
+        S2H_CHECK_HIP(hipStreamEndCapture(origin, &graph));
+      } catch (...) {
+        hipGraph_t g2 = nullptr;
+        (void)hipStreamEndCapture(origin, &g2);
+        if (g2) (void)hipGraphDestroy(g2);
+        timing_resume(was);
+        throw;
+      }
+      timing_resume(was);
+      hipGraphExec_t ex = nullptr;
+      const hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      S2H_CHECK_HIP(e);
+      it = G.exec.emplace(key.str(), ex).first;
+    }
+    S2H_CHECK_HIP(hipEventRecord(G.event(0), user));
+    S2H_CHECK_HIP(hipStreamWaitEvent(origin, G.event(0), 0));
+    S2H_CHECK_HIP(hipGraphLaunch(it->second, origin));
+    S2H_CHECK_HIP(hipEventRecord(G.event(1), origin));
+    S2H_CHECK_HIP(hipStreamWaitEvent(user, G.event(1), 0));
+  });
+}
+
+int s2let_hip_autotune(const s2let_parameters_t *p, int mode, void *stream) {
+  return guarded([&] {
+    require_mw(p, "s2let_hip_autotune");
+    Context &C = Context::get();
+    std::lock_guard<std::mutex> lk(C.mu);
+    hipStream_t st = (hipStream_t)stream;
+    Plan &P = C.plan(p, mode, C.stream());
+    const int L = p->L;
+    const size_t nlm = (size_t)L * L, nw = s2let_n_wav(p), ns = s2let_n_scal(p);
+    DevBuf flm, fw, fs;
+    flm.alloc(nlm * CB);
+    fw.alloc(nw * CB);
+    fs.alloc(ns * CB);
+    S2H_CHECK_HIP(hipMemsetAsync(flm.p, 0, nlm * CB, st));
+    S2H_CHECK_HIP(hipMemsetAsync(fw.p, 0, nw * CB, st));
+    S2H_CHECK_HIP(hipMemsetAsync(fs.p, 0, ns * CB, st));
+    hipEvent_t e0, e1;
+    S2H_CHECK_HIP(hipEventCreate(&e0));
+    S2H_CHECK_HIP(hipEventCreate(&e1));
+    const bool was = timing_pause();
+    auto time_it = [&](auto &&fn) {
+      fn();  // warm-up
+      S2H_CHECK_HIP(hipEventRecord(e0, st));
+      for (int r = 0; r < 3; ++r) fn();
+      S2H_CHECK_HIP(hipEventRecord(e1, st));
+      S2H_CHECK_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      S2H_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+      return ms;
+    };
+    try {
+      // forward contraction: m' splits of the top band-limit (the launch covers every group)
+      Tune t = C.tune(L);
+      float best = 1e30f;
+      int bs = 0;
+      for (int s : {1, 2, 4, 8}) {
+        if (s > kMaxSplits || (s > 1 && L < 32 * s)) continue;
+        t.splits = s;
+        C.set_tune(L, t);
+        const float ms = time_it([&] { run_synthesis(P, fw.p, fs.p, flm.p, st); });
+        if (ms < best) {
+          best = ms;
+          bs = s;
+        }
+      }
+      t.splits = bs;
+      C.set_tune(L, t);
+      // row stage: LDS share per block, per band-limit of the plan's groups
+      std::vector<int> Ls;
+      for (const Group &g : P.groups)
+        if (std::find(Ls.begin(), Ls.end(), g.L) == Ls.end()) Ls.push_back(g.L);
+      for (int Lg : Ls) {
+        Tune tg = C.tune(Lg);
+        float bt = 1e30f;
+        int bd = 4;
+        for (int dv : {2, 4, 8}) {
+          tg.rows_div = dv;
+          C.set_tune(Lg, tg);
+          const float ms = time_it([&] { run_analysis(P, flm.p, fw.p, fs.p, st); });
+          if (ms < bt) {
+            bt = ms;
+            bd = dv;
+          }
+        }
+        tg.rows_div = bd;
+        C.set_tune(Lg, tg);
+      }
+    } catch (...) {
+      timing_resume(was);
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    timing_resume(was);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
   });
 }
 

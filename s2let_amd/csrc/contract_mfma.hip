@@ -398,14 +398,13 @@ void launch_inv_mfma(const void *x, const T *dt, void *Fbase, const InvJobs<T> &
 template void launch_inv_mfma<double>(const void *, const double *, void *, const InvJobs<double> &, hipStream_t);
 template void launch_inv_mfma<float>(const void *, const float *, void *, const InvJobs<float> &, hipStream_t);
 
-int fwd_splits(int L) { return std::max(1, std::min(8, L / 64)); }
+int fwd_splits_default(int L) { return std::max(1, std::min(kMaxSplits, L / 64)); }
 
 template <typename T>
 void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *const *Efs, const T *dt, int L,
-                     const void *Jbase, void *y, void *parts, hipStream_t st) {
+                     const void *Jbase, void *y, void *parts, int S, hipStream_t st) {
   const int nb = (L + 15) / 16;
   const int npairs = (nb + 1) / 2;
-  const int S = fwd_splits(L);
   constexpr int MC = FwdTile<T>::MC;
   int chunk = (L + S - 1) / S;
   chunk = (chunk + MC - 1) / MC * MC;
@@ -413,12 +412,11 @@ void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *
   cx<double> *out = (S == 1) ? (cx<double> *)y : (cx<double> *)parts;
   hipLaunchKernelGGL((k_fwd3<T>), grid, dim3(256), 0, st, sgs, nsg, cols, Efs, dt, L, (const cx<T> *)Jbase, out, chunk);
 }
-template void launch_fwd_mfma<double>(const SuperDev *, int, const ColDev *, const double *const *, const double *, int, const void *, void *, void *, hipStream_t);
-template void launch_fwd_mfma<float>(const SuperDev *, int, const ColDev *, const float *const *, const float *, int, const void *, void *, void *, hipStream_t);
+template void launch_fwd_mfma<double>(const SuperDev *, int, const ColDev *, const double *const *, const double *, int, const void *, void *, void *, int, hipStream_t);
+template void launch_fwd_mfma<float>(const SuperDev *, int, const ColDev *, const float *const *, const float *, int, const void *, void *, void *, int, hipStream_t);
 
-void launch_sum_parts(int L, const void *parts, void *y, hipStream_t st) {
-  const int S = fwd_splits(L);
-  if (S == 1) return;
+void launch_sum_parts(int L, int S, const void *parts, void *y, hipStream_t st) {
+  if (S <= 1) return;
   const size_t n = (size_t)L * L;
   hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0, st,
                      (const cx<double> *)parts, S, n, (cx<double> *)y);

@@ -102,9 +102,11 @@ template <typename T>
 void launch_inv_mfma(const void *x, const T *dt, void *Fbase, const InvJobs<T> &jobs, hipStream_t st);
 template <typename T>
 void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *const *Efs, const T *dt, int L,
-                     const void *Jbase, void *y, void *parts, hipStream_t st);
-int fwd_splits(int L);  // m' splits of the forward contraction (partials: fwd_splits(L) * L^2 complex128)
-void launch_sum_parts(int L, const void *parts, void *y, hipStream_t st);  // y = sum of the split partials
+                     const void *Jbase, void *y, void *parts, int splits, hipStream_t st);
+// m' splits of the forward contraction (partials: splits * L^2 complex128, at most kMaxSplits)
+constexpr int kMaxSplits = 8;
+int fwd_splits_default(int L);
+void launch_sum_parts(int L, int splits, const void *parts, void *y, hipStream_t st);  // y = sum of the partials
 int mfma_selftest();
 
 // Grouped launches: one launch runs the jobs of several column groups (each job

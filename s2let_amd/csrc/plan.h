@@ -89,6 +89,12 @@ struct ShtPlan {
   const void *ei = nullptr;
 };
 
+// Launch-shape choices per band-limit (defaults, or the autotuner's picks).
+struct Tune {
+  int splits = 0;    // m' splits of the forward contraction (0: default)
+  int rows_div = 0;  // row stage: LDS per block = device LDS / rows_div (0: default 4)
+};
+
 class Context {
  public:
   static Context &get();
@@ -101,6 +107,15 @@ class Context {
   Plan &plan(const s2let_parameters_t *p, int mode, hipStream_t st);
   ShtPlan &sht(int L, int spin, int mode, hipStream_t st);
   size_t lds_bytes() const { return lds_bytes_; }
+  Tune tune(int L) const {
+    auto it = tune_.find(L);
+    return it == tune_.end() ? Tune{} : it->second;
+  }
+  void set_tune(int L, Tune t) { tune_[L] = t; }
+  int splits(int L) const {
+    const int s = tune(L).splits;
+    return s > 0 ? s : fwd_splits_default(L);
+  }
   hipStream_t stream();
   void release();
   std::mutex mu;
@@ -115,11 +130,14 @@ class Context {
   std::map<int, std::unique_ptr<LTables>> lt_;
   std::map<std::string, std::unique_ptr<Plan>> plans_;
   std::map<std::tuple<int, int, int>, std::unique_ptr<ShtPlan>> shts_;
+  std::map<int, Tune> tune_;
   hipStream_t stream_ = nullptr;
 };
 
 // pipelines (pipeline.hip)
 void timing_enable(bool on);
+bool timing_pause();  // stops recording without clearing; returns the previous state
+void timing_resume(bool was_on);
 std::string timing_report();
 void run_analysis(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st);
 void run_synthesis(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st);

@@ -420,7 +420,7 @@ Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
   pl.spec.alloc(pl.spec_elems * esz);
   pl.work.alloc(pl.spec_elems * esz);
   if (f32) pl.flm32.alloc((size_t)L * L * sizeof(cx<float>));
-  pl.parts.alloc((size_t)fwd_splits(L) * L * L * sizeof(cx<double>));
+  pl.parts.alloc((size_t)kMaxSplits * L * L * sizeof(cx<double>));
   S2H_CHECK_HIP(hipStreamSynchronize(st));
   auto &ref = *P;
   plans_[key] = std::move(P);
@@ -469,7 +469,7 @@ ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st) {
   s.spec.alloc(ne * esz);
   s.work.alloc(ne * esz);
   if (f32) s.flm32.alloc((size_t)L * L * sizeof(cx<float>));
-  s.parts.alloc((size_t)fwd_splits(L) * L * L * sizeof(cx<double>));
+  s.parts.alloc((size_t)kMaxSplits * L * L * sizeof(cx<double>));
   S2H_CHECK_HIP(hipStreamSynchronize(st));
   auto &ref = *S;
   shts_[key] = std::move(S);

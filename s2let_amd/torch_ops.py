@@ -15,7 +15,7 @@ import torch
 from . import FP64, MIXED, Parameters, check, lib, n_scal, n_wav, parameters  # noqa: F401
 
 __all__ = ["analysis", "synthesis", "analysis_adjoint", "synthesis_adjoint", "mw_alm2map", "mw_map2alm",
-           "WaveletAnalysis", "wavelet_analysis", "FP64", "MIXED", "parameters"]
+           "batch", "autotune", "WaveletAnalysis", "wavelet_analysis", "FP64", "MIXED", "parameters"]
 
 
 def _stream():
@@ -61,6 +61,29 @@ def synthesis_adjoint(flm, p: Parameters, mode=FP64, out=None):
     check(lib().s2let_hip_synthesis_adjoint_lm2wav(_dev(fw), _dev(fs), _dev(flm), ctypes.byref(p), int(mode),
                                                    _stream()))
     return fw, fs
+
+
+def batch(op, outs_a, outs_b, ins_a, ins_b, params, mode=FP64, graph=True):
+    """A batch of maps of uneven shape (s2let_hip_batch_graph / s2let_hip_batch):
+    op is one of the OP_* codes; map k uses params[k] and the k-th tensors of
+    each list (outs_b / ins_b may be None for ops with one output / input).
+    graph=True replays the batch as one cached HIP graph (captured on the first
+    call with the same shapes and tensors)."""
+    n = len(params)
+    P = (Parameters * n)(*params)
+
+    def arr(ts):
+        if ts is None:
+            return None
+        return (ctypes.c_void_p * n)(*[_dev(t).value for t in ts])
+
+    fn = lib().s2let_hip_batch_graph if graph else lib().s2let_hip_batch
+    check(fn(int(op), n, arr(outs_a), arr(outs_b), arr(ins_a), arr(ins_b), P, int(mode), _stream()))
+
+
+def autotune(p: Parameters, mode=FP64):
+    """s2let_hip_autotune on the current stream: picks launch shapes for p's band-limits."""
+    check(lib().s2let_hip_autotune(ctypes.byref(p), int(mode), _stream()))
 
 
 def mw_alm2map(flm, L, spin=0, mode=FP64):
