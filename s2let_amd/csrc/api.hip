@@ -359,32 +359,33 @@ int s2let_hip_autotune(const s2let_parameters_t *p, int mode, void *stream) {
     };
     try {
       // forward contraction: m' splits of the top band-limit (the launch covers every group)
-      Tune t = C.tune(L);
+      const int esz = (mode == S2LET_HIP_MIXED) ? 4 : 8;
+      Tune t = C.tune(L, esz);
       float best = 1e30f;
       int bs = 0;
       for (int s : {1, 2, 4, 8}) {
         if (s > kMaxSplits || (s > 1 && L < 32 * s)) continue;
         t.splits = s;
-        C.set_tune(L, t);
-        const float ms = time_it([&] { run_synthesis(P, fw.p, fs.p, flm.p, st); });
+        C.set_tune(L, esz, t);
+        const float ms = time_it([&] { run_synthesis(P, fw.p, fs.p, flm.p, st); });  // the contraction's launch
         if (ms < best) {
           best = ms;
           bs = s;
         }
       }
       t.splits = bs;
-      C.set_tune(L, t);
+      C.set_tune(L, esz, t);
       // row stage: LDS share per block, per band-limit of the plan's groups
       std::vector<int> Ls;
       for (const Group &g : P.groups)
         if (std::find(Ls.begin(), Ls.end(), g.L) == Ls.end()) Ls.push_back(g.L);
       for (int Lg : Ls) {
-        Tune tg = C.tune(Lg);
+        Tune tg = C.tune(Lg, esz);
         float bt = 1e30f;
         int bd = 4;
         for (int dv : {2, 4, 8}) {
           tg.rows_div = dv;
-          C.set_tune(Lg, tg);
+          C.set_tune(Lg, esz, tg);
           const float ms = time_it([&] { run_analysis(P, flm.p, fw.p, fs.p, st); });
           if (ms < bt) {
             bt = ms;
@@ -392,7 +393,7 @@ int s2let_hip_autotune(const s2let_parameters_t *p, int mode, void *stream) {
           }
         }
         tg.rows_div = bd;
-        C.set_tune(Lg, tg);
+        C.set_tune(Lg, esz, tg);
       }
     } catch (...) {
       timing_resume(was);

@@ -163,7 +163,7 @@ RowArgs<T> row_args(int L, size_t nlines, int in_c, int out_c, int sign, double 
   };
   // a quarter of the LDS per block (several blocks per CU overlap their staging
   // with the matrix products of the others), more only if one line needs it
-  const int div = C.tune(L).rows_div > 0 ? C.tune(L).rows_div : 4;
+  const int div = C.tune(L, (int)sizeof(T)).rows_div > 0 ? C.tune(L, (int)sizeof(T)).rows_div : 4;
   size_t cap = std::min(lds_budget(C), C.lds_bytes() / div);
   if (!fits(1, cap)) cap = lds_budget(C);
   if (!fits(1, cap)) cap = C.lds_bytes();
@@ -225,11 +225,11 @@ void contract_fwd_all(Plan &P, const ColDev *cols, const void *Jbase, void *flm,
   {
     Stage s_("contract_fwd", st);
     launch_fwd_mfma<T>(P.d_supers.as<SuperDev>(), (int)P.supers.size(), cols, P.d_efs.as<const T *>(),
-                       Sel<T>::delta(C), P.p.L, Jbase, flm, P.parts.p, C.splits(P.p.L), st);
+                       Sel<T>::delta(C), P.p.L, Jbase, flm, P.parts.p, C.splits(P.p.L, (int)sizeof(T)), st);
   }
-  if (C.splits(P.p.L) > 1) {
+  if (C.splits(P.p.L, (int)sizeof(T)) > 1) {
     Stage s_("contract_fwd_sum", st);
-    launch_sum_parts(P.p.L, C.splits(P.p.L), P.parts.p, flm, st);
+    launch_sum_parts(P.p.L, C.splits(P.p.L, (int)sizeof(T)), P.parts.p, flm, st);
   }
 }
 
@@ -435,8 +435,8 @@ void sht_forward_t(ShtPlan &S, const void *f, void *flm, hipStream_t st) {
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("rows", st); rows_one<double, T, T>(L, f, S.spec.p, 0, 1, -1, kTwoPi / M, st); }
   { Stage s_("beta_fwd", st); theta_one<T>(L, col, 2, S.spec.p, S.work.p, st); }
-  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, C.splits(L), st); }
-  if (C.splits(L) > 1) { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, C.splits(L), S.parts.p, flm, st); }
+  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, C.splits(L, (int)sizeof(T)), st); }
+  if (C.splits(L, (int)sizeof(T)) > 1) { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, C.splits(L, (int)sizeof(T)), S.parts.p, flm, st); }
 }
 
 template <typename T>
@@ -446,8 +446,8 @@ void sht_inverse_adjoint_t(ShtPlan &S, const void *f, void *flm, hipStream_t st)
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("rows", st); rows_one<double, T, T>(L, f, S.spec.p, 0, 1, -1, 1.0, st); }
   { Stage s_("beta_inv_adj", st); theta_one<T>(L, col, 1, S.spec.p, S.work.p, st); }
-  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, C.splits(L), st); }
-  if (C.splits(L) > 1) { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, C.splits(L), S.parts.p, flm, st); }
+  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, C.splits(L, (int)sizeof(T)), st); }
+  if (C.splits(L, (int)sizeof(T)) > 1) { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, C.splits(L, (int)sizeof(T)), S.parts.p, flm, st); }
 }
 
 template <typename T>

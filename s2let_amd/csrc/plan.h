@@ -107,13 +107,14 @@ class Context {
   Plan &plan(const s2let_parameters_t *p, int mode, hipStream_t st);
   ShtPlan &sht(int L, int spin, int mode, hipStream_t st);
   size_t lds_bytes() const { return lds_bytes_; }
-  Tune tune(int L) const {
-    auto it = tune_.find(L);
+  // launch shapes per (band-limit, precision of the intermediate data: 4 or 8 bytes)
+  Tune tune(int L, int esz) const {
+    auto it = tune_.find({L, esz});
     return it == tune_.end() ? Tune{} : it->second;
   }
-  void set_tune(int L, Tune t) { tune_[L] = t; }
-  int splits(int L) const {
-    const int s = tune(L).splits;
+  void set_tune(int L, int esz, Tune t) { tune_[{L, esz}] = t; }
+  int splits(int L, int esz) const {
+    const int s = tune(L, esz).splits;
     return s > 0 ? s : fwd_splits_default(L);
   }
   hipStream_t stream();
@@ -130,7 +131,7 @@ class Context {
   std::map<int, std::unique_ptr<LTables>> lt_;
   std::map<std::string, std::unique_ptr<Plan>> plans_;
   std::map<std::tuple<int, int, int>, std::unique_ptr<ShtPlan>> shts_;
-  std::map<int, Tune> tune_;
+  std::map<std::pair<int, int>, Tune> tune_;
   hipStream_t stream_ = nullptr;
 };
 

@@ -115,13 +115,20 @@ void build_theta_tables(int L, const double *cs, const double *sn, double *th, h
 // B (the samples) is staged in LDS split by class and re/im; A fragments are
 // read from the (L2-resident) table.  FP32: products accumulate in FP32 per
 // K chunk of 32 and the chunk sums are added in FP64.
+// FP32 mode: with F64 set the products accumulate in FP32 over kThetaRun K
+// chunks (kThetaRun * BK terms) and these partial sums are added in FP64; without
+// it the whole reduction (L terms) accumulates in FP32 (used for L <= kThetaF32MaxL,
+// where it keeps the transform within 5e-7 of FP64 and frees 64 VGPRs)
+constexpr int kThetaRun = 4;
+constexpr int kThetaF32MaxL = 512;
+
 template <typename T>
 struct ThetaTile {
   static constexpr int BK = sizeof(T) == 8 ? 16 : 32;
   static constexpr int SJ = 32 + (sizeof(T) == 8 ? 8 : 16);  // padded LDS row
 };
 
-template <typename T>
+template <typename T, bool F64>
 __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
   constexpr int BK = ThetaTile<T>::BK, SJ = ThetaTile<T>::SJ;
   __shared__ T sB[2][2][2][BK][SJ];  // [buffer][class][re/im][k][j]
@@ -158,8 +165,9 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
   for (int r = 0; r < 2; ++r)
 #pragma unroll
     for (int qq = 0; qq < 4; ++qq) acc[r][qq] = Mfma<T>::zero();
+  constexpr bool f64sums = sizeof(T) == 4 && F64;
   double hs[2][4][4];
-  if constexpr (sizeof(T) == 4) {
+  if constexpr (f64sums) {
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -222,7 +230,8 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq) acc[r][qq] = Mfma<T>::mma(acur[s][r], bv[qq], acc[r][qq]);
     }
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (f64sums) {
+      if ((ch + 1) % kThetaRun == 0 || ch + 1 == nch)
 #pragma unroll
       for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -248,7 +257,7 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
         const int i = i0w + 16 * r + Mfma<T>::row(k4, v);
         if (i >= L || mi >= M) continue;
         double re, im;
-        if constexpr (sizeof(T) == 4) {
+        if constexpr (f64sums) {
           re = hs[r][qq][v];
           im = hs[r][qq + 2][v];
         } else {
@@ -275,7 +284,12 @@ template int theta_blocks<float>(const ThetaArgs<float> &);
 template <typename T>
 void launch_theta(const ThetaJobs<T> &jobs, hipStream_t st) {
   if (jobs.n == 0 || jobs.nblk == 0) return;
-  hipLaunchKernelGGL((k_theta<T>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
+  int Lmax = 0;
+  for (int k = 0; k < jobs.n; ++k) Lmax = std::max(Lmax, jobs.j[k].a.L);
+  if (sizeof(T) == 4 && Lmax <= kThetaF32MaxL)
+    hipLaunchKernelGGL((k_theta<T, false>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
+  else
+    hipLaunchKernelGGL((k_theta<T, true>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
 }
 template void launch_theta<double>(const ThetaJobs<double> &, hipStream_t);
 template void launch_theta<float>(const ThetaJobs<float> &, hipStream_t);
