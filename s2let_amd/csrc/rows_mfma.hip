@@ -18,7 +18,10 @@
 
 namespace s2h {
 
-// one (row tile, up to two column tiles) item of a product: acc[u] += F_mt * S[tile u]
+// one (row tile, up to two column tiles) item of a product: acc[u] += F_mt * S[tile u].
+// In FP32, up to kRowsKR K-steps the item's A fragments are all loaded first (one
+// latency per item); FP64 and longer reductions stream them four at a time.
+constexpr int kRowsKR = 20;
 template <typename T>
 __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const T *S, int cs, int t0, bool two,
                                 int lane, typename Mfma<T>::acc &a0, typename Mfma<T>::acc &a1) {
@@ -28,6 +31,18 @@ __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const 
   const T *b1 = b0 + 16 * cs;
   a0 = Mfma<T>::zero();
   a1 = Mfma<T>::zero();
+  if (sizeof(T) == 4 && nks <= kRowsKR) {
+    T av[kRowsKR];
+#pragma unroll
+    for (int s = 0; s < kRowsKR; ++s) av[s] = (s < nks) ? ga[s * 64] : (T)0;
+#pragma unroll
+    for (int s = 0; s < kRowsKR; ++s) {
+      if (s >= nks) break;
+      a0 = Mfma<T>::mma(av[s], b0[4 * s], a0);
+      if (two) a1 = Mfma<T>::mma(av[s], b1[4 * s], a1);
+    }
+    return;
+  }
   for (int s0 = 0; s0 < nks; s0 += 4) {  // nks is a multiple of 4
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -91,7 +106,7 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
     for (int it = 0; it < IMAX; ++it) {
       const int item = wave + 4 * it;
       if (item >= nitems) break;
-      const int mt = item / ntp, tp = item - mt * ntp;
+      const int tp = item / nmt, mt = item - tp * nmt;
       dft_item<T>(F, nks, mt, S, d.CSA, 2 * tp, 2 * tp + 1 < nt, lane, r0[it], r1[it]);
     }
     __syncthreads();  // every column of the first product has been read
@@ -100,7 +115,7 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
     for (int it = 0; it < IMAX; ++it) {
       const int item = wave + 4 * it;
       if (item >= nitems) break;
-      const int mt = item / ntp, tp = item - mt * ntp;
+      const int tp = item / nmt, mt = item - tp * nmt;
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int col = 16 * (2 * tp + u) + r16;
@@ -132,7 +147,7 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
     const int ncol = nl * M1, nt = (ncol + 15) / 16, ntp = (nt + 1) / 2, nitems = nmt * ntp;
     const T scale = a.scale;
     for (int item = wave; item < nitems; item += 4) {
-      const int mt = item / ntp, tp = item - mt * ntp;
+      const int tp = item / nmt, mt = item - tp * nmt;
       acc_t r0, r1;
       dft_item<T>(F, nks, mt, S, d.CSC, 2 * tp, 2 * tp + 1 < nt, lane, r0, r1);
 #pragma unroll
