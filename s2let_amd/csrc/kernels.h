@@ -123,7 +123,7 @@ struct ThetaJob {
 template <typename T>
 struct ThetaJobs {
   int n, nblk;
-  ThetaJob<T> j[kMaxJobs];
+  ThetaJob<T> j[kMaxJobs];  // blocks of job k: theta_blocks(a, theta_rows(max L of the jobs))
 };
 template <typename T>
 struct RowJob {
@@ -153,7 +153,9 @@ struct MixJobs {
 // kind 0 inverse, 1 inverse adjoint, 2 forward, 3 forward adjoint.
 void build_theta_tables(int L, const double *cs, const double *sn, double *th, hipStream_t st);
 template <typename T>
-int theta_blocks(const ThetaArgs<T> &a);
+int theta_rows(int Lmax);  // output rows per block of the launch, for the largest L among its jobs
+template <typename T>
+int theta_blocks(const ThetaArgs<T> &a, int rows);
 template <typename T>
 void launch_theta(const ThetaJobs<T> &jobs, hipStream_t st);
 template <typename TI, typename TO, typename T>

@@ -238,24 +238,28 @@ void contract_fwd_all(Plan &P, const ColDev *cols, const void *Jbase, void *flm,
 template <typename T>
 void theta_groups(Plan &P, const ColDev *cols, int kind, const DevBuf &in, DevBuf &out, const char *name,
                   hipStream_t st) {
-  ThetaJobs<T> J{};
-  auto flush = [&]() {
+  // job lists of at most kMaxJobs groups, largest band-limit first; the rows per
+  // block of a launch follow its largest band-limit
+  std::vector<size_t> order;
+  for (size_t gi = P.groups.size(); gi-- > 0;) order.push_back(gi);
+  for (size_t k0 = 0; k0 < order.size(); k0 += kMaxJobs) {
+    const size_t k1 = std::min(order.size(), k0 + kMaxJobs);
+    int Lmax = 0;
+    for (size_t k = k0; k < k1; ++k) Lmax = std::max(Lmax, P.groups[order[k]].L);
+    const int rows = theta_rows<T>(Lmax);
+    ThetaJobs<T> J{};
+    for (size_t k = k0; k < k1; ++k) {
+      const Group &g = P.groups[order[k]];
+      ThetaJob<T> &j = J.j[J.n++];
+      j.a = theta_args<T>(g.L, cols + g.col0, g.ncol, kind);
+      j.in = at<T>(in, g.spec_off);
+      j.out = at<T>(out, g.spec_off);
+      j.blk0 = J.nblk;
+      J.nblk += theta_blocks<T>(j.a, rows);
+    }
     Stage s_(name, st);
     launch_theta<T>(J, st);
-    J.n = 0;
-    J.nblk = 0;
-  };
-  for (size_t gi = P.groups.size(); gi-- > 0;) {  // largest band-limit first
-    const Group &g = P.groups[gi];
-    ThetaJob<T> &j = J.j[J.n++];
-    j.a = theta_args<T>(g.L, cols + g.col0, g.ncol, kind);
-    j.in = at<T>(in, g.spec_off);
-    j.out = at<T>(out, g.spec_off);
-    j.blk0 = J.nblk;
-    J.nblk += theta_blocks<T>(j.a);
-    if (J.n == kMaxJobs) flush();
   }
-  if (J.n) flush();
 }
 
 // row DFTs of the wavelet groups (all lines of a group are contiguous, M each)
@@ -387,7 +391,7 @@ void theta_one(int L, const ColDev *col, int kind, const void *in, void *out, hi
   J.j[0].in = in;
   J.j[0].out = out;
   J.j[0].blk0 = 0;
-  J.nblk = theta_blocks<T>(J.j[0].a);
+  J.nblk = theta_blocks<T>(J.j[0].a, theta_rows<T>(L));
   launch_theta<T>(J, st);
 }
 

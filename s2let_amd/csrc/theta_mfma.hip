@@ -128,7 +128,8 @@ struct ThetaTile {
   static constexpr int SJ = 32 + (sizeof(T) == 8 ? 8 : 16);  // padded LDS row
 };
 
-template <typename T, bool F64>
+// RT row tiles of 16 per wave: a block covers 32 RT rows x 64 consecutive m.
+template <typename T, bool F64, int RT>
 __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
   constexpr int BK = ThetaTile<T>::BK, SJ = ThetaTile<T>::SJ;
   __shared__ T sB[2][2][2][BK][SJ];  // [buffer][class][re/im][k][j]
@@ -146,30 +147,30 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
   cx<T> *__restrict__ out = static_cast<cx<T> *>(jobs.j[jb].out);
   w -= jobs.j[jb].blk0;
   const int L = a.L, M = 2 * L - 1;
-  const int nti = (L + 63) / 64, ntm = (M + 63) / 64;
+  const int nti = (L + 32 * RT - 1) / (32 * RT), ntm = (M + 63) / 64;
   const int ti = w % nti, rest = w / nti, tm = rest % ntm, c = rest / ntm;
-  const int i0 = ti * 64, mi0 = tm * 64;
+  const int i0 = ti * 32 * RT, mi0 = tm * 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cls_lo = (mi0 - (L - 1)) & 1;  // parity of m at mi0 (class of ee = 0 is cls_lo ^ (n & 1))
   const ColDev col = a.cols[c];
   const int par0 = (cls_lo + col.n) & 1;
   const int cls = wave >> 1;
-  const int i0w = i0 + 32 * (wave & 1);
+  const int i0w = i0 + 16 * RT * (wave & 1);
   const int r16 = lane & 15, k4 = lane >> 4;
   const cx<T> *__restrict__ inc = in + (size_t)c * L * M;
   const T *__restrict__ AT = a.AT[cls];
   const int in_row0 = a.in_row[0], in_row1 = a.in_row[1], in_e0 = a.in_e[0], in_e1 = a.in_e[1];
 
-  acc_t acc[2][4];
+  acc_t acc[RT][4];
 #pragma unroll
-  for (int r = 0; r < 2; ++r)
+  for (int r = 0; r < RT; ++r)
 #pragma unroll
     for (int qq = 0; qq < 4; ++qq) acc[r][qq] = Mfma<T>::zero();
   constexpr bool f64sums = sizeof(T) == 4 && F64;
-  double hs[2][4][4];
+  double hs[RT][4][4];
   if constexpr (f64sums) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int r = 0; r < RT; ++r)
 #pragma unroll
       for (int qq = 0; qq < 4; ++qq)
 #pragma unroll
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
   constexpr int UPT = BK * 64 / 256;  // complex samples staged per thread per chunk
   const int nch = (L + BK - 1) / BK;
   cx<T> breg[UPT];
-  T anx[BK / 4][2], acur[BK / 4][2];
+  T anx[BK / 4][RT], acur[BK / 4][RT];
   auto load_chunk = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < UPT; ++u) {
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
     for (int s = 0; s < BK / 4; ++s) {
       const int k = k0 + 4 * s + k4;
 #pragma unroll
-      for (int r = 0; r < 2; ++r) {
+      for (int r = 0; r < RT; ++r) {
         const int i = i0w + 16 * r + r16;
         anx[s][r] = (k < L && i < L) ? AT[(size_t)k * L + i] : (T)0;
       }
@@ -214,8 +215,8 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
     }
 #pragma unroll
     for (int s = 0; s < BK / 4; ++s) {
-      acur[s][0] = anx[s][0];
-      acur[s][1] = anx[s][1];
+#pragma unroll
+      for (int r = 0; r < RT; ++r) acur[s][r] = anx[s][r];
     }
     __syncthreads();
     if (ch + 1 < nch) load_chunk(k0 + BK);
@@ -226,14 +227,14 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
 #pragma unroll
       for (int qq = 0; qq < 4; ++qq) bv[qq] = sB[buf][cls][qq >> 1][kk][16 * (qq & 1) + r16];
 #pragma unroll
-      for (int r = 0; r < 2; ++r)
+      for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq) acc[r][qq] = Mfma<T>::mma(acur[s][r], bv[qq], acc[r][qq]);
     }
     if constexpr (f64sums) {
       if ((ch + 1) % kThetaRun == 0 || ch + 1 == nch)
 #pragma unroll
-      for (int r = 0; r < 2; ++r)
+      for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq) {
 #pragma unroll
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
   const int zc = a.z_e[cls], orow = a.out_row[cls], oe = a.out_e[cls];
   cx<T> *__restrict__ outc = out + (size_t)c * L * M;
 #pragma unroll
-  for (int r = 0; r < 2; ++r)
+  for (int r = 0; r < RT; ++r)
 #pragma unroll
     for (int qq = 0; qq < 2; ++qq) {
       const int j = 16 * qq + r16, ee = 2 * j + (cls ^ par0), mi = mi0 + ee;
@@ -273,23 +274,31 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
     }
 }
 
+// rows per block: 128 (FP32 accumulation, 4 row tiles per wave) or 64
 template <typename T>
-int theta_blocks(const ThetaArgs<T> &a) {
-  const int M = 2 * a.L - 1;
-  return ((a.L + 63) / 64) * ((M + 63) / 64) * a.ncol;
+int theta_rows(int Lmax) {
+  return (sizeof(T) == 4 && Lmax <= kThetaF32MaxL) ? 128 : 64;
 }
-template int theta_blocks<double>(const ThetaArgs<double> &);
-template int theta_blocks<float>(const ThetaArgs<float> &);
+
+template <typename T>
+int theta_blocks(const ThetaArgs<T> &a, int rows) {
+  const int M = 2 * a.L - 1;
+  return ((a.L + rows - 1) / rows) * ((M + 63) / 64) * a.ncol;
+}
+template int theta_blocks<double>(const ThetaArgs<double> &, int);
+template int theta_rows<double>(int);
+template int theta_rows<float>(int);
+template int theta_blocks<float>(const ThetaArgs<float> &, int);
 
 template <typename T>
 void launch_theta(const ThetaJobs<T> &jobs, hipStream_t st) {
   if (jobs.n == 0 || jobs.nblk == 0) return;
   int Lmax = 0;
   for (int k = 0; k < jobs.n; ++k) Lmax = std::max(Lmax, jobs.j[k].a.L);
-  if (sizeof(T) == 4 && Lmax <= kThetaF32MaxL)
-    hipLaunchKernelGGL((k_theta<T, false>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
+  if (theta_rows<T>(Lmax) == 128)
+    hipLaunchKernelGGL((k_theta<T, false, 4>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
   else
-    hipLaunchKernelGGL((k_theta<T, true>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
+    hipLaunchKernelGGL((k_theta<T, true, 2>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
 }
 template void launch_theta<double>(const ThetaJobs<double> &, hipStream_t);
 template void launch_theta<float>(const ThetaJobs<float> &, hipStream_t);

@@ -83,3 +83,54 @@ def test_autograd_backward_is_adjoint(ops, oracle):
             # cancellation ~ eps * loss / h
             tol = 1e-6 * abs(an) + 20 * np.finfo(float).eps * l0 / h
             assert abs(fd - an) < tol, (idx, d, fd, an, tol)
+
+
+def real_field_alm(rng, L, lmin=2):
+    """Harmonics of a real Gaussian field with C_l = (l+1)^-2 for l >= lmin:
+    a_l,-m = (-1)^m conj(a_lm), a_l0 real."""
+    a = np.zeros(L * L, np.complex128)
+    for l in range(lmin, L):
+        sd = 1.0 / (l + 1.0)
+        a[l * l + l] = rng.standard_normal() * sd
+        m = np.arange(1, l + 1)
+        v = (rng.standard_normal(l) + 1j * rng.standard_normal(l)) * sd / np.sqrt(2)
+        a[l * l + l + m] = v
+        a[l * l + l - m] = ((-1.0) ** m) * np.conj(v)
+    return a
+
+
+def test_c4_spin2_gradient(ops):
+    """c4: L=256, N=4, B=2, J_min=2, spin 2 (E/B fields, seed 4; target seed 40).
+    Loss sum_j ||W_j(f) - T_j||^2 (+ the scaling part); its autograd gradient
+    (hand-written adjoint) against a central finite difference (FP64, step 1e-6)
+    along two random directions."""
+    import torch
+    L, N = 256, 4
+    p = ops.parameters(L, N, 2.0, 2, 2)
+    rng = np.random.default_rng(4)
+    E, B = real_field_alm(rng, L), real_field_alm(rng, L)
+    flm = -(E + 1j * B)
+    rt = np.random.default_rng(40)
+    Et, Bt = real_field_alm(rt, L), real_field_alm(rt, L)
+    tw, ts = ops.analysis(torch.from_numpy(-(Et + 1j * Bt)).cuda(), p)
+    x = torch.from_numpy(flm).cuda().requires_grad_(True)
+    w, s = ops.wavelet_analysis(x, p)
+    loss = (w - tw).abs().pow(2).sum() + (s - ts).abs().pow(2).sum()
+    loss.backward()
+    g = x.grad.detach()
+
+    def lossf(v):
+        w2, s2 = ops.analysis(v, p)
+        return float(((w2 - tw).abs().pow(2).sum() + (s2 - ts).abs().pow(2).sum()).item())
+
+    h = 1e-6
+    l0 = float(loss.item())
+    for k in range(2):
+        d = torch.from_numpy(rng.standard_normal(L * L) + 1j * rng.standard_normal(L * L)).cuda()
+        d[:4] = 0
+        d /= torch.linalg.vector_norm(d)
+        fd = (lossf(x.detach() + h * d) - lossf(x.detach() - h * d)) / (2 * h)
+        an = float(torch.real(torch.vdot(g, d)).item())  # d/dt L(f + t d) = Re <grad, d>
+        tol = 1e-6 * abs(an) + 20 * np.finfo(float).eps * l0 / h
+        print("c4 directional derivative", k, fd, an, tol, flush=True)
+        assert abs(fd - an) < tol, (k, fd, an, tol)
