@@ -79,27 +79,55 @@ __global__ __launch_bounds__(256) void k_mix(MixJobs jobs) {
   for (int i = threadIdx.x; i < nin * nout; i += blockDim.x) sW[i] = mk<T>((T)W[i].x, (T)W[i].y);
   __syncthreads();
   const size_t b0 = blockIdx.x - J.blk0, nbj = (jb + 1 < jobs.n ? jobs.j[jb + 1].blk0 : jobs.nblk) - J.blk0;
-  for (size_t pt = b0 * blockDim.x + threadIdx.x; pt < J.npts; pt += nbj * blockDim.x) {
-    cx<T> v[NMAX];
+  // two sample points per thread and iteration (twice the loads in flight); the
+  // f_wav side (FP64) is streamed with non-temporal loads / stores
+  const size_t step = nbj * blockDim.x;
+  for (size_t p0 = b0 * blockDim.x + threadIdx.x; p0 < J.npts; p0 += 2 * step) {
+    const size_t pts[2] = {p0, p0 + step};
+    const bool ok1 = pts[1] < J.npts;
+    cx<T> v[2][NMAX];
 #pragma unroll
-    for (int j = 0; j < NMAX; ++j) {
-      v[j] = mk<T>(0, 0);
-      if (j < nin) {
-        const cx<TI> u = in[j * J.in_stride + pt];
-        v[j] = mk<T>((T)u.x, (T)u.y);
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j) {
+        v[h][j] = mk<T>(0, 0);
+        if (j < nin && (h == 0 || ok1)) {
+          const cx<TI> *src = in + j * J.in_stride + pts[h];
+          TI ux, uy;
+          if constexpr (sizeof(TI) == 8) {
+            ux = __builtin_nontemporal_load(&src->x);
+            uy = __builtin_nontemporal_load(&src->y);
+          } else {
+            ux = src->x;
+            uy = src->y;
+          }
+          v[h][j] = mk<T>((T)ux, (T)uy);
+        }
       }
-    }
     for (int k = 0; k < nout; ++k) {
-      T ax = 0, ay = 0;
+      T ax[2] = {0, 0}, ay[2] = {0, 0};
 #pragma unroll
       for (int j = 0; j < NMAX; ++j) {
         if (j < nin) {
           const cx<T> w = sW[k * nin + j];
-          ax += v[j].x * w.x - v[j].y * w.y;
-          ay += v[j].x * w.y + v[j].y * w.x;
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            ax[h] += v[h][j].x * w.x - v[h][j].y * w.y;
+            ay[h] += v[h][j].x * w.y + v[h][j].y * w.x;
+          }
         }
       }
-      out[k * J.out_stride + pt] = mk<TO>((TO)ax, (TO)ay);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (h == 1 && !ok1) break;
+        cx<TO> *dst = out + k * J.out_stride + pts[h];
+        if constexpr (sizeof(TO) == 8) {
+          __builtin_nontemporal_store((TO)ax[h], &dst->x);
+          __builtin_nontemporal_store((TO)ay[h], &dst->y);
+        } else {
+          *dst = mk<TO>((TO)ax[h], (TO)ay[h]);
+        }
+      }
     }
   }
 }
