@@ -46,8 +46,8 @@ struct ThetaArgs {
 // for exponent sign -1 (f) and +1 (b), stored as MFMA A fragments:
 // F[((mt K/4) + s) 64 + lane] = G[16 mt + lane % 16][4 s + lane / 16].
 // The lines of a block are staged in LDS as columns of the two products, with
-// the reduction index fastest: column stride CSA = K1 + 4 (first product) and
-// CSC = K2 + 4 (second).  perm[q] is the LDS offset, within a line, of the real
+// the reduction index fastest: column stride CSA = K1 + 5 (first product) and
+// CSC = K2 + 5 (second).  perm[q] is the LDS offset, within a line, of the real
 // part of input sample q.  tw[k] = e^{-2 pi i k / M}.
 template <typename T>
 struct Dft2 {

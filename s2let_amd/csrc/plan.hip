@@ -140,7 +140,7 @@ const DftTables &Context::dft(int n) {
   }
   auto t = std::make_unique<DftTables>();
   const int R1 = round_up(2 * m1, 16), K1 = round_up(2 * m1, 16), R2 = round_up(2 * m2, 16), K2 = round_up(2 * m2, 16);
-  const int CSA = K1 + 4, CSC = K2 + 4;
+  const int CSA = K1 + 5, CSC = K2 + 5;  // odd strides: consecutive columns in consecutive LDS banks
   const size_t s1 = (size_t)R1 * K1, s2 = (size_t)R2 * K2;
   std::vector<double> g64, part64;
   std::vector<float> g32, part32;

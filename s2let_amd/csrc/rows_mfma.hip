@@ -18,6 +18,20 @@
 
 namespace s2h {
 
+// q = n / d, r = n % d for 0 <= n < 2^22, 0 < d < 2^12 through the FP32 reciprocal
+// (one correction step makes it exact)
+__device__ inline void divmod_small(int n, int d, float inv, int &q, int &r) {
+  q = (int)((float)n * inv);
+  r = n - q * d;
+  if (r < 0) {
+    --q;
+    r += d;
+  } else if (r >= d) {
+    ++q;
+    r -= d;
+  }
+}
+
 // one (row tile, up to two column tiles) item of a product: acc[u] += F_mt * S[tile u].
 // In FP32, up to kRowsKR K-steps the item's A fragments are all loaded first (one
 // latency per item); FP64 and longer reductions stream them four at a time.
@@ -100,13 +114,15 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
   if (two_level) {
     const T *__restrict__ F = fwd ? d.F1f : d.F1b;
     const int nks = d.K1 / 4, nmt = d.R1 / 16;
+    const float inmt = 1.0f / (float)nmt;
     const int ncol = nl * M2, nt = (ncol + 15) / 16, ntp = (nt + 1) / 2, nitems = nmt * ntp;
     acc_t r0[IMAX], r1[IMAX];
 #pragma unroll
     for (int it = 0; it < IMAX; ++it) {
       const int item = wave + 4 * it;
       if (item >= nitems) break;
-      const int tp = item / nmt, mt = item - tp * nmt;
+      int tp, mt;
+      divmod_small(item, nmt, inmt, tp, mt);
       dft_item<T>(F, nks, mt, S, d.CSA, 2 * tp, 2 * tp + 1 < nt, lane, r0[it], r1[it]);
     }
     __syncthreads();  // every column of the first product has been read
@@ -115,12 +131,14 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
     for (int it = 0; it < IMAX; ++it) {
       const int item = wave + 4 * it;
       if (item >= nitems) break;
-      const int tp = item / nmt, mt = item - tp * nmt;
+      int tp, mt;
+      divmod_small(item, nmt, inmt, tp, mt);
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int col = 16 * (2 * tp + u) + r16;
         if (col >= ncol) continue;
-        const int line = col / M2, n2 = col - line * M2;
+        int line, n2;
+        divmod_small(col, M2, 1.0f / (float)M2, line, n2);
         const acc_t &r = u ? r1[it] : r0[it];
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
@@ -144,17 +162,20 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
   {  // second product and output
     const T *__restrict__ F = fwd ? d.F2f : d.F2b;
     const int nks = d.K2 / 4, nmt = d.R2 / 16;
+    const float inmt = 1.0f / (float)nmt;
     const int ncol = nl * M1, nt = (ncol + 15) / 16, ntp = (nt + 1) / 2, nitems = nmt * ntp;
     const T scale = a.scale;
     for (int item = wave; item < nitems; item += 4) {
-      const int tp = item / nmt, mt = item - tp * nmt;
+      int tp, mt;
+      divmod_small(item, nmt, inmt, tp, mt);
       acc_t r0, r1;
       dft_item<T>(F, nks, mt, S, d.CSC, 2 * tp, 2 * tp + 1 < nt, lane, r0, r1);
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int col = 16 * (2 * tp + u) + r16;
         if (col >= ncol) continue;
-        const int line = col / M1, k1 = col - line * M1;
+        int line, k1;
+        divmod_small(col, M1, 1.0f / (float)M1, line, k1);
         const acc_t &r = u ? r1 : r0;
         cx<TO> *dst = out + (line0 + line) * M;
 #pragma unroll
