@@ -97,6 +97,9 @@ void s2let_allocate_lmn_f_wav(S2LET_COMPLEX(double) **f_wav_lmn, S2LET_COMPLEX(d
                               const s2let_parameters_t *parameters);
 void s2let_allocate_f_wav(S2LET_COMPLEX(double) **f_wav, S2LET_COMPLEX(double) **f_scal,
                           const s2let_parameters_t *parameters);
+void s2let_allocate_mw_real(double **f, int L);                           /* s2let_alloc.c:28 */
+void s2let_allocate_f_wav_real(double **f_wav, double **f_scal,
+                               const s2let_parameters_t *parameters);     /* s2let_alloc.c:122 */
 
 /* ---- harmonic tiling multiplies (GPU kernels) ---- */
 void s2let_analysis_lm2lmn(S2LET_COMPLEX(double) *f_wav_lmn, S2LET_COMPLEX(double) *f_scal_lm,
@@ -140,9 +143,32 @@ void s2let_synthesis_adjoint_px2wav(S2LET_COMPLEX(double) *f_wav, S2LET_COMPLEX(
                                     const S2LET_COMPLEX(double) *f,
                                     const s2let_parameters_t *parameters); /* s2let_synthesis_adjoint.c:465 */
 
-/* ---- MW spherical harmonic transforms used by pys2let (s2let_mw.c:21-40) ---- */
+/* ---- real signals.  Real maps and real wavelet coefficients (double), the
+ * harmonic coefficients of a real signal (complex, f_{l,-m} = (-1)^m conj(f_lm));
+ * array sizes as the complex variants.  Pixel-space variants use spin-0 MW
+ * transforms whatever parameters->spin is, as the reference does. ---- */
+void s2let_analysis_lm2wav_real(double *f_wav, double *f_scal, const S2LET_COMPLEX(double) *flm,
+                                const s2let_parameters_t *parameters);    /* s2let_analysis.c:371 */
+void s2let_synthesis_wav2lm_real(S2LET_COMPLEX(double) *flm, const double *f_wav, const double *f_scal,
+                                 const s2let_parameters_t *parameters);   /* s2let_synthesis.c:386 */
+void s2let_analysis_px2wav_real(double *f_wav, double *f_scal, const double *f,
+                                const s2let_parameters_t *parameters);    /* s2let_analysis.c:497 */
+void s2let_synthesis_wav2px_real(double *f, const double *f_wav, const double *f_scal,
+                                 const s2let_parameters_t *parameters);   /* s2let_synthesis.c:513 */
+void s2let_analysis_adjoint_wav2lm_real(S2LET_COMPLEX(double) *flm, const double *f_wav, const double *f_scal,
+                                        const s2let_parameters_t *parameters); /* s2let_analysis_adjoint.c:402 */
+void s2let_analysis_adjoint_wav2px_real(double *f, const double *f_wav, const double *f_scal,
+                                        const s2let_parameters_t *parameters); /* s2let_analysis_adjoint.c:530 */
+void s2let_synthesis_adjoint_lm2wav_real(double *f_wav, double *f_scal, const S2LET_COMPLEX(double) *flm,
+                                         const s2let_parameters_t *parameters); /* s2let_synthesis_adjoint.c:380 */
+void s2let_synthesis_adjoint_px2wav_real(double *f_wav, double *f_scal, const double *f,
+                                         const s2let_parameters_t *parameters); /* s2let_synthesis_adjoint.c:506 */
+
+/* ---- MW spherical harmonic transforms used by pys2let (s2let_mw.c:21-70) ---- */
 void s2let_mw_alm2map(S2LET_COMPLEX(double) *f, const S2LET_COMPLEX(double) *flm, int L, int spin);
 void s2let_mw_map2alm(S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *f, int L, int spin);
+void s2let_mw_alm2map_real(double *f, const S2LET_COMPLEX(double) *flm, int L); /* s2let_mw.c:51 */
+void s2let_mw_map2alm_real(S2LET_COMPLEX(double) *flm, const double *f, int L); /* s2let_mw.c:66 */
 
 /* ---------------------------------------------------------------------------
  * Extension: device-pointer entry points (no reference counterpart).
@@ -169,6 +195,17 @@ int s2let_hip_analysis_adjoint_wav2lm(void *flm, const void *f_wav, const void *
                                       const s2let_parameters_t *parameters, int mode, void *stream);
 int s2let_hip_synthesis_adjoint_lm2wav(void *f_wav, void *f_scal, const void *flm,
                                        const s2let_parameters_t *parameters, int mode, void *stream);
+/* Real-signal variants (device pointers): f_wav / f_scal real double arrays,
+ * flm complex128 (s2let_analysis.c:371, s2let_synthesis.c:386,
+ * s2let_analysis_adjoint.c:402, s2let_synthesis_adjoint.c:380). */
+int s2let_hip_analysis_lm2wav_real(double *f_wav, double *f_scal, const void *flm,
+                                   const s2let_parameters_t *parameters, int mode, void *stream);
+int s2let_hip_synthesis_wav2lm_real(void *flm, const double *f_wav, const double *f_scal,
+                                    const s2let_parameters_t *parameters, int mode, void *stream);
+int s2let_hip_analysis_adjoint_wav2lm_real(void *flm, const double *f_wav, const double *f_scal,
+                                           const s2let_parameters_t *parameters, int mode, void *stream);
+int s2let_hip_synthesis_adjoint_lm2wav_real(double *f_wav, double *f_scal, const void *flm,
+                                            const s2let_parameters_t *parameters, int mode, void *stream);
 int s2let_hip_mw_alm2map(void *f, const void *flm, int L, int spin, int mode, void *stream);
 int s2let_hip_mw_map2alm(void *flm, const void *f, int L, int spin, int mode, void *stream);
 int s2let_hip_mw_alm2map_adjoint(void *flm, const void *f, int L, int spin, int mode, void *stream);

@@ -88,6 +88,15 @@ for _n in ("analysis_lm2wav", "synthesis_wav2lm", "analysis_adjoint_wav2lm", "sy
 for _n in ("mw_alm2map", "mw_map2alm", "mw_alm2map_adjoint", "mw_map2alm_adjoint"):
     _SIGS["s2let_hip_" + _n] = (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int, ctypes.c_void_p])
+for _n in ("analysis_lm2wav_real", "synthesis_wav2lm_real", "analysis_px2wav_real", "synthesis_wav2px_real",
+           "analysis_adjoint_wav2lm_real", "analysis_adjoint_wav2px_real", "synthesis_adjoint_lm2wav_real",
+           "synthesis_adjoint_px2wav_real"):
+    _SIGS["s2let_" + _n] = (None, [ctypes.c_void_p] * 4)
+for _n in ("mw_alm2map_real", "mw_map2alm_real"):
+    _SIGS["s2let_" + _n] = (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int])
+for _n in ("analysis_lm2wav_real", "synthesis_wav2lm_real", "analysis_adjoint_wav2lm_real",
+           "synthesis_adjoint_lm2wav_real"):
+    _SIGS["s2let_hip_" + _n] = (ctypes.c_int, [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_void_p])
 _SIGS["s2let_hip_selftest"] = (ctypes.c_int, [])
 _SIGS["s2let_hip_timing"] = (None, [ctypes.c_int])
 _SIGS["s2let_hip_timing_report"] = (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int])
@@ -145,6 +154,10 @@ def _cin(a):
 
 def _ref(p):
     return ctypes.byref(p)
+
+
+def _rin(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
 
 
 # ----------------------------------------------------------------------------- sizes
@@ -264,6 +277,84 @@ def mw_map2alm(f, L, spin=0):
     f = _cin(f)
     flm = np.zeros(L * L, np.complex128)
     lib().s2let_mw_map2alm(_ptr(flm), _ptr(f), int(L), int(spin))
+    return flm
+
+
+# ----------------------------------------------------------------------------- real signals
+# Real maps / wavelet coefficients (float64) and the harmonic coefficients of a
+# real signal (s2let_analysis.c:371-523, s2let_synthesis.c:386-540,
+# s2let_analysis_adjoint.c:402-556, s2let_synthesis_adjoint.c:380-532, s2let_mw.c:51-70).
+def _real_wav(p):
+    return np.zeros(n_wav(p), np.float64), np.zeros(n_scal(p), np.float64)
+
+
+def analysis_lm2wav_real(flm, p):
+    flm = _cin(flm)
+    fw, fs = _real_wav(p)
+    lib().s2let_analysis_lm2wav_real(_ptr(fw), _ptr(fs), _ptr(flm), _ref(p))
+    return fw, fs
+
+
+def synthesis_wav2lm_real(f_wav, f_scal, p):
+    a, b = _rin(f_wav), _rin(f_scal)
+    flm = np.zeros(p.L * p.L, np.complex128)
+    lib().s2let_synthesis_wav2lm_real(_ptr(flm), _ptr(a), _ptr(b), _ref(p))
+    return flm
+
+
+def analysis_px2wav_real(f, p):
+    f = _rin(f)
+    fw, fs = _real_wav(p)
+    lib().s2let_analysis_px2wav_real(_ptr(fw), _ptr(fs), _ptr(f), _ref(p))
+    return fw, fs
+
+
+def synthesis_wav2px_real(f_wav, f_scal, p):
+    a, b = _rin(f_wav), _rin(f_scal)
+    f = np.zeros(p.L * (2 * p.L - 1), np.float64)
+    lib().s2let_synthesis_wav2px_real(_ptr(f), _ptr(a), _ptr(b), _ref(p))
+    return f
+
+
+def analysis_adjoint_wav2lm_real(f_wav, f_scal, p):
+    a, b = _rin(f_wav), _rin(f_scal)
+    flm = np.zeros(p.L * p.L, np.complex128)
+    lib().s2let_analysis_adjoint_wav2lm_real(_ptr(flm), _ptr(a), _ptr(b), _ref(p))
+    return flm
+
+
+def analysis_adjoint_wav2px_real(f_wav, f_scal, p):
+    a, b = _rin(f_wav), _rin(f_scal)
+    f = np.zeros(p.L * (2 * p.L - 1), np.float64)
+    lib().s2let_analysis_adjoint_wav2px_real(_ptr(f), _ptr(a), _ptr(b), _ref(p))
+    return f
+
+
+def synthesis_adjoint_lm2wav_real(flm, p):
+    flm = _cin(flm)
+    fw, fs = _real_wav(p)
+    lib().s2let_synthesis_adjoint_lm2wav_real(_ptr(fw), _ptr(fs), _ptr(flm), _ref(p))
+    return fw, fs
+
+
+def synthesis_adjoint_px2wav_real(f, p):
+    f = _rin(f)
+    fw, fs = _real_wav(p)
+    lib().s2let_synthesis_adjoint_px2wav_real(_ptr(fw), _ptr(fs), _ptr(f), _ref(p))
+    return fw, fs
+
+
+def mw_alm2map_real(flm, L):
+    flm = _cin(flm)
+    f = np.zeros(L * (2 * L - 1), np.float64)
+    lib().s2let_mw_alm2map_real(_ptr(f), _ptr(flm), int(L))
+    return f
+
+
+def mw_map2alm_real(f, L):
+    f = _rin(f)
+    flm = np.zeros(L * L, np.complex128)
+    lib().s2let_mw_map2alm_real(_ptr(flm), _ptr(f), int(L))
     return flm
 
 

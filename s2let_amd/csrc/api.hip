@@ -136,6 +136,56 @@ std::string param_key(const s2let_parameters_t *p) {
   return b;
 }
 
+
+// ---- real signals (s2let_analysis.c:371-523, s2let_synthesis.c:386-540,
+// s2let_analysis_adjoint.c:402-556, s2let_synthesis_adjoint.c:380-532).  The
+// transforms of a real signal are the complex ones restricted to Hermitian
+// harmonics (f_{l,-m} = (-1)^m conj(f_lm)) and real samples: every S2LET
+// wavelet obeys Psi_{l,-n} = (-1)^n conj(Psi_{ln}), so wavelet coefficients of
+// a real signal are real, and the reference's n >= 0 storage with the implied
+// f_{l,m,-n} = (-1)^{m+n} conj(f_{l,-m,n}) is the complex transform on that
+// subspace.  The real variants promote real inputs to complex, run the complex
+// kernels and keep real parts (on a non-Hermitian flm: the transform of its
+// Hermitian part).
+struct RealWork {
+  Plan &P;
+  size_t nwav, nscal, npx;
+  explicit RealWork(Plan &pl) : P(pl) {
+    const int L = pl.p.L;
+    nwav = pl.n_wav_used;
+    const int Ls = scal_bandlimit(&pl.p);
+    nscal = (size_t)Ls * (2 * Ls - 1);
+    npx = (size_t)L * (2 * L - 1);
+    if (pl.r_wav.bytes < nwav * CB) pl.r_wav.alloc(nwav * CB);
+    if (pl.r_scal.bytes < nscal * CB) pl.r_scal.alloc(nscal * CB);
+    if (pl.r_map.bytes < npx * CB) pl.r_map.alloc(npx * CB);
+    const size_t nr = std::max(nwav, npx);
+    if (pl.r_real.bytes < nr * sizeof(double)) pl.r_real.alloc(nr * sizeof(double));
+    if (pl.r_real2.bytes < std::max(nscal, (size_t)L * L) * CB) pl.r_real2.alloc(std::max(nscal, (size_t)L * L) * CB);
+  }
+};
+
+// device-pointer real transforms on plan P (f_wav, f_scal, maps real; flm complex128)
+void real_analysis(Plan &P, RealWork &w, const void *flm, double *f_wav, double *f_scal, hipStream_t st) {
+  run_analysis(P, flm, P.r_wav.p, P.r_scal.p, st);
+  launch_cx2re(P.r_wav.p, f_wav, w.nwav, st);
+  launch_cx2re(P.r_scal.p, f_scal, w.nscal, st);
+}
+void real_synthesis(Plan &P, RealWork &w, const double *f_wav, const double *f_scal, void *flm, hipStream_t st) {
+  launch_re2cx(f_wav, P.r_wav.p, w.nwav, st);
+  launch_re2cx(f_scal, P.r_scal.p, w.nscal, st);
+  run_synthesis(P, P.r_wav.p, P.r_scal.p, flm, st);
+}
+void real_analysis_adjoint(Plan &P, RealWork &w, const double *f_wav, const double *f_scal, void *flm, hipStream_t st) {
+  launch_re2cx(f_wav, P.r_wav.p, w.nwav, st);
+  launch_re2cx(f_scal, P.r_scal.p, w.nscal, st);
+  run_analysis_adjoint(P, P.r_wav.p, P.r_scal.p, flm, st);
+}
+void real_synthesis_adjoint(Plan &P, RealWork &w, const void *flm, double *f_wav, double *f_scal, hipStream_t st) {
+  run_synthesis_adjoint(P, flm, P.r_wav.p, P.r_scal.p, st);
+  launch_cx2re(P.r_wav.p, f_wav, w.nwav, st);
+  launch_cx2re(P.r_scal.p, f_scal, w.nscal, st);
+}
 }  // namespace
 }  // namespace s2h
 
@@ -220,6 +270,44 @@ int s2let_hip_synthesis_adjoint_lm2wav(void *f_wav, void *f_scal, const void *fl
   return guarded([&] {
     S2H_DEV_PLAN
     run_synthesis_adjoint(P, flm, f_wav, f_scal, st);
+    S2H_CHECK_HIP(hipGetLastError());
+  });
+}
+
+// ---------------------------------------------------------------- real signals (device pointers)
+int s2let_hip_analysis_lm2wav_real(double *f_wav, double *f_scal, const void *flm, const s2let_parameters_t *p,
+                                   int mode, void *stream) {
+  return guarded([&] {
+    S2H_DEV_PLAN
+    RealWork w(P);
+    real_analysis(P, w, flm, f_wav, f_scal, st);
+    S2H_CHECK_HIP(hipGetLastError());
+  });
+}
+int s2let_hip_synthesis_wav2lm_real(void *flm, const double *f_wav, const double *f_scal, const s2let_parameters_t *p,
+                                    int mode, void *stream) {
+  return guarded([&] {
+    S2H_DEV_PLAN
+    RealWork w(P);
+    real_synthesis(P, w, f_wav, f_scal, flm, st);
+    S2H_CHECK_HIP(hipGetLastError());
+  });
+}
+int s2let_hip_analysis_adjoint_wav2lm_real(void *flm, const double *f_wav, const double *f_scal,
+                                           const s2let_parameters_t *p, int mode, void *stream) {
+  return guarded([&] {
+    S2H_DEV_PLAN
+    RealWork w(P);
+    real_analysis_adjoint(P, w, f_wav, f_scal, flm, st);
+    S2H_CHECK_HIP(hipGetLastError());
+  });
+}
+int s2let_hip_synthesis_adjoint_lm2wav_real(double *f_wav, double *f_scal, const void *flm,
+                                            const s2let_parameters_t *p, int mode, void *stream) {
+  return guarded([&] {
+    S2H_DEV_PLAN
+    RealWork w(P);
+    real_synthesis_adjoint(P, w, flm, f_wav, f_scal, st);
     S2H_CHECK_HIP(hipGetLastError());
   });
 }
@@ -652,4 +740,167 @@ void s2let_synthesis_adjoint_lm2lmn(std::complex<double> *f_wav_lmn, std::comple
   host_call(__func__, [&] { lmn_common(3, f_wav_lmn, f_scal_lm, flm, nullptr, wav_lm, scal_l, p); });
 }
 
+
+// ---------------------------------------------------------------- real signals (host pointers, FP64)
+constexpr size_t DB = sizeof(double);
+
+void s2let_analysis_lm2wav_real(double *f_wav, double *f_scal, const std::complex<double> *flm,
+                                const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    RealWork w(P);
+    h2d(P.h_flm.p, flm, io.nlm * CB, st);
+    real_analysis(P, w, P.h_flm.p, P.r_real.as<double>(), P.r_real2.as<double>(), st);
+    d2h(f_wav, P.r_real.p, w.nwav * DB, st);
+    d2h(f_scal, P.r_real2.p, w.nscal * DB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_synthesis_wav2lm_real(std::complex<double> *flm, const double *f_wav, const double *f_scal,
+                                 const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    RealWork w(P);
+    h2d(P.r_real.p, f_wav, w.nwav * DB, st);
+    h2d(P.r_real2.p, f_scal, w.nscal * DB, st);
+    real_synthesis(P, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_flm.p, st);
+    d2h(flm, P.h_flm.p, io.nlm * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_analysis_px2wav_real(double *f_wav, double *f_scal, const double *f, const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    RealWork w(P);
+    ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st);  // real maps: spin 0
+    h2d(P.r_real.p, f, w.npx * DB, st);
+    launch_re2cx(P.r_real.as<double>(), P.r_map.p, w.npx, st);
+    run_sht_forward(S, P.r_map.p, P.h_tmp.p, st);
+    real_analysis(P, w, P.h_tmp.p, P.r_real.as<double>(), P.r_real2.as<double>(), st);
+    d2h(f_wav, P.r_real.p, w.nwav * DB, st);
+    d2h(f_scal, P.r_real2.p, w.nscal * DB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_synthesis_wav2px_real(double *f, const double *f_wav, const double *f_scal, const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    RealWork w(P);
+    ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st);
+    h2d(P.r_real.p, f_wav, w.nwav * DB, st);
+    h2d(P.r_real2.p, f_scal, w.nscal * DB, st);
+    real_synthesis(P, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_tmp.p, st);
+    run_sht_inverse(S, P.h_tmp.p, P.r_map.p, st);
+    launch_cx2re(P.r_map.p, P.r_real.as<double>(), w.npx, st);
+    d2h(f, P.r_real.p, w.npx * DB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_analysis_adjoint_wav2lm_real(std::complex<double> *flm, const double *f_wav, const double *f_scal,
+                                        const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    RealWork w(P);
+    h2d(P.r_real.p, f_wav, w.nwav * DB, st);
+    h2d(P.r_real2.p, f_scal, w.nscal * DB, st);
+    real_analysis_adjoint(P, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_flm.p, st);
+    d2h(flm, P.h_flm.p, io.nlm * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_analysis_adjoint_wav2px_real(double *f, const double *f_wav, const double *f_scal,
+                                        const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    RealWork w(P);
+    ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st);
+    h2d(P.r_real.p, f_wav, w.nwav * DB, st);
+    h2d(P.r_real2.p, f_scal, w.nscal * DB, st);
+    real_analysis_adjoint(P, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_tmp.p, st);
+    run_sht_forward_adjoint(S, P.h_tmp.p, P.r_map.p, st);
+    launch_cx2re(P.r_map.p, P.r_real.as<double>(), w.npx, st);
+    d2h(f, P.r_real.p, w.npx * DB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_synthesis_adjoint_lm2wav_real(double *f_wav, double *f_scal, const std::complex<double> *flm,
+                                         const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    RealWork w(P);
+    h2d(P.h_flm.p, flm, io.nlm * CB, st);
+    real_synthesis_adjoint(P, w, P.h_flm.p, P.r_real.as<double>(), P.r_real2.as<double>(), st);
+    d2h(f_wav, P.r_real.p, w.nwav * DB, st);
+    d2h(f_scal, P.r_real2.p, w.nscal * DB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_synthesis_adjoint_px2wav_real(double *f_wav, double *f_scal, const double *f,
+                                         const s2let_parameters_t *p) {
+  require_mw(p, __func__);
+  host_call(__func__, [&] {
+    S2H_HOST_PLAN
+    RealWork w(P);
+    ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st);
+    h2d(P.r_real.p, f, w.npx * DB, st);
+    launch_re2cx(P.r_real.as<double>(), P.r_map.p, w.npx, st);
+    run_sht_inverse_adjoint(S, P.r_map.p, P.h_tmp.p, st);
+    real_synthesis_adjoint(P, w, P.h_tmp.p, P.r_real.as<double>(), P.r_real2.as<double>(), st);
+    d2h(f_wav, P.r_real.p, w.nwav * DB, st);
+    d2h(f_scal, P.r_real2.p, w.nscal * DB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_mw_alm2map_real(double *f, const std::complex<double> *flm, int L) {
+  host_call(__func__, [&] {
+    Context &C = Context::get();
+    std::lock_guard<std::mutex> lk(C.mu);
+    hipStream_t st = C.stream();
+    ShtPlan &S = C.sht(L, 0, S2LET_HIP_FP64, st);
+    const size_t nlm = (size_t)L * L, npx = (size_t)L * (2 * L - 1);
+    DevBuf a, b, r;
+    a.alloc(nlm * CB);
+    b.alloc(npx * CB);
+    r.alloc(npx * DB);
+    h2d(a.p, flm, nlm * CB, st);
+    run_sht_inverse(S, a.p, b.p, st);
+    launch_cx2re(b.p, r.as<double>(), npx, st);
+    d2h(f, r.p, npx * DB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_mw_map2alm_real(std::complex<double> *flm, const double *f, int L) {
+  host_call(__func__, [&] {
+    Context &C = Context::get();
+    std::lock_guard<std::mutex> lk(C.mu);
+    hipStream_t st = C.stream();
+    ShtPlan &S = C.sht(L, 0, S2LET_HIP_FP64, st);
+    const size_t nlm = (size_t)L * L, npx = (size_t)L * (2 * L - 1);
+    DevBuf a, b, r;
+    a.alloc(nlm * CB);
+    b.alloc(npx * CB);
+    r.alloc(npx * DB);
+    h2d(r.p, f, npx * DB, st);
+    launch_re2cx(r.as<double>(), b.p, npx, st);
+    run_sht_forward(S, b.p, a.p, st);
+    d2h(flm, a.p, nlm * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
 }  // extern "C"

@@ -63,6 +63,16 @@ __global__ void k_cast_cx(const cx<A> *__restrict__ a, cx<B> *__restrict__ b, si
     b[i] = mk<B>((B)a[i].x, (B)a[i].y);
 }
 
+// ------------------------------------------------------------------ real signals
+__global__ void k_re2cx(const double *__restrict__ a, cx<double> *__restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = mk<double>(a[i], 0.0);
+}
+__global__ void k_cx2re(const cx<double> *__restrict__ a, double *__restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i].x;
+}
+
 // ------------------------------------------------------------------ gamma mixing
 // per job: out_k[pt] = sum_j W[k][j] in_j[pt] (complex, nout x nin matrix W, nin <= NMAX),
 // kMixBlocks blocks per job
@@ -148,6 +158,13 @@ void launch_risbo_half(const double *in, double *out, int n, int rmax, int symro
 
 void launch_to_f32(const double *a, float *b, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_to_f32, dim3(grid1d(n)), dim3(256), 0, st, a, b, n);
+}
+
+void launch_re2cx(const double *a, void *b, size_t n, hipStream_t st) {
+  if (n) hipLaunchKernelGGL(k_re2cx, dim3(grid1d(n)), dim3(256), 0, st, a, (cx<double> *)b, n);
+}
+void launch_cx2re(const void *a, double *b, size_t n, hipStream_t st) {
+  if (n) hipLaunchKernelGGL(k_cx2re, dim3(grid1d(n)), dim3(256), 0, st, (const cx<double> *)a, b, n);
 }
 
 template <typename A, typename B>

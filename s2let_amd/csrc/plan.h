@@ -76,6 +76,8 @@ struct Plan {
   std::vector<size_t> gamma_off;  // per group: offsets of its 4 mixing matrices in d_gamma
   // staging for the host-pointer API
   DevBuf h_flm, h_wav, h_scal, h_map, h_tmp;
+  // real-signal variants: complex work copies of f_wav / f_scal / a map and real staging
+  DevBuf r_wav, r_scal, r_map, r_real, r_real2;
 };
 
 // Single spin-s MW harmonic transform plan (s2let_mw_alm2map / map2alm and

@@ -313,6 +313,11 @@ int s2let_n_wav_j(int j, const s2let_parameters_t *p) {
 void s2let_allocate_mw(std::complex<double> **f, int L) {
   *f = (std::complex<double> *)calloc((size_t)L * (2 * L - 1), sizeof(std::complex<double>));
 }
+void s2let_allocate_mw_real(double **f, int L) { *f = (double *)calloc((size_t)L * (2 * L - 1), sizeof(double)); }
+void s2let_allocate_f_wav_real(double **a, double **b, const s2let_parameters_t *p) {
+  *a = (double *)calloc(s2let_n_wav(p), sizeof(double));
+  *b = (double *)calloc(s2let_n_scal(p), sizeof(double));
+}
 void s2let_allocate_lm(std::complex<double> **flm, int L) {
   *flm = (std::complex<double> *)calloc((size_t)L * L, sizeof(std::complex<double>));
 }
