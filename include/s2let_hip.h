@@ -116,6 +116,23 @@ void s2let_synthesis_adjoint_lm2lmn(S2LET_COMPLEX(double) *f_wav_lmn, S2LET_COMP
                                     const S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *wav_lm,
                                     const double *scal_l,
                                     const s2let_parameters_t *parameters); /* s2let_synthesis_adjoint.c:28 */
+/* real signals: Wigner coefficients stored for n >= 0 only, row n (l >= n) at
+ * offset n L_j^2 - (n-1) n (2n-1)/6 of its scale; s2let_n_lmn_wav and
+ * s2let_allocate_lmn_f_wav size them when parameters->reality is set. */
+void s2let_analysis_lm2lmn_real(S2LET_COMPLEX(double) *f_wav_lmn, S2LET_COMPLEX(double) *f_scal_lm,
+                                const S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *wav_lm,
+                                const double *scal_l, const s2let_parameters_t *parameters); /* s2let_analysis.c:102 */
+void s2let_synthesis_lmn2lm_real(S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *f_wav_lmn,
+                                 const S2LET_COMPLEX(double) *f_scal_lm, const S2LET_COMPLEX(double) *wav_lm,
+                                 const double *scal_l, const s2let_parameters_t *parameters); /* s2let_synthesis.c:101 */
+void s2let_analysis_adjoint_lmn2lm_real(S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *f_wav_lmn,
+                                        const S2LET_COMPLEX(double) *f_scal_lm, const S2LET_COMPLEX(double) *wav_lm,
+                                        const double *scal_l,
+                                        const s2let_parameters_t *parameters); /* s2let_analysis_adjoint.c:101 */
+void s2let_synthesis_adjoint_lm2lmn_real(S2LET_COMPLEX(double) *f_wav_lmn, S2LET_COMPLEX(double) *f_scal_lm,
+                                         const S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *wav_lm,
+                                         const double *scal_l,
+                                         const s2let_parameters_t *parameters); /* s2let_synthesis_adjoint.c:103 */
 
 /* ---- the transform (hot path) ---- */
 void s2let_analysis_lm2wav(S2LET_COMPLEX(double) *f_wav, S2LET_COMPLEX(double) *f_scal,

@@ -249,6 +249,50 @@ def synthesis_adjoint_lm2lmn(flm, p):
     return wl, sl
 
 
+def flmn_size_real(L, N):
+    return lib().oracle_flmn_size_real(int(L), int(N))
+
+
+def elmn2ind_real(el, m, n, L):
+    return lib().oracle_elmn2ind_real(int(el), int(m), int(n), int(L))
+
+
+def n_lmn_wav_real(p):
+    return lib().oracle_n_lmn_wav_real(ctypes.byref(p))
+
+
+def analysis_lm2lmn_real(flm, p):
+    psi, phi = tiling_wavelet(p)
+    wl, sl = _c(n_lmn_wav_real(p)), _c(n_lm_scal(p))
+    flm = _in(flm)
+    lib().oracle_analysis_lm2lmn_real(_p(wl), _p(sl), _p(flm), _p(psi), _p(phi), ctypes.byref(p))
+    return wl, sl
+
+
+def synthesis_adjoint_lm2lmn_real(flm, p):
+    psi, phi = tiling_wavelet(p)
+    wl, sl = _c(n_lmn_wav_real(p)), _c(n_lm_scal(p))
+    flm = _in(flm)
+    lib().oracle_synthesis_adjoint_lm2lmn_real(_p(wl), _p(sl), _p(flm), _p(psi), _p(phi), ctypes.byref(p))
+    return wl, sl
+
+
+def synthesis_lmn2lm_real(f_wav_lmn, f_scal_lm, p):
+    psi, phi = tiling_wavelet(p)
+    flm = _c(p.L * p.L)
+    a, b = _in(f_wav_lmn), _in(f_scal_lm)
+    lib().oracle_synthesis_lmn2lm_real(_p(flm), _p(a), _p(b), _p(psi), _p(phi), ctypes.byref(p))
+    return flm
+
+
+def analysis_adjoint_lmn2lm_real(f_wav_lmn, f_scal_lm, p):
+    psi, phi = tiling_wavelet(p)
+    flm = _c(p.L * p.L)
+    a, b = _in(f_wav_lmn), _in(f_scal_lm)
+    lib().oracle_analysis_adjoint_lmn2lm_real(_p(flm), _p(a), _p(b), _p(psi), _p(phi), ctypes.byref(p))
+    return flm
+
+
 def _wav_pair(p):
     return _c(n_wav(p)), _c(n_scal(p))
 

@@ -1215,6 +1215,105 @@ void oracle_synthesis_adjoint_px2wav(ocplx *f_wav, ocplx *f_scal, const ocplx *f
   free(flm);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Real-signal harmonic multiplies (n >= 0 storage)                          */
+/* ------------------------------------------------------------------------ */
+
+/* SO3 compact storage of a real signal keeps rows n = 0..N-1 (l >= n), L^2 - n^2
+   entries each; derived like the complex order from the size formula
+   N L^2 - (N-1) N (2N-1) / 6 (the SO3 library is absent). */
+int oracle_flmn_size_real(int L, int N) { return N * L * L - (N - 1) * N * (2 * N - 1) / 6; }
+int oracle_elmn2ind_real(int el, int m, int n, int L) {
+  int off = 0;
+  for (int np = 0; np < n; ++np) off += L * L - np * np;
+  return off + (el * el - n * n) + (el + m);
+}
+/* s2let_helper.c:37-56 with reality set */
+int oracle_n_lmn_wav_real(const oracle_params_t *p) {
+  int J = oracle_j_max(p), total = 0, bl = p->L, Nn = p->N;
+  for (int j = p->J_min; j <= J; ++j) {
+    if (!p->upsample) {
+      bl = OMIN(oracle_bandlimit(j, p), p->L);
+      Nn = OMIN(p->N, bl);
+    }
+    total += oracle_flmn_size_real(bl, Nn);
+  }
+  return total;
+}
+
+/* s2let_analysis.c:102-160 (adjoint_synthesis = 0) and s2let_synthesis_adjoint.c:103-162 (= 1) */
+static void or_lm2lmn_real(ocplx *f_wav_lmn, ocplx *f_scal_lm, const ocplx *flm, const ocplx *wav_lm,
+                           const double *scal_l, const oracle_params_t *p, int adjoint_synthesis) {
+  int L = p->L, N = p->N, J = oracle_j_max(p), bandlimit = L, Nj = N, offset = 0;
+  for (int j = p->J_min; j <= J; ++j) {
+    int Ns = N; /* storage N of this scale */
+    if (!p->upsample) or_scale(p, j, &bandlimit, &Ns);
+    for (int n = 1 - Nj % 2; n < Nj; n += 2) /* the loop runs with the outer N (s2let_analysis.c:137) */
+      for (int el = n; el < bandlimit; ++el) {
+        ocplx psi = conj(wav_lm[(size_t)j * L * L + el * el + el + n]);
+        if (!adjoint_synthesis) psi = 8 * OPI * OPI / (2 * el + 1) * psi;
+        for (int m = -el; m <= el; ++m)
+          f_wav_lmn[offset + oracle_elmn2ind_real(el, m, n, bandlimit)] = flm[el * el + el + m] * psi;
+      }
+    offset += oracle_flmn_size_real(bandlimit, Ns);
+  }
+  if (!p->upsample) bandlimit = OMIN(oracle_bandlimit(p->J_min - 1, p), L);
+  for (int el = 0; el < bandlimit; ++el) {
+    double phi = sqrt(4.0 * OPI / (2 * el + 1)) * scal_l[el];
+    for (int m = -el; m <= el; ++m) f_scal_lm[el * el + el + m] = flm[el * el + el + m] * phi;
+  }
+}
+
+/* s2let_synthesis.c:101-170 (adjoint_analysis = 0) and s2let_analysis_adjoint.c:101-176 (= 1) */
+static void or_lmn2lm_real(ocplx *flm, const ocplx *f_wav_lmn, const ocplx *f_scal_lm, const ocplx *wav_lm,
+                           const double *scal_l, const oracle_params_t *p, int adjoint_analysis) {
+  int L = p->L, N = p->N, J = oracle_j_max(p), bandlimit = L, Nj = N, offset = 0;
+  for (int i = 0; i < L * L; ++i) flm[i] = 0;
+  for (int j = p->J_min; j <= J; ++j) {
+    int Ns = N;
+    if (!p->upsample) or_scale(p, j, &bandlimit, &Ns);
+    for (int n = 1 - Nj % 2; n < Nj; n += 2)
+      for (int el = n; el < bandlimit; ++el) {
+        ocplx psi = wav_lm[(size_t)j * L * L + el * el + el + n], npsi = 0;
+        if (adjoint_analysis) psi = 8 * OPI * OPI / (2 * el + 1) * psi;
+        if (n) {
+          npsi = wav_lm[(size_t)j * L * L + el * el + el - n];
+          if (adjoint_analysis) npsi = 8 * OPI * OPI / (2 * el + 1) * npsi;
+        }
+        for (int m = -el; m <= el; ++m) {
+          flm[el * el + el + m] += f_wav_lmn[offset + oracle_elmn2ind_real(el, m, n, bandlimit)] * psi;
+          if (n) {
+            int sign = (m + n) % 2 ? -1 : 1;
+            flm[el * el + el + m] += sign * conj(f_wav_lmn[offset + oracle_elmn2ind_real(el, -m, n, bandlimit)]) * npsi;
+          }
+        }
+      }
+    offset += oracle_flmn_size_real(bandlimit, Ns);
+  }
+  if (!p->upsample) bandlimit = OMIN(oracle_bandlimit(p->J_min - 1, p), L);
+  for (int el = 0; el < bandlimit; ++el) {
+    double phi = sqrt(4.0 * OPI / (2 * el + 1)) * scal_l[el];
+    for (int m = -el; m <= el; ++m) flm[el * el + el + m] += f_scal_lm[el * el + el + m] * phi;
+  }
+}
+
+void oracle_analysis_lm2lmn_real(ocplx *f_wav_lmn, ocplx *f_scal_lm, const ocplx *flm, const ocplx *wav_lm,
+                                 const double *scal_l, const oracle_params_t *p) {
+  or_lm2lmn_real(f_wav_lmn, f_scal_lm, flm, wav_lm, scal_l, p, 0);
+}
+void oracle_synthesis_adjoint_lm2lmn_real(ocplx *f_wav_lmn, ocplx *f_scal_lm, const ocplx *flm,
+                                          const ocplx *wav_lm, const double *scal_l, const oracle_params_t *p) {
+  or_lm2lmn_real(f_wav_lmn, f_scal_lm, flm, wav_lm, scal_l, p, 1);
+}
+void oracle_synthesis_lmn2lm_real(ocplx *flm, const ocplx *f_wav_lmn, const ocplx *f_scal_lm, const ocplx *wav_lm,
+                                  const double *scal_l, const oracle_params_t *p) {
+  or_lmn2lm_real(flm, f_wav_lmn, f_scal_lm, wav_lm, scal_l, p, 0);
+}
+void oracle_analysis_adjoint_lmn2lm_real(ocplx *flm, const ocplx *f_wav_lmn, const ocplx *f_scal_lm,
+                                         const ocplx *wav_lm, const double *scal_l, const oracle_params_t *p) {
+  or_lmn2lm_real(flm, f_wav_lmn, f_scal_lm, wav_lm, scal_l, p, 1);
+}
+
 /* s2let_transform_axisym_mw.c (multires analysis) with s2let_transform_axisym_lm.c kernels */
 void oracle_axisym_wav_analysis_mw_multires(ocplx *f_wav, ocplx *f_scal, const ocplx *f, const oracle_params_t *p) {
   int L = p->L, J = oracle_j_max(p);

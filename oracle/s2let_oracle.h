@@ -103,6 +103,19 @@ void oracle_synthesis_adjoint_lm2lmn(ocplx *f_wav_lmn, ocplx *f_scal_lm, const o
                                      const ocplx *wav_lm, const double *scal_l,
                                      const oracle_params_t *p);
 
+/* real-signal multiplies: n >= 0 storage, row n at offset sum_{n'<n} (L^2 - n'^2) */
+int oracle_flmn_size_real(int L, int N);
+int oracle_elmn2ind_real(int el, int m, int n, int L);
+int oracle_n_lmn_wav_real(const oracle_params_t *p);
+void oracle_analysis_lm2lmn_real(ocplx *f_wav_lmn, ocplx *f_scal_lm, const ocplx *flm,
+                                 const ocplx *wav_lm, const double *scal_l, const oracle_params_t *p);
+void oracle_synthesis_lmn2lm_real(ocplx *flm, const ocplx *f_wav_lmn, const ocplx *f_scal_lm,
+                                  const ocplx *wav_lm, const double *scal_l, const oracle_params_t *p);
+void oracle_analysis_adjoint_lmn2lm_real(ocplx *flm, const ocplx *f_wav_lmn, const ocplx *f_scal_lm,
+                                         const ocplx *wav_lm, const double *scal_l, const oracle_params_t *p);
+void oracle_synthesis_adjoint_lm2lmn_real(ocplx *f_wav_lmn, ocplx *f_scal_lm, const ocplx *flm,
+                                          const ocplx *wav_lm, const double *scal_l, const oracle_params_t *p);
+
 /* S2LET drivers */
 void oracle_analysis_lm2wav(ocplx *f_wav, ocplx *f_scal, const ocplx *flm, const oracle_params_t *p);
 void oracle_synthesis_wav2lm(ocplx *flm, const ocplx *f_wav, const ocplx *f_scal, const oracle_params_t *p);

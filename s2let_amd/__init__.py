@@ -81,6 +81,9 @@ for _n in ("analysis_lm2wav", "synthesis_wav2lm", "analysis_px2wav", "synthesis_
     _SIGS["s2let_" + _n] = (None, [ctypes.c_void_p] * 4)
 for _n in ("analysis_lm2lmn", "synthesis_lmn2lm", "analysis_adjoint_lmn2lm", "synthesis_adjoint_lm2lmn"):
     _SIGS["s2let_" + _n] = (None, [ctypes.c_void_p] * 6)
+for _n in ("analysis_lm2lmn_real", "synthesis_lmn2lm_real", "analysis_adjoint_lmn2lm_real",
+           "synthesis_adjoint_lm2lmn_real"):
+    _SIGS["s2let_" + _n] = (None, [ctypes.c_void_p] * 6)
 for _n in ("mw_alm2map", "mw_map2alm"):
     _SIGS["s2let_" + _n] = (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
 for _n in ("analysis_lm2wav", "synthesis_wav2lm", "analysis_adjoint_wav2lm", "synthesis_adjoint_lm2wav"):
@@ -358,10 +361,17 @@ def mw_map2alm_real(f, L):
     return flm
 
 
+def _reality(p, r):
+    """Copy of p with its reality flag set to r (sizes the Wigner-coefficient arrays)."""
+    q = Parameters.from_buffer_copy(p)
+    q.reality = int(r)
+    return q
+
+
 def analysis_lm2lmn(flm, p):
     psi, phi = tiling_wavelet(p)
     flm = _cin(flm)
-    wl, sl = np.zeros(n_lmn_wav(p), np.complex128), np.zeros(n_lm_scal(p), np.complex128)
+    wl, sl = np.zeros(n_lmn_wav(_reality(p, 0)), np.complex128), np.zeros(n_lm_scal(p), np.complex128)
     lib().s2let_analysis_lm2lmn(_ptr(wl), _ptr(sl), _ptr(flm), _ptr(psi), _ptr(phi), _ref(p))
     return wl, sl
 
@@ -385,8 +395,41 @@ def analysis_adjoint_lmn2lm(f_wav_lmn, f_scal_lm, p):
 def synthesis_adjoint_lm2lmn(flm, p):
     psi, phi = tiling_wavelet(p)
     flm = _cin(flm)
-    wl, sl = np.zeros(n_lmn_wav(p), np.complex128), np.zeros(n_lm_scal(p), np.complex128)
+    wl, sl = np.zeros(n_lmn_wav(_reality(p, 0)), np.complex128), np.zeros(n_lm_scal(p), np.complex128)
     lib().s2let_synthesis_adjoint_lm2lmn(_ptr(wl), _ptr(sl), _ptr(flm), _ptr(psi), _ptr(phi), _ref(p))
+    return wl, sl
+
+
+def analysis_lm2lmn_real(flm, p):
+    """Real-signal Wigner coefficients, n >= 0 only (s2let_analysis.c:102-160)."""
+    psi, phi = tiling_wavelet(p)
+    flm = _cin(flm)
+    wl, sl = np.zeros(n_lmn_wav(_reality(p, 1)), np.complex128), np.zeros(n_lm_scal(p), np.complex128)
+    lib().s2let_analysis_lm2lmn_real(_ptr(wl), _ptr(sl), _ptr(flm), _ptr(psi), _ptr(phi), _ref(p))
+    return wl, sl
+
+
+def synthesis_lmn2lm_real(f_wav_lmn, f_scal_lm, p):
+    psi, phi = tiling_wavelet(p)
+    a, b = _cin(f_wav_lmn), _cin(f_scal_lm)
+    flm = np.zeros(p.L * p.L, np.complex128)
+    lib().s2let_synthesis_lmn2lm_real(_ptr(flm), _ptr(a), _ptr(b), _ptr(psi), _ptr(phi), _ref(p))
+    return flm
+
+
+def analysis_adjoint_lmn2lm_real(f_wav_lmn, f_scal_lm, p):
+    psi, phi = tiling_wavelet(p)
+    a, b = _cin(f_wav_lmn), _cin(f_scal_lm)
+    flm = np.zeros(p.L * p.L, np.complex128)
+    lib().s2let_analysis_adjoint_lmn2lm_real(_ptr(flm), _ptr(a), _ptr(b), _ptr(psi), _ptr(phi), _ref(p))
+    return flm
+
+
+def synthesis_adjoint_lm2lmn_real(flm, p):
+    psi, phi = tiling_wavelet(p)
+    flm = _cin(flm)
+    wl, sl = np.zeros(n_lmn_wav(_reality(p, 1)), np.complex128), np.zeros(n_lm_scal(p), np.complex128)
+    lib().s2let_synthesis_adjoint_lm2lmn_real(_ptr(wl), _ptr(sl), _ptr(flm), _ptr(psi), _ptr(phi), _ref(p))
     return wl, sl
 
 

@@ -670,10 +670,13 @@ void s2let_mw_map2alm(std::complex<double> *flm, const std::complex<double> *f, 
 // over the per-scale compact blocks.
 static void lmn_common(int which, std::complex<double> *a, std::complex<double> *b, const std::complex<double> *c,
                        const std::complex<double> *d, const std::complex<double> *wav_lm, const double *scal_l,
-                       const s2let_parameters_t *p) {
+                       const s2let_parameters_t *pin, bool real) {
   Context &C = Context::get();
   std::lock_guard<std::mutex> lk(C.mu);
   hipStream_t st = C.stream();
+  s2let_parameters_t pp = *pin;  // the complex forms ignore reality, the real forms imply it
+  pp.reality = real ? 1 : 0;
+  const s2let_parameters_t *p = &pp;
   const int L = p->L, J = j_max(p);
   const size_t nlmn = (size_t)s2let_n_lmn_wav(p), nls = (size_t)s2let_n_lm_scal(p), nlm = (size_t)L * L;
   const int Ls = scal_bandlimit(p);
@@ -701,12 +704,22 @@ static void lmn_common(int which, std::complex<double> *a, std::complex<double> 
     scale_shape(p, j, &Lj, &Nj);
     const void *psi_j = (const char *)dpsi.p + (size_t)j * L * L * CB;
     void *blk = (char *)dw.p + off * CB;
-    if (which == 0) launch_lm2lmn(blk, dflm.p, psi_j, Lj, Nj, p->spin, 0, st);
-    if (which == 3) launch_lm2lmn(blk, dflm.p, psi_j, Lj, Nj, p->spin, 1, st);
-    if (which == 1) launch_lmn2lm(dflm.p, blk, psi_j, Lj, Nj, p->spin, 0, st);
-    if (which == 2) launch_lmn2lm(dflm.p, blk, psi_j, Lj, Nj, p->spin, 1, st);
-    off += (size_t)flmn_size(Lj, Nj);
+    if (real) {
+      if (which == 0) launch_lm2lmn_real(blk, dflm.p, psi_j, Lj, Nj, 0, st);
+      if (which == 3) launch_lm2lmn_real(blk, dflm.p, psi_j, Lj, Nj, 1, st);
+      if (which == 1) launch_lmn2lm_real(dflm.p, blk, psi_j, Lj, Nj, 0, st);
+      if (which == 2) launch_lmn2lm_real(dflm.p, blk, psi_j, Lj, Nj, 1, st);
+      off += (size_t)flmn_size_real(Lj, Nj);
+    } else {
+      if (which == 0) launch_lm2lmn(blk, dflm.p, psi_j, Lj, Nj, p->spin, 0, st);
+      if (which == 3) launch_lm2lmn(blk, dflm.p, psi_j, Lj, Nj, p->spin, 1, st);
+      if (which == 1) launch_lmn2lm(dflm.p, blk, psi_j, Lj, Nj, p->spin, 0, st);
+      if (which == 2) launch_lmn2lm(dflm.p, blk, psi_j, Lj, Nj, p->spin, 1, st);
+      off += (size_t)flmn_size(Lj, Nj);
+    }
   }
+  // the real forms run the scaling multiply from l = 0 (s2let_analysis.c:155-160): phi vanishes
+  // below |spin|, so starting at |spin| gives the same values
   if (to_lmn) {
     launch_scal_mult(ds.p, dflm.p, dphi.as<double>(), Ls, p->spin, 0, st);
     d2h(a, dw.p, nlmn * CB, st);
@@ -722,24 +735,46 @@ static void lmn_common(int which, std::complex<double> *a, std::complex<double> 
 void s2let_analysis_lm2lmn(std::complex<double> *f_wav_lmn, std::complex<double> *f_scal_lm,
                            const std::complex<double> *flm, const std::complex<double> *wav_lm, const double *scal_l,
                            const s2let_parameters_t *p) {
-  host_call(__func__, [&] { lmn_common(0, f_wav_lmn, f_scal_lm, flm, nullptr, wav_lm, scal_l, p); });
+  host_call(__func__, [&] { lmn_common(0, f_wav_lmn, f_scal_lm, flm, nullptr, wav_lm, scal_l, p, false); });
 }
 void s2let_synthesis_lmn2lm(std::complex<double> *flm, const std::complex<double> *f_wav_lmn,
                             const std::complex<double> *f_scal_lm, const std::complex<double> *wav_lm,
                             const double *scal_l, const s2let_parameters_t *p) {
-  host_call(__func__, [&] { lmn_common(1, flm, nullptr, f_wav_lmn, f_scal_lm, wav_lm, scal_l, p); });
+  host_call(__func__, [&] { lmn_common(1, flm, nullptr, f_wav_lmn, f_scal_lm, wav_lm, scal_l, p, false); });
 }
 void s2let_analysis_adjoint_lmn2lm(std::complex<double> *flm, const std::complex<double> *f_wav_lmn,
                                    const std::complex<double> *f_scal_lm, const std::complex<double> *wav_lm,
                                    const double *scal_l, const s2let_parameters_t *p) {
-  host_call(__func__, [&] { lmn_common(2, flm, nullptr, f_wav_lmn, f_scal_lm, wav_lm, scal_l, p); });
+  host_call(__func__, [&] { lmn_common(2, flm, nullptr, f_wav_lmn, f_scal_lm, wav_lm, scal_l, p, false); });
 }
 void s2let_synthesis_adjoint_lm2lmn(std::complex<double> *f_wav_lmn, std::complex<double> *f_scal_lm,
                                     const std::complex<double> *flm, const std::complex<double> *wav_lm,
                                     const double *scal_l, const s2let_parameters_t *p) {
-  host_call(__func__, [&] { lmn_common(3, f_wav_lmn, f_scal_lm, flm, nullptr, wav_lm, scal_l, p); });
+  host_call(__func__, [&] { lmn_common(3, f_wav_lmn, f_scal_lm, flm, nullptr, wav_lm, scal_l, p, false); });
 }
 
+// real-signal storage: n >= 0 only (s2let_analysis.c:102, s2let_synthesis.c:101,
+// s2let_analysis_adjoint.c:101, s2let_synthesis_adjoint.c:103)
+void s2let_analysis_lm2lmn_real(std::complex<double> *f_wav_lmn, std::complex<double> *f_scal_lm,
+                                const std::complex<double> *flm, const std::complex<double> *wav_lm,
+                                const double *scal_l, const s2let_parameters_t *p) {
+  host_call(__func__, [&] { lmn_common(0, f_wav_lmn, f_scal_lm, flm, nullptr, wav_lm, scal_l, p, true); });
+}
+void s2let_synthesis_lmn2lm_real(std::complex<double> *flm, const std::complex<double> *f_wav_lmn,
+                                 const std::complex<double> *f_scal_lm, const std::complex<double> *wav_lm,
+                                 const double *scal_l, const s2let_parameters_t *p) {
+  host_call(__func__, [&] { lmn_common(1, flm, nullptr, f_wav_lmn, f_scal_lm, wav_lm, scal_l, p, true); });
+}
+void s2let_analysis_adjoint_lmn2lm_real(std::complex<double> *flm, const std::complex<double> *f_wav_lmn,
+                                        const std::complex<double> *f_scal_lm, const std::complex<double> *wav_lm,
+                                        const double *scal_l, const s2let_parameters_t *p) {
+  host_call(__func__, [&] { lmn_common(2, flm, nullptr, f_wav_lmn, f_scal_lm, wav_lm, scal_l, p, true); });
+}
+void s2let_synthesis_adjoint_lm2lmn_real(std::complex<double> *f_wav_lmn, std::complex<double> *f_scal_lm,
+                                         const std::complex<double> *flm, const std::complex<double> *wav_lm,
+                                         const double *scal_l, const s2let_parameters_t *p) {
+  host_call(__func__, [&] { lmn_common(3, f_wav_lmn, f_scal_lm, flm, nullptr, wav_lm, scal_l, p, true); });
+}
 
 // ---------------------------------------------------------------- real signals (host pointers, FP64)
 constexpr size_t DB = sizeof(double);

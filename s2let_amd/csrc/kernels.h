@@ -172,5 +172,8 @@ void launch_mix(const MixJobs &jobs, hipStream_t st);
 void launch_lm2lmn(void *wlmn, const void *flm, const void *psi_j, int Lj, int Nj, int spin, int mode, hipStream_t st);
 void launch_lmn2lm(void *flm, const void *wlmn, const void *psi_j, int Lj, int Nj, int spin, int mode, hipStream_t st);
 void launch_scal_mult(void *a, const void *b, const double *phi, int Ls, int spin, int add, hipStream_t st);
+// real-signal storage (n >= 0 rows only): mode 0 analysis / synthesis, 1 the adjoints
+void launch_lm2lmn_real(void *wlmn, const void *flm, const void *psi_j, int Lj, int Nj, int mode, hipStream_t st);
+void launch_lmn2lm_real(void *flm, const void *wlmn, const void *psi_j, int Lj, int Nj, int mode, hipStream_t st);
 
 }  // namespace s2h

@@ -71,6 +71,60 @@ __global__ void k_lmn2lm(cx<double> *__restrict__ flm, const cx<double> *__restr
   }
 }
 
+// Real signals: only n >= 0 is stored, row n (of L_j^2 - n^2 entries, l >= n)
+// at offset n L_j^2 - (n-1) n (2n-1) / 6.
+__device__ __host__ inline int real_row_offset(int n, int Lj) { return n * Lj * Lj - (n - 1) * n * (2 * n - 1) / 6; }
+
+// mode 0: W = flm * 8pi^2/(2l+1) conj(psi) (s2let_analysis.c:139-148); mode 1:
+// W = flm * conj(psi) (s2let_synthesis_adjoint.c:139-148); n >= 0 with N's parity.
+__global__ void k_lm2lmn_real(cx<double> *__restrict__ wlmn, const cx<double> *__restrict__ flm,
+                              const cx<double> *__restrict__ psi_j, int Lj, int Nj, int mode) {
+  for (int n = 1 - (Nj & 1); n < Nj && n < Lj; n += 2) {
+    const int off = real_row_offset(n, Lj);
+    const int cnt = Lj * Lj - n * n;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += gridDim.x * blockDim.x) {
+      const int q = e + n * n;
+      int el = (int)sqrt((double)q);
+      while (el * el > q) --el;
+      while ((el + 1) * (el + 1) <= q) ++el;
+      const int m = q - el * el - el;
+      const cx<double> ps = psi_j[el * el + el + n];
+      cx<double> pc = mk<double>(ps.x, -ps.y);
+      if (mode == 0) pc = cscale(pc, 8 * M_PI * M_PI / (2 * el + 1));
+      wlmn[off + e] = cmul(flm[el * el + el + m], pc);
+    }
+  }
+}
+
+// flm[l][m] += sum_{n >= 0} W[l,m,n] f(psi_n) + [n > 0] (-1)^{m+n} conj(W[l,-m,n]) f(psi_{-n});
+// mode 0: f = identity (s2let_synthesis.c:139-160), mode 1: f = 8pi^2/(2l+1) x
+// (s2let_analysis_adjoint.c:139-160).
+__global__ void k_lmn2lm_real(cx<double> *__restrict__ flm, const cx<double> *__restrict__ wlmn,
+                              const cx<double> *__restrict__ psi_j, int Lj, int Nj, int mode) {
+  const int cnt = Lj * Lj;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
+    int el = (int)sqrt((double)q);
+    while (el * el > q) --el;
+    while ((el + 1) * (el + 1) <= q) ++el;
+    const int m = q - el * el - el;
+    cx<double> acc = flm[q];
+    for (int n = 1 - (Nj & 1); n < Nj && n <= el; n += 2) {
+      const int base = real_row_offset(n, Lj) + el * el - n * n + el;
+      cx<double> ps = psi_j[el * el + el + n];
+      if (mode == 1) ps = cscale(ps, 8 * M_PI * M_PI / (2 * el + 1));
+      acc = cadd(acc, cmul(wlmn[base + m], ps));
+      if (n) {
+        cx<double> nps = psi_j[el * el + el - n];
+        if (mode == 1) nps = cscale(nps, 8 * M_PI * M_PI / (2 * el + 1));
+        const cx<double> w = wlmn[base - m];
+        const double sg = ((m + n) & 1) ? -1.0 : 1.0;
+        acc = cadd(acc, cmul(mk<double>(sg * w.x, sg * -w.y), nps));
+      }
+    }
+    flm[q] = acc;
+  }
+}
+
 // f_scal_lm = flm * sqrt(4 pi/(2l+1)) * phi_l (l in [|s|, Ls)), or flm += f_scal_lm * (...) when add
 __global__ void k_scal_mult(cx<double> *__restrict__ a, const cx<double> *__restrict__ b, const double *__restrict__ phi,
                             int Ls, int spin, int add) {
@@ -93,6 +147,14 @@ void launch_lm2lmn(void *wlmn, const void *flm, const void *psi_j, int Lj, int N
 void launch_lmn2lm(void *flm, const void *wlmn, const void *psi_j, int Lj, int Nj, int spin, int mode, hipStream_t st) {
   hipLaunchKernelGGL(k_lmn2lm, dim3(grid1d((size_t)Lj * Lj)), dim3(256), 0, st, (cx<double> *)flm,
                      (const cx<double> *)wlmn, (const cx<double> *)psi_j, Lj, Nj, spin, mode);
+}
+void launch_lm2lmn_real(void *wlmn, const void *flm, const void *psi_j, int Lj, int Nj, int mode, hipStream_t st) {
+  hipLaunchKernelGGL(k_lm2lmn_real, dim3(grid1d((size_t)Lj * Lj)), dim3(256), 0, st, (cx<double> *)wlmn,
+                     (const cx<double> *)flm, (const cx<double> *)psi_j, Lj, Nj, mode);
+}
+void launch_lmn2lm_real(void *flm, const void *wlmn, const void *psi_j, int Lj, int Nj, int mode, hipStream_t st) {
+  hipLaunchKernelGGL(k_lmn2lm_real, dim3(grid1d((size_t)Lj * Lj)), dim3(256), 0, st, (cx<double> *)flm,
+                     (const cx<double> *)wlmn, (const cx<double> *)psi_j, Lj, Nj, mode);
 }
 void launch_scal_mult(void *a, const void *b, const double *phi, int Ls, int spin, int add, hipStream_t st) {
   hipLaunchKernelGGL(k_scal_mult, dim3(grid1d((size_t)Ls * Ls)), dim3(256), 0, st, (cx<double> *)a,

@@ -185,6 +185,8 @@ void tiling_wavelet(cd *psi, double *phi, const s2let_parameters_t *p) {
 }
 
 int flmn_size(int L, int N) { return ((2 * N - 1) * (3 * L * L - N * (N - 1))) / 3; }
+// n >= 0 rows only (real signals): sum_{n<N} (L^2 - n^2)
+int flmn_size_real(int L, int N) { return N * L * L - (N - 1) * N * (2 * N - 1) / 6; }
 int f_size(int L, int N) { return L * (2 * L - 1) * (2 * N - 1); }
 
 int scal_bandlimit(const s2let_parameters_t *p) {
@@ -288,7 +290,7 @@ int s2let_n_lmn_wav(const s2let_parameters_t *p) {
       bl = std::min(bandlimit(j, p), p->L);
       n = std::min(p->N, bl);  // not parity adjusted (s2let_helper.c:51)
     }
-    total += flmn_size(bl, n);
+    total += p->reality ? flmn_size_real(bl, n) : flmn_size(bl, n);  // SO3 storage follows reality
   }
   return total;
 }

@@ -21,6 +21,7 @@ void tiling_direction(cd *s_elm, const s2let_parameters_t *p);
 void tiling_wavelet(cd *psi, double *phi, const s2let_parameters_t *p);
 
 int flmn_size(int L, int N);
+int flmn_size_real(int L, int N);
 int f_size(int L, int N);
 int scal_bandlimit(const s2let_parameters_t *p);
 void scale_shape(const s2let_parameters_t *p, int j, int *Lj, int *Nj);

@@ -48,6 +48,8 @@ def test_tiling_and_sizes_match_oracle(s2, oracle, L, N, B, J_min, spin, up):
     assert s2.n_scal(p) == oracle.n_scal(q)
     assert s2.n_lmn_wav(p) == oracle.n_lmn_wav(q)
     assert s2.n_lm_scal(p) == oracle.n_lm_scal(q)
+    # real signals: the reality flag selects the n >= 0 storage (s2let_so3.h:33 -> SO3 size)
+    assert s2.n_lmn_wav(s2._reality(p, 1)) == oracle.n_lmn_wav_real(q)
 
 
 @pytest.mark.parametrize("wavtype", [2, 3])

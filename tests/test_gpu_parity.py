@@ -106,6 +106,23 @@ def test_lmn_api_matches_oracle(s2, oracle, L, N, J_min, spin, up):
     assert maxrel(s2.analysis_adjoint_lmn2lm(wl, sl, p), oracle.analysis_adjoint_lmn2lm(wl, sl, q)) < 1e-15
 
 
+@pytest.mark.parametrize("L,N,J_min,spin,up", CASES)
+def test_real_lmn_api_matches_oracle(s2, oracle, L, N, J_min, spin, up):
+    """Real-signal multiplies (n >= 0 storage), bit for bit."""
+    p, q = s2.parameters(L, N, 2.0, J_min, spin, up), oracle.params(L, N, 2.0, J_min, spin, up)
+    rng = np.random.default_rng(6)
+    flm = crandn(rng, L * L)
+    a, b = s2.analysis_lm2lmn_real(flm, p)
+    c, d = oracle.analysis_lm2lmn_real(flm, q)
+    assert np.array_equal(a, c) and np.array_equal(b, d)
+    a, b = s2.synthesis_adjoint_lm2lmn_real(flm, p)
+    c, d = oracle.synthesis_adjoint_lm2lmn_real(flm, q)
+    assert np.array_equal(a, c) and np.array_equal(b, d)
+    wl, sl = crandn(rng, a.size), crandn(rng, b.size)
+    assert np.array_equal(s2.synthesis_lmn2lm_real(wl, sl, p), oracle.synthesis_lmn2lm_real(wl, sl, q))
+    assert np.array_equal(s2.analysis_adjoint_lmn2lm_real(wl, sl, p), oracle.analysis_adjoint_lmn2lm_real(wl, sl, q))
+
+
 @pytest.mark.parametrize("L,spin", [(1, 0), (2, 0), (9, 1), (33, -2), (128, 2)])
 def test_mw_sht_matches_oracle(s2, oracle, L, spin):
     rng = np.random.default_rng(L + spin)

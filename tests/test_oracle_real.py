@@ -165,3 +165,50 @@ def test_hermitian_analysis_is_real(oracle, L, N, J_min, spin, up):
     # a non-Hermitian flm is not: the property tests something
     fw2, _ = oracle.analysis_lm2wav(crandn(rng, L * L), p)
     assert np.abs(fw2.imag).max() > 1e-3 * np.abs(fw2).max()
+
+
+def to_half(oracle, g, p):
+    """n >= 0 rows of complex-layout Wigner coefficients, in the real-signal layout."""
+    out, off_r = [], 0
+    h = np.zeros(oracle.n_lmn_wav_real(p), complex)
+    for off, Lj, Nj in scales(oracle, p):
+        for n in range(0, Nj):
+            for l in range(n, Lj):
+                for m in range(-l, l + 1):
+                    h[off_r + oracle.elmn2ind_real(l, m, n, Lj)] = g[off + oracle.elmn2ind(l, m, n, Lj, Nj)]
+        off_r += oracle.flmn_size_real(Lj, Nj)
+    assert off_r == h.size
+    return h
+
+
+@pytest.mark.parametrize("L,N", [(8, 1), (8, 4), (9, 5), (5, 5), (5, 6)])
+def test_real_storage_index(oracle, L, N):
+    """The n >= 0 compact order enumerates 0 .. size-1 once; size = sum_{n<N} (L^2 - n^2)."""
+    seen = sorted(oracle.elmn2ind_real(l, m, n, L) for n in range(min(N, L)) for l in range(n, L)
+                  for m in range(-l, l + 1))
+    assert seen == list(range(oracle.flmn_size_real(L, N)))
+    assert oracle.flmn_size_real(L, N) == sum(L * L - n * n for n in range(min(N, L)))
+
+
+@pytest.mark.parametrize("L,N,J_min,spin,up", STATES)
+def test_real_lmn_multiplies(oracle, L, N, J_min, spin, up):
+    """The real multiplies (s2let_analysis.c:102-160 and siblings, in the oracle's C)
+    against the complex ones: the to-lmn forms are the n >= 0 rows bit for bit; the
+    to-lm forms equal the complex sums on coefficients of a real signal."""
+    p = oracle.params(L, N, 2.0, J_min, spin, up)
+    rng = np.random.default_rng(5 * L + N)
+    flm = hermitian_flm(rng, L)
+    flm[:spin * spin] = 0
+    a, b = oracle.analysis_lm2lmn_real(flm, p)
+    c, d = oracle.analysis_lm2lmn(flm, p)
+    assert np.array_equal(a, to_half(oracle, c, p)) and np.array_equal(b, d)
+    a, b = oracle.synthesis_adjoint_lm2lmn_real(flm, p)
+    c, d = oracle.synthesis_adjoint_lm2lmn(flm, p)
+    assert np.array_equal(a, to_half(oracle, c, p)) and np.array_equal(b, d)
+    g = symmetrise(oracle, crandn(rng, oracle.n_lmn_wav(p)), p)
+    s = crandn(rng, oracle.n_lm_scal(p))
+    for real_fn, cx_fn in ((oracle.synthesis_lmn2lm_real, oracle.synthesis_lmn2lm),
+                           (oracle.analysis_adjoint_lmn2lm_real, oracle.analysis_adjoint_lmn2lm)):
+        want = cx_fn(g, s, p)
+        got = real_fn(to_half(oracle, g, p), s, p)
+        assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
