@@ -160,6 +160,34 @@ void s2let_synthesis_adjoint_px2wav(S2LET_COMPLEX(double) *f_wav, S2LET_COMPLEX(
                                     const S2LET_COMPLEX(double) *f,
                                     const s2let_parameters_t *parameters); /* s2let_synthesis_adjoint.c:465 */
 
+/* ---- manual tiling: user wavelets wav_lm ((J+1) L^2, scales j = 0..J), scaling
+ * weights scal_l (L), band-limits min(wav_bandlimits[j], L) and min(scal_bandlimit, L),
+ * N_j = min(N, L_j) with N's parity, lower band-limit 0.  f_wav holds
+ * sum_j L_j (2L_j - 1)(2N_j - 1) samples, f_scal Ls (2Ls - 1).  The to-flm forms add
+ * into flm (they do not clear it).  The two manual adjoints weight with conj(wav_lm)
+ * and wav_lm respectively, as the reference does. ---- */
+void s2let_analysis_lm2wav_manual(S2LET_COMPLEX(double) *f_wav, S2LET_COMPLEX(double) *f_scal,
+                                  const S2LET_COMPLEX(double) *flm, const double *scal_l,
+                                  const S2LET_COMPLEX(double) *wav_lm, const int scal_bandlimit,
+                                  const int *wav_bandlimits, int J, int L, int spin, int N); /* s2let_analysis.c:188 */
+void s2let_synthesis_wav2lm_manual(S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *f_wav,
+                                   const S2LET_COMPLEX(double) *f_scal, const double *scal_l,
+                                   const S2LET_COMPLEX(double) *wav_lm, const int scal_bandlimit,
+                                   const int *wav_bandlimits, int J, int L, int spin, int N); /* s2let_synthesis.c:202 */
+void s2let_analysis_adjoint_wav2lm_manual(S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *f_wav,
+                                          const S2LET_COMPLEX(double) *f_scal, const double *scal_l,
+                                          const S2LET_COMPLEX(double) *wav_lm, const int scal_bandlimit,
+                                          const int *wav_bandlimits, int J, int L, int spin,
+                                          int N); /* s2let_analysis_adjoint.c:202 */
+void s2let_synthesis_adjoint_lm2wav_manual(S2LET_COMPLEX(double) *f_wav, S2LET_COMPLEX(double) *f_scal,
+                                           const S2LET_COMPLEX(double) *flm, const double *scal_l,
+                                           const S2LET_COMPLEX(double) *wav_lm, const int scal_bandlimit,
+                                           const int *wav_bandlimits, int J, int L, int spin,
+                                           int N); /* s2let_synthesis_adjoint.c:189 */
+void s2let_allocate_f_wav_manual(S2LET_COMPLEX(double) **f_wav, S2LET_COMPLEX(double) **f_scal, int *wav_bandlimits,
+                                 int scal_bandlimit, int N, int J,
+                                 const s2let_parameters_t *parameters); /* s2let_alloc.c:148 */
+
 /* ---- real signals.  Real maps and real wavelet coefficients (double), the
  * harmonic coefficients of a real signal (complex, f_{l,-m} = (-1)^m conj(f_lm));
  * array sizes as the complex variants.  Pixel-space variants use spin-0 MW

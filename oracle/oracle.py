@@ -293,6 +293,55 @@ def analysis_adjoint_lmn2lm_real(f_wav_lmn, f_scal_lm, p):
     return flm
 
 
+class Manual:
+    """A manual tiling: wavelets wav_lm ((J+1) L^2), scal_l (L), band-limits."""
+
+    def __init__(self, wav_lm, scal_l, scal_bl, wav_bl, L, spin, N):
+        self.wav_lm = np.ascontiguousarray(wav_lm, dtype=np.complex128)
+        self.scal_l = np.ascontiguousarray(scal_l, dtype=np.float64)
+        self.wav_bl = np.ascontiguousarray(wav_bl, dtype=np.int32)
+        self.scal_bl, self.L, self.spin, self.N = int(scal_bl), int(L), int(spin), int(N)
+        self.J = len(self.wav_bl) - 1
+
+    def args(self):
+        return (_p(self.scal_l), _p(self.wav_lm), self.scal_bl, _p(self.wav_bl), self.J, self.L, self.spin, self.N)
+
+    def n_wav(self):
+        return lib().oracle_n_wav_manual(_p(self.wav_bl), self.J, self.L, self.N)
+
+    def n_scal(self):
+        b = min(self.scal_bl, self.L)
+        return b * (2 * b - 1)
+
+
+def analysis_lm2wav_manual(flm, m):
+    fw, fs = _c(m.n_wav()), _c(m.n_scal())
+    flm = _in(flm)
+    lib().oracle_analysis_lm2wav_manual(_p(fw), _p(fs), _p(flm), *m.args())
+    return fw, fs
+
+
+def synthesis_adjoint_lm2wav_manual(flm, m):
+    fw, fs = _c(m.n_wav()), _c(m.n_scal())
+    flm = _in(flm)
+    lib().oracle_synthesis_adjoint_lm2wav_manual(_p(fw), _p(fs), _p(flm), *m.args())
+    return fw, fs
+
+
+def synthesis_wav2lm_manual(f_wav, f_scal, m, flm=None):
+    out = _c(m.L * m.L) if flm is None else np.array(flm, dtype=np.complex128)
+    a, b = _in(f_wav), _in(f_scal)
+    lib().oracle_synthesis_wav2lm_manual(_p(out), _p(a), _p(b), *m.args())
+    return out
+
+
+def analysis_adjoint_wav2lm_manual(f_wav, f_scal, m, flm=None):
+    out = _c(m.L * m.L) if flm is None else np.array(flm, dtype=np.complex128)
+    a, b = _in(f_wav), _in(f_scal)
+    lib().oracle_analysis_adjoint_wav2lm_manual(_p(out), _p(a), _p(b), *m.args())
+    return out
+
+
 def _wav_pair(p):
     return _c(n_wav(p)), _c(n_scal(p))
 

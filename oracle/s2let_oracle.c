@@ -1314,6 +1314,110 @@ void oracle_analysis_adjoint_lmn2lm_real(ocplx *flm, const ocplx *f_wav_lmn, con
   or_lmn2lm_real(flm, f_wav_lmn, f_scal_lm, wav_lm, scal_l, p, 1);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Manual tiling (user wavelets and band-limits, scales j = 0..J, L0 = 0)    */
+/* ------------------------------------------------------------------------ */
+
+static void or_manual_scale(const int *wav_bl, int j, int L, int N, int *Lj, int *Nj) {
+  *Lj = OMIN(wav_bl[j], L);
+  int n = OMIN(N, *Lj);
+  n += (n + N) % 2;
+  *Nj = n;
+}
+
+/* s2let_analysis.c:188-276 (kind 0) and s2let_synthesis_adjoint.c:189-290 (kind 1: psi
+   unconjugated, no 8pi^2/(2l+1), adjoint transforms) */
+static void or_manual_to_wav(int kind, ocplx *f_wav, ocplx *f_scal, const ocplx *flm, const double *scal_l,
+                             const ocplx *wav_lm, int scal_bl, const int *wav_bl, int J, int L, int spin, int N) {
+  int bl = OMIN(scal_bl, L);
+  ocplx *f_scal_lm = (ocplx *)calloc((size_t)bl * bl, sizeof(ocplx));
+  for (int el = OABS(spin); el < bl; ++el) {
+    double phi = sqrt(4.0 * OPI / (2 * el + 1)) * scal_l[el];
+    for (int m = -el; m <= el; ++m) f_scal_lm[el * el + el + m] = flm[el * el + el + m] * phi;
+  }
+  if (kind == 0) oracle_ssht_inverse(f_scal, f_scal_lm, bl, 0);
+  else oracle_ssht_adjoint_forward(f_scal, f_scal_lm, bl, 0);
+  free(f_scal_lm);
+  int off = 0;
+  for (int j = 0; j <= J; ++j) {
+    int Lj, Nj;
+    or_manual_scale(wav_bl, j, L, N, &Lj, &Nj);
+    ocplx *flmn = (ocplx *)calloc(oracle_flmn_size(Lj, Nj), sizeof(ocplx));
+    for (int n = -Nj + 1; n < Nj; n += 2)
+      for (int el = OMAX(OABS(spin), OABS(n)); el < Lj; ++el) {
+        ocplx w = wav_lm[(size_t)j * L * L + el * el + el + n];
+        ocplx psi = kind == 0 ? 8 * OPI * OPI / (2 * el + 1) * conj(w) : w;
+        for (int m = -el; m <= el; ++m) flmn[oracle_elmn2ind(el, m, n, Lj, Nj)] = flm[el * el + el + m] * psi;
+      }
+    if (kind == 0) oracle_so3_inverse(f_wav + off, flmn, Lj, Nj, 0, or_nmode(N));
+    else oracle_so3_adjoint_forward(f_wav + off, flmn, Lj, Nj, 0, or_nmode(N));
+    free(flmn);
+    off += or_f_size(Lj, Nj);
+  }
+}
+
+/* s2let_synthesis.c:202-289 (kind 0) and s2let_analysis_adjoint.c:202-290 (kind 1: psi
+   conjugated with 8pi^2/(2l+1), adjoint transforms); both accumulate into flm */
+static void or_manual_to_lm(int kind, ocplx *flm, const ocplx *f_wav, const ocplx *f_scal, const double *scal_l,
+                            const ocplx *wav_lm, int scal_bl, const int *wav_bl, int J, int L, int spin, int N) {
+  int bl = OMIN(scal_bl, L);
+  ocplx *f_scal_lm = (ocplx *)calloc((size_t)bl * bl, sizeof(ocplx));
+  if (kind == 0) oracle_ssht_forward(f_scal_lm, f_scal, bl, 0);
+  else oracle_ssht_adjoint_inverse(f_scal_lm, f_scal, bl, 0);
+  for (int el = OABS(spin); el < bl; ++el) {
+    double phi = sqrt(4.0 * OPI / (2 * el + 1)) * scal_l[el];
+    for (int m = -el; m <= el; ++m) flm[el * el + el + m] += f_scal_lm[el * el + el + m] * phi;
+  }
+  free(f_scal_lm);
+  int off = 0;
+  for (int j = 0; j <= J; ++j) {
+    int Lj, Nj;
+    or_manual_scale(wav_bl, j, L, N, &Lj, &Nj);
+    ocplx *flmn = (ocplx *)calloc(oracle_flmn_size(Lj, Nj), sizeof(ocplx));
+    if (kind == 0) oracle_so3_forward(flmn, f_wav + off, Lj, Nj, 0, or_nmode(N));
+    else oracle_so3_adjoint_inverse(flmn, f_wav + off, Lj, Nj, 0, or_nmode(N));
+    for (int n = -Nj + 1; n < Nj; n += 2)
+      for (int el = OMAX(OABS(spin), OABS(n)); el < Lj; ++el) {
+        ocplx w = wav_lm[(size_t)j * L * L + el * el + el + n];
+        ocplx psi = kind == 0 ? w : 8 * OPI * OPI / (2 * el + 1) * conj(w);
+        for (int m = -el; m <= el; ++m) flm[el * el + el + m] += flmn[oracle_elmn2ind(el, m, n, Lj, Nj)] * psi;
+      }
+    free(flmn);
+    off += or_f_size(Lj, Nj);
+  }
+}
+
+void oracle_analysis_lm2wav_manual(ocplx *f_wav, ocplx *f_scal, const ocplx *flm, const double *scal_l,
+                                   const ocplx *wav_lm, int scal_bl, const int *wav_bl, int J, int L, int spin,
+                                   int N) {
+  or_manual_to_wav(0, f_wav, f_scal, flm, scal_l, wav_lm, scal_bl, wav_bl, J, L, spin, N);
+}
+void oracle_synthesis_adjoint_lm2wav_manual(ocplx *f_wav, ocplx *f_scal, const ocplx *flm, const double *scal_l,
+                                            const ocplx *wav_lm, int scal_bl, const int *wav_bl, int J, int L,
+                                            int spin, int N) {
+  or_manual_to_wav(1, f_wav, f_scal, flm, scal_l, wav_lm, scal_bl, wav_bl, J, L, spin, N);
+}
+void oracle_synthesis_wav2lm_manual(ocplx *flm, const ocplx *f_wav, const ocplx *f_scal, const double *scal_l,
+                                    const ocplx *wav_lm, int scal_bl, const int *wav_bl, int J, int L, int spin,
+                                    int N) {
+  or_manual_to_lm(0, flm, f_wav, f_scal, scal_l, wav_lm, scal_bl, wav_bl, J, L, spin, N);
+}
+void oracle_analysis_adjoint_wav2lm_manual(ocplx *flm, const ocplx *f_wav, const ocplx *f_scal, const double *scal_l,
+                                           const ocplx *wav_lm, int scal_bl, const int *wav_bl, int J, int L,
+                                           int spin, int N) {
+  or_manual_to_lm(1, flm, f_wav, f_scal, scal_l, wav_lm, scal_bl, wav_bl, J, L, spin, N);
+}
+/* the f_wav samples the manual transforms write: sum_j L_j (2L_j - 1)(2N_j - 1) */
+int oracle_n_wav_manual(const int *wav_bl, int J, int L, int N) {
+  int tot = 0;
+  for (int j = 0; j <= J; ++j) {
+    int Lj, Nj;
+    or_manual_scale(wav_bl, j, L, N, &Lj, &Nj);
+    tot += or_f_size(Lj, Nj);
+  }
+  return tot;
+}
+
 /* s2let_transform_axisym_mw.c (multires analysis) with s2let_transform_axisym_lm.c kernels */
 void oracle_axisym_wav_analysis_mw_multires(ocplx *f_wav, ocplx *f_scal, const ocplx *f, const oracle_params_t *p) {
   int L = p->L, J = oracle_j_max(p);

@@ -116,6 +116,19 @@ void oracle_analysis_adjoint_lmn2lm_real(ocplx *flm, const ocplx *f_wav_lmn, con
 void oracle_synthesis_adjoint_lm2lmn_real(ocplx *f_wav_lmn, ocplx *f_scal_lm, const ocplx *flm,
                                           const ocplx *wav_lm, const double *scal_l, const oracle_params_t *p);
 
+/* manual tiling: scales j = 0..J, user wavelets wav_lm ((J+1) L^2) and scal_l (L) */
+void oracle_analysis_lm2wav_manual(ocplx *f_wav, ocplx *f_scal, const ocplx *flm, const double *scal_l,
+                                   const ocplx *wav_lm, int scal_bl, const int *wav_bl, int J, int L, int spin, int N);
+void oracle_synthesis_wav2lm_manual(ocplx *flm, const ocplx *f_wav, const ocplx *f_scal, const double *scal_l,
+                                    const ocplx *wav_lm, int scal_bl, const int *wav_bl, int J, int L, int spin, int N);
+void oracle_analysis_adjoint_wav2lm_manual(ocplx *flm, const ocplx *f_wav, const ocplx *f_scal,
+                                           const double *scal_l, const ocplx *wav_lm, int scal_bl,
+                                           const int *wav_bl, int J, int L, int spin, int N);
+void oracle_synthesis_adjoint_lm2wav_manual(ocplx *f_wav, ocplx *f_scal, const ocplx *flm, const double *scal_l,
+                                            const ocplx *wav_lm, int scal_bl, const int *wav_bl, int J, int L,
+                                            int spin, int N);
+int oracle_n_wav_manual(const int *wav_bl, int J, int L, int N);
+
 /* S2LET drivers */
 void oracle_analysis_lm2wav(ocplx *f_wav, ocplx *f_scal, const ocplx *flm, const oracle_params_t *p);
 void oracle_synthesis_wav2lm(ocplx *flm, const ocplx *f_wav, const ocplx *f_scal, const oracle_params_t *p);

@@ -100,6 +100,10 @@ for _n in ("mw_alm2map_real", "mw_map2alm_real"):
 for _n in ("analysis_lm2wav_real", "synthesis_wav2lm_real", "analysis_adjoint_wav2lm_real",
            "synthesis_adjoint_lm2wav_real"):
     _SIGS["s2let_hip_" + _n] = (ctypes.c_int, [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_void_p])
+for _n in ("analysis_lm2wav_manual", "synthesis_wav2lm_manual", "analysis_adjoint_wav2lm_manual",
+           "synthesis_adjoint_lm2wav_manual"):
+    _SIGS["s2let_" + _n] = (None, [ctypes.c_void_p] * 5 + [ctypes.c_int, ctypes.c_void_p] + [ctypes.c_int] * 4)
+_SIGS["s2let_allocate_f_wav_manual"] = (None, [ctypes.c_void_p] * 3 + [ctypes.c_int] * 3 + [ctypes.c_void_p])
 _SIGS["s2let_hip_selftest"] = (ctypes.c_int, [])
 _SIGS["s2let_hip_timing"] = (None, [ctypes.c_int])
 _SIGS["s2let_hip_timing_report"] = (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int])
@@ -431,6 +435,66 @@ def synthesis_adjoint_lm2lmn_real(flm, p):
     wl, sl = np.zeros(n_lmn_wav(_reality(p, 1)), np.complex128), np.zeros(n_lm_scal(p), np.complex128)
     lib().s2let_synthesis_adjoint_lm2lmn_real(_ptr(wl), _ptr(sl), _ptr(flm), _ptr(psi), _ptr(phi), _ref(p))
     return wl, sl
+
+
+# ----------------------------------------------------------------------------- manual tiling
+class ManualTiling:
+    """User wavelets wav_lm ((J+1) L^2), scaling weights scal_l (L) and band-limits for
+    the *_manual entry points (s2let_analysis.c:188-276 and siblings)."""
+
+    def __init__(self, wav_lm, scal_l, scal_bandlimit, wav_bandlimits, L, spin, N):
+        self.wav_lm = np.ascontiguousarray(wav_lm, dtype=np.complex128)
+        self.scal_l = np.ascontiguousarray(scal_l, dtype=np.float64)
+        self.wav_bl = np.ascontiguousarray(wav_bandlimits, dtype=np.int32)
+        self.scal_bl, self.L, self.spin, self.N = int(scal_bandlimit), int(L), int(spin), int(N)
+        self.J = self.wav_bl.size - 1
+        assert self.wav_lm.size == (self.J + 1) * self.L * self.L and self.scal_l.size >= self.L
+
+    def _args(self):
+        return (_ptr(self.scal_l), _ptr(self.wav_lm), self.scal_bl, _ptr(self.wav_bl), self.J, self.L, self.spin,
+                self.N)
+
+    def n_wav(self):
+        tot = 0
+        for b in self.wav_bl:
+            Lj = min(int(b), self.L)
+            Nj = min(self.N, Lj)
+            Nj += (Nj + self.N) % 2
+            tot += Lj * (2 * Lj - 1) * (2 * Nj - 1)
+        return tot
+
+    def n_scal(self):
+        b = min(self.scal_bl, self.L)
+        return b * (2 * b - 1)
+
+
+def analysis_lm2wav_manual(flm, t):
+    flm = _cin(flm)
+    fw, fs = np.zeros(t.n_wav(), np.complex128), np.zeros(t.n_scal(), np.complex128)
+    lib().s2let_analysis_lm2wav_manual(_ptr(fw), _ptr(fs), _ptr(flm), *t._args())
+    return fw, fs
+
+
+def synthesis_wav2lm_manual(f_wav, f_scal, t, flm=None):
+    """Adds into flm (a copy of it is returned; zeros when None), as the reference does."""
+    a, b = _cin(f_wav), _cin(f_scal)
+    out = np.zeros(t.L * t.L, np.complex128) if flm is None else np.array(flm, dtype=np.complex128)
+    lib().s2let_synthesis_wav2lm_manual(_ptr(out), _ptr(a), _ptr(b), *t._args())
+    return out
+
+
+def analysis_adjoint_wav2lm_manual(f_wav, f_scal, t, flm=None):
+    a, b = _cin(f_wav), _cin(f_scal)
+    out = np.zeros(t.L * t.L, np.complex128) if flm is None else np.array(flm, dtype=np.complex128)
+    lib().s2let_analysis_adjoint_wav2lm_manual(_ptr(out), _ptr(a), _ptr(b), *t._args())
+    return out
+
+
+def synthesis_adjoint_lm2wav_manual(flm, t):
+    flm = _cin(flm)
+    fw, fs = np.zeros(t.n_wav(), np.complex128), np.zeros(t.n_scal(), np.complex128)
+    lib().s2let_synthesis_adjoint_lm2wav_manual(_ptr(fw), _ptr(fs), _ptr(flm), *t._args())
+    return fw, fs
 
 
 def timing(enable=True):

@@ -77,7 +77,7 @@ struct HostIO {
     const int L = pl.p.L;
     nlm = (size_t)L * L;
     nwav = nwav_used(pl);
-    const int Ls = scal_bandlimit(&pl.p);
+    const int Ls = pl.groups[0].L;  // scaling-function band-limit
     nscal = (size_t)Ls * (2 * Ls - 1);
     npx = (size_t)L * (2 * L - 1);
     pl.h_flm.alloc(nlm * CB);
@@ -153,7 +153,7 @@ struct RealWork {
   explicit RealWork(Plan &pl) : P(pl) {
     const int L = pl.p.L;
     nwav = pl.n_wav_used;
-    const int Ls = scal_bandlimit(&pl.p);
+    const int Ls = pl.groups[0].L;  // scaling-function band-limit
     nscal = (size_t)Ls * (2 * Ls - 1);
     npx = (size_t)L * (2 * L - 1);
     if (pl.r_wav.bytes < nwav * CB) pl.r_wav.alloc(nwav * CB);
@@ -775,6 +775,85 @@ void s2let_synthesis_adjoint_lm2lmn_real(std::complex<double> *f_wav_lmn, std::c
                                          const double *scal_l, const s2let_parameters_t *p) {
   host_call(__func__, [&] { lmn_common(3, f_wav_lmn, f_scal_lm, flm, nullptr, wav_lm, scal_l, p, true); });
 }
+// ---------------------------------------------------------------- manual tiling (host pointers, FP64)
+// s2let_analysis.c:188-276, s2let_synthesis.c:202-289, s2let_analysis_adjoint.c:202-290,
+// s2let_synthesis_adjoint.c:189-290.  The to-flm forms accumulate into flm as the
+// reference does; its manual adjoints weight with conj(psi) where the plain adjoints use
+// psi (and vice versa), which is kept: they are the exact adjoints of the manual
+// transforms built on conj(psi).
+static Plan &manual_plan(Context &C, const std::complex<double> *wav_lm, const double *scal_l, int scal_bl,
+                         const int *wav_bl, int J, int L, int spin, int N, bool conj_psi, hipStream_t st) {
+  ManualTiling m{L, N, spin, J, scal_bl, wav_bl, wav_lm, scal_l, conj_psi};
+  return C.plan_manual(m, S2LET_HIP_FP64, st);
+}
+
+#define S2H_MANUAL_PLAN(conj_psi)                                                                        \
+  Context &C = Context::get();                                                                           \
+  std::lock_guard<std::mutex> lk(C.mu);                                                                  \
+  hipStream_t st = C.stream();                                                                           \
+  Plan &P = manual_plan(C, wav_lm, scal_l, scal_bandlimit, wav_bandlimits, J, L, spin, N, conj_psi, st); \
+  HostIO io(P);
+
+void s2let_analysis_lm2wav_manual(std::complex<double> *f_wav, std::complex<double> *f_scal,
+                                  const std::complex<double> *flm, const double *scal_l,
+                                  const std::complex<double> *wav_lm, const int scal_bandlimit,
+                                  const int *wav_bandlimits, int J, int L, int spin, int N) {
+  host_call(__func__, [&] {
+    S2H_MANUAL_PLAN(false)
+    h2d(P.h_flm.p, flm, io.nlm * CB, st);
+    run_analysis(P, P.h_flm.p, P.h_wav.p, P.h_scal.p, st);
+    d2h(f_wav, P.h_wav.p, io.nwav * CB, st);
+    d2h(f_scal, P.h_scal.p, io.nscal * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_synthesis_wav2lm_manual(std::complex<double> *flm, const std::complex<double> *f_wav,
+                                   const std::complex<double> *f_scal, const double *scal_l,
+                                   const std::complex<double> *wav_lm, const int scal_bandlimit,
+                                   const int *wav_bandlimits, int J, int L, int spin, int N) {
+  host_call(__func__, [&] {
+    S2H_MANUAL_PLAN(false)
+    h2d(P.h_wav.p, f_wav, io.nwav * CB, st);
+    h2d(P.h_scal.p, f_scal, io.nscal * CB, st);
+    h2d(P.h_tmp.p, flm, io.nlm * CB, st);
+    run_synthesis(P, P.h_wav.p, P.h_scal.p, P.h_flm.p, st);
+    launch_add_cx(P.h_tmp.p, P.h_flm.p, io.nlm, st);  // flm += (s2let_synthesis.c:250-284)
+    d2h(flm, P.h_tmp.p, io.nlm * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_analysis_adjoint_wav2lm_manual(std::complex<double> *flm, const std::complex<double> *f_wav,
+                                          const std::complex<double> *f_scal, const double *scal_l,
+                                          const std::complex<double> *wav_lm, const int scal_bandlimit,
+                                          const int *wav_bandlimits, int J, int L, int spin, int N) {
+  host_call(__func__, [&] {
+    S2H_MANUAL_PLAN(true)
+    h2d(P.h_wav.p, f_wav, io.nwav * CB, st);
+    h2d(P.h_scal.p, f_scal, io.nscal * CB, st);
+    h2d(P.h_tmp.p, flm, io.nlm * CB, st);
+    run_analysis_adjoint(P, P.h_wav.p, P.h_scal.p, P.h_flm.p, st);
+    launch_add_cx(P.h_tmp.p, P.h_flm.p, io.nlm, st);  // flm += (s2let_analysis_adjoint.c:251-287)
+    d2h(flm, P.h_tmp.p, io.nlm * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void s2let_synthesis_adjoint_lm2wav_manual(std::complex<double> *f_wav, std::complex<double> *f_scal,
+                                           const std::complex<double> *flm, const double *scal_l,
+                                           const std::complex<double> *wav_lm, const int scal_bandlimit,
+                                           const int *wav_bandlimits, int J, int L, int spin, int N) {
+  host_call(__func__, [&] {
+    S2H_MANUAL_PLAN(true)
+    h2d(P.h_flm.p, flm, io.nlm * CB, st);
+    run_synthesis_adjoint(P, P.h_flm.p, P.h_wav.p, P.h_scal.p, st);
+    d2h(f_wav, P.h_wav.p, io.nwav * CB, st);
+    d2h(f_scal, P.h_scal.p, io.nscal * CB, st);
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+  });
+}
+
 
 // ---------------------------------------------------------------- real signals (host pointers, FP64)
 constexpr size_t DB = sizeof(double);

@@ -73,6 +73,11 @@ __global__ void k_cx2re(const cx<double> *__restrict__ a, double *__restrict__ b
     b[i] = a[i].x;
 }
 
+__global__ void k_add_cx(cx<double> *__restrict__ a, const cx<double> *__restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    a[i] = cadd(a[i], b[i]);
+}
+
 // ------------------------------------------------------------------ gamma mixing
 // per job: out_k[pt] = sum_j W[k][j] in_j[pt] (complex, nout x nin matrix W, nin <= NMAX),
 // kMixBlocks blocks per job
@@ -160,6 +165,9 @@ void launch_to_f32(const double *a, float *b, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_to_f32, dim3(grid1d(n)), dim3(256), 0, st, a, b, n);
 }
 
+void launch_add_cx(void *a, const void *b, size_t n, hipStream_t st) {
+  if (n) hipLaunchKernelGGL(k_add_cx, dim3(grid1d(n)), dim3(256), 0, st, (cx<double> *)a, (const cx<double> *)b, n);
+}
 void launch_re2cx(const double *a, void *b, size_t n, hipStream_t st) {
   if (n) hipLaunchKernelGGL(k_re2cx, dim3(grid1d(n)), dim3(256), 0, st, a, (cx<double> *)b, n);
 }

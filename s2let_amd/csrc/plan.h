@@ -97,6 +97,16 @@ struct Tune {
   int rows_div = 0;  // row stage: LDS per block = device LDS / rows_div (0: default 4)
 };
 
+// A user-supplied tiling (the reference's *_manual entry points).
+struct ManualTiling {
+  int L, N, spin, J;  // scales j = 0..J
+  int Ls;             // scaling-function band-limit (min'd with L)
+  const int *bl;      // J + 1 wavelet band-limits (min'd with L)
+  const cd *psi;      // (J + 1) L^2 wavelet harmonics
+  const double *phi;  // L scaling-function weights
+  bool conj_psi;      // use conj(psi) (the reference's manual adjoints)
+};
+
 class Context {
  public:
   static Context &get();
@@ -107,6 +117,7 @@ class Context {
   const DftTables &dft(int n);
   const LTables &ltab(int L);
   Plan &plan(const s2let_parameters_t *p, int mode, hipStream_t st);
+  Plan &plan_manual(const ManualTiling &m, int mode, hipStream_t st);
   ShtPlan &sht(int L, int spin, int mode, hipStream_t st);
   size_t lds_bytes() const { return lds_bytes_; }
   // launch shapes per (band-limit, precision of the intermediate data: 4 or 8 bytes)
@@ -125,6 +136,7 @@ class Context {
 
  private:
   Context();
+  void finish_plan(Plan &pl, hipStream_t st);
   int dev_ = 0;
   size_t lds_bytes_ = 65536;
   int dl_L_ = 0, t64_L_ = 0, t32_L_ = 0;

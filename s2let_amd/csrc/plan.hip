@@ -272,6 +272,53 @@ static std::string plan_key(const s2let_parameters_t *p, int mode) {
   return o.str();
 }
 
+namespace {
+
+// Appends columns and groups to a plan under construction.
+struct Columns {
+  Plan &pl;
+  size_t spec = 0, out = 0;
+  explicit Columns(Plan &p) : pl(p) {}
+  // weights b (L entries, real), phases of the analysis (conj(Psi)) and synthesis (Psi) sides
+  void add(int n, int lmin, const std::vector<double> &b, cd pa, cd ps) {
+    ColDev c{};
+    c.n = n;
+    c.lmin = lmin;
+    c.boff = (int)pl.bvals.size();
+    c.ph_re = pa.real();
+    c.ph_im = pa.imag();
+    pl.cols_ana.push_back(c);
+    c.ph_re = ps.real();
+    c.ph_im = ps.imag();
+    pl.cols_syn.push_back(c);
+    pl.bvals.insert(pl.bvals.end(), b.begin(), b.end());
+  }
+  // scaling function: spin-0 MW transform at Ls of flm * sqrt(4pi/(2l+1)) phi_l, l >= |spin|
+  // (s2let_analysis.c:318-324)
+  void scaling(int L, int Ls, int aspin, const std::vector<double> &phi) {
+    Group g{Ls, 0, 1, (int)pl.cols_ana.size(), spec, 0, -1};
+    std::vector<double> b(L, 0.0);
+    for (int l = aspin; l < Ls; ++l)
+      b[l] = std::sqrt((2.0 * l + 1) / (4.0 * M_PI)) * (std::sqrt(4.0 * M_PI / (2 * l + 1)) * phi[l]);
+    add(0, aspin, b, cd(1, 0), cd(1, 0));
+    spec += (size_t)Ls * (2 * Ls - 1);
+    pl.groups.push_back(g);
+  }
+  Group open(int Lj, int Nj, int j) { return Group{Lj, 2 * Nj - 1, 0, (int)pl.cols_ana.size(), spec, out, j}; }
+  void close(Group &g, int Nj) {
+    spec += (size_t)g.ncol * g.L * (2 * g.L - 1);
+    out += (size_t)g.L * (2 * g.L - 1) * (2 * Nj - 1);
+    pl.maxcol = std::max(pl.maxcol, g.ncol);
+    pl.groups.push_back(g);
+  }
+  void finish() {
+    pl.spec_elems = spec;
+    pl.n_wav_used = out;
+  }
+};
+
+}  // namespace
+
 Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
   const std::string key = plan_key(p, mode);
   auto it = plans_.find(key);
@@ -291,35 +338,12 @@ Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
   tiling_wavelet(pl.psi.data(), pl.phi.data(), p);
   const cd nu = (p->N % 2) ? cd(1, 0) : cd(0, 1);
   const int aspin = std::abs(p->spin);
-  const double pi = M_PI;
-  size_t spec = 0, out = 0;
-  auto add_col = [&](int n, int lmin, const std::vector<double> &b, double ar, double ai, double sr, double si) {
-    ColDev c{};
-    c.n = n;
-    c.lmin = lmin;
-    c.boff = (int)pl.bvals.size();
-    c.ph_re = ar;
-    c.ph_im = ai;
-    pl.cols_ana.push_back(c);
-    c.ph_re = sr;
-    c.ph_im = si;
-    pl.cols_syn.push_back(c);
-    pl.bvals.insert(pl.bvals.end(), b.begin(), b.end());
-  };
-  // scaling function: spin-0 MW transform at L_s (s2let_analysis.c:318-324)
-  {
-    const int Ls = scal_bandlimit(p);
-    Group g{Ls, 0, 1, (int)pl.cols_ana.size(), spec, 0, -1};
-    std::vector<double> b(L, 0.0);
-    for (int l = aspin; l < Ls; ++l) b[l] = std::sqrt((2.0 * l + 1) / (4.0 * pi)) * (std::sqrt(4.0 * pi / (2 * l + 1)) * pl.phi[l]);
-    add_col(0, aspin, b, 1, 0, 1, 0);
-    spec += (size_t)Ls * (2 * Ls - 1);
-    pl.groups.push_back(g);
-  }
+  Columns cb(pl);
+  cb.scaling(L, scal_bandlimit(p), aspin, pl.phi);
   for (int j = p->J_min; j <= J; ++j) {
     int Lj, Nj;
     scale_shape(p, j, &Lj, &Nj);
-    Group g{Lj, 2 * Nj - 1, 0, (int)pl.cols_ana.size(), spec, out, j};
+    Group g = cb.open(Lj, Nj, j);
     const int L0 = el0(j, p);
     for (int n = -Nj + 1; n < Nj; n += 2) {
       // Psi^j_ln = nu * r(l): analysis uses conj(Psi) (phase i^n conj(nu)), synthesis Psi (i^n nu)
@@ -332,17 +356,105 @@ Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
       double ir, ii;
       ipow(n, ir, ii);
       const cd in(ir, ii);
-      const cd pa = in * std::conj(nu), ps = in * nu;
-      add_col(n, lmin, b, pa.real(), pa.imag(), ps.real(), ps.imag());
+      cb.add(n, lmin, b, in * std::conj(nu), in * nu);
       g.ncol++;
     }
-    spec += (size_t)g.ncol * Lj * (2 * Lj - 1);
-    out += (size_t)Lj * (2 * Lj - 1) * (2 * Nj - 1);
-    pl.maxcol = std::max(pl.maxcol, g.ncol);
-    pl.groups.push_back(g);
+    cb.close(g, Nj);
   }
-  pl.spec_elems = spec;
-  pl.n_wav_used = out;
+  cb.finish();
+  finish_plan(pl, st);
+  auto &ref = *P;
+  plans_[key] = std::move(P);
+  return ref;
+}
+
+// Manual tiling (s2let_analysis.c:188-276): scales j = 0..J at L_j = min(wav_bandlimits[j], L),
+// N_j = min(N, L_j) with N's parity, lower band-limit 0, and any complex Psi^j_ln.  Each
+// (j, n) becomes one column when Psi^j_.n is real or imaginary throughout, else two
+// (Re Psi with phase 1, Im Psi with phase i): the transform is linear in the weights, the
+// two columns read / write the same gamma plane, and the split is exact.
+Plan &Context::plan_manual(const ManualTiling &m, int mode, hipStream_t st) {
+  const int L = m.L, N = m.N, J = m.J, aspin = std::abs(m.spin);
+  if (L < 1 || N < 1 || N > 16 || J < 0 || aspin >= L || m.Ls < 0) {
+    set_error(-1, "unsupported manual tiling (need L >= 1, 1 <= N <= 16, J >= 0, |spin| < L)");
+    throw ArgFailure();
+  }
+  for (int j = 0; j <= J; ++j)
+    if (m.bl[j] < 1) {
+      set_error(-1, "manual tiling: wavelet band-limits must be >= 1");
+      throw ArgFailure();
+    }
+  // content key: sizes, band-limits and a hash of the weights
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](const void *d, size_t n) {
+    const unsigned char *c = static_cast<const unsigned char *>(d);
+    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+  };
+  mix(m.psi, (size_t)(J + 1) * L * L * sizeof(cd));
+  mix(m.phi, (size_t)L * sizeof(double));
+  std::ostringstream o;
+  o << "manual," << L << ',' << N << ',' << J << ',' << m.spin << ',' << m.Ls << ',' << m.conj_psi << ',' << mode;
+  for (int j = 0; j <= J; ++j) o << ',' << m.bl[j];
+  o << ',' << h;
+  const std::string key = o.str();
+  auto it = plans_.find(key);
+  if (it != plans_.end()) return *it->second;
+  auto P = std::make_unique<Plan>();
+  Plan &pl = *P;
+  pl.p = s2let_parameters_t{};
+  pl.p.L = L;
+  pl.p.N = N;
+  pl.p.spin = m.spin;
+  pl.p.B = 2.0;
+  pl.mode = mode;
+  pl.J = J;
+  pl.psi.assign(m.psi, m.psi + (size_t)(J + 1) * L * L);
+  if (m.conj_psi)
+    for (cd &v : pl.psi) v = std::conj(v);
+  pl.phi.assign(m.phi, m.phi + L);
+  Columns cb(pl);
+  cb.scaling(L, std::min(m.Ls, L), aspin, pl.phi);
+  for (int j = 0; j <= J; ++j) {
+    const int Lj = std::min(m.bl[j], L);
+    int Nj = std::min(N, Lj);
+    Nj += (Nj + N) % 2;
+    Group g = cb.open(Lj, Nj, j);
+    for (int n = -Nj + 1; n < Nj; n += 2) {
+      const int lmin = std::max(aspin, std::abs(n));
+      std::vector<double> br(L, 0.0), bi(L, 0.0);
+      bool any_re = false, any_im = false;
+      for (int l = lmin; l < Lj; ++l) {
+        const cd ps = pl.psi[(size_t)j * L * L + (size_t)l * l + l + n];
+        br[l] = ps.real();
+        bi[l] = ps.imag();
+        any_re |= ps.real() != 0.0;
+        any_im |= ps.imag() != 0.0;
+      }
+      double ir, ii;
+      ipow(n, ir, ii);
+      const cd in(ir, ii), one(1, 0), im(0, 1);
+      if (any_re || !any_im) {
+        cb.add(n, lmin, br, in, in);
+        g.ncol++;
+      }
+      if (any_im) {
+        cb.add(n, lmin, bi, in * std::conj(im), in * im);
+        g.ncol++;
+      }
+      (void)one;
+    }
+    cb.close(g, Nj);
+  }
+  cb.finish();
+  finish_plan(pl, st);
+  auto &ref = *P;
+  plans_[key] = std::move(P);
+  return ref;
+}
+
+void Context::finish_plan(Plan &pl, hipStream_t st) {
+  const int L = pl.p.L;
+  const int mode = pl.mode;
   // spectrum plane of every column, and the equal-L runs of columns (MFMA column tiles)
   for (const Group &g : pl.groups)
     for (int k = 0; k < g.ncol; ++k) {
@@ -422,10 +534,8 @@ Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
   if (f32) pl.flm32.alloc((size_t)L * L * sizeof(cx<float>));
   pl.parts.alloc((size_t)kMaxSplits * L * L * sizeof(cx<double>));
   S2H_CHECK_HIP(hipStreamSynchronize(st));
-  auto &ref = *P;
-  plans_[key] = std::move(P);
-  return ref;
 }
+
 
 ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st) {
   auto key = std::make_tuple(L, spin, mode);

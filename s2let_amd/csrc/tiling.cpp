@@ -315,6 +315,15 @@ int s2let_n_wav_j(int j, const s2let_parameters_t *p) {
 void s2let_allocate_mw(std::complex<double> **f, int L) {
   *f = (std::complex<double> *)calloc((size_t)L * (2 * L - 1), sizeof(std::complex<double>));
 }
+// s2let_alloc.c:148-174: N and, with upsample, L come from parameters
+void s2let_allocate_f_wav_manual(std::complex<double> **f_wav, std::complex<double> **f_scal, int *wav_bandlimits,
+                                 int scal_bandlimit, int N, int J, const s2let_parameters_t *p) {
+  (void)N;
+  size_t total = 0;
+  for (int j = 0; j <= J; ++j) total += (size_t)f_size(p->upsample ? p->L : wav_bandlimits[j], p->N);
+  *f_wav = (std::complex<double> *)calloc(total, sizeof(std::complex<double>));
+  *f_scal = (std::complex<double> *)calloc((size_t)scal_bandlimit * (2 * scal_bandlimit - 1), sizeof(std::complex<double>));
+}
 void s2let_allocate_mw_real(double **f, int L) { *f = (double *)calloc((size_t)L * (2 * L - 1), sizeof(double)); }
 void s2let_allocate_f_wav_real(double **a, double **b, const s2let_parameters_t *p) {
   *a = (double *)calloc(s2let_n_wav(p), sizeof(double));
