@@ -36,9 +36,14 @@ class Params(ctypes.Structure):
     ]
 
 
-def params(L, N, B=2.0, J_min=0, spin=0, upsample=0, original_spin=0):
+def params(L, N, B=2.0, J_min=0, spin=0, upsample=0, original_spin=0, sampling=0):
+    """sampling: 0 MW, 1 MWSS (S2LET_SAMPLING_MW_SS)."""
     return Params(0, 0, int(upsample), float(B), int(L), int(J_min), int(N), int(spin),
-                  int(original_spin), 0, 0)
+                  int(original_spin), int(sampling), 0)
+
+
+def map_size(L, ss=0):
+    return (L + 1) * 2 * L if ss else L * (2 * L - 1)
 
 
 def build():
@@ -161,59 +166,59 @@ def dl_beta_full(L, beta):
     return planes
 
 
-def ssht_inverse(flm, L, spin=0):
-    f = _c(L * (2 * L - 1))
+def ssht_inverse(flm, L, spin=0, ss=0):
+    f = _c(map_size(L, ss))
     flm = _in(flm)
-    lib().oracle_ssht_inverse(_p(f), _p(flm), int(L), int(spin))
+    lib().oracle_ssht_inverse_s(_p(f), _p(flm), int(L), int(spin), int(ss))
     return f
 
 
-def ssht_forward(f, L, spin=0):
+def ssht_forward(f, L, spin=0, ss=0):
     flm = _c(L * L)
     f = _in(f)
-    lib().oracle_ssht_forward(_p(flm), _p(f), int(L), int(spin))
+    lib().oracle_ssht_forward_s(_p(flm), _p(f), int(L), int(spin), int(ss))
     return flm
 
 
-def ssht_adjoint_inverse(f, L, spin=0):
+def ssht_adjoint_inverse(f, L, spin=0, ss=0):
     flm = _c(L * L)
     f = _in(f)
-    lib().oracle_ssht_adjoint_inverse(_p(flm), _p(f), int(L), int(spin))
+    lib().oracle_ssht_adjoint_inverse_s(_p(flm), _p(f), int(L), int(spin), int(ss))
     return flm
 
 
-def ssht_adjoint_forward(flm, L, spin=0):
-    f = _c(L * (2 * L - 1))
+def ssht_adjoint_forward(flm, L, spin=0, ss=0):
+    f = _c(map_size(L, ss))
     flm = _in(flm)
-    lib().oracle_ssht_adjoint_forward(_p(f), _p(flm), int(L), int(spin))
+    lib().oracle_ssht_adjoint_forward_s(_p(f), _p(flm), int(L), int(spin), int(ss))
     return f
 
 
-def so3_inverse(flmn, L, N, L0=0, nmode=0):
-    f = _c(L * (2 * L - 1) * (2 * N - 1))
+def so3_inverse(flmn, L, N, L0=0, nmode=0, ss=0):
+    f = _c(map_size(L, ss) * (2 * N - 1))
     flmn = _in(flmn)
-    lib().oracle_so3_inverse(_p(f), _p(flmn), int(L), int(N), int(L0), int(nmode))
+    lib().oracle_so3_inverse_s(_p(f), _p(flmn), int(L), int(N), int(L0), int(nmode), int(ss))
     return f
 
 
-def so3_forward(f, L, N, L0=0, nmode=0):
+def so3_forward(f, L, N, L0=0, nmode=0, ss=0):
     flmn = _c(flmn_size(L, N))
     f = _in(f)
-    lib().oracle_so3_forward(_p(flmn), _p(f), int(L), int(N), int(L0), int(nmode))
+    lib().oracle_so3_forward_s(_p(flmn), _p(f), int(L), int(N), int(L0), int(nmode), int(ss))
     return flmn
 
 
-def so3_adjoint_inverse(f, L, N, L0=0, nmode=0):
+def so3_adjoint_inverse(f, L, N, L0=0, nmode=0, ss=0):
     flmn = _c(flmn_size(L, N))
     f = _in(f)
-    lib().oracle_so3_adjoint_inverse(_p(flmn), _p(f), int(L), int(N), int(L0), int(nmode))
+    lib().oracle_so3_adjoint_inverse_s(_p(flmn), _p(f), int(L), int(N), int(L0), int(nmode), int(ss))
     return flmn
 
 
-def so3_adjoint_forward(flmn, L, N, L0=0, nmode=0):
-    f = _c(L * (2 * L - 1) * (2 * N - 1))
+def so3_adjoint_forward(flmn, L, N, L0=0, nmode=0, ss=0):
+    f = _c(map_size(L, ss) * (2 * N - 1))
     flmn = _in(flmn)
-    lib().oracle_so3_adjoint_forward(_p(f), _p(flmn), int(L), int(N), int(L0), int(nmode))
+    lib().oracle_so3_adjoint_forward_s(_p(f), _p(flmn), int(L), int(N), int(L0), int(nmode), int(ss))
     return f
 
 
@@ -368,7 +373,7 @@ def analysis_px2wav(f, p):
 
 
 def synthesis_wav2px(f_wav, f_scal, p):
-    f = _c(p.L * (2 * p.L - 1))
+    f = _c(map_size(p.L, p.sampling_scheme))
     a, b = _in(f_wav), _in(f_scal)
     lib().oracle_synthesis_wav2px(_p(f), _p(a), _p(b), ctypes.byref(p))
     return f
@@ -382,7 +387,7 @@ def analysis_adjoint_wav2lm(f_wav, f_scal, p):
 
 
 def analysis_adjoint_wav2px(f_wav, f_scal, p):
-    f = _c(p.L * (2 * p.L - 1))
+    f = _c(map_size(p.L, p.sampling_scheme))
     a, b = _in(f_wav), _in(f_scal)
     lib().oracle_analysis_adjoint_wav2px(_p(f), _p(a), _p(b), ctypes.byref(p))
     return f

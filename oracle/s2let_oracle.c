@@ -297,6 +297,9 @@ void oracle_tiling_wavelet(ocplx *psi, double *phi, const oracle_params_t *p) {
 
 int oracle_flmn_size(int L, int N) { return ((2 * N - 1) * (3 * L * L - N * (N - 1))) / 3; }
 static int or_f_size(int L, int N) { return L * (2 * L - 1) * (2 * N - 1); }
+/* samples of one SO(3) signal: MW L (2L-1) (2N-1), MWSS (L+1) 2L (2N-1) (s2let_helper.c:5-20) */
+static int or_f_size_s(int L, int N, int ss) { return (ss ? (L + 1) * 2 * L : L * (2 * L - 1)) * (2 * N - 1); }
+static int or_ss(const oracle_params_t *p) { return p->sampling_scheme == 1; }
 
 int oracle_elmn2ind(int el, int m, int n, int L, int N) {
   int off = 0;
@@ -316,7 +319,7 @@ int oracle_n_lm_scal(const oracle_params_t *p) {
 /* s2let_helper.c:65-76 */
 int oracle_n_scal(const oracle_params_t *p) {
   int b = or_scal_bandlimit(p);
-  return b * (2 * b - 1);
+  return or_ss(p) ? (b + 1) * 2 * b : b * (2 * b - 1);
 }
 /* s2let_helper.c:37-56 (N per scale is min(N, L_j), not parity adjusted) */
 int oracle_n_lmn_wav(const oracle_params_t *p) {
@@ -335,7 +338,7 @@ int oracle_n_wav(const oracle_params_t *p) {
   int J = oracle_j_max(p), total = 0, bl = p->L;
   for (int j = p->J_min; j <= J; ++j) {
     if (!p->upsample) bl = OMIN(oracle_bandlimit(j, p), p->L);
-    total += or_f_size(bl, p->N);
+    total += or_f_size_s(bl, p->N, or_ss(p));
   }
   return total;
 }
@@ -539,49 +542,70 @@ static inline ocplx or_ipow(int k) {
 }
 static inline int or_sgn(int k) { return (k & 1) ? -1 : 1; }
 
-/* folded spectrum F -> map g[t][p] (t < L): 2-D inverse DFT of the full spectrum */
-static void or_spec_to_map(ocplx *map, const ocplx *F, int L, int n) {
-  int M = 2 * L - 1;
-  ocplx *S = (ocplx *)calloc((size_t)M * M, sizeof(ocplx)); /* S[(m'+L-1)*M + m+L-1] */
+/* Sampling geometry.  MW (McEwen & Wiaux 2011): theta_t = pi (2t+1)/(2L-1), t < L,
+   phi_p = 2 pi p/(2L-1); the theta samples extend to a (2L-1)-periodic torus with
+   theta_{M-1-t} = 2 pi - theta_t.  MWSS (the symmetric variant, S2LET_SAMPLING_MW_SS):
+   theta_t = pi t / L, t <= L (both poles), phi_p = pi p / L, p < 2L; the torus has 2L
+   samples with theta_{2L-t} = 2 pi - theta_t.  Mt: torus length, Mp: phi samples, nt:
+   theta rows kept, h: theta_0 in units such that e^{i m' theta_t} = e^{i m' h} w^{m' t}. */
+typedef struct {
+  int L, ss, Mt, Mp, nt;
+  double h;
+} or_geo;
+static or_geo or_geo_of(int L, int ss) {
+  or_geo g;
+  g.L = L;
+  g.ss = ss;
+  g.Mt = ss ? 2 * L : 2 * L - 1;
+  g.Mp = g.Mt;
+  g.nt = ss ? L + 1 : L;
+  g.h = ss ? 0.0 : OPI / (2 * L - 1);
+  return g;
+}
+/* the stored theta row whose samples extend torus row t >= nt */
+static int or_ext_row(const or_geo *g, int t) { return g->ss ? g->Mt - t : g->Mt - 1 - t; }
+static int or_map_size(int L, int ss) { return ss ? (L + 1) * 2 * L : L * (2 * L - 1); }
+
+/* folded spectrum F[m'][m] (m' in [0,L), m centred, 2L-1 wide) -> map g[t][p] (t < nt):
+   2-D inverse DFT of the unfolded spectrum (S(-m') = (-1)^{m+n} S(m')) */
+static void or_spec_to_map(ocplx *map, const ocplx *F, int L, int n, int ss) {
+  or_geo G = or_geo_of(L, ss);
+  const int Mm = 2 * L - 1, Mt = G.Mt, Mp = G.Mp;
+  ocplx *X = (ocplx *)calloc((size_t)Mt * Mp, sizeof(ocplx)); /* X[m' mod Mt][m mod Mp] */
   for (int mp = 0; mp < L; ++mp)
     for (int m = -(L - 1); m <= L - 1; ++m) {
-      ocplx v = or_ipow(n - m) * F[(size_t)mp * M + m + L - 1];
-      S[(size_t)(mp + L - 1) * M + m + L - 1] = v * cexp(I * mp * OPI / M);
-      if (mp > 0) S[(size_t)(-mp + L - 1) * M + m + L - 1] = or_sgn(m + n) * v * cexp(-I * mp * OPI / M);
+      ocplx v = or_ipow(n - m) * F[(size_t)mp * Mm + m + L - 1];
+      X[(size_t)(mp % Mt) * Mp + (m + Mp) % Mp] += v * cexp(I * mp * G.h);
+      if (mp > 0) X[(size_t)((Mt - mp) % Mt) * Mp + (m + Mp) % Mp] += or_sgn(m + n) * v * cexp(-I * mp * G.h);
     }
-  /* reorder to DFT index order: k -> k mod M */
-  ocplx *X = (ocplx *)calloc((size_t)M * M, sizeof(ocplx));
-  for (int a = -(L - 1); a <= L - 1; ++a)
-    for (int b = -(L - 1); b <= L - 1; ++b)
-      X[(size_t)((a + M) % M) * M + (b + M) % M] = S[(size_t)(a + L - 1) * M + b + L - 1];
   or_fft P;
-  or_fft_init(&P, M);
-  ocplx *s1 = malloc(sizeof(ocplx) * M), *s2 = malloc(sizeof(ocplx) * M);
-  for (int r = 0; r < M; ++r) or_fft_line(&P, X + (size_t)r * M, 1, +1, s1, s2);   /* over m */
-  for (int c = 0; c < M; ++c) or_fft_line(&P, X + c, M, +1, s1, s2);               /* over m' */
-  for (int t = 0; t < L; ++t) memcpy(map + (size_t)t * M, X + (size_t)t * M, sizeof(ocplx) * M);
+  or_fft_init(&P, Mp);
+  ocplx *s1 = malloc(sizeof(ocplx) * Mp), *s2 = malloc(sizeof(ocplx) * Mp);
+  for (int r = 0; r < Mt; ++r) or_fft_line(&P, X + (size_t)r * Mp, 1, +1, s1, s2); /* over m */
+  for (int c = 0; c < Mp; ++c) or_fft_line(&P, X + c, Mp, +1, s1, s2);            /* over m' (Mt == Mp) */
+  for (int t = 0; t < G.nt; ++t) memcpy(map + (size_t)t * Mp, X + (size_t)t * Mp, sizeof(ocplx) * Mp);
   free(s1);
   free(s2);
   or_fft_free(&P);
-  free(S);
   free(X);
 }
 
-/* adjoint of or_spec_to_map: map (t < L) -> folded F */
-static void or_map_to_spec_adj(ocplx *F, const ocplx *map, int L, int n) {
-  int M = 2 * L - 1;
-  ocplx *X = (ocplx *)calloc((size_t)M * M, sizeof(ocplx));
-  for (int t = 0; t < L; ++t) memcpy(X + (size_t)t * M, map + (size_t)t * M, sizeof(ocplx) * M);
+/* adjoint of or_spec_to_map: map (t < nt) -> folded F */
+static void or_map_to_spec_adj(ocplx *F, const ocplx *map, int L, int n, int ss) {
+  or_geo G = or_geo_of(L, ss);
+  const int Mm = 2 * L - 1, Mt = G.Mt, Mp = G.Mp;
+  ocplx *X = (ocplx *)calloc((size_t)Mt * Mp, sizeof(ocplx));
+  for (int t = 0; t < G.nt; ++t) memcpy(X + (size_t)t * Mp, map + (size_t)t * Mp, sizeof(ocplx) * Mp);
   or_fft P;
-  or_fft_init(&P, M);
-  ocplx *s1 = malloc(sizeof(ocplx) * M), *s2 = malloc(sizeof(ocplx) * M);
-  for (int r = 0; r < M; ++r) or_fft_line(&P, X + (size_t)r * M, 1, -1, s1, s2);
-  for (int c = 0; c < M; ++c) or_fft_line(&P, X + c, M, -1, s1, s2);
+  or_fft_init(&P, Mp);
+  ocplx *s1 = malloc(sizeof(ocplx) * Mp), *s2 = malloc(sizeof(ocplx) * Mp);
+  for (int r = 0; r < Mt; ++r) or_fft_line(&P, X + (size_t)r * Mp, 1, -1, s1, s2);
+  for (int c = 0; c < Mp; ++c) or_fft_line(&P, X + c, Mp, -1, s1, s2);
   for (int mp = 0; mp < L; ++mp)
     for (int m = -(L - 1); m <= L - 1; ++m) {
-      ocplx a = X[(size_t)mp * M + (m + M) % M] * cexp(-I * mp * OPI / M);
-      if (mp > 0) a += or_sgn(m + n) * X[(size_t)((M - mp) % M) * M + (m + M) % M] * cexp(I * mp * OPI / M);
-      F[(size_t)mp * M + m + L - 1] = conj(or_ipow(n - m)) * a;
+      ocplx a = X[(size_t)(mp % Mt) * Mp + (m + Mp) % Mp] * cexp(-I * mp * G.h);
+      if (mp > 0) a += or_sgn(m + n) * X[(size_t)((Mt - mp) % Mt) * Mp + (m + Mp) % Mp] * cexp(I * mp * G.h);
+      F[(size_t)mp * Mm + m + L - 1] = conj(or_ipow(n - m)) * a;
     }
   free(s1);
   free(s2);
@@ -604,44 +628,49 @@ static int or_pow2_at_least(int x) {
 }
 
 /*
- * Forward MW quadrature for one column n: map g[t][p] (t < L) ->
- * folded Jf[m'][m] (m' in [0,L)), where
+ * Forward quadrature for one column n: map g[t][p] (t < nt) -> folded Jf[m'][m]
+ * (m' in [0,L)), where
  *   J(m, m') = int_0^pi G_m(theta) e^{i m' theta} sin theta d theta,
  *   Jf(m', m) = J(m, m') + (-1)^{m+n} J(m, -m') for m' > 0, Jf(0, m) = J(m, 0),
- * G_m(theta_t) = (2 pi / M) sum_p g[t][p] e^{-i m phi_p}, extended to t in [L, M)
- * by G_m(theta_{M-1-t}) (-1)^{m+n}.
+ * G_m(theta_t) = (2 pi / Mp) sum_p g[t][p] e^{-i m phi_p} (|m| < L), extended to the
+ * torus rows t >= nt by G_m(theta_ext(t)) (-1)^{m+n}; its Fourier coefficients on the
+ * torus, |m''| < L (MWSS: the Nyquist coefficient m'' = L is not used), are convolved
+ * with the exact weights I(p).
  */
-static void or_map_to_jf(ocplx *Jf, const ocplx *map, int L, int n) {
-  int M = 2 * L - 1;
+static void or_map_to_jf(ocplx *Jf, const ocplx *map, int L, int n, int ss) {
+  or_geo Gg = or_geo_of(L, ss);
+  const int Mm = 2 * L - 1, Mt = Gg.Mt, Mp = Gg.Mp;
   or_fft P;
-  or_fft_init(&P, M);
+  or_fft_init(&P, Mp);
   int Pc = or_pow2_at_least(4 * L - 3);
   or_fft Q;
   or_fft_init(&Q, Pc);
-  ocplx *G = (ocplx *)calloc((size_t)M * M, sizeof(ocplx)); /* G[t][m mod M] */
-  ocplx *s1 = malloc(sizeof(ocplx) * (Pc > M ? Pc : M)), *s2 = malloc(sizeof(ocplx) * (Pc > M ? Pc : M));
-  for (int t = 0; t < L; ++t) {
-    memcpy(G + (size_t)t * M, map + (size_t)t * M, sizeof(ocplx) * M);
-    or_fft_line(&P, G + (size_t)t * M, 1, -1, s1, s2);
-    for (int k = 0; k < M; ++k) G[(size_t)t * M + k] *= 2.0 * OPI / M;
+  ocplx *G = (ocplx *)calloc((size_t)Mt * Mp, sizeof(ocplx)); /* G[t][m mod Mp] */
+  ocplx *s1 = malloc(sizeof(ocplx) * (Pc > Mt ? Pc : Mt)), *s2 = malloc(sizeof(ocplx) * (Pc > Mt ? Pc : Mt));
+  for (int t = 0; t < Gg.nt; ++t) {
+    memcpy(G + (size_t)t * Mp, map + (size_t)t * Mp, sizeof(ocplx) * Mp);
+    or_fft_line(&P, G + (size_t)t * Mp, 1, -1, s1, s2);
+    for (int k = 0; k < Mp; ++k) G[(size_t)t * Mp + k] *= 2.0 * OPI / Mp;
   }
-  for (int t = L; t < M; ++t)
+  for (int t = Gg.nt; t < Mt; ++t)
     for (int m = -(L - 1); m <= L - 1; ++m)
-      G[(size_t)t * M + (m + M) % M] = or_sgn(m + n) * G[(size_t)(M - 1 - t) * M + (m + M) % M];
+      G[(size_t)t * Mp + (m + Mp) % Mp] = or_sgn(m + n) * G[(size_t)or_ext_row(&Gg, t) * Mp + (m + Mp) % Mp];
   /* weights spectrum, circular period Pc */
   ocplx *wI = (ocplx *)calloc(Pc, sizeof(ocplx));
   for (int q = -2 * (L - 1); q <= 2 * (L - 1); ++q) wI[(q + Pc) % Pc] = or_wI(q);
   or_fft_line(&Q, wI, 1, -1, s1, s2);
   ocplx *col = (ocplx *)malloc(sizeof(ocplx) * Pc);
+  or_fft Pt;
+  or_fft_init(&Pt, Mt);
   for (int m = -(L - 1); m <= L - 1; ++m) {
-    int mi = (m + M) % M;
-    /* theta DFT: Fm[m''] = (1/M) e^{-i m'' pi/M} sum_t G[t][m] e^{-2 pi i m'' t / M} */
-    for (int t = 0; t < M; ++t) s1[t] = G[(size_t)t * M + mi];
-    or_fft_rec(s1, 1, s2, M, P.fac, P.tw, M, -1);
+    int mi = (m + Mp) % Mp;
+    /* theta DFT: Fm[m''] = (1/Mt) e^{-i m'' h} sum_t G[t][m] e^{-2 pi i m'' t / Mt} */
+    for (int t = 0; t < Mt; ++t) s1[t] = G[(size_t)t * Mp + mi];
+    or_fft_rec(s1, 1, s2, Mt, Pt.fac, Pt.tw, Mt, -1);
     /* J(m') = sum_{m''} Fm[m''] I(m'' + m') = sum_k R[k] I(m' - k), R[k] = Fm[-k] */
     for (int k = 0; k < Pc; ++k) col[k] = 0;
     for (int mpp = -(L - 1); mpp <= L - 1; ++mpp) {
-      ocplx fm = s2[(mpp + M) % M] * cexp(-I * mpp * OPI / M) / M;
+      ocplx fm = s2[(mpp + Mt) % Mt] * cexp(-I * mpp * Gg.h) / Mt;
       col[(-mpp + Pc) % Pc] = fm;
     }
     or_fft_line(&Q, col, 1, -1, s1, s2);
@@ -651,7 +680,7 @@ static void or_map_to_jf(ocplx *Jf, const ocplx *map, int L, int n) {
     for (int mp = 0; mp < L; ++mp) {
       ocplx v = col[mp];
       if (mp > 0) v += or_sgn(m + n) * col[(Pc - mp) % Pc];
-      Jf[(size_t)mp * M + m + L - 1] = v;
+      Jf[(size_t)mp * Mm + m + L - 1] = v;
     }
   }
   free(col);
@@ -660,29 +689,33 @@ static void or_map_to_jf(ocplx *Jf, const ocplx *map, int L, int n) {
   free(s2);
   free(G);
   or_fft_free(&P);
+  or_fft_free(&Pt);
   or_fft_free(&Q);
 }
 
 /* adjoint of or_map_to_jf: folded Jf -> map */
-static void or_jf_to_map_adj(ocplx *map, const ocplx *Jf, int L, int n) {
-  int M = 2 * L - 1;
+static void or_jf_to_map_adj(ocplx *map, const ocplx *Jf, int L, int n, int ss) {
+  or_geo Gg = or_geo_of(L, ss);
+  const int Mm = 2 * L - 1, Mt = Gg.Mt, Mp = Gg.Mp;
   or_fft P;
-  or_fft_init(&P, M);
+  or_fft_init(&P, Mp);
+  or_fft Pt;
+  or_fft_init(&Pt, Mt);
   int Pc = or_pow2_at_least(4 * L - 3);
   or_fft Q;
   or_fft_init(&Q, Pc);
-  ocplx *G = (ocplx *)calloc((size_t)M * M, sizeof(ocplx));
-  ocplx *s1 = malloc(sizeof(ocplx) * (Pc > M ? Pc : M)), *s2 = malloc(sizeof(ocplx) * (Pc > M ? Pc : M));
+  ocplx *G = (ocplx *)calloc((size_t)Mt * Mp, sizeof(ocplx));
+  ocplx *s1 = malloc(sizeof(ocplx) * (Pc > Mt ? Pc : Mt)), *s2 = malloc(sizeof(ocplx) * (Pc > Mt ? Pc : Mt));
   ocplx *wI = (ocplx *)calloc(Pc, sizeof(ocplx));
   for (int q = -2 * (L - 1); q <= 2 * (L - 1); ++q) wI[(q + Pc) % Pc] = or_wI(q);
   or_fft_line(&Q, wI, 1, -1, s1, s2);
   ocplx *col = (ocplx *)malloc(sizeof(ocplx) * Pc);
   for (int m = -(L - 1); m <= L - 1; ++m) {
-    int mi = (m + M) % M;
+    int mi = (m + Mp) % Mp;
     /* unfold^H */
     for (int k = 0; k < Pc; ++k) col[k] = 0;
     for (int mp = 0; mp < L; ++mp) {
-      ocplx v = Jf[(size_t)mp * M + m + L - 1];
+      ocplx v = Jf[(size_t)mp * Mm + m + L - 1];
       col[mp] += v;
       if (mp > 0) col[(Pc - mp) % Pc] += or_sgn(m + n) * v;
     }
@@ -692,21 +725,25 @@ static void or_jf_to_map_adj(ocplx *map, const ocplx *Jf, int L, int n) {
     or_fft_line(&Q, col, 1, +1, s1, s2);
     for (int k = 0; k < Pc; ++k) col[k] /= Pc;
     /* R[k] = Fm[-k] -> Fm~[m''] = col[-m''], then phase^H and theta-DFT^H */
-    for (int t = 0; t < M; ++t) s1[t] = 0;
+    for (int t = 0; t < Mt; ++t) s1[t] = 0;
     for (int mpp = -(L - 1); mpp <= L - 1; ++mpp)
-      s1[(mpp + M) % M] = col[(-mpp + Pc) % Pc] * cexp(I * mpp * OPI / M) / M;
-    or_fft_rec(s1, 1, s2, M, P.fac, P.tw, M, +1);
-    for (int t = 0; t < M; ++t) G[(size_t)t * M + mi] = s2[t];
+      s1[(mpp + Mt) % Mt] = col[(-mpp + Pc) % Pc] * cexp(I * mpp * Gg.h) / Mt;
+    or_fft_rec(s1, 1, s2, Mt, Pt.fac, Pt.tw, Mt, +1);
+    for (int t = 0; t < Mt; ++t) G[(size_t)t * Mp + mi] = s2[t];
   }
   /* extension^H, then phi-DFT^H */
-  for (int t = 0; t < L - 1; ++t)
+  for (int t = Gg.nt; t < Mt; ++t)
     for (int m = -(L - 1); m <= L - 1; ++m) {
-      int mi = (m + M) % M;
-      G[(size_t)t * M + mi] += or_sgn(m + n) * G[(size_t)(M - 1 - t) * M + mi];
+      int mi = (m + Mp) % Mp;
+      G[(size_t)or_ext_row(&Gg, t) * Mp + mi] += or_sgn(m + n) * G[(size_t)t * Mp + mi];
     }
-  for (int t = 0; t < L; ++t) {
-    or_fft_line(&P, G + (size_t)t * M, 1, +1, s1, s2);
-    for (int k = 0; k < M; ++k) map[(size_t)t * M + k] = G[(size_t)t * M + k] * (2.0 * OPI / M);
+  for (int t = 0; t < Gg.nt; ++t) {
+    for (int k = 0; k < Mp; ++k) {  /* only |m| < L carry data */
+      int m = k <= L - 1 ? k : k - Mp;
+      if (m < -(L - 1)) G[(size_t)t * Mp + k] = 0;
+    }
+    or_fft_line(&P, G + (size_t)t * Mp, 1, +1, s1, s2);
+    for (int k = 0; k < Mp; ++k) map[(size_t)t * Mp + k] = G[(size_t)t * Mp + k] * (2.0 * OPI / Mp);
   }
   free(col);
   free(wI);
@@ -714,6 +751,7 @@ static void or_jf_to_map_adj(ocplx *map, const ocplx *Jf, int L, int n) {
   free(s2);
   free(G);
   or_fft_free(&P);
+  or_fft_free(&Pt);
   or_fft_free(&Q);
 }
 
@@ -788,24 +826,25 @@ static double *or_ssht_weights(int L, int spin) {
   return w;
 }
 
-/* ssht_core_mw_inverse_sov_sym (called at s2let_analysis.c:324, s2let_synthesis.c:489) */
-void oracle_ssht_inverse(ocplx *f, const ocplx *flm, int L, int spin) {
+/* ssht_core_mw_inverse_sov_sym (called at s2let_analysis.c:324, s2let_synthesis.c:489);
+   ss: ssht_core_mw_inverse_sov_sym_ss (s2let_analysis.c:327) */
+void oracle_ssht_inverse_s(ocplx *f, const ocplx *flm, int L, int spin, int ss) {
   int M = 2 * L - 1, n = -spin;
   double *w = or_ssht_weights(L, spin);
   or_col col = {n, abs(spin), w};
   ocplx *F = (ocplx *)calloc((size_t)L * M, sizeof(ocplx));
   ocplx *x = (ocplx *)flm;
   or_contract_inv(&F, &x, &col, 1, L);
-  or_spec_to_map(f, F, L, n);
+  or_spec_to_map(f, F, L, n, ss);
   free(F);
   free(w);
 }
 
 /* ssht_adjoint_mw_inverse_sov_sym (s2let_analysis_adjoint.c:336, s2let_synthesis_adjoint.c:480) */
-void oracle_ssht_adjoint_inverse(ocplx *flm, const ocplx *f, int L, int spin) {
+void oracle_ssht_adjoint_inverse_s(ocplx *flm, const ocplx *f, int L, int spin, int ss) {
   int M = 2 * L - 1, n = -spin;
   ocplx *F = (ocplx *)calloc((size_t)L * M, sizeof(ocplx));
-  or_map_to_spec_adj(F, f, L, n);
+  or_map_to_spec_adj(F, f, L, n, ss);
   or_col col = {n, abs(spin), NULL};
   ocplx *y = flm;
   or_contract_fwd(&y, &F, &col, 1, L);
@@ -814,11 +853,12 @@ void oracle_ssht_adjoint_inverse(ocplx *flm, const ocplx *f, int L, int spin) {
   free(F);
 }
 
-/* ssht_core_mw_forward_sov_conv_sym (s2let_analysis.c:471, s2let_synthesis.c:335) */
-void oracle_ssht_forward(ocplx *flm, const ocplx *f, int L, int spin) {
+/* ssht_core_mw_forward_sov_conv_sym (s2let_analysis.c:471, s2let_synthesis.c:335);
+   ss: ssht_core_mw_forward_sov_conv_sym_ss (s2let_analysis.c:474) */
+void oracle_ssht_forward_s(ocplx *flm, const ocplx *f, int L, int spin, int ss) {
   int M = 2 * L - 1, n = -spin;
   ocplx *Jf = (ocplx *)calloc((size_t)L * M, sizeof(ocplx));
-  or_map_to_jf(Jf, f, L, n);
+  or_map_to_jf(Jf, f, L, n, ss);
   or_col col = {n, abs(spin), NULL};
   ocplx *y = flm;
   or_contract_fwd(&y, &Jf, &col, 1, L);
@@ -831,7 +871,7 @@ void oracle_ssht_forward(ocplx *flm, const ocplx *f, int L, int spin) {
 }
 
 /* ssht_adjoint_mw_forward_sov_sym (s2let_analysis_adjoint.c:506, s2let_synthesis_adjoint.c:328) */
-void oracle_ssht_adjoint_forward(ocplx *f, const ocplx *flm, int L, int spin) {
+void oracle_ssht_adjoint_forward_s(ocplx *f, const ocplx *flm, int L, int spin, int ss) {
   int M = 2 * L - 1, n = -spin;
   ocplx *x = (ocplx *)calloc((size_t)L * L, sizeof(ocplx));
   for (int l = abs(spin); l < L; ++l)
@@ -840,10 +880,19 @@ void oracle_ssht_adjoint_forward(ocplx *f, const ocplx *flm, int L, int spin) {
   or_col col = {n, abs(spin), w};
   ocplx *Jf = (ocplx *)calloc((size_t)L * M, sizeof(ocplx));
   or_contract_inv(&Jf, &x, &col, 1, L);
-  or_jf_to_map_adj(f, Jf, L, n);
+  or_jf_to_map_adj(f, Jf, L, n, ss);
   free(Jf);
   free(x);
   free(w);
+}
+
+void oracle_ssht_inverse(ocplx *f, const ocplx *flm, int L, int spin) { oracle_ssht_inverse_s(f, flm, L, spin, 0); }
+void oracle_ssht_adjoint_inverse(ocplx *flm, const ocplx *f, int L, int spin) {
+  oracle_ssht_adjoint_inverse_s(flm, f, L, spin, 0);
+}
+void oracle_ssht_forward(ocplx *flm, const ocplx *f, int L, int spin) { oracle_ssht_forward_s(flm, f, L, spin, 0); }
+void oracle_ssht_adjoint_forward(ocplx *f, const ocplx *flm, int L, int spin) {
+  oracle_ssht_adjoint_forward_s(f, flm, L, spin, 0);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -857,8 +906,9 @@ static int or_n_active(int n, int nmode) {
 }
 
 /* so3_core_inverse_direct (called at s2let_analysis.c:348) */
-void oracle_so3_inverse(ocplx *f, const ocplx *flmn, int L, int N, int L0, int nmode) {
+void oracle_so3_inverse_s(ocplx *f, const ocplx *flmn, int L, int N, int L0, int nmode, int ss) {
   int M = 2 * L - 1, G = 2 * N - 1, nc = 0;
+  const size_t NP = (size_t)or_map_size(L, ss); /* samples of one gamma slice */
   or_col cols[64];
   ocplx *F[64], *x[64];
   double *w = (double *)calloc(L, sizeof(double));
@@ -875,14 +925,14 @@ void oracle_so3_inverse(ocplx *f, const ocplx *flmn, int L, int N, int L0, int n
     ++nc;
   }
   or_contract_inv(F, x, cols, nc, L);
-  memset(f, 0, sizeof(ocplx) * (size_t)or_f_size(L, N));
-  ocplx *map = (ocplx *)malloc(sizeof(ocplx) * (size_t)L * M);
+  memset(f, 0, sizeof(ocplx) * NP * G);
+  ocplx *map = (ocplx *)malloc(sizeof(ocplx) * NP);
   for (int c = 0; c < nc; ++c) {
-    or_spec_to_map(map, F[c], L, cols[c].n);
+    or_spec_to_map(map, F[c], L, cols[c].n, ss);
     for (int g = 0; g < G; ++g) {
       ocplx ph = cexp(I * cols[c].n * 2.0 * OPI * g / G);
-      ocplx *fg = f + (size_t)g * L * M;
-      for (size_t i = 0; i < (size_t)L * M; ++i) fg[i] += ph * map[i];
+      ocplx *fg = f + (size_t)g * NP;
+      for (size_t i = 0; i < NP; ++i) fg[i] += ph * map[i];
     }
     free(F[c]);
     free(x[c]);
@@ -892,24 +942,25 @@ void oracle_so3_inverse(ocplx *f, const ocplx *flmn, int L, int N, int L0, int n
 }
 
 /* adjoint of oracle_so3_inverse (so3_adjoint_inverse_direct, s2let_analysis_adjoint.c:371) */
-void oracle_so3_adjoint_inverse(ocplx *flmn, const ocplx *f, int L, int N, int L0, int nmode) {
+void oracle_so3_adjoint_inverse_s(ocplx *flmn, const ocplx *f, int L, int N, int L0, int nmode, int ss) {
   int M = 2 * L - 1, G = 2 * N - 1, nc = 0;
+  const size_t NP = (size_t)or_map_size(L, ss); /* samples of one gamma slice */
   or_col cols[64];
   ocplx *F[64], *y[64];
-  ocplx *map = (ocplx *)malloc(sizeof(ocplx) * (size_t)L * M);
+  ocplx *map = (ocplx *)malloc(sizeof(ocplx) * NP);
   for (int n = -N + 1; n <= N - 1; ++n) {
     if (!or_n_active(n, nmode)) continue;
     cols[nc].n = n;
     cols[nc].lmin = OMAX(L0, abs(n));
     cols[nc].w = NULL;
-    for (size_t i = 0; i < (size_t)L * M; ++i) map[i] = 0;
+    for (size_t i = 0; i < NP; ++i) map[i] = 0;
     for (int g = 0; g < G; ++g) {
       ocplx ph = cexp(-I * n * 2.0 * OPI * g / G);
-      const ocplx *fg = f + (size_t)g * L * M;
-      for (size_t i = 0; i < (size_t)L * M; ++i) map[i] += ph * fg[i];
+      const ocplx *fg = f + (size_t)g * NP;
+      for (size_t i = 0; i < NP; ++i) map[i] += ph * fg[i];
     }
     F[nc] = (ocplx *)calloc((size_t)L * M, sizeof(ocplx));
-    or_map_to_spec_adj(F[nc], map, L, n);
+    or_map_to_spec_adj(F[nc], map, L, n, ss);
     y[nc] = (ocplx *)calloc((size_t)L * L, sizeof(ocplx));
     ++nc;
   }
@@ -929,24 +980,25 @@ void oracle_so3_adjoint_inverse(ocplx *flmn, const ocplx *f, int L, int N, int L
 /* so3_core_forward_via_ssht (called at s2let_synthesis.c:361):
  * f^l_mn = i^{n-m} sum_{m'} Delta^l_{m'm} Delta^l_{m'n} J_n(m, m'),
  * J_n from F_n(a,b) = (2 pi/(2N-1)) sum_g f e^{-i n gamma_g} by the MW quadrature. */
-void oracle_so3_forward(ocplx *flmn, const ocplx *f, int L, int N, int L0, int nmode) {
+void oracle_so3_forward_s(ocplx *flmn, const ocplx *f, int L, int N, int L0, int nmode, int ss) {
   int M = 2 * L - 1, G = 2 * N - 1, nc = 0;
+  const size_t NP = (size_t)or_map_size(L, ss); /* samples of one gamma slice */
   or_col cols[64];
   ocplx *Jf[64], *y[64];
-  ocplx *map = (ocplx *)malloc(sizeof(ocplx) * (size_t)L * M);
+  ocplx *map = (ocplx *)malloc(sizeof(ocplx) * NP);
   for (int n = -N + 1; n <= N - 1; ++n) {
     if (!or_n_active(n, nmode)) continue;
     cols[nc].n = n;
     cols[nc].lmin = OMAX(L0, abs(n));
     cols[nc].w = NULL;
-    for (size_t i = 0; i < (size_t)L * M; ++i) map[i] = 0;
+    for (size_t i = 0; i < NP; ++i) map[i] = 0;
     for (int g = 0; g < G; ++g) {
       ocplx ph = cexp(-I * n * 2.0 * OPI * g / G) * (2.0 * OPI / G);
-      const ocplx *fg = f + (size_t)g * L * M;
-      for (size_t i = 0; i < (size_t)L * M; ++i) map[i] += ph * fg[i];
+      const ocplx *fg = f + (size_t)g * NP;
+      for (size_t i = 0; i < NP; ++i) map[i] += ph * fg[i];
     }
     Jf[nc] = (ocplx *)calloc((size_t)L * M, sizeof(ocplx));
-    or_map_to_jf(Jf[nc], map, L, n);
+    or_map_to_jf(Jf[nc], map, L, n, ss);
     y[nc] = (ocplx *)calloc((size_t)L * L, sizeof(ocplx));
     ++nc;
   }
@@ -963,8 +1015,9 @@ void oracle_so3_forward(ocplx *flmn, const ocplx *f, int L, int N, int L0, int n
 }
 
 /* adjoint of oracle_so3_forward (so3_adjoint_forward_direct, s2let_synthesis_adjoint.c:352) */
-void oracle_so3_adjoint_forward(ocplx *f, const ocplx *flmn, int L, int N, int L0, int nmode) {
+void oracle_so3_adjoint_forward_s(ocplx *f, const ocplx *flmn, int L, int N, int L0, int nmode, int ss) {
   int M = 2 * L - 1, G = 2 * N - 1, nc = 0;
+  const size_t NP = (size_t)or_map_size(L, ss); /* samples of one gamma slice */
   or_col cols[64];
   ocplx *Jf[64], *x[64];
   double *w = (double *)calloc(L, sizeof(double));
@@ -982,20 +1035,33 @@ void oracle_so3_adjoint_forward(ocplx *f, const ocplx *flmn, int L, int N, int L
     ++nc;
   }
   or_contract_inv(Jf, x, cols, nc, L);
-  memset(f, 0, sizeof(ocplx) * (size_t)or_f_size(L, N));
-  ocplx *map = (ocplx *)malloc(sizeof(ocplx) * (size_t)L * M);
+  memset(f, 0, sizeof(ocplx) * NP * G);
+  ocplx *map = (ocplx *)malloc(sizeof(ocplx) * NP);
   for (int c = 0; c < nc; ++c) {
-    or_jf_to_map_adj(map, Jf[c], L, cols[c].n);
+    or_jf_to_map_adj(map, Jf[c], L, cols[c].n, ss);
     for (int g = 0; g < G; ++g) {
       ocplx ph = cexp(I * cols[c].n * 2.0 * OPI * g / G) * (2.0 * OPI / G);
-      ocplx *fg = f + (size_t)g * L * M;
-      for (size_t i = 0; i < (size_t)L * M; ++i) fg[i] += ph * map[i];
+      ocplx *fg = f + (size_t)g * NP;
+      for (size_t i = 0; i < NP; ++i) fg[i] += ph * map[i];
     }
     free(Jf[c]);
     free(x[c]);
   }
   free(map);
   free(w);
+}
+
+void oracle_so3_inverse(ocplx *f, const ocplx *flmn, int L, int N, int L0, int nmode) {
+  oracle_so3_inverse_s(f, flmn, L, N, L0, nmode, 0);
+}
+void oracle_so3_adjoint_inverse(ocplx *flmn, const ocplx *f, int L, int N, int L0, int nmode) {
+  oracle_so3_adjoint_inverse_s(flmn, f, L, N, L0, nmode, 0);
+}
+void oracle_so3_forward(ocplx *flmn, const ocplx *f, int L, int N, int L0, int nmode) {
+  oracle_so3_forward_s(flmn, f, L, N, L0, nmode, 0);
+}
+void oracle_so3_adjoint_forward(ocplx *f, const ocplx *flmn, int L, int N, int L0, int nmode) {
+  oracle_so3_adjoint_forward_s(f, flmn, L, N, L0, nmode, 0);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1100,13 +1166,13 @@ void oracle_analysis_lm2wav(ocplx *f_wav, ocplx *f_scal, const ocplx *flm, const
   f_scal_lm = (ocplx *)calloc(oracle_n_lm_scal(p), sizeof(ocplx));
   oracle_analysis_lm2lmn(f_wav_lmn, f_scal_lm, flm, wav_lm, scal_l, p);
   int bl = or_scal_bandlimit(p);
-  oracle_ssht_inverse(f_scal, f_scal_lm, bl, 0);
+  oracle_ssht_inverse_s(f_scal, f_scal_lm, bl, 0, or_ss(p));
   int off = 0, off_lmn = 0, J = oracle_j_max(p), Lj, Nj;
   for (int j = p->J_min; j <= J; ++j) {
     or_scale(p, j, &Lj, &Nj);
-    oracle_so3_inverse(f_wav + off, f_wav_lmn + off_lmn, Lj, Nj, oracle_L0(j, p), or_nmode(p->N));
+    oracle_so3_inverse_s(f_wav + off, f_wav_lmn + off_lmn, Lj, Nj, oracle_L0(j, p), or_nmode(p->N), or_ss(p));
     off_lmn += oracle_flmn_size(Lj, Nj);
-    off += or_f_size(Lj, Nj);
+    off += or_f_size_s(Lj, Nj, or_ss(p));
   }
   free(wav_lm);
   free(scal_l);
@@ -1122,13 +1188,13 @@ void oracle_synthesis_wav2lm(ocplx *flm, const ocplx *f_wav, const ocplx *f_scal
   f_wav_lmn = (ocplx *)calloc(oracle_n_lmn_wav(p), sizeof(ocplx));
   f_scal_lm = (ocplx *)calloc(oracle_n_lm_scal(p), sizeof(ocplx));
   int bl = or_scal_bandlimit(p);
-  oracle_ssht_forward(f_scal_lm, f_scal, bl, 0);
+  oracle_ssht_forward_s(f_scal_lm, f_scal, bl, 0, or_ss(p));
   int off = 0, off_lmn = 0, J = oracle_j_max(p), Lj, Nj;
   for (int j = p->J_min; j <= J; ++j) {
     or_scale(p, j, &Lj, &Nj);
-    oracle_so3_forward(f_wav_lmn + off_lmn, f_wav + off, Lj, Nj, oracle_L0(j, p), or_nmode(p->N));
+    oracle_so3_forward_s(f_wav_lmn + off_lmn, f_wav + off, Lj, Nj, oracle_L0(j, p), or_nmode(p->N), or_ss(p));
     off_lmn += oracle_flmn_size(Lj, Nj);
-    off += or_f_size(Lj, Nj);
+    off += or_f_size_s(Lj, Nj, or_ss(p));
   }
   oracle_synthesis_lmn2lm(flm, f_wav_lmn, f_scal_lm, wav_lm, scal_l, p);
   free(wav_lm);
@@ -1140,7 +1206,7 @@ void oracle_synthesis_wav2lm(ocplx *flm, const ocplx *f_wav, const ocplx *f_scal
 /* s2let_analysis.c:456-483 */
 void oracle_analysis_px2wav(ocplx *f_wav, ocplx *f_scal, const ocplx *f, const oracle_params_t *p) {
   ocplx *flm = (ocplx *)calloc((size_t)p->L * p->L, sizeof(ocplx));
-  oracle_ssht_forward(flm, f, p->L, p->spin);
+  oracle_ssht_forward_s(flm, f, p->L, p->spin, or_ss(p));
   oracle_analysis_lm2wav(f_wav, f_scal, flm, p);
   free(flm);
 }
@@ -1149,7 +1215,7 @@ void oracle_analysis_px2wav(ocplx *f_wav, ocplx *f_scal, const ocplx *f, const o
 void oracle_synthesis_wav2px(ocplx *f, const ocplx *f_wav, const ocplx *f_scal, const oracle_params_t *p) {
   ocplx *flm = (ocplx *)calloc((size_t)p->L * p->L, sizeof(ocplx));
   oracle_synthesis_wav2lm(flm, f_wav, f_scal, p);
-  oracle_ssht_inverse(f, flm, p->L, p->spin);
+  oracle_ssht_inverse_s(f, flm, p->L, p->spin, or_ss(p));
   free(flm);
 }
 
@@ -1161,13 +1227,13 @@ void oracle_analysis_adjoint_wav2lm(ocplx *flm, const ocplx *f_wav, const ocplx 
   f_wav_lmn = (ocplx *)calloc(oracle_n_lmn_wav(p), sizeof(ocplx));
   f_scal_lm = (ocplx *)calloc(oracle_n_lm_scal(p), sizeof(ocplx));
   int bl = or_scal_bandlimit(p);
-  oracle_ssht_adjoint_inverse(f_scal_lm, f_scal, bl, 0);
+  oracle_ssht_adjoint_inverse_s(f_scal_lm, f_scal, bl, 0, or_ss(p));
   int off = 0, off_lmn = 0, J = oracle_j_max(p), Lj, Nj;
   for (int j = p->J_min; j <= J; ++j) {
     or_scale(p, j, &Lj, &Nj);
-    oracle_so3_adjoint_inverse(f_wav_lmn + off_lmn, f_wav + off, Lj, Nj, oracle_L0(j, p), or_nmode(p->N));
+    oracle_so3_adjoint_inverse_s(f_wav_lmn + off_lmn, f_wav + off, Lj, Nj, oracle_L0(j, p), or_nmode(p->N), or_ss(p));
     off_lmn += oracle_flmn_size(Lj, Nj);
-    off += or_f_size(Lj, Nj);
+    off += or_f_size_s(Lj, Nj, or_ss(p));
   }
   oracle_analysis_adjoint_lmn2lm(flm, f_wav_lmn, f_scal_lm, wav_lm, scal_l, p);
   free(wav_lm);
@@ -1180,7 +1246,7 @@ void oracle_analysis_adjoint_wav2lm(ocplx *flm, const ocplx *f_wav, const ocplx 
 void oracle_analysis_adjoint_wav2px(ocplx *f, const ocplx *f_wav, const ocplx *f_scal, const oracle_params_t *p) {
   ocplx *flm = (ocplx *)calloc((size_t)p->L * p->L, sizeof(ocplx));
   oracle_analysis_adjoint_wav2lm(flm, f_wav, f_scal, p);
-  oracle_ssht_adjoint_forward(f, flm, p->L, p->spin);
+  oracle_ssht_adjoint_forward_s(f, flm, p->L, p->spin, or_ss(p));
   free(flm);
 }
 
@@ -1193,13 +1259,13 @@ void oracle_synthesis_adjoint_lm2wav(ocplx *f_wav, ocplx *f_scal, const ocplx *f
   f_scal_lm = (ocplx *)calloc(oracle_n_lm_scal(p), sizeof(ocplx));
   oracle_synthesis_adjoint_lm2lmn(f_wav_lmn, f_scal_lm, flm, wav_lm, scal_l, p);
   int bl = or_scal_bandlimit(p);
-  oracle_ssht_adjoint_forward(f_scal, f_scal_lm, bl, 0);
+  oracle_ssht_adjoint_forward_s(f_scal, f_scal_lm, bl, 0, or_ss(p));
   int off = 0, off_lmn = 0, J = oracle_j_max(p), Lj, Nj;
   for (int j = p->J_min; j <= J; ++j) {
     or_scale(p, j, &Lj, &Nj);
-    oracle_so3_adjoint_forward(f_wav + off, f_wav_lmn + off_lmn, Lj, Nj, oracle_L0(j, p), or_nmode(p->N));
+    oracle_so3_adjoint_forward_s(f_wav + off, f_wav_lmn + off_lmn, Lj, Nj, oracle_L0(j, p), or_nmode(p->N), or_ss(p));
     off_lmn += oracle_flmn_size(Lj, Nj);
-    off += or_f_size(Lj, Nj);
+    off += or_f_size_s(Lj, Nj, or_ss(p));
   }
   free(wav_lm);
   free(scal_l);
@@ -1210,7 +1276,7 @@ void oracle_synthesis_adjoint_lm2wav(ocplx *f_wav, ocplx *f_scal, const ocplx *f
 /* s2let_synthesis_adjoint.c:465-492 */
 void oracle_synthesis_adjoint_px2wav(ocplx *f_wav, ocplx *f_scal, const ocplx *f, const oracle_params_t *p) {
   ocplx *flm = (ocplx *)calloc((size_t)p->L * p->L, sizeof(ocplx));
-  oracle_ssht_adjoint_inverse(flm, f, p->L, p->spin);
+  oracle_ssht_adjoint_inverse_s(flm, f, p->L, p->spin, or_ss(p));
   oracle_synthesis_adjoint_lm2wav(f_wav, f_scal, flm, p);
   free(flm);
 }

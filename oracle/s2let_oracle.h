@@ -48,7 +48,7 @@ typedef struct {
   int N;
   int spin;
   int original_spin;
-  int sampling_scheme; /* 0 = MW (only MW is restated) */
+  int sampling_scheme; /* 0 = MW, 1 = MWSS */
   int dl_method;       /* 0 = Risbo */
 } oracle_params_t;
 
@@ -84,6 +84,16 @@ void oracle_ssht_inverse(ocplx *f, const ocplx *flm, int L, int spin);
 void oracle_ssht_forward(ocplx *flm, const ocplx *f, int L, int spin);
 void oracle_ssht_adjoint_inverse(ocplx *flm, const ocplx *f, int L, int spin);
 void oracle_ssht_adjoint_forward(ocplx *f, const ocplx *flm, int L, int spin);
+
+/* ss = 1: MWSS sampling (theta_t = pi t / L, t <= L; phi_p = pi p / L, p < 2L) */
+void oracle_ssht_inverse_s(ocplx *f, const ocplx *flm, int L, int spin, int ss);
+void oracle_ssht_forward_s(ocplx *flm, const ocplx *f, int L, int spin, int ss);
+void oracle_ssht_adjoint_inverse_s(ocplx *flm, const ocplx *f, int L, int spin, int ss);
+void oracle_ssht_adjoint_forward_s(ocplx *f, const ocplx *flm, int L, int spin, int ss);
+void oracle_so3_inverse_s(ocplx *f, const ocplx *flmn, int L, int N, int L0, int nmode, int ss);
+void oracle_so3_forward_s(ocplx *flmn, const ocplx *f, int L, int N, int L0, int nmode, int ss);
+void oracle_so3_adjoint_inverse_s(ocplx *flmn, const ocplx *f, int L, int N, int L0, int nmode, int ss);
+void oracle_so3_adjoint_forward_s(ocplx *f, const ocplx *flmn, int L, int N, int L0, int nmode, int ss);
 
 /* SO3 (MW, compact, negative-n first); nmode 0 = all, 1 = even only, 2 = odd only */
 void oracle_so3_inverse(ocplx *f, const ocplx *flmn, int L, int N, int L0, int nmode);
