@@ -1,6 +1,6 @@
 /*
  * s2let_hip.h -- C ABI of the MI355X-native S2LET directional/spin wavelet
- * transform (MW sampling).  Drop-in for the reference's C API: same symbol
+ * transform (MW and MWSS sampling).  Drop-in for the reference's C API: same symbol
  * names, same s2let_parameters_t layout, same buffer layouts and sizes, same
  * void return and exit(1)-on-unsupported-sampling behaviour.  Host pointers in
  * and out; each call copies to the GPU, runs hand-written HIP kernels and
@@ -98,6 +98,8 @@ void s2let_allocate_lmn_f_wav(S2LET_COMPLEX(double) **f_wav_lmn, S2LET_COMPLEX(d
 void s2let_allocate_f_wav(S2LET_COMPLEX(double) **f_wav, S2LET_COMPLEX(double) **f_scal,
                           const s2let_parameters_t *parameters);
 void s2let_allocate_mw_real(double **f, int L);                           /* s2let_alloc.c:28 */
+void s2let_allocate_mwss(S2LET_COMPLEX(double) **f, int L);               /* s2let_alloc.c:37 */
+void s2let_allocate_mwss_real(double **f, int L);                         /* s2let_alloc.c:46 */
 void s2let_allocate_f_wav_real(double **f_wav, double **f_scal,
                                const s2let_parameters_t *parameters);     /* s2let_alloc.c:122 */
 

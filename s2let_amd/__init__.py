@@ -66,6 +66,7 @@ _SIGS = {
     "s2let_n_lm_scal": (ctypes.c_int, [ctypes.c_void_p]),
     "s2let_n_wav_j": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
     "s2let_n_gamma": (ctypes.c_int, [ctypes.c_void_p]),
+    "s2let_n_px": (ctypes.c_int, [ctypes.c_void_p]),
     "s2let_hip_last_error": (ctypes.c_int, []),
     "s2let_hip_last_error_string": (ctypes.c_char_p, []),
     "s2let_hip_version": (ctypes.c_char_p, []),
@@ -192,6 +193,11 @@ def n_lmn_wav(p):
     return lib().s2let_n_lmn_wav(_ref(p))
 
 
+def n_px(p):
+    """Samples of one map: L (2L - 1) on the MW grid, (L + 1) 2L on the MWSS grid."""
+    return lib().s2let_n_px(_ref(p))
+
+
 def n_lm_scal(p):
     return lib().s2let_n_lm_scal(_ref(p))
 
@@ -240,7 +246,7 @@ def analysis_px2wav(f, p):
 
 def synthesis_wav2px(f_wav, f_scal, p):
     a, b = _cin(f_wav), _cin(f_scal)
-    f = np.zeros(p.L * (2 * p.L - 1), np.complex128)
+    f = np.zeros(n_px(p), np.complex128)
     lib().s2let_synthesis_wav2px(_ptr(f), _ptr(a), _ptr(b), _ref(p))
     return f
 
@@ -254,7 +260,7 @@ def analysis_adjoint_wav2lm(f_wav, f_scal, p):
 
 def analysis_adjoint_wav2px(f_wav, f_scal, p):
     a, b = _cin(f_wav), _cin(f_scal)
-    f = np.zeros(p.L * (2 * p.L - 1), np.complex128)
+    f = np.zeros(n_px(p), np.complex128)
     lib().s2let_analysis_adjoint_wav2px(_ptr(f), _ptr(a), _ptr(b), _ref(p))
     return f
 
@@ -318,7 +324,7 @@ def analysis_px2wav_real(f, p):
 
 def synthesis_wav2px_real(f_wav, f_scal, p):
     a, b = _rin(f_wav), _rin(f_scal)
-    f = np.zeros(p.L * (2 * p.L - 1), np.float64)
+    f = np.zeros(n_px(p), np.float64)
     lib().s2let_synthesis_wav2px_real(_ptr(f), _ptr(a), _ptr(b), _ref(p))
     return f
 
@@ -332,7 +338,7 @@ def analysis_adjoint_wav2lm_real(f_wav, f_scal, p):
 
 def analysis_adjoint_wav2px_real(f_wav, f_scal, p):
     a, b = _rin(f_wav), _rin(f_scal)
-    f = np.zeros(p.L * (2 * p.L - 1), np.float64)
+    f = np.zeros(n_px(p), np.float64)
     lib().s2let_analysis_adjoint_wav2px_real(_ptr(f), _ptr(a), _ptr(b), _ref(p))
     return f
 

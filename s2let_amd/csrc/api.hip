@@ -26,9 +26,9 @@ void clear_error() {
 namespace {
 
 // The reference's only detected error: unsupported sampling scheme -> message + exit(1)
-// (s2let_error.h:13-22, raised at e.g. s2let_analysis.c:330-331).
+// (s2let_error.h:13-22, raised at e.g. s2let_analysis.c:330-331).  MW and MWSS are supported.
 void require_mw(const s2let_parameters_t *p, const char *fn) {
-  if (p->sampling_scheme != S2LET_SAMPLING_MW) {
+  if (p->sampling_scheme != S2LET_SAMPLING_MW && p->sampling_scheme != S2LET_SAMPLING_MW_SS) {
     printf("ERROR: %s.\n", "Sampling scheme not supported");
     printf("ERROR: %s <%s> %s %s %s %d.\n", "Occurred in function", fn, "of file", __FILE__, "on line", __LINE__);
     exit(1);
@@ -78,8 +78,8 @@ struct HostIO {
     nlm = (size_t)L * L;
     nwav = nwav_used(pl);
     const int Ls = pl.groups[0].L;  // scaling-function band-limit
-    nscal = (size_t)Ls * (2 * Ls - 1);
-    npx = (size_t)L * (2 * L - 1);
+    nscal = map_elems(Ls, pl.ss);
+    npx = map_elems(L, pl.ss);
     pl.h_flm.alloc(nlm * CB);
     pl.h_wav.alloc(nwav * CB);
     pl.h_scal.alloc(nscal * CB);
@@ -154,8 +154,8 @@ struct RealWork {
     const int L = pl.p.L;
     nwav = pl.n_wav_used;
     const int Ls = pl.groups[0].L;  // scaling-function band-limit
-    nscal = (size_t)Ls * (2 * Ls - 1);
-    npx = (size_t)L * (2 * L - 1);
+    nscal = map_elems(Ls, pl.ss);
+    npx = map_elems(L, pl.ss);
     if (pl.r_wav.bytes < nwav * CB) pl.r_wav.alloc(nwav * CB);
     if (pl.r_scal.bytes < nscal * CB) pl.r_scal.alloc(nscal * CB);
     if (pl.r_map.bytes < npx * CB) pl.r_map.alloc(npx * CB);
@@ -578,7 +578,7 @@ void s2let_analysis_px2wav(std::complex<double> *f_wav, std::complex<double> *f_
   require_mw(p, __func__);
   host_call(__func__, [&] {
     S2H_HOST_PLAN
-    ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st);
+    ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st, P.ss);
     h2d(P.h_map.p, f, io.npx * CB, st);
     run_sht_forward(S, P.h_map.p, P.h_tmp.p, st);
     run_analysis(P, P.h_tmp.p, P.h_wav.p, P.h_scal.p, st);
@@ -593,7 +593,7 @@ void s2let_synthesis_wav2px(std::complex<double> *f, const std::complex<double> 
   require_mw(p, __func__);
   host_call(__func__, [&] {
     S2H_HOST_PLAN
-    ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st);
+    ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st, P.ss);
     h2d(P.h_wav.p, f_wav, io.nwav * CB, st);
     h2d(P.h_scal.p, f_scal, io.nscal * CB, st);
     run_synthesis(P, P.h_wav.p, P.h_scal.p, P.h_tmp.p, st);
@@ -608,7 +608,7 @@ void s2let_analysis_adjoint_wav2px(std::complex<double> *f, const std::complex<d
   require_mw(p, __func__);
   host_call(__func__, [&] {
     S2H_HOST_PLAN
-    ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st);
+    ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st, P.ss);
     h2d(P.h_wav.p, f_wav, io.nwav * CB, st);
     h2d(P.h_scal.p, f_scal, io.nscal * CB, st);
     run_analysis_adjoint(P, P.h_wav.p, P.h_scal.p, P.h_tmp.p, st);
@@ -623,7 +623,7 @@ void s2let_synthesis_adjoint_px2wav(std::complex<double> *f_wav, std::complex<do
   require_mw(p, __func__);
   host_call(__func__, [&] {
     S2H_HOST_PLAN
-    ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st);
+    ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st, P.ss);
     h2d(P.h_map.p, f, io.npx * CB, st);
     run_sht_inverse_adjoint(S, P.h_map.p, P.h_tmp.p, st);
     run_synthesis_adjoint(P, P.h_tmp.p, P.h_wav.p, P.h_scal.p, st);
@@ -891,7 +891,7 @@ void s2let_analysis_px2wav_real(double *f_wav, double *f_scal, const double *f, 
   host_call(__func__, [&] {
     S2H_HOST_PLAN
     RealWork w(P);
-    ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st);  // real maps: spin 0
+    ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st, P.ss);  // real maps: spin 0
     h2d(P.r_real.p, f, w.npx * DB, st);
     launch_re2cx(P.r_real.as<double>(), P.r_map.p, w.npx, st);
     run_sht_forward(S, P.r_map.p, P.h_tmp.p, st);
@@ -907,7 +907,7 @@ void s2let_synthesis_wav2px_real(double *f, const double *f_wav, const double *f
   host_call(__func__, [&] {
     S2H_HOST_PLAN
     RealWork w(P);
-    ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st);
+    ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st, P.ss);
     h2d(P.r_real.p, f_wav, w.nwav * DB, st);
     h2d(P.r_real2.p, f_scal, w.nscal * DB, st);
     real_synthesis(P, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_tmp.p, st);
@@ -938,7 +938,7 @@ void s2let_analysis_adjoint_wav2px_real(double *f, const double *f_wav, const do
   host_call(__func__, [&] {
     S2H_HOST_PLAN
     RealWork w(P);
-    ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st);
+    ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st, P.ss);
     h2d(P.r_real.p, f_wav, w.nwav * DB, st);
     h2d(P.r_real2.p, f_scal, w.nscal * DB, st);
     real_analysis_adjoint(P, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_tmp.p, st);
@@ -969,7 +969,7 @@ void s2let_synthesis_adjoint_px2wav_real(double *f_wav, double *f_scal, const do
   host_call(__func__, [&] {
     S2H_HOST_PLAN
     RealWork w(P);
-    ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st);
+    ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st, P.ss);
     h2d(P.r_real.p, f, w.npx * DB, st);
     launch_re2cx(P.r_real.as<double>(), P.r_map.p, w.npx, st);
     run_sht_inverse_adjoint(S, P.r_map.p, P.h_tmp.p, st);

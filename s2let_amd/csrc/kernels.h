@@ -31,11 +31,15 @@ struct SuperDev {
 // L x L (stored transposed, [k][i]) and D_in / D_out multiplying one row by
 // i^{in_e} / i^{out_e} (row -1: none).  phase = i^{-m} ph_c, or i^{m} conj(ph_c)
 // when conj_ph is set (adjoint kinds).
+// R output rows and K reduction rows of A (MW: both L; MWSS: L + 1 theta rows on
+// the sample side), column c's in / out plane at c * plane, rows of 2L - 1 (m).
 template <typename T>
 struct ThetaArgs {
   int L;
   int ncol;
   int conj_ph;
+  int R, K;
+  size_t plane;
   const ColDev *cols;
   const T *AT[2];
   int z_e[2], in_row[2], in_e[2], out_row[2], out_e[2];
@@ -58,11 +62,19 @@ struct Dft2 {
   const int *perm;
 };
 
+// Lines: rows_pp lines per plane, planes at a stride of `plane` entries; a line
+// holds in_len (out_len) entries on the input (output) side: 2L - 1 centred m
+// values (m = -(L-1) .. L-1) or d.M samples (MW 2L - 1, MWSS 2L).  A centred
+// side maps m to DFT bin m mod M; MWSS bin L (the Nyquist bin) reads as zero and
+// is not written.
 template <typename T>
 struct RowArgs {
   int L;
   int lpb;  // lines per block
   size_t nlines;
+  int rows_pp;
+  size_t plane;
+  int in_len, out_len;
   int in_centred, out_centred, sign;
   T scale;
   Dft2<T> d;
@@ -155,6 +167,9 @@ struct MixJobs {
 // theta stage (theta_mfma.hip).  Tables: 8 L x L matrices [kind][class],
 // kind 0 inverse, 1 inverse adjoint, 2 forward, 3 forward adjoint.
 void build_theta_tables(int L, const double *cs, const double *sn, double *th, hipStream_t st);
+// MWSS: the same 8 matrices for theta_t = pi t / L, t <= L (blocks of L (L + 1));
+// cs[j] = cos(pi j / L), sn[j] = sin(pi j / L), j < 2L
+void build_theta_tables_ss(int L, const double *cs, const double *sn, double *th, hipStream_t st);
 template <typename T>
 int theta_rows(int Lmax);  // output rows per block of the launch, for the largest L among its jobs
 template <typename T>

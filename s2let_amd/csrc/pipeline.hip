@@ -121,16 +121,21 @@ size_t lds_budget(Context &C) {
 // (m + n odd): K_inv = i A diag(-i, 1, ...), K_fwd = -i diag(i, 1, ...) A; the
 // adjoints conjugate and transpose these.
 template <typename T>
-ThetaArgs<T> theta_args(int L, const ColDev *cols, int ncol, int kind) {
-  const LTables &lt = Context::get().ltab(L);
+ThetaArgs<T> theta_args(int L, const ColDev *cols, int ncol, int kind, int ss) {
+  const LTables &lt = Context::get().ltab(L, ss);
   ThetaArgs<T> a{};
   a.L = L;
   a.ncol = ncol;
   a.cols = cols;
   a.conj_ph = kind & 1;
+  // the sample side has L (MW) or L + 1 (MWSS) theta rows
+  const bool to_samples = (kind == 0 || kind == 3);
+  a.R = to_samples ? L + ss : L;
+  a.K = to_samples ? L : L + ss;
+  a.plane = map_elems(L, ss);
   const T *th = Sel<T>::theta(lt);
   for (int c = 0; c < 2; ++c) {
-    a.AT[c] = th + (size_t)(kind * 2 + c) * L * L;
+    a.AT[c] = th + (size_t)(kind * 2 + c) * L * (L + ss);
     a.z_e[c] = 0;
     a.in_row[c] = a.out_row[c] = -1;
     a.in_e[c] = a.out_e[c] = 0;
@@ -145,12 +150,21 @@ ThetaArgs<T> theta_args(int L, const ColDev *cols, int ncol, int kind) {
 }
 
 template <typename T>
-RowArgs<T> row_args(int L, size_t nlines, int in_c, int out_c, int sign, double scale) {
+RowArgs<T> row_args(int L, size_t nlines, int in_c, int out_c, int sign, double scale, int ss) {
   Context &C = Context::get();
-  const LTables &lt = C.ltab(L);
+  const LTables &lt = C.ltab(L, ss);
   RowArgs<T> a{};
   a.L = L;
   a.nlines = nlines;
+  a.rows_pp = L + ss;
+  a.plane = map_elems(L, ss);
+  const int Mp = 2 * L - 1 + ss;
+  a.in_len = in_c ? 2 * L - 1 : Mp;
+  a.out_len = out_c ? 2 * L - 1 : Mp;
+  if (nlines >= (1u << 22) || a.rows_pp >= 4096) {
+    set_error(-3, "row stage: too many lines or theta rows for one launch");
+    throw ArgFailure();
+  }
   a.in_centred = in_c;
   a.out_centred = out_c;
   a.sign = sign;
@@ -251,7 +265,7 @@ void theta_groups(Plan &P, const ColDev *cols, int kind, const DevBuf &in, DevBu
     for (size_t k = k0; k < k1; ++k) {
       const Group &g = P.groups[order[k]];
       ThetaJob<T> &j = J.j[J.n++];
-      j.a = theta_args<T>(g.L, cols + g.col0, g.ncol, kind);
+      j.a = theta_args<T>(g.L, cols + g.col0, g.ncol, kind, P.ss);
       j.in = at<T>(in, g.spec_off);
       j.out = at<T>(out, g.spec_off);
       j.blk0 = J.nblk;
@@ -274,9 +288,9 @@ void rows_groups(Plan &P, const DevBuf &in, DevBuf &out, int in_c, int out_c, in
   };
   for (const Group &g : P.groups) {  // small band-limits (long single-level DFT blocks) first
     if (g.ngamma == 0) continue;
-    const int M = 2 * g.L - 1;
+    const int M = 2 * g.L - 1 + P.ss;  // samples per line
     RowJob<T> &j = J.j[J.n++];
-    j.a = row_args<T>(g.L, (size_t)g.ncol * g.L, in_c, out_c, sign, scaled ? kTwoPi / M : 1.0);
+    j.a = row_args<T>(g.L, (size_t)g.ncol * (g.L + P.ss), in_c, out_c, sign, scaled ? kTwoPi / M : 1.0, P.ss);
     j.in = at<T>(in, g.spec_off);
     j.out = at<T>(out, g.spec_off);
     j.blk0 = J.nblk;
@@ -290,10 +304,10 @@ void rows_groups(Plan &P, const DevBuf &in, DevBuf &out, int in_c, int out_c, in
 template <typename TI, typename TO, typename T>
 void rows_scal(Plan &P, const void *in, void *out, int in_c, int out_c, int sign, bool scaled, hipStream_t st) {
   const Group &g = P.groups[0];
-  const int M = 2 * g.L - 1;
+  const int M = 2 * g.L - 1 + P.ss;
   RowJobs<T> J{};
   J.n = 1;
-  J.j[0].a = row_args<T>(g.L, (size_t)g.L, in_c, out_c, sign, scaled ? kTwoPi / M : 1.0);
+  J.j[0].a = row_args<T>(g.L, (size_t)(g.L + P.ss), in_c, out_c, sign, scaled ? kTwoPi / M : 1.0, P.ss);
   J.j[0].in = in;
   J.j[0].out = out;
   J.j[0].blk0 = 0;
@@ -318,7 +332,7 @@ void mix_groups(Plan &P, int which, bool to_wav, const void *src_wav, void *dst_
   for (size_t gi = P.groups.size(); gi-- > 0;) {  // largest band-limit first
     const Group &g = P.groups[gi];
     if (g.ngamma == 0) continue;
-    const size_t LM = (size_t)g.L * (2 * g.L - 1);
+    const size_t LM = map_elems(g.L, P.ss);
     MixJob &j = J.j[J.n++];
     j.W = P.d_gamma.as<cx<double>>() + P.gamma_off[gi] + (size_t)which * g.ngamma * g.ncol;
     if (to_wav) {
@@ -384,10 +398,10 @@ void synthesis_adjoint_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hi
 
 // ---- single spin-s transform at L (one column)
 template <typename T>
-void theta_one(int L, const ColDev *col, int kind, const void *in, void *out, hipStream_t st) {
+void theta_one(int L, const ColDev *col, int kind, const void *in, void *out, int ss, hipStream_t st) {
   ThetaJobs<T> J{};
   J.n = 1;
-  J.j[0].a = theta_args<T>(L, col, 1, kind);
+  J.j[0].a = theta_args<T>(L, col, 1, kind, ss);
   J.j[0].in = in;
   J.j[0].out = out;
   J.j[0].blk0 = 0;
@@ -396,10 +410,10 @@ void theta_one(int L, const ColDev *col, int kind, const void *in, void *out, hi
 }
 
 template <typename TI, typename TO, typename T>
-void rows_one(int L, const void *in, void *out, int in_c, int out_c, int sign, double scale, hipStream_t st) {
+void rows_one(int L, const void *in, void *out, int in_c, int out_c, int sign, double scale, int ss, hipStream_t st) {
   RowJobs<T> J{};
   J.n = 1;
-  J.j[0].a = row_args<T>(L, (size_t)L, in_c, out_c, sign, scale);
+  J.j[0].a = row_args<T>(L, (size_t)(L + ss), in_c, out_c, sign, scale, ss);
   J.j[0].in = in;
   J.j[0].out = out;
   J.j[0].blk0 = 0;
@@ -428,17 +442,17 @@ void sht_inverse_t(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
     Stage s_("contract_inv", st);
     launch_inv_mfma<T>(x, Sel<T>::delta(C), S.spec.p, J, st);
   }
-  { Stage s_("beta_inv", st); theta_one<T>(L, col, 0, S.spec.p, S.work.p, st); }
-  { Stage s_("rows", st); rows_one<T, double, T>(L, S.work.p, f, 1, 0, +1, 1.0, st); }
+  { Stage s_("beta_inv", st); theta_one<T>(L, col, 0, S.spec.p, S.work.p, S.ss, st); }
+  { Stage s_("rows", st); rows_one<T, double, T>(L, S.work.p, f, 1, 0, +1, 1.0, S.ss, st); }
 }
 
 template <typename T>
 void sht_forward_t(ShtPlan &S, const void *f, void *flm, hipStream_t st) {
   Context &C = Context::get();
-  const int L = S.L, M = 2 * L - 1;
+  const int L = S.L, M = 2 * L - 1 + S.ss;
   const ColDev *col = S.d_col.as<ColDev>();
-  { Stage s_("rows", st); rows_one<double, T, T>(L, f, S.spec.p, 0, 1, -1, kTwoPi / M, st); }
-  { Stage s_("beta_fwd", st); theta_one<T>(L, col, 2, S.spec.p, S.work.p, st); }
+  { Stage s_("rows", st); rows_one<double, T, T>(L, f, S.spec.p, 0, 1, -1, kTwoPi / M, S.ss, st); }
+  { Stage s_("beta_fwd", st); theta_one<T>(L, col, 2, S.spec.p, S.work.p, S.ss, st); }
   { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, C.splits(L, (int)sizeof(T)), st); }
   if (C.splits(L, (int)sizeof(T)) > 1) { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, C.splits(L, (int)sizeof(T)), S.parts.p, flm, st); }
 }
@@ -448,8 +462,8 @@ void sht_inverse_adjoint_t(ShtPlan &S, const void *f, void *flm, hipStream_t st)
   Context &C = Context::get();
   const int L = S.L;
   const ColDev *col = S.d_col.as<ColDev>();
-  { Stage s_("rows", st); rows_one<double, T, T>(L, f, S.spec.p, 0, 1, -1, 1.0, st); }
-  { Stage s_("beta_inv_adj", st); theta_one<T>(L, col, 1, S.spec.p, S.work.p, st); }
+  { Stage s_("rows", st); rows_one<double, T, T>(L, f, S.spec.p, 0, 1, -1, 1.0, S.ss, st); }
+  { Stage s_("beta_inv_adj", st); theta_one<T>(L, col, 1, S.spec.p, S.work.p, S.ss, st); }
   { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, C.splits(L, (int)sizeof(T)), st); }
   if (C.splits(L, (int)sizeof(T)) > 1) { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, C.splits(L, (int)sizeof(T)), S.parts.p, flm, st); }
 }
@@ -457,7 +471,7 @@ void sht_inverse_adjoint_t(ShtPlan &S, const void *f, void *flm, hipStream_t st)
 template <typename T>
 void sht_forward_adjoint_t(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
   Context &C = Context::get();
-  const int L = S.L, M = 2 * L - 1;
+  const int L = S.L, M = 2 * L - 1 + S.ss;
   const ColDev *col = S.d_col.as<ColDev>();
   {
     const void *x = sht_input<T>(S, flm, st);
@@ -468,8 +482,8 @@ void sht_forward_adjoint_t(ShtPlan &S, const void *flm, void *f, hipStream_t st)
     Stage s_("contract_inv", st);
     launch_inv_mfma<T>(x, Sel<T>::delta(C), S.spec.p, J, st);
   }
-  { Stage s_("beta_fwd_adj", st); theta_one<T>(L, col, 3, S.spec.p, S.work.p, st); }
-  { Stage s_("rows", st); rows_one<T, double, T>(L, S.work.p, f, 1, 0, +1, kTwoPi / M, st); }
+  { Stage s_("beta_fwd_adj", st); theta_one<T>(L, col, 3, S.spec.p, S.work.p, S.ss, st); }
+  { Stage s_("rows", st); rows_one<T, double, T>(L, S.work.p, f, 1, 0, +1, kTwoPi / M, S.ss, st); }
 }
 
 }  // namespace

@@ -32,13 +32,18 @@ struct DftTables {  // two-level DFT of one length M (see Dft2)
   DevBuf g64, g32, tw64, tw32, perm;
 };
 
-struct LTables {  // per band-limit L
+struct LTables {  // per band-limit L and sampling (MW / MWSS)
   int L;
+  int ss;
   DevBuf th64, th32;  // theta-stage matrices, 8 x L x L (see build_theta_tables)
   const DftTables *dM;
 };
 
 // A set of columns sharing one band-limit (see GroupDev).
+// Samples of one MW (ss = 0) or MWSS (ss = 1) map at band-limit L: L x (2L - 1) or
+// (L + 1) x 2L.  Also the per-column plane stride of the plans' scratch buffers.
+inline size_t map_elems(int L, int ss) { return ss ? (size_t)(L + 1) * 2 * L : (size_t)L * (2 * L - 1); }
+
 struct Group {
   int L;
   int ngamma;     // 2N_j - 1 for a wavelet scale, 0 for a single spin-s transform
@@ -51,7 +56,8 @@ struct Group {
 
 struct Plan {
   s2let_parameters_t p;
-  int mode;  // S2LET_HIP_FP64 / S2LET_HIP_MIXED
+  int mode;    // S2LET_HIP_FP64 / S2LET_HIP_MIXED
+  int ss = 0;  // MWSS sampling
   int J;
   std::vector<cd> psi;      // (J+1) L^2
   std::vector<double> phi;  // L
@@ -83,7 +89,7 @@ struct Plan {
 // Single spin-s MW harmonic transform plan (s2let_mw_alm2map / map2alm and
 // the px2wav / wav2px front ends).
 struct ShtPlan {
-  int L, spin, mode;
+  int L, spin, mode, ss = 0;
   ColDev col;
   std::vector<double> bvals;
   DevBuf d_col, d_b64, d_b32, spec, work, flm32, d_super, parts;
@@ -115,10 +121,10 @@ class Context {
   const double *tile64() const { return tile64_.as<double>(); }    // tiled copies
   const float *tile32() const { return tile32_.as<float>(); }
   const DftTables &dft(int n);
-  const LTables &ltab(int L);
+  const LTables &ltab(int L, int ss = 0);
   Plan &plan(const s2let_parameters_t *p, int mode, hipStream_t st);
   Plan &plan_manual(const ManualTiling &m, int mode, hipStream_t st);
-  ShtPlan &sht(int L, int spin, int mode, hipStream_t st);
+  ShtPlan &sht(int L, int spin, int mode, hipStream_t st, int ss = 0);
   size_t lds_bytes() const { return lds_bytes_; }
   // launch shapes per (band-limit, precision of the intermediate data: 4 or 8 bytes)
   Tune tune(int L, int esz) const {
@@ -144,7 +150,7 @@ class Context {
   std::map<int, std::unique_ptr<DftTables>> dft_;
   std::map<int, std::unique_ptr<LTables>> lt_;
   std::map<std::string, std::unique_ptr<Plan>> plans_;
-  std::map<std::tuple<int, int, int>, std::unique_ptr<ShtPlan>> shts_;
+  std::map<std::tuple<int, int, int, int>, std::unique_ptr<ShtPlan>> shts_;
   std::map<std::pair<int, int>, Tune> tune_;
   hipStream_t stream_ = nullptr;
 };

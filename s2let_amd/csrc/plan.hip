@@ -203,17 +203,20 @@ const DftTables &Context::dft(int n) {
   return ref;
 }
 
-const LTables &Context::ltab(int L) {
-  auto it = lt_.find(L);
+const LTables &Context::ltab(int L, int ss) {
+  const int key = 2 * L + (ss ? 1 : 0);
+  auto it = lt_.find(key);
   if (it != lt_.end()) return *it->second;
   auto t = std::make_unique<LTables>();
   t->L = L;
-  const int M = 2 * L - 1;
+  t->ss = ss ? 1 : 0;
+  // MW: cos / sin (pi j / M), j < 2M, M = 2L - 1 (every m theta_t = pi m (2t+1) / M is one of these);
+  // MWSS: cos / sin (pi j / L), j < 2L (m theta_t = pi m t / L)
+  const int M = 2 * L - 1, D = ss ? L : M, nang = 2 * D;
   const long double pi = 3.14159265358979323846264338327950288L;
-  // cos / sin (pi j / M), j < 2M: every theta_t multiple m theta_t = pi m (2t+1) / M is one of these
-  std::vector<double> cs(2 * M), sn(2 * M);
-  for (int j = 0; j < 2 * M; ++j) {
-    const long double a = pi * (long double)j / (long double)M;
+  std::vector<double> cs(nang), sn(nang);
+  for (int j = 0; j < nang; ++j) {
+    const long double a = pi * (long double)j / (long double)D;
     cs[j] = (double)cosl(a);
     sn[j] = (double)sinl(a);
   }
@@ -221,16 +224,17 @@ const LTables &Context::ltab(int L) {
   DevBuf dcs, dsn;
   upload(dcs, cs, st);
   upload(dsn, sn, st);
-  const size_t n = 8 * (size_t)L * L;
+  const size_t n = 8 * (size_t)L * (L + t->ss);
   t->th64.alloc(n * sizeof(double));
   t->th32.alloc(n * sizeof(float));
-  build_theta_tables(L, dcs.as<double>(), dsn.as<double>(), t->th64.as<double>(), st);
+  if (t->ss) build_theta_tables_ss(L, dcs.as<double>(), dsn.as<double>(), t->th64.as<double>(), st);
+  else build_theta_tables(L, dcs.as<double>(), dsn.as<double>(), t->th64.as<double>(), st);
   launch_to_f32(t->th64.as<double>(), t->th32.as<float>(), n, st);
   S2H_CHECK_HIP(hipGetLastError());
   S2H_CHECK_HIP(hipStreamSynchronize(st));
-  t->dM = &dft(M);
+  t->dM = &dft(M + t->ss);
   auto &ref = *t;
-  lt_[L] = std::move(t);
+  lt_[key] = std::move(t);
   return ref;
 }
 
@@ -268,7 +272,7 @@ static std::string plan_key(const s2let_parameters_t *p, int mode) {
   std::ostringstream o;
   o.precision(17);
   o << p->L << ',' << p->N << ',' << p->B << ',' << p->J_min << ',' << p->spin << ',' << p->original_spin << ','
-    << p->upsample << ',' << kernel_type() << ',' << mode;
+    << p->upsample << ',' << p->sampling_scheme << ',' << kernel_type() << ',' << mode;
   return o.str();
 }
 
@@ -301,13 +305,13 @@ struct Columns {
     for (int l = aspin; l < Ls; ++l)
       b[l] = std::sqrt((2.0 * l + 1) / (4.0 * M_PI)) * (std::sqrt(4.0 * M_PI / (2 * l + 1)) * phi[l]);
     add(0, aspin, b, cd(1, 0), cd(1, 0));
-    spec += (size_t)Ls * (2 * Ls - 1);
+    spec += map_elems(Ls, pl.ss);
     pl.groups.push_back(g);
   }
   Group open(int Lj, int Nj, int j) { return Group{Lj, 2 * Nj - 1, 0, (int)pl.cols_ana.size(), spec, out, j}; }
   void close(Group &g, int Nj) {
-    spec += (size_t)g.ncol * g.L * (2 * g.L - 1);
-    out += (size_t)g.L * (2 * g.L - 1) * (2 * Nj - 1);
+    spec += (size_t)g.ncol * map_elems(g.L, pl.ss);
+    out += map_elems(g.L, pl.ss) * (2 * Nj - 1);
     pl.maxcol = std::max(pl.maxcol, g.ncol);
     pl.groups.push_back(g);
   }
@@ -327,10 +331,15 @@ Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
     set_error(-1, "unsupported parameters (need L >= 1, 1 <= N <= 16, B > 1, |spin| < L)");
     throw ArgFailure();
   }
+  if (p->sampling_scheme != S2LET_SAMPLING_MW && p->sampling_scheme != S2LET_SAMPLING_MW_SS) {
+    set_error(-1, "unsupported sampling scheme (MW or MWSS)");
+    throw ArgFailure();
+  }
   auto P = std::make_unique<Plan>();
   Plan &pl = *P;
   pl.p = *p;
   pl.mode = mode;
+  pl.ss = p->sampling_scheme == S2LET_SAMPLING_MW_SS;
   pl.J = j_max(p);
   const int L = p->L, J = pl.J;
   pl.psi.assign((size_t)(J + 1) * L * L, cd(0, 0));
@@ -458,7 +467,7 @@ void Context::finish_plan(Plan &pl, hipStream_t st) {
   // spectrum plane of every column, and the equal-L runs of columns (MFMA column tiles)
   for (const Group &g : pl.groups)
     for (int k = 0; k < g.ncol; ++k) {
-      const long long off = (long long)g.spec_off + (long long)k * g.L * (2 * g.L - 1);
+      const long long off = (long long)g.spec_off + (long long)k * (long long)map_elems(g.L, pl.ss);
       pl.cols_ana[g.col0 + k].out_off = off;
       pl.cols_syn[g.col0 + k].out_off = off;
     }
@@ -477,7 +486,7 @@ void Context::finish_plan(Plan &pl, hipStream_t st) {
   }
   const bool f32 = (mode == S2LET_HIP_MIXED);
   ensure_delta(L, f32, st);
-  for (const Group &g : pl.groups) ltab(g.L);
+  for (const Group &g : pl.groups) ltab(g.L, pl.ss);
   // gamma mixing matrices per scale: [inv (G x nc), inv_adj (nc x G), fwd (nc x G), fwd_adj (G x nc)]
   std::vector<cx<double>> gm;
   for (const Group &g : pl.groups) {
@@ -537,8 +546,9 @@ void Context::finish_plan(Plan &pl, hipStream_t st) {
 }
 
 
-ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st) {
-  auto key = std::make_tuple(L, spin, mode);
+ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st, int ss) {
+  ss = ss ? 1 : 0;
+  auto key = std::make_tuple(L, spin, mode, ss);
   auto it = shts_.find(key);
   if (it != shts_.end()) return *it->second;
   if (L < 1 || std::abs(spin) >= L) {
@@ -550,6 +560,7 @@ ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st) {
   s.L = L;
   s.spin = spin;
   s.mode = mode;
+  s.ss = ss;
   const int n = -spin;
   s.bvals.assign(L, 0.0);
   for (int l = std::abs(spin); l < L; ++l) s.bvals[l] = ((spin & 1) ? -1.0 : 1.0) * std::sqrt((2.0 * l + 1) / (4.0 * M_PI));
@@ -560,7 +571,7 @@ ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st) {
   ipow(n, s.col.ph_re, s.col.ph_im);
   const bool f32 = (mode == S2LET_HIP_MIXED);
   ensure_delta(L, f32, st);
-  ltab(L);
+  ltab(L, ss);
   s.col.out_off = 0;
   std::vector<ColDev> cv{s.col};
   upload(s.d_col, cv, st);
@@ -575,7 +586,7 @@ ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st) {
     s.ei = ei[0];
   }
   const size_t esz = f32 ? sizeof(cx<float>) : sizeof(cx<double>);
-  const size_t ne = (size_t)L * (2 * L - 1);
+  const size_t ne = map_elems(L, ss);
   s.spec.alloc(ne * esz);
   s.work.alloc(ne * esz);
   if (f32) s.flm32.alloc((size_t)L * L * sizeof(cx<float>));

@@ -1,4 +1,5 @@
-// phi / alpha DFTs of the MW grid lines (length M = 2L - 1) on the matrix cores.
+// phi / alpha DFTs of the grid lines (length M = 2L - 1 on the MW grid, 2L on the
+// MWSS grid) on the matrix cores.
 //
 // M = M1 M2 (M1 the largest divisor <= sqrt M; M1 = 1 for prime M).  With
 // n = M2 n1 + n2 and k = k1 + M1 k2 the DFT X[k] = sum_n x[n] w^{nk}, w = e^{sign 2 pi i / M},
@@ -77,7 +78,14 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
   const cx<TI> *__restrict__ in = static_cast<const cx<TI> *>(jobs.j[jb].in);
   cx<TO> *__restrict__ out = static_cast<cx<TO> *>(jobs.j[jb].out);
   const Dft2<T> &d = a.d;
-  const int M = d.M, M1 = d.M1, M2 = d.M2, L = a.L;
+  const int M = d.M, M1 = d.M1, M2 = d.M2, half = a.L - 1, Mh = M - half;
+  const float inv_rpp = 1.0f / (float)a.rows_pp;
+  // entry offset of line `ln` of this job: plane ln / rows_pp, row ln % rows_pp
+  auto line_at = [&](int ln, int len) -> size_t {
+    int c, r;
+    divmod_small(ln, a.rows_pp, inv_rpp, c, r);
+    return (size_t)c * a.plane + (size_t)r * len;
+  };
   T *S = reinterpret_cast<T *>(smem_raw);
   const size_t line0 = (size_t)(blockIdx.x - jobs.j[jb].blk0) * a.lpb;
   const int nl = (int)min((size_t)a.lpb, a.nlines - line0);
@@ -94,15 +102,18 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
     for (int c = tid; c < ncolp; c += 256)
       for (int kk = (c < ncol) ? P2 : 0; kk < K; ++kk) S[c * cs + kk] = (T)0;
     for (int k = 0; k < nl; ++k) {
-      const cx<TI> *src = in + (line0 + k) * M;
+      const cx<TI> *src = in + line_at((int)line0 + k, a.in_len);
       T *dst = S + (size_t)k * cw * cs;
       for (int q = tid; q < M; q += 256) {
+        // DFT bin q <- centred m = q (q <= L-1) or q - M (q >= M - (L-1)); MWSS bin L is zero
         int sidx = q;
+        bool ok = true;
         if (a.in_centred) {
-          sidx = q + L - 1;
-          if (sidx >= M) sidx -= M;
+          if (q <= half) sidx = q + half;
+          else if (q >= Mh) sidx = q - Mh;
+          else ok = false;
         }
-        const cx<TI> v = src[sidx];
+        const cx<TI> v = ok ? src[sidx] : mk<TI>(0, 0);
         const int o = d.perm[q];
         dst[o] = (T)v.x;
         dst[o + 1] = (T)v.y;
@@ -177,15 +188,16 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
         int line, k1;
         divmod_small(col, M1, 1.0f / (float)M1, line, k1);
         const acc_t &r = u ? r1 : r0;
-        cx<TO> *dst = out + (line0 + line) * M;
+        cx<TO> *dst = out + line_at((int)line0 + line, a.out_len);
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
           const int k2 = 8 * mt + 2 * k4 + p;
           if (k2 >= M2) continue;
           int o = k1 + M1 * k2;
-          if (a.out_centred) {
-            o += L - 1;
-            if (o >= M) o -= M;
+          if (a.out_centred) {  // bin o -> centred m; the MWSS Nyquist bin is dropped
+            if (o <= half) o += half;
+            else if (o >= Mh) o -= Mh;
+            else continue;
           }
           dst[o] = mk<TO>((TO)(r[2 * p] * scale), (TO)(r[2 * p + 1] * scale));
         }

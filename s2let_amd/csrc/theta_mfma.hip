@@ -106,6 +106,87 @@ void build_theta_tables(int L, const double *cs, const double *sn, double *th, h
   hipLaunchKernelGGL(k_theta_tables, dim3(g), dim3(256), 0, st, L, cs, sn, th);
 }
 
+// MWSS: theta_t = pi t / L, t = 0..L; the torus has 2L samples (theta_{2L-t} =
+// 2 pi - theta_t, t = 0 and t = L map to themselves), so c_t(m'') is 1 at t = 0,
+// (-1)^{m''} at t = L and e^{-i m'' theta_t} + sigma e^{i m'' theta_t} between; the
+// Fourier coefficients |m''| < L enter (the torus Nyquist term m'' = L does not).
+// Blocks of L (L + 1): kinds 0 and 3 [m'][t] (R = L + 1), kinds 1 and 2 [t][m'] (R = L).
+__global__ void k_theta_tables_ss(int L, const double *__restrict__ cs, const double *__restrict__ sn,
+                                  double *__restrict__ th) {
+  const int T2 = 2 * L, Lt = L + 1;
+  const size_t B = (size_t)L * Lt;
+  const size_t total = 2 * B;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int cls = (int)(e / B);
+    const int rem = (int)(e - (size_t)cls * B);
+    const int mp = rem / Lt, t = rem - mp * Lt;  // (m', t)
+    const double sg = cls ? -1.0 : 1.0;
+    double ainv;
+    if (mp == 0) ainv = 1.0;
+    else {
+      const int j = (int)(((long long)mp * t) % T2);
+      ainv = cls ? 2.0 * sn[j] : 2.0 * cs[j];
+    }
+    th[(size_t)(0 * 2 + cls) * B + (size_t)mp * Lt + t] = ainv;  // kind 0: [k=m'][i=t]
+    th[(size_t)(1 * 2 + cls) * B + (size_t)t * L + mp] = ainv;   // kind 1: [k=t][i=m']
+    auto ct = [&](int mpp, double &re, double &im) {
+      if (t == 0) {
+        re = 1.0;
+        im = 0.0;
+        return;
+      }
+      if (t == L) {
+        re = (mpp & 1) ? -1.0 : 1.0;
+        im = 0.0;
+        return;
+      }
+      long long q = ((long long)mpp * t) % T2;
+      if (q < 0) q += T2;
+      const double c = cs[q], s = sn[q];
+      re = c + sg * c;
+      im = -s + sg * s;
+    };
+    auto Jsum = [&](int j, double &re, double &im) {  // 2L J_t[j] = sum_{m''} c_t(m'') I(j + m'')
+      re = 0.0;
+      im = 0.0;
+      for (int mpp = -(L - 1); mpp <= L - 1; ++mpp) {
+        const int p = j + mpp;
+        double cr, ci;
+        if (p == 1 || p == -1) {
+          ct(mpp, cr, ci);
+          const double ii = (p == 1) ? 1.5707963267948966192 : -1.5707963267948966192;
+          re += -ci * ii;
+          im += cr * ii;
+        } else if (!(p & 1)) {
+          ct(mpp, cr, ci);
+          const double w = Ire(p);
+          re += cr * w;
+          im += ci * w;
+        }
+      }
+    };
+    double kr, ki;
+    Jsum(mp, kr, ki);
+    if (mp > 0) {
+      double r2, i2;
+      Jsum(-mp, r2, i2);
+      kr += sg * r2;
+      ki += sg * i2;
+    }
+    kr /= T2;
+    ki /= T2;
+    const double afwd = (cls == 0 || mp == 0) ? kr : -ki;
+    th[(size_t)(2 * 2 + cls) * B + (size_t)t * L + mp] = afwd;   // kind 2: [k=t][i=m']
+    th[(size_t)(3 * 2 + cls) * B + (size_t)mp * Lt + t] = afwd;  // kind 3: [k=m'][i=t]
+  }
+}
+
+void build_theta_tables_ss(int L, const double *cs, const double *sn, double *th, hipStream_t st) {
+  const size_t total = 2 * (size_t)L * (L + 1);
+  const unsigned g = (unsigned)std::min<size_t>((total + 255) / 256, 65535 * 4);
+  hipLaunchKernelGGL(k_theta_tables_ss, dim3(g), dim3(256), 0, st, L, cs, sn, th);
+}
+
 // ------------------------------------------------------------------ GEMM
 // out_c[i][mi] = phase(c, m) z_cls D_out[i] sum_k A_cls[i][k] D_in[k] in_c[k][mi]
 // for every column c, rows i < L, k < L, mi < M (m = mi - (L-1)), cls = parity
@@ -146,8 +227,8 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
   const cx<T> *__restrict__ in = static_cast<const cx<T> *>(jobs.j[jb].in);
   cx<T> *__restrict__ out = static_cast<cx<T> *>(jobs.j[jb].out);
   w -= jobs.j[jb].blk0;
-  const int L = a.L, M = 2 * L - 1;
-  const int nti = (L + 32 * RT - 1) / (32 * RT), ntm = (M + 63) / 64;
+  const int L = a.L, M = 2 * L - 1, R = a.R, K = a.K;
+  const int nti = (R + 32 * RT - 1) / (32 * RT), ntm = (M + 63) / 64;
   const int ti = w % nti, rest = w / nti, tm = rest % ntm, c = rest / ntm;
   const int i0 = ti * 32 * RT, mi0 = tm * 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -157,7 +238,7 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
   const int cls = wave >> 1;
   const int i0w = i0 + 16 * RT * (wave & 1);
   const int r16 = lane & 15, k4 = lane >> 4;
-  const cx<T> *__restrict__ inc = in + (size_t)c * L * M;
+  const cx<T> *__restrict__ inc = in + (size_t)c * a.plane;
   const T *__restrict__ AT = a.AT[cls];
   const int in_row0 = a.in_row[0], in_row1 = a.in_row[1], in_e0 = a.in_e[0], in_e1 = a.in_e[1];
 
@@ -181,7 +262,7 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
   // fragments (L2) are loaded while the current chunk's MFMAs run; the samples
   // go through a double-buffered LDS tile (one barrier per chunk).
   constexpr int UPT = BK * 64 / 256;  // complex samples staged per thread per chunk
-  const int nch = (L + BK - 1) / BK;
+  const int nch = (K + BK - 1) / BK;
   cx<T> breg[UPT];
   T anx[BK / 4][RT], acur[BK / 4][RT];
   auto load_chunk = [&](int k0) {
@@ -189,7 +270,7 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
     for (int u = 0; u < UPT; ++u) {
       const int e = tid + 256 * u, kk = e >> 6, ee = e & 63;
       const int k = k0 + kk, mi = mi0 + ee;
-      breg[u] = (k < L && mi < M) ? inc[(size_t)k * M + mi] : mk<T>(0, 0);
+      breg[u] = (k < K && mi < M) ? inc[(size_t)k * M + mi] : mk<T>(0, 0);
     }
 #pragma unroll
     for (int s = 0; s < BK / 4; ++s) {
@@ -197,7 +278,7 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
 #pragma unroll
       for (int r = 0; r < RT; ++r) {
         const int i = i0w + 16 * r + r16;
-        anx[s][r] = (k < L && i < L) ? AT[(size_t)k * L + i] : (T)0;
+        anx[s][r] = (k < K && i < R) ? AT[(size_t)k * R + i] : (T)0;
       }
     }
   };
@@ -246,7 +327,7 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
 
   const cx<double> ph = mk<double>(col.ph_re, col.ph_im);
   const int zc = a.z_e[cls], orow = a.out_row[cls], oe = a.out_e[cls];
-  cx<T> *__restrict__ outc = out + (size_t)c * L * M;
+  cx<T> *__restrict__ outc = out + (size_t)c * a.plane;
 #pragma unroll
   for (int r = 0; r < RT; ++r)
 #pragma unroll
@@ -256,7 +337,7 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = i0w + 16 * r + Mfma<T>::row(k4, v);
-        if (i >= L || mi >= M) continue;
+        if (i >= R || mi >= M) continue;
         double re, im;
         if constexpr (f64sums) {
           re = hs[r][qq][v];
@@ -283,7 +364,7 @@ int theta_rows(int Lmax) {
 template <typename T>
 int theta_blocks(const ThetaArgs<T> &a, int rows) {
   const int M = 2 * a.L - 1;
-  return ((a.L + rows - 1) / rows) * ((M + 63) / 64) * a.ncol;
+  return ((a.R + rows - 1) / rows) * ((M + 63) / 64) * a.ncol;
 }
 template int theta_blocks<double>(const ThetaArgs<double> &, int);
 template int theta_rows<double>(int);

@@ -275,9 +275,11 @@ double s2let_tiling_wavelet_check_identity(std::complex<double> *psi, double *ph
 }
 
 // sizes: s2let_helper.c:5-107 (MW only: L theta rows x 2L-1 phi columns)
-int s2let_n_phi(const s2let_parameters_t *p) { return 2 * p->L - 1; }
-int s2let_n_theta(const s2let_parameters_t *p) { return p->L; }
-int s2let_n_px(const s2let_parameters_t *p) { return p->L * (2 * p->L - 1); }
+// MWSS (s2let_helper.c:5-17): 2L phi samples, L + 1 theta samples
+static bool mwss(const s2let_parameters_t *p) { return p->sampling_scheme == S2LET_SAMPLING_MW_SS; }
+int s2let_n_phi(const s2let_parameters_t *p) { return mwss(p) ? 2 * p->L : 2 * p->L - 1; }
+int s2let_n_theta(const s2let_parameters_t *p) { return mwss(p) ? p->L + 1 : p->L; }
+int s2let_n_px(const s2let_parameters_t *p) { return s2let_n_phi(p) * s2let_n_theta(p); }
 int s2let_n_lm(const s2let_parameters_t *p) { return p->L * p->L; }
 int s2let_n_lm_scal(const s2let_parameters_t *p) {
   int b = scal_bandlimit(p);
@@ -297,19 +299,20 @@ int s2let_n_lmn_wav(const s2let_parameters_t *p) {
 int s2let_n_gamma(const s2let_parameters_t *p) { return 2 * p->N - 1; }
 int s2let_n_scal(const s2let_parameters_t *p) {
   int b = scal_bandlimit(p);
-  return b * (2 * b - 1);
+  return mwss(p) ? 2 * b * (b + 1) : b * (2 * b - 1);
 }
 int s2let_n_wav(const s2let_parameters_t *p) {
   int total = 0, bl = p->L;
   for (int j = p->J_min; j <= j_max(p); ++j) {
     if (!p->upsample) bl = std::min(bandlimit(j, p), p->L);
-    total += f_size(bl, p->N);  // 2N-1 gamma slices per scale (s2let_helper.c:87-93)
+    // 2N-1 gamma slices per scale (s2let_helper.c:87-93)
+    total += mwss(p) ? 2 * bl * (bl + 1) * (2 * p->N - 1) : f_size(bl, p->N);
   }
   return total;
 }
 int s2let_n_wav_j(int j, const s2let_parameters_t *p) {
   int bl = p->upsample ? p->L : std::min(bandlimit(j, p), p->L);
-  return f_size(bl, p->N);
+  return mwss(p) ? 2 * bl * (bl + 1) * (2 * p->N - 1) : f_size(bl, p->N);
 }
 
 void s2let_allocate_mw(std::complex<double> **f, int L) {
@@ -320,10 +323,19 @@ void s2let_allocate_f_wav_manual(std::complex<double> **f_wav, std::complex<doub
                                  int scal_bandlimit, int N, int J, const s2let_parameters_t *p) {
   (void)N;
   size_t total = 0;
-  for (int j = 0; j <= J; ++j) total += (size_t)f_size(p->upsample ? p->L : wav_bandlimits[j], p->N);
+  for (int j = 0; j <= J; ++j) {
+    const int b = p->upsample ? p->L : wav_bandlimits[j];
+    total += mwss(p) ? (size_t)2 * b * (b + 1) * (2 * p->N - 1) : (size_t)f_size(b, p->N);
+  }
   *f_wav = (std::complex<double> *)calloc(total, sizeof(std::complex<double>));
-  *f_scal = (std::complex<double> *)calloc((size_t)scal_bandlimit * (2 * scal_bandlimit - 1), sizeof(std::complex<double>));
+  const size_t ns = mwss(p) ? (size_t)2 * scal_bandlimit * (scal_bandlimit + 1)
+                            : (size_t)scal_bandlimit * (2 * scal_bandlimit - 1);
+  *f_scal = (std::complex<double> *)calloc(ns, sizeof(std::complex<double>));
 }
+void s2let_allocate_mwss(std::complex<double> **f, int L) {
+  *f = (std::complex<double> *)calloc((size_t)2 * L * (L + 1), sizeof(std::complex<double>));
+}
+void s2let_allocate_mwss_real(double **f, int L) { *f = (double *)calloc((size_t)2 * L * (L + 1), sizeof(double)); }
 void s2let_allocate_mw_real(double **f, int L) { *f = (double *)calloc((size_t)L * (2 * L - 1), sizeof(double)); }
 void s2let_allocate_f_wav_real(double **a, double **b, const s2let_parameters_t *p) {
   *a = (double *)calloc(s2let_n_wav(p), sizeof(double));

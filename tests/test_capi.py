@@ -50,6 +50,11 @@ def test_tiling_and_sizes_match_oracle(s2, oracle, L, N, B, J_min, spin, up):
     assert s2.n_lm_scal(p) == oracle.n_lm_scal(q)
     # real signals: the reality flag selects the n >= 0 storage (s2let_so3.h:33 -> SO3 size)
     assert s2.n_lmn_wav(s2._reality(p, 1)) == oracle.n_lmn_wav_real(q)
+    # MWSS sampling (s2let_helper.c:5-20): (L + 1) x 2L maps
+    p_ss = s2.parameters(L, N, B, J_min, spin, up, sampling_scheme=1)
+    q_ss = oracle.params(L, N, B, J_min, spin, up, sampling=1)
+    assert s2.n_wav(p_ss) == oracle.n_wav(q_ss) and s2.n_scal(p_ss) == oracle.n_scal(q_ss)
+    assert s2.n_px(p_ss) == (L + 1) * 2 * L and s2.n_px(p) == L * (2 * L - 1)
 
 
 @pytest.mark.parametrize("wavtype", [2, 3])
