@@ -190,6 +190,91 @@ void s2let_allocate_f_wav_manual(S2LET_COMPLEX(double) **f_wav, S2LET_COMPLEX(do
                                  int scal_bandlimit, int N, int J,
                                  const s2let_parameters_t *parameters); /* s2let_alloc.c:148 */
 
+/* ---- axisymmetric wavelets (spin 0, MW sampling; the directional transform at N = 1).
+ * Full resolution: every scale at L; multires: scale j at min(B^{j+1}, L).  The lm forms
+ * take the caller's kernels wav_lm ((J+1) L) and scal_lm (L); their synthesis adds into flm. */
+void s2let_transform_axisym_lm_allocate_f_wav(S2LET_COMPLEX(double) **f_wav_lm, S2LET_COMPLEX(double) **f_scal_lm,
+                                              const s2let_parameters_t *parameters); /* s2let_transform_axisym_lm.c:25 */
+void s2let_transform_axisym_lm_allocate_f_wav_multires(S2LET_COMPLEX(double) **f_wav_lm,
+                                                       S2LET_COMPLEX(double) **f_scal_lm,
+                                                       const s2let_parameters_t *parameters); /* :45 */
+void s2let_transform_axisym_lm_allocate_wav(double **wav_lm, double **scal_lm,
+                                            const s2let_parameters_t *parameters); /* :72 */
+void s2let_transform_axisym_lm_wav(double *wav_lm, double *scal_lm, const s2let_parameters_t *parameters); /* :91 */
+void s2let_transform_axisym_lm_wav_analysis(S2LET_COMPLEX(double) *f_wav_lm, S2LET_COMPLEX(double) *f_scal_lm,
+                                            const S2LET_COMPLEX(double) *flm, const double *wav_lm,
+                                            const double *scal_lm, const s2let_parameters_t *parameters); /* :133 */
+void s2let_transform_axisym_lm_wav_synthesis(S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *f_wav_lm,
+                                             const S2LET_COMPLEX(double) *f_scal_lm, const double *wav_lm,
+                                             const double *scal_lm, const s2let_parameters_t *parameters); /* :178 */
+void s2let_transform_axisym_lm_wav_analysis_multires(S2LET_COMPLEX(double) *f_wav_lm,
+                                                     S2LET_COMPLEX(double) *f_scal_lm,
+                                                     const S2LET_COMPLEX(double) *flm, const double *wav_lm,
+                                                     const double *scal_lm,
+                                                     const s2let_parameters_t *parameters); /* :226 */
+void s2let_transform_axisym_lm_wav_synthesis_multires(S2LET_COMPLEX(double) *flm,
+                                                      const S2LET_COMPLEX(double) *f_wav_lm,
+                                                      const S2LET_COMPLEX(double) *f_scal_lm, const double *wav_lm,
+                                                      const double *scal_lm,
+                                                      const s2let_parameters_t *parameters); /* :275 */
+/* s2let_transform_axisym_mw.c */
+void s2let_transform_axisym_allocate_mw_f_wav(S2LET_COMPLEX(double) **f_wav, S2LET_COMPLEX(double) **f_scal,
+                                              const s2let_parameters_t *parameters); /* :23 */
+void s2let_transform_axisym_allocate_mw_f_wav_multires(S2LET_COMPLEX(double) **f_wav, S2LET_COMPLEX(double) **f_scal,
+                                                       const s2let_parameters_t *parameters); /* :45 */
+void s2let_transform_axisym_allocate_mw_f_wav_real(double **f_wav, double **f_scal,
+                                                   const s2let_parameters_t *parameters); /* :74 */
+void s2let_transform_axisym_allocate_mw_f_wav_multires_real(double **f_wav, double **f_scal,
+                                                            const s2let_parameters_t *parameters); /* :95 */
+void s2let_transform_axisym_wav_analysis_mw(S2LET_COMPLEX(double) *f_wav, S2LET_COMPLEX(double) *f_scal,
+                                            const S2LET_COMPLEX(double) *f, const s2let_parameters_t *parameters); /* :124 */
+void s2let_transform_axisym_wav_analysis_adjoint_mw(S2LET_COMPLEX(double) *f, const S2LET_COMPLEX(double) *f_wav,
+                                                    const S2LET_COMPLEX(double) *f_scal,
+                                                    const s2let_parameters_t *parameters); /* :188 */
+void s2let_transform_axisym_wav_synthesis_mw(S2LET_COMPLEX(double) *f, const S2LET_COMPLEX(double) *f_wav,
+                                             const S2LET_COMPLEX(double) *f_scal,
+                                             const s2let_parameters_t *parameters); /* :254 */
+void s2let_transform_axisym_wav_synthesis_adjoint_mw(S2LET_COMPLEX(double) *f_wav, S2LET_COMPLEX(double) *f_scal,
+                                                     const S2LET_COMPLEX(double) *f,
+                                                     const s2let_parameters_t *parameters); /* :320 */
+void s2let_transform_axisym_wav_analysis_mw_multires(S2LET_COMPLEX(double) *f_wav, S2LET_COMPLEX(double) *f_scal,
+                                                     const S2LET_COMPLEX(double) *f,
+                                                     const s2let_parameters_t *parameters); /* :384 */
+void s2let_transform_axisym_wav_analysis_adjoint_mw_multires(S2LET_COMPLEX(double) *f,
+                                                             const S2LET_COMPLEX(double) *f_wav,
+                                                             const S2LET_COMPLEX(double) *f_scal,
+                                                             const s2let_parameters_t *parameters); /* :451 */
+void s2let_transform_axisym_wav_synthesis_mw_multires(S2LET_COMPLEX(double) *f, const S2LET_COMPLEX(double) *f_wav,
+                                                      const S2LET_COMPLEX(double) *f_scal,
+                                                      const s2let_parameters_t *parameters); /* :521 */
+void s2let_transform_axisym_wav_synthesis_adjoint_mw_multires(S2LET_COMPLEX(double) *f_wav,
+                                                              S2LET_COMPLEX(double) *f_scal,
+                                                              const S2LET_COMPLEX(double) *f,
+                                                              const s2let_parameters_t *parameters); /* :591 */
+void s2let_transform_axisym_wav_analysis_mw_real(double *f_wav, double *f_scal, const double *f,
+                                                 const s2let_parameters_t *parameters); /* :659 */
+void s2let_transform_axisym_wav_analysis_adjoint_mw_real(double *f, const double *f_wav, const double *f_scal,
+                                                         const s2let_parameters_t *parameters); /* :723 */
+void s2let_transform_axisym_wav_synthesis_mw_real(double *f, const double *f_wav, const double *f_scal,
+                                                  const s2let_parameters_t *parameters); /* :788 */
+void s2let_transform_axisym_wav_synthesis_adjoint_mw_real(double *f_wav, double *f_scal, const double *f,
+                                                          const s2let_parameters_t *parameters); /* :852 */
+void s2let_transform_axisym_wav_analysis_mw_multires_real(double *f_wav, double *f_scal, const double *f,
+                                                          const s2let_parameters_t *parameters); /* :915 */
+void s2let_transform_axisym_wav_analysis_adjoint_mw_multires_real(double *f, const double *f_wav,
+                                                                  const double *f_scal,
+                                                                  const s2let_parameters_t *parameters); /* :981 */
+void s2let_transform_axisym_wav_synthesis_mw_multires_real(double *f, const double *f_wav, const double *f_scal,
+                                                           const s2let_parameters_t *parameters); /* :1051 */
+void s2let_transform_axisym_wav_synthesis_adjoint_mw_multires_real(double *f_wav, double *f_scal, const double *f,
+                                                                   const s2let_parameters_t *parameters); /* :1120 */
+/* zero samples whose integer part is below threshold[j - J_min] in magnitude (the reference
+ * compares C's int abs() of the value) */
+void s2let_transform_axisym_wav_hardthreshold_real(double *g_wav, const double *threshold,
+                                                   const s2let_parameters_t *parameters); /* :1183 */
+void s2let_transform_axisym_wav_hardthreshold_multires_real(double *g_wav, const double *threshold,
+                                                            const s2let_parameters_t *parameters); /* :1210 */
+
 /* ---- real signals.  Real maps and real wavelet coefficients (double), the
  * harmonic coefficients of a real signal (complex, f_{l,-m} = (-1)^m conj(f_lm));
  * array sizes as the complex variants.  Pixel-space variants use spin-0 MW

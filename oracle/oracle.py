@@ -407,6 +407,49 @@ def synthesis_adjoint_px2wav(f, p):
     return fw, fs
 
 
+def axisym_lm_wav(p):
+    J = j_max(p)
+    wav, scal = np.zeros((J + 1) * p.L), np.zeros(p.L)
+    lib().oracle_axisym_lm_wav(_p(wav), _p(scal), ctypes.byref(p))
+    return wav, scal
+
+
+def _axisym_sizes(p, multires):
+    bl = [min(bandlimit(j, p), p.L) if multires else p.L for j in range(p.J_min, j_max(p) + 1)]
+    bs = min(bandlimit(p.J_min - 1, p), p.L) if multires else p.L
+    return bl, bs
+
+
+def axisym_lm_wav_analysis(flm, wav_lm, scal_lm, p, multires=0):
+    bl, bs = _axisym_sizes(p, multires)
+    fw, fs = _c(sum(b * b for b in bl)), _c(bs * bs)
+    lib().oracle_axisym_lm_wav_analysis(_p(fw), _p(fs), _p(_in(flm)), _p(np.ascontiguousarray(wav_lm, float)),
+                                        _p(np.ascontiguousarray(scal_lm, float)), ctypes.byref(p), int(multires))
+    return fw, fs
+
+
+def axisym_lm_wav_synthesis(f_wav_lm, f_scal_lm, wav_lm, scal_lm, p, multires=0, flm=None):
+    out = _c(p.L * p.L) if flm is None else np.array(flm, dtype=np.complex128)
+    lib().oracle_axisym_lm_wav_synthesis(_p(out), _p(_in(f_wav_lm)), _p(_in(f_scal_lm)),
+                                         _p(np.ascontiguousarray(wav_lm, float)),
+                                         _p(np.ascontiguousarray(scal_lm, float)), ctypes.byref(p), int(multires))
+    return out
+
+
+def axisym_mw(kind, multires, x, y, p):
+    """kind 0 analysis (x = f), 1 analysis adjoint (x, y = f_wav, f_scal), 2 synthesis
+    (x, y = f_wav, f_scal), 3 synthesis adjoint (x = f)."""
+    bl, bs = _axisym_sizes(p, multires)
+    nw, ns, nf = sum(b * (2 * b - 1) for b in bl), bs * (2 * bs - 1), p.L * (2 * p.L - 1)
+    if kind in (0, 3):
+        fw, fs = _c(nw), _c(ns)
+        lib().oracle_axisym_mw(int(kind), int(multires), _p(fw), _p(fs), _p(_in(x)), ctypes.byref(p))
+        return fw, fs
+    f = _c(nf)
+    lib().oracle_axisym_mw(int(kind), int(multires), _p(f), _p(_in(x)), _p(_in(y)), ctypes.byref(p))
+    return f
+
+
 def axisym_wav_analysis_mw_multires(f, p):
     J = j_max(p)
     tot = sum(min(bandlimit(j, p), p.L) * (2 * min(bandlimit(j, p), p.L) - 1)

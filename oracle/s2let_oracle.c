@@ -1484,34 +1484,121 @@ int oracle_n_wav_manual(const int *wav_bl, int J, int L, int N) {
   return tot;
 }
 
-/* s2let_transform_axisym_mw.c (multires analysis) with s2let_transform_axisym_lm.c kernels */
-void oracle_axisym_wav_analysis_mw_multires(ocplx *f_wav, ocplx *f_scal, const ocplx *f, const oracle_params_t *p) {
+/* ------------------------------------------------------------------------ */
+/* Axisymmetric wavelets (s2let_transform_axisym_lm.c, s2let_transform_axisym_mw.c) */
+/* ------------------------------------------------------------------------ */
+
+/* s2let_transform_axisym_lm.c:91-120 */
+void oracle_axisym_lm_wav(double *wav_lm, double *scal_lm, const oracle_params_t *p) {
   int L = p->L, J = oracle_j_max(p);
   double *kappa = (double *)calloc((size_t)(J + 1) * L, sizeof(double));
   double *kappa0 = (double *)calloc(L, sizeof(double));
   oracle_tiling_axisym(kappa, kappa0, p);
-  ocplx *flm = (ocplx *)calloc((size_t)L * L, sizeof(ocplx));
-  ocplx *tmp = (ocplx *)calloc((size_t)L * L, sizeof(ocplx));
-  oracle_ssht_forward(flm, f, L, 0);
-  int bl = OMIN(oracle_bandlimit(p->J_min - 1, p), L);
-  for (int l = 0; l < bl; ++l) {
-    double scal0 = sqrt((4.0 * OPI) / (2.0 * l + 1.0)) * (sqrt((2 * l + 1) / (4.0 * OPI)) * kappa0[l]);
-    for (int m = -l; m <= l; ++m) tmp[l * l + l + m] = flm[l * l + l + m] * scal0;
-  }
-  oracle_ssht_inverse(f_scal, tmp, bl, 0);
-  int off = 0;
-  for (int j = p->J_min; j <= J; ++j) {
-    bl = OMIN(oracle_bandlimit(j, p), L);
-    for (int l = 0; l < bl; ++l) {
-      double wav0 = sqrt((4.0 * OPI) / (2.0 * l + 1.0)) * (sqrt((2 * l + 1) / (4.0 * OPI)) * kappa[l + j * L]);
-      for (int m = -l; m <= l; ++m) tmp[l * l + l + m] = flm[l * l + l + m] * wav0;
-    }
-    oracle_ssht_inverse(f_wav + off, tmp, bl, 0);
-    for (int i = 0; i < bl * (2 * bl - 1); ++i) f_wav[off + i] /= sqrt(2 * OPI);
-    off += bl * (2 * bl - 1);
-  }
+  for (int j = p->J_min; j <= J; j++)
+    for (int l = 0; l < L; l++) wav_lm[j * L + l] = sqrt((2 * l + 1) / (4.0 * OPI)) * kappa[l + j * L];
+  for (int l = 0; l < L; l++) scal_lm[l] = sqrt((2 * l + 1) / (4.0 * OPI)) * kappa0[l];
   free(kappa);
   free(kappa0);
+}
+
+static int or_axisym_bl(const oracle_params_t *p, int j, int multires) {
+  return multires ? OMIN(oracle_bandlimit(j, p), p->L) : p->L;
+}
+
+/* s2let_transform_axisym_lm.c:133-176 (full) and :226-273 (multires) */
+void oracle_axisym_lm_wav_analysis(ocplx *f_wav_lm, ocplx *f_scal_lm, const ocplx *flm, const double *wav_lm,
+                                   const double *scal_lm, const oracle_params_t *p, int multires) {
+  int L = p->L, J = oracle_j_max(p), offset = 0;
+  for (int j = p->J_min; j <= J; j++) {
+    int bl = or_axisym_bl(p, j, multires);
+    for (int l = 0; l < bl; l++) {
+      double wav0 = sqrt((4.0 * OPI) / (2.0 * l + 1.0)) * wav_lm[j * L + l];
+      for (int m = -l; m <= l; m++) f_wav_lm[offset + l * l + l + m] = flm[l * l + l + m] * wav0;
+    }
+    offset += bl * bl;
+  }
+  int bl = or_axisym_bl(p, p->J_min - 1, multires);
+  for (int l = 0; l < bl; l++) {
+    double scal0 = sqrt((4.0 * OPI) / (2.0 * l + 1.0)) * scal_lm[l];
+    for (int m = -l; m <= l; m++) f_scal_lm[l * l + l + m] = flm[l * l + l + m] * scal0;
+  }
+}
+
+/* s2let_transform_axisym_lm.c:178-224 (full) and :275-324 (multires); adds into flm */
+void oracle_axisym_lm_wav_synthesis(ocplx *flm, const ocplx *f_wav_lm, const ocplx *f_scal_lm, const double *wav_lm,
+                                    const double *scal_lm, const oracle_params_t *p, int multires) {
+  int L = p->L, J = oracle_j_max(p), offset = 0;
+  for (int j = p->J_min; j <= J; j++) {
+    int bl = or_axisym_bl(p, j, multires);
+    for (int l = 0; l < bl; l++) {
+      double wav0 = sqrt((4.0 * OPI) / (2.0 * l + 1.0)) * wav_lm[j * L + l];
+      for (int m = -l; m <= l; m++) flm[l * l + l + m] += f_wav_lm[offset + l * l + l + m] * wav0;
+    }
+    offset += bl * bl;
+  }
+  int bl = or_axisym_bl(p, p->J_min - 1, multires);
+  for (int l = 0; l < bl; l++) {
+    double scal0 = sqrt((4.0 * OPI) / (2.0 * l + 1.0)) * scal_lm[l];
+    for (int m = -l; m <= l; m++) flm[l * l + l + m] += f_scal_lm[l * l + l + m] * scal0;
+  }
+}
+
+/* s2let_transform_axisym_mw.c: kind 0 analysis (:124-186, multires :384-449), 1 analysis
+   adjoint (:188-252, :451-519), 2 synthesis (:254-318, :521-589), 3 synthesis adjoint
+   (:320-382, :591-657); spin-0 MW transforms, wavelet maps scaled by 1/sqrt(2 pi) */
+void oracle_axisym_mw(int kind, int multires, ocplx *a, ocplx *b, ocplx *c, const oracle_params_t *p) {
+  int L = p->L, J = oracle_j_max(p), nsc = J + 1 - p->J_min;
+  double *wav_lm = (double *)calloc((size_t)(J + 1) * L, sizeof(double));
+  double *scal_lm = (double *)calloc(L, sizeof(double));
+  oracle_axisym_lm_wav(wav_lm, scal_lm, p);
+  int nwl = 0;
+  for (int j = p->J_min; j <= J; ++j) nwl += or_axisym_bl(p, j, multires) * or_axisym_bl(p, j, multires);
+  int bs = or_axisym_bl(p, p->J_min - 1, multires);
+  ocplx *flm = (ocplx *)calloc((size_t)L * L, sizeof(ocplx));
+  ocplx *f_wav_lm = (ocplx *)calloc(nwl, sizeof(ocplx));
+  ocplx *f_scal_lm = (ocplx *)calloc((size_t)L * L, sizeof(ocplx));
+  const double r2pi = sqrt(2 * OPI);
+  if (kind == 0 || kind == 3) { /* a = f_wav, b = f_scal, c = f */
+    if (kind == 0) oracle_ssht_forward(flm, c, L, 0);
+    else oracle_ssht_adjoint_inverse(flm, c, L, 0);
+    oracle_axisym_lm_wav_analysis(f_wav_lm, f_scal_lm, flm, wav_lm, scal_lm, p, multires);
+    if (kind == 0) oracle_ssht_inverse(b, f_scal_lm, bs, 0);
+    else oracle_ssht_adjoint_forward(b, f_scal_lm, bs, 0);
+    int off = 0, offl = 0;
+    for (int j = p->J_min; j <= J; ++j) {
+      int bl = or_axisym_bl(p, j, multires);
+      if (kind == 0) oracle_ssht_inverse(a + off, f_wav_lm + offl, bl, 0);
+      else oracle_ssht_adjoint_forward(a + off, f_wav_lm + offl, bl, 0);
+      for (int i = 0; i < bl * (2 * bl - 1); ++i) a[off + i] = kind == 0 ? a[off + i] / r2pi : a[off + i] * r2pi;
+      offl += bl * bl;
+      off += bl * (2 * bl - 1);
+    }
+  } else { /* a = f, b = f_wav, c = f_scal */
+    if (kind == 2) oracle_ssht_forward(f_scal_lm, c, bs, 0);
+    else oracle_ssht_adjoint_inverse(f_scal_lm, c, bs, 0);
+    int off = 0, offl = 0;
+    for (int j = p->J_min; j <= J; ++j) {
+      int bl = or_axisym_bl(p, j, multires), n = bl * (2 * bl - 1);
+      ocplx *w = (ocplx *)malloc(sizeof(ocplx) * n);
+      for (int i = 0; i < n; ++i) w[i] = kind == 2 ? b[off + i] * r2pi : b[off + i] / r2pi;
+      if (kind == 2) oracle_ssht_forward(f_wav_lm + offl, w, bl, 0);
+      else oracle_ssht_adjoint_inverse(f_wav_lm + offl, w, bl, 0);
+      free(w);
+      offl += bl * bl;
+      off += n;
+    }
+    oracle_axisym_lm_wav_synthesis(flm, f_wav_lm, f_scal_lm, wav_lm, scal_lm, p, multires);
+    if (kind == 2) oracle_ssht_inverse(a, flm, L, 0);
+    else oracle_ssht_adjoint_forward(a, flm, L, 0);
+  }
+  (void)nsc;
+  free(wav_lm);
+  free(scal_lm);
   free(flm);
-  free(tmp);
+  free(f_wav_lm);
+  free(f_scal_lm);
+}
+
+void oracle_axisym_wav_analysis_mw_multires(ocplx *f_wav, ocplx *f_scal, const ocplx *f, const oracle_params_t *p) {
+  oracle_axisym_mw(0, 1, f_wav, f_scal, (ocplx *)f, p);
 }

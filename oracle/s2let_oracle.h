@@ -139,6 +139,16 @@ void oracle_synthesis_adjoint_lm2wav_manual(ocplx *f_wav, ocplx *f_scal, const o
                                             int spin, int N);
 int oracle_n_wav_manual(const int *wav_bl, int J, int L, int N);
 
+/* axisymmetric wavelets: kind 0 analysis (a=f_wav, b=f_scal, c=f), 1 analysis adjoint
+   (a=f, b=f_wav, c=f_scal), 2 synthesis (a=f, b=f_wav, c=f_scal), 3 synthesis adjoint
+   (a=f_wav, b=f_scal, c=f); multires 0/1 */
+void oracle_axisym_lm_wav(double *wav_lm, double *scal_lm, const oracle_params_t *p);
+void oracle_axisym_lm_wav_analysis(ocplx *f_wav_lm, ocplx *f_scal_lm, const ocplx *flm, const double *wav_lm,
+                                   const double *scal_lm, const oracle_params_t *p, int multires);
+void oracle_axisym_lm_wav_synthesis(ocplx *flm, const ocplx *f_wav_lm, const ocplx *f_scal_lm, const double *wav_lm,
+                                    const double *scal_lm, const oracle_params_t *p, int multires);
+void oracle_axisym_mw(int kind, int multires, ocplx *a, ocplx *b, ocplx *c, const oracle_params_t *p);
+
 /* S2LET drivers */
 void oracle_analysis_lm2wav(ocplx *f_wav, ocplx *f_scal, const ocplx *flm, const oracle_params_t *p);
 void oracle_synthesis_wav2lm(ocplx *flm, const ocplx *f_wav, const ocplx *f_scal, const oracle_params_t *p);

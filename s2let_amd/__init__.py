@@ -105,6 +105,15 @@ for _n in ("analysis_lm2wav_manual", "synthesis_wav2lm_manual", "analysis_adjoin
            "synthesis_adjoint_lm2wav_manual"):
     _SIGS["s2let_" + _n] = (None, [ctypes.c_void_p] * 5 + [ctypes.c_int, ctypes.c_void_p] + [ctypes.c_int] * 4)
 _SIGS["s2let_allocate_f_wav_manual"] = (None, [ctypes.c_void_p] * 3 + [ctypes.c_int] * 3 + [ctypes.c_void_p])
+for _n in ("lm_wav_analysis", "lm_wav_synthesis", "lm_wav_analysis_multires", "lm_wav_synthesis_multires"):
+    _SIGS["s2let_transform_axisym_" + _n] = (None, [ctypes.c_void_p] * 6)
+for _n in ("lm_allocate_f_wav", "lm_allocate_f_wav_multires", "lm_allocate_wav", "lm_wav", "allocate_mw_f_wav",
+           "allocate_mw_f_wav_multires", "allocate_mw_f_wav_real", "allocate_mw_f_wav_multires_real",
+           "wav_hardthreshold_real", "wav_hardthreshold_multires_real"):
+    _SIGS["s2let_transform_axisym_" + _n] = (None, [ctypes.c_void_p] * 3)
+for _k in ("analysis", "analysis_adjoint", "synthesis", "synthesis_adjoint"):
+    for _s in ("", "_multires", "_real", "_multires_real"):
+        _SIGS["s2let_transform_axisym_wav_%s_mw%s" % (_k, _s)] = (None, [ctypes.c_void_p] * 4)
 _SIGS["s2let_hip_selftest"] = (ctypes.c_int, [])
 _SIGS["s2let_hip_timing"] = (None, [ctypes.c_int])
 _SIGS["s2let_hip_timing_report"] = (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int])
@@ -501,6 +510,81 @@ def synthesis_adjoint_lm2wav_manual(flm, t):
     fw, fs = np.zeros(t.n_wav(), np.complex128), np.zeros(t.n_scal(), np.complex128)
     lib().s2let_synthesis_adjoint_lm2wav_manual(_ptr(fw), _ptr(fs), _ptr(flm), *t._args())
     return fw, fs
+
+
+# ----------------------------------------------------------------------------- axisymmetric wavelets
+# s2let_transform_axisym_lm.c, s2let_transform_axisym_mw.c (spin 0, MW sampling)
+def _axisym_sizes(p, multires):
+    bl = [min(bandlimit(j, p), p.L) if multires else p.L for j in range(p.J_min, j_max(p) + 1)]
+    bs = min(bandlimit(p.J_min - 1, p), p.L) if multires else p.L
+    return bl, bs
+
+
+def axisym_lm_wav(p):
+    """Kernels (wav_lm ((J+1) L), scal_lm (L)) of s2let_transform_axisym_lm_wav."""
+    wav, scal = np.zeros((j_max(p) + 1) * p.L), np.zeros(p.L)
+    lib().s2let_transform_axisym_lm_wav(_ptr(wav), _ptr(scal), _ref(p))
+    return wav, scal
+
+
+def axisym_lm_wav_analysis(flm, wav_lm, scal_lm, p, multires=False):
+    bl, bs = _axisym_sizes(p, multires)
+    fw, fs = np.zeros(sum(b * b for b in bl), np.complex128), np.zeros(bs * bs, np.complex128)
+    fn = "s2let_transform_axisym_lm_wav_analysis" + ("_multires" if multires else "")
+    getattr(lib(), fn)(_ptr(fw), _ptr(fs), _ptr(_cin(flm)), _ptr(_rin(wav_lm)), _ptr(_rin(scal_lm)), _ref(p))
+    return fw, fs
+
+
+def axisym_lm_wav_synthesis(f_wav_lm, f_scal_lm, wav_lm, scal_lm, p, multires=False, flm=None):
+    """Adds into flm (a copy is returned; zeros when None), as the reference does."""
+    out = np.zeros(p.L * p.L, np.complex128) if flm is None else np.array(flm, dtype=np.complex128)
+    fn = "s2let_transform_axisym_lm_wav_synthesis" + ("_multires" if multires else "")
+    getattr(lib(), fn)(_ptr(out), _ptr(_cin(f_wav_lm)), _ptr(_cin(f_scal_lm)), _ptr(_rin(wav_lm)),
+                       _ptr(_rin(scal_lm)), _ref(p))
+    return out
+
+
+def _axisym_fn(kind, multires, real):
+    return "s2let_transform_axisym_wav_%s_mw%s%s" % (kind, "_multires" if multires else "", "_real" if real else "")
+
+
+def axisym_wav_analysis_mw(f, p, multires=False, real=False):
+    bl, bs = _axisym_sizes(p, multires)
+    dt, cv = (np.float64, _rin) if real else (np.complex128, _cin)
+    fw, fs = np.zeros(sum(b * (2 * b - 1) for b in bl), dt), np.zeros(bs * (2 * bs - 1), dt)
+    getattr(lib(), _axisym_fn("analysis", multires, real))(_ptr(fw), _ptr(fs), _ptr(cv(f)), _ref(p))
+    return fw, fs
+
+
+def axisym_wav_synthesis_adjoint_mw(f, p, multires=False, real=False):
+    bl, bs = _axisym_sizes(p, multires)
+    dt, cv = (np.float64, _rin) if real else (np.complex128, _cin)
+    fw, fs = np.zeros(sum(b * (2 * b - 1) for b in bl), dt), np.zeros(bs * (2 * bs - 1), dt)
+    getattr(lib(), _axisym_fn("synthesis_adjoint", multires, real))(_ptr(fw), _ptr(fs), _ptr(cv(f)), _ref(p))
+    return fw, fs
+
+
+def axisym_wav_synthesis_mw(f_wav, f_scal, p, multires=False, real=False):
+    dt, cv = (np.float64, _rin) if real else (np.complex128, _cin)
+    f = np.zeros(p.L * (2 * p.L - 1), dt)
+    getattr(lib(), _axisym_fn("synthesis", multires, real))(_ptr(f), _ptr(cv(f_wav)), _ptr(cv(f_scal)), _ref(p))
+    return f
+
+
+def axisym_wav_analysis_adjoint_mw(f_wav, f_scal, p, multires=False, real=False):
+    dt, cv = (np.float64, _rin) if real else (np.complex128, _cin)
+    f = np.zeros(p.L * (2 * p.L - 1), dt)
+    getattr(lib(), _axisym_fn("analysis_adjoint", multires, real))(_ptr(f), _ptr(cv(f_wav)), _ptr(cv(f_scal)),
+                                                                   _ref(p))
+    return f
+
+
+def axisym_wav_hardthreshold_real(g_wav, threshold, p, multires=False):
+    """In place on a copy: samples with |trunc(x)| < threshold[j - J_min] become 0."""
+    g = np.array(g_wav, dtype=np.float64)
+    fn = "s2let_transform_axisym_wav_hardthreshold%s_real" % ("_multires" if multires else "")
+    getattr(lib(), fn)(_ptr(g), _ptr(_rin(threshold)), _ref(p))
+    return g
 
 
 def timing(enable=True):

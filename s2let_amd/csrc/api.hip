@@ -1017,4 +1017,175 @@ void s2let_mw_map2alm_real(std::complex<double> *flm, const double *f, int L) {
     S2H_CHECK_HIP(hipStreamSynchronize(st));
   });
 }
+
+// ---------------------------------------------------------------- axisymmetric wavelets
+// s2let_transform_axisym_mw.c, s2let_transform_axisym_lm.c.  The axisymmetric transform is
+// the directional one at N = 1 and spin 0: Psi^j_{l0} = sqrt((2l+1)/8pi^2) kappa_j(l) and the
+// single gamma slice give f_wav_j = sum_lm f_lm kappa_j(l) Y_lm / sqrt(2 pi), the reference's
+// SSHT^-1(f_lm sqrt(4pi/(2l+1)) wav_lm) / sqrt(2 pi) (s2let_transform_axisym_mw.c:150-165);
+// full resolution is the upsampled schedule, multiresolution the band-limited one.  The
+// reference's axisymmetric functions always use MW sampling and spin 0.
+static s2let_parameters_t axisym_params(const s2let_parameters_t *p, bool multires) {
+  s2let_parameters_t q = *p;
+  q.N = 1;
+  q.spin = 0;
+  q.original_spin = 0;
+  q.upsample = multires ? 0 : 1;
+  q.reality = 0;
+  q.sampling_scheme = S2LET_SAMPLING_MW;
+  return q;
+}
+
+#define S2H_AXISYM(name, target, multires, T_a, a, T_b, b, T_c, c)                 \
+  void name(T_a a, T_b b, T_c c, const s2let_parameters_t *p) {                    \
+    const s2let_parameters_t q = axisym_params(p, multires);                      \
+    target(a, b, c, &q);                                                           \
+  }
+
+typedef std::complex<double> cxd;
+S2H_AXISYM(s2let_transform_axisym_wav_analysis_mw, s2let_analysis_px2wav, false, cxd *, f_wav, cxd *, f_scal,
+           const cxd *, f)
+S2H_AXISYM(s2let_transform_axisym_wav_analysis_adjoint_mw, s2let_analysis_adjoint_wav2px, false, cxd *, f,
+           const cxd *, f_wav, const cxd *, f_scal)
+S2H_AXISYM(s2let_transform_axisym_wav_synthesis_mw, s2let_synthesis_wav2px, false, cxd *, f, const cxd *, f_wav,
+           const cxd *, f_scal)
+S2H_AXISYM(s2let_transform_axisym_wav_synthesis_adjoint_mw, s2let_synthesis_adjoint_px2wav, false, cxd *, f_wav,
+           cxd *, f_scal, const cxd *, f)
+S2H_AXISYM(s2let_transform_axisym_wav_analysis_mw_multires, s2let_analysis_px2wav, true, cxd *, f_wav, cxd *,
+           f_scal, const cxd *, f)
+S2H_AXISYM(s2let_transform_axisym_wav_analysis_adjoint_mw_multires, s2let_analysis_adjoint_wav2px, true, cxd *, f,
+           const cxd *, f_wav, const cxd *, f_scal)
+S2H_AXISYM(s2let_transform_axisym_wav_synthesis_mw_multires, s2let_synthesis_wav2px, true, cxd *, f, const cxd *,
+           f_wav, const cxd *, f_scal)
+S2H_AXISYM(s2let_transform_axisym_wav_synthesis_adjoint_mw_multires, s2let_synthesis_adjoint_px2wav, true, cxd *,
+           f_wav, cxd *, f_scal, const cxd *, f)
+S2H_AXISYM(s2let_transform_axisym_wav_analysis_mw_real, s2let_analysis_px2wav_real, false, double *, f_wav,
+           double *, f_scal, const double *, f)
+S2H_AXISYM(s2let_transform_axisym_wav_analysis_adjoint_mw_real, s2let_analysis_adjoint_wav2px_real, false,
+           double *, f, const double *, f_wav, const double *, f_scal)
+S2H_AXISYM(s2let_transform_axisym_wav_synthesis_mw_real, s2let_synthesis_wav2px_real, false, double *, f,
+           const double *, f_wav, const double *, f_scal)
+S2H_AXISYM(s2let_transform_axisym_wav_synthesis_adjoint_mw_real, s2let_synthesis_adjoint_px2wav_real, false,
+           double *, f_wav, double *, f_scal, const double *, f)
+S2H_AXISYM(s2let_transform_axisym_wav_analysis_mw_multires_real, s2let_analysis_px2wav_real, true, double *, f_wav,
+           double *, f_scal, const double *, f)
+S2H_AXISYM(s2let_transform_axisym_wav_analysis_adjoint_mw_multires_real, s2let_analysis_adjoint_wav2px_real, true,
+           double *, f, const double *, f_wav, const double *, f_scal)
+S2H_AXISYM(s2let_transform_axisym_wav_synthesis_mw_multires_real, s2let_synthesis_wav2px_real, true, double *, f,
+           const double *, f_wav, const double *, f_scal)
+S2H_AXISYM(s2let_transform_axisym_wav_synthesis_adjoint_mw_multires_real, s2let_synthesis_adjoint_px2wav_real,
+           true, double *, f_wav, double *, f_scal, const double *, f)
+#undef S2H_AXISYM
+
+// harmonic-space multiplies with the caller's kernels (s2let_transform_axisym_lm.c:133-324):
+// f_wav_lm[j] = f_lm sqrt(4pi/(2l+1)) wav_lm[j][l], f_scal_lm = f_lm sqrt(4pi/(2l+1)) scal_lm[l];
+// the synthesis adds into flm.  Multiresolution: scale j keeps l < L_j, the scaling part l < L_s.
+static void axisym_lm(bool synthesis, bool multires, cxd *flm_io, cxd *fw, cxd *fs, const cxd *flm_in,
+                      const double *wav_lm, const double *scal_lm, const s2let_parameters_t *p) {
+  Context &C = Context::get();
+  std::lock_guard<std::mutex> lk(C.mu);
+  hipStream_t st = C.stream();
+  const int L = p->L, J = j_max(p), nsc = J + 1 - p->J_min;
+  std::vector<int> bl(nsc);
+  size_t nw = 0;
+  for (int j = p->J_min; j <= J; ++j) {
+    bl[j - p->J_min] = multires ? std::min(bandlimit(j, p), L) : L;
+    nw += (size_t)bl[j - p->J_min] * bl[j - p->J_min];
+  }
+  const int Ls = multires ? std::min(bandlimit(p->J_min - 1, p), L) : L;
+  DevBuf dflm, dw, ds, dk, dsk;
+  dflm.alloc((size_t)L * L * CB);
+  dw.alloc(nw * CB);
+  ds.alloc((size_t)Ls * Ls * CB);
+  dk.alloc((size_t)(J + 1) * L * sizeof(double));
+  dsk.alloc((size_t)L * sizeof(double));
+  h2d(dk.p, wav_lm, (size_t)(J + 1) * L * sizeof(double), st);
+  h2d(dsk.p, scal_lm, (size_t)L * sizeof(double), st);
+  if (synthesis) {
+    h2d(dflm.p, flm_io, (size_t)L * L * CB, st);
+    h2d(dw.p, fw, nw * CB, st);
+    h2d(ds.p, fs, (size_t)Ls * Ls * CB, st);
+  } else {
+    h2d(dflm.p, flm_in, (size_t)L * L * CB, st);
+  }
+  size_t off = 0;
+  for (int j = p->J_min; j <= J; ++j) {
+    const int b = bl[j - p->J_min];
+    void *blk = (char *)dw.p + off * CB;
+    const double *kj = dk.as<double>() + (size_t)j * L;
+    if (synthesis) launch_scal_mult(dflm.p, blk, kj, b, 0, 1, st);
+    else launch_scal_mult(blk, dflm.p, kj, b, 0, 0, st);
+    off += (size_t)b * b;
+  }
+  if (synthesis) {
+    launch_scal_mult(dflm.p, ds.p, dsk.as<double>(), Ls, 0, 1, st);
+    d2h(flm_io, dflm.p, (size_t)L * L * CB, st);
+  } else {
+    launch_scal_mult(ds.p, dflm.p, dsk.as<double>(), Ls, 0, 0, st);
+    d2h(fw, dw.p, nw * CB, st);
+    d2h(fs, ds.p, (size_t)Ls * Ls * CB, st);
+  }
+  S2H_CHECK_HIP(hipGetLastError());
+  S2H_CHECK_HIP(hipStreamSynchronize(st));
+}
+
+void s2let_transform_axisym_lm_wav_analysis(cxd *f_wav_lm, cxd *f_scal_lm, const cxd *flm, const double *wav_lm,
+                                            const double *scal_lm, const s2let_parameters_t *p) {
+  host_call(__func__, [&] { axisym_lm(false, false, nullptr, f_wav_lm, f_scal_lm, flm, wav_lm, scal_lm, p); });
+}
+void s2let_transform_axisym_lm_wav_synthesis(cxd *flm, const cxd *f_wav_lm, const cxd *f_scal_lm,
+                                             const double *wav_lm, const double *scal_lm,
+                                             const s2let_parameters_t *p) {
+  host_call(__func__, [&] {
+    axisym_lm(true, false, flm, const_cast<cxd *>(f_wav_lm), const_cast<cxd *>(f_scal_lm), nullptr, wav_lm, scal_lm,
+              p);
+  });
+}
+void s2let_transform_axisym_lm_wav_analysis_multires(cxd *f_wav_lm, cxd *f_scal_lm, const cxd *flm,
+                                                     const double *wav_lm, const double *scal_lm,
+                                                     const s2let_parameters_t *p) {
+  host_call(__func__, [&] { axisym_lm(false, true, nullptr, f_wav_lm, f_scal_lm, flm, wav_lm, scal_lm, p); });
+}
+void s2let_transform_axisym_lm_wav_synthesis_multires(cxd *flm, const cxd *f_wav_lm, const cxd *f_scal_lm,
+                                                      const double *wav_lm, const double *scal_lm,
+                                                      const s2let_parameters_t *p) {
+  host_call(__func__, [&] {
+    axisym_lm(true, true, flm, const_cast<cxd *>(f_wav_lm), const_cast<cxd *>(f_scal_lm), nullptr, wav_lm, scal_lm,
+              p);
+  });
+}
+
+// s2let_transform_axisym_mw.c:1183-1230: zero real wavelet samples below a per-scale threshold
+static void axisym_threshold(double *g_wav, const double *threshold, const s2let_parameters_t *p, bool multires) {
+  Context &C = Context::get();
+  std::lock_guard<std::mutex> lk(C.mu);
+  hipStream_t st = C.stream();
+  const int L = p->L, J = j_max(p);
+  size_t n = 0;
+  std::vector<size_t> cnt;
+  for (int j = p->J_min; j <= J; ++j) {
+    const int b = multires ? std::min(bandlimit(j, p), L) : L;
+    cnt.push_back((size_t)b * (2 * b - 1));
+    n += cnt.back();
+  }
+  DevBuf d;
+  d.alloc(n * sizeof(double));
+  h2d(d.p, g_wav, n * sizeof(double), st);
+  size_t off = 0;
+  for (size_t k = 0; k < cnt.size(); ++k) {
+    launch_hardthreshold(d.as<double>() + off, cnt[k], threshold[k], st);
+    off += cnt[k];
+  }
+  d2h(g_wav, d.p, n * sizeof(double), st);
+  S2H_CHECK_HIP(hipGetLastError());
+  S2H_CHECK_HIP(hipStreamSynchronize(st));
+}
+void s2let_transform_axisym_wav_hardthreshold_real(double *g_wav, const double *threshold,
+                                                   const s2let_parameters_t *p) {
+  host_call(__func__, [&] { axisym_threshold(g_wav, threshold, p, false); });
+}
+void s2let_transform_axisym_wav_hardthreshold_multires_real(double *g_wav, const double *threshold,
+                                                            const s2let_parameters_t *p) {
+  host_call(__func__, [&] { axisym_threshold(g_wav, threshold, p, true); });
+}
 }  // extern "C"

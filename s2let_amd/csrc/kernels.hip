@@ -73,6 +73,15 @@ __global__ void k_cx2re(const cx<double> *__restrict__ a, double *__restrict__ b
     b[i] = a[i].x;
 }
 
+// g[i] = 0 where |trunc(g[i])| < thr: the reference compares C's int abs() of the
+// value (s2let_transform_axisym_mw.c:1192-1194), i.e. after truncation to an integer
+__global__ void k_hardthreshold(double *__restrict__ g, size_t n, double thr) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double t = trunc(g[i]);
+    if (fabs(t) < thr) g[i] = 0.0;
+  }
+}
+
 __global__ void k_add_cx(cx<double> *__restrict__ a, const cx<double> *__restrict__ b, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     a[i] = cadd(a[i], b[i]);
@@ -165,6 +174,9 @@ void launch_to_f32(const double *a, float *b, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_to_f32, dim3(grid1d(n)), dim3(256), 0, st, a, b, n);
 }
 
+void launch_hardthreshold(double *g, size_t n, double thr, hipStream_t st) {
+  if (n) hipLaunchKernelGGL(k_hardthreshold, dim3(grid1d(n)), dim3(256), 0, st, g, n, thr);
+}
 void launch_add_cx(void *a, const void *b, size_t n, hipStream_t st) {
   if (n) hipLaunchKernelGGL(k_add_cx, dim3(grid1d(n)), dim3(256), 0, st, (cx<double> *)a, (const cx<double> *)b, n);
 }

@@ -332,6 +332,71 @@ void s2let_allocate_f_wav_manual(std::complex<double> **f_wav, std::complex<doub
                             : (size_t)scal_bandlimit * (2 * scal_bandlimit - 1);
   *f_scal = (std::complex<double> *)calloc(ns, sizeof(std::complex<double>));
 }
+// ---- axisymmetric wavelets (s2let_transform_axisym_lm.c:25-120, s2let_transform_axisym_mw.c:23-120)
+void s2let_transform_axisym_lm_allocate_f_wav(std::complex<double> **f_wav_lm, std::complex<double> **f_scal_lm,
+                                              const s2let_parameters_t *p) {
+  const int L = p->L, J = j_max(p);
+  *f_wav_lm = (std::complex<double> *)calloc((size_t)(J + 1 - p->J_min) * L * L, sizeof(std::complex<double>));
+  *f_scal_lm = (std::complex<double> *)calloc((size_t)L * L, sizeof(std::complex<double>));
+}
+void s2let_transform_axisym_lm_allocate_f_wav_multires(std::complex<double> **f_wav_lm,
+                                                       std::complex<double> **f_scal_lm, const s2let_parameters_t *p) {
+  const int L = p->L, J = j_max(p);
+  size_t total = 0;
+  for (int j = p->J_min; j <= J; ++j) {
+    const int b = std::min(bandlimit(j, p), L);
+    total += (size_t)b * b;
+  }
+  *f_wav_lm = (std::complex<double> *)calloc(total, sizeof(std::complex<double>));
+  const int b = std::min(bandlimit(p->J_min - 1, p), L);
+  *f_scal_lm = (std::complex<double> *)calloc((size_t)b * b, sizeof(std::complex<double>));
+}
+void s2let_transform_axisym_lm_allocate_wav(double **wav_lm, double **scal_lm, const s2let_parameters_t *p) {
+  *wav_lm = (double *)calloc((size_t)(j_max(p) + 1) * p->L, sizeof(double));
+  *scal_lm = (double *)calloc(p->L, sizeof(double));
+}
+// kernels sqrt((2l+1)/4pi) kappa_j(l), sqrt((2l+1)/4pi) kappa_0(l) (s2let_transform_axisym_lm.c:91-120)
+void s2let_transform_axisym_lm_wav(double *wav_lm, double *scal_lm, const s2let_parameters_t *p) {
+  const int L = p->L, J = j_max(p);
+  std::vector<double> kappa((size_t)(J + 1) * L, 0.0), kappa0(L, 0.0);
+  tiling_axisym(kappa.data(), kappa0.data(), p);
+  for (int j = p->J_min; j <= J; j++)
+    for (int l = 0; l < L; l++) wav_lm[j * L + l] = sqrt((2 * l + 1) / (4.0 * M_PI)) * kappa[l + j * L];
+  for (int l = 0; l < L; l++) scal_lm[l] = sqrt((2 * l + 1) / (4.0 * M_PI)) * kappa0[l];
+}
+static size_t axisym_wav_px(const s2let_parameters_t *p, bool multires) {
+  const int L = p->L, J = j_max(p);
+  size_t total = 0;
+  for (int j = p->J_min; j <= J; ++j) {
+    const int b = multires ? std::min(bandlimit(j, p), L) : L;
+    total += (size_t)b * (2 * b - 1);
+  }
+  return total;
+}
+static size_t axisym_scal_px(const s2let_parameters_t *p, bool multires) {
+  const int b = multires ? std::min(bandlimit(p->J_min - 1, p), p->L) : p->L;
+  return (size_t)b * (2 * b - 1);
+}
+void s2let_transform_axisym_allocate_mw_f_wav(std::complex<double> **f_wav, std::complex<double> **f_scal,
+                                              const s2let_parameters_t *p) {
+  *f_wav = (std::complex<double> *)calloc(axisym_wav_px(p, false), sizeof(std::complex<double>));
+  *f_scal = (std::complex<double> *)calloc(axisym_scal_px(p, false), sizeof(std::complex<double>));
+}
+void s2let_transform_axisym_allocate_mw_f_wav_multires(std::complex<double> **f_wav, std::complex<double> **f_scal,
+                                                       const s2let_parameters_t *p) {
+  *f_wav = (std::complex<double> *)calloc(axisym_wav_px(p, true), sizeof(std::complex<double>));
+  *f_scal = (std::complex<double> *)calloc(axisym_scal_px(p, true), sizeof(std::complex<double>));
+}
+void s2let_transform_axisym_allocate_mw_f_wav_real(double **f_wav, double **f_scal, const s2let_parameters_t *p) {
+  *f_wav = (double *)calloc(axisym_wav_px(p, false), sizeof(double));
+  *f_scal = (double *)calloc(axisym_scal_px(p, false), sizeof(double));
+}
+void s2let_transform_axisym_allocate_mw_f_wav_multires_real(double **f_wav, double **f_scal,
+                                                            const s2let_parameters_t *p) {
+  *f_wav = (double *)calloc(axisym_wav_px(p, true), sizeof(double));
+  *f_scal = (double *)calloc(axisym_scal_px(p, true), sizeof(double));
+}
+
 void s2let_allocate_mwss(std::complex<double> **f, int L) {
   *f = (std::complex<double> *)calloc((size_t)2 * L * (L + 1), sizeof(std::complex<double>));
 }

@@ -72,3 +72,12 @@ def test_other_kernels_match_oracle(s2, oracle, wavtype):
     finally:
         s2.switch_wavtype(1)
         oracle.switch_wavtype(1)
+
+
+@pytest.mark.parametrize("L,B,J_min", [(16, 2.0, 2), (64, 3.0, 1)])
+def test_axisym_kernels_match_oracle(s2, oracle, L, B, J_min):
+    """s2let_transform_axisym_lm_wav (host tiling code), bit for bit."""
+    p, q = s2.parameters(L, 1, B, J_min), oracle.params(L, 1, B, J_min)
+    a, b = s2.axisym_lm_wav(p)
+    c, d = oracle.axisym_lm_wav(q)
+    assert np.array_equal(a, c) and np.array_equal(b, d)
