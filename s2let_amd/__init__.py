@@ -223,6 +223,17 @@ def tiling_wavelet(p):
     return psi, phi
 
 
+def n_wav_j(j, p):
+    return lib().s2let_n_wav_j(int(j), _ref(p))
+
+
+def tiling_direction(p):
+    """s_lm of the directional tiling, L^2 entries (s2let_tiling.c:281-319)."""
+    s = np.zeros(p.L * p.L, np.complex128)
+    lib().s2let_tiling_direction(_ptr(s), _ref(p))
+    return s
+
+
 def tiling_axisym(p):
     J = j_max(p)
     kappa = np.zeros((J + 1) * p.L)
