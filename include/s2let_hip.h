@@ -296,6 +296,21 @@ void s2let_synthesis_adjoint_lm2wav_real(double *f_wav, double *f_scal, const S2
 void s2let_synthesis_adjoint_px2wav_real(double *f_wav, double *f_scal, const double *f,
                                          const s2let_parameters_t *parameters); /* s2let_synthesis_adjoint.c:506 */
 
+/* ---- FITS maps (s2let_fits.c): an empty primary HDU and one binary table of double
+ * columns (SIGNAL, or Q_POLARISATION and U_POLARISATION) with the keyword L (and MWSS = 1);
+ * the readers accept 4- or 8-byte real columns and exit with a FITS status code on a
+ * missing file or keyword, as the reference does. ---- */
+int s2let_fits_mw_read_bandlimit(char *filename);                               /* s2let_fits.c:17 */
+int s2let_fits_hpx_read_nside(char *filename);                                  /* s2let_fits.c:52 */
+void s2let_fits_mw_write_map(char *filename, double *f, int L);                 /* s2let_fits.c:87 */
+void s2let_fits_mwss_write_map(char *filename, double *f, int L);               /* s2let_fits.c:130 */
+void s2let_fits_mw_read_map(double *f, char *filename, int L);                  /* s2let_fits.c:177 */
+void s2let_fits_mwss_read_map(double *f, char *filename, int L);                /* s2let_fits.c:236 */
+void s2let_fits_mw_write_spin_maps(char *filename, double *fQ, double *fU, int L);   /* s2let_fits.c:302 */
+void s2let_fits_mwss_write_spin_maps(char *filename, double *fQ, double *fU, int L); /* s2let_fits.c:348 */
+void s2let_fits_mw_read_spin_maps(double *fQ, double *fU, char *filename, int L);    /* s2let_fits.c:398 */
+void s2let_fits_mwss_read_spin_maps(double *fQ, double *fU, char *filename, int L);  /* s2let_fits.c:471 */
+
 /* ---- MW spherical harmonic transforms used by pys2let (s2let_mw.c:21-70) ---- */
 void s2let_mw_alm2map(S2LET_COMPLEX(double) *f, const S2LET_COMPLEX(double) *flm, int L, int spin);
 void s2let_mw_map2alm(S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *f, int L, int spin);

@@ -114,6 +114,15 @@ for _n in ("lm_allocate_f_wav", "lm_allocate_f_wav_multires", "lm_allocate_wav",
 for _k in ("analysis", "analysis_adjoint", "synthesis", "synthesis_adjoint"):
     for _s in ("", "_multires", "_real", "_multires_real"):
         _SIGS["s2let_transform_axisym_wav_%s_mw%s" % (_k, _s)] = (None, [ctypes.c_void_p] * 4)
+_SIGS["s2let_fits_mw_read_bandlimit"] = (ctypes.c_int, [ctypes.c_char_p])
+_SIGS["s2let_fits_hpx_read_nside"] = (ctypes.c_int, [ctypes.c_char_p])
+for _n in ("mw", "mwss"):
+    _SIGS["s2let_fits_%s_write_map" % _n] = (None, [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int])
+    _SIGS["s2let_fits_%s_read_map" % _n] = (None, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int])
+    _SIGS["s2let_fits_%s_write_spin_maps" % _n] = (None, [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                        ctypes.c_int])
+    _SIGS["s2let_fits_%s_read_spin_maps" % _n] = (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
+                                                       ctypes.c_int])
 _SIGS["s2let_hip_selftest"] = (ctypes.c_int, [])
 _SIGS["s2let_hip_timing"] = (None, [ctypes.c_int])
 _SIGS["s2let_hip_timing_report"] = (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int])
@@ -596,6 +605,41 @@ def axisym_wav_hardthreshold_real(g_wav, threshold, p, multires=False):
     fn = "s2let_transform_axisym_wav_hardthreshold%s_real" % ("_multires" if multires else "")
     getattr(lib(), fn)(_ptr(g), _ptr(_rin(threshold)), _ref(p))
     return g
+
+
+# ----------------------------------------------------------------------------- FITS maps (s2let_fits.c)
+def _npx(L, mwss):
+    return (L + 1) * 2 * L if mwss else L * (2 * L - 1)
+
+
+def fits_write_map(filename, f, L, mwss=False):
+    getattr(lib(), "s2let_fits_%s_write_map" % ("mwss" if mwss else "mw"))(str(filename).encode(), _ptr(_rin(f)), int(L))
+
+
+def fits_read_map(filename, L, mwss=False):
+    f = np.zeros(_npx(L, mwss))
+    getattr(lib(), "s2let_fits_%s_read_map" % ("mwss" if mwss else "mw"))(_ptr(f), str(filename).encode(), int(L))
+    return f
+
+
+def fits_write_spin_maps(filename, fQ, fU, L, mwss=False):
+    getattr(lib(), "s2let_fits_%s_write_spin_maps" % ("mwss" if mwss else "mw"))(str(filename).encode(), _ptr(_rin(fQ)),
+                                                                       _ptr(_rin(fU)), int(L))
+
+
+def fits_read_spin_maps(filename, L, mwss=False):
+    q, u = np.zeros(_npx(L, mwss)), np.zeros(_npx(L, mwss))
+    getattr(lib(), "s2let_fits_%s_read_spin_maps" % ("mwss" if mwss else "mw"))(_ptr(q), _ptr(u), str(filename).encode(),
+                                                                      int(L))
+    return q, u
+
+
+def fits_read_bandlimit(filename):
+    return lib().s2let_fits_mw_read_bandlimit(str(filename).encode())
+
+
+def fits_read_nside(filename):
+    return lib().s2let_fits_hpx_read_nside(str(filename).encode())
 
 
 def timing(enable=True):
