@@ -22,7 +22,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libs2let_amd.so")
+# S2LET_AMD_LIB selects another build of the library (e.g. to compare two builds in one process tree)
+LIB_PATH = os.environ.get("S2LET_AMD_LIB", os.path.join(_HERE, "libs2let_amd.so"))
 _lib = None
 
 FP64 = 0

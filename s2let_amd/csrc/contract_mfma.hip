@@ -129,7 +129,11 @@ __global__ void k_etab(const double *__restrict__ dtab, const ColDev *__restrict
 // Block: 4 waves x 16 |m| (one m-block each), MP consecutive m'.  Grid: x = m-group
 // of 64 |m|, y = m' group.  F_c = Fbase + cols[c].out_off, layout [m'][m + L - 1].
 // x has Lx^2 entries (Lx >= L).
-template <typename T, int MP>
+// F64 (FP32 data only): the l sums are flushed into FP64 every 32 l; without it they
+// accumulate in FP32 (band-limits <= kInvF32MaxL, where the transform stays within
+// 1e-6 of FP64 and the 64 FP64 registers go)
+constexpr int kInvF32MaxL = 512;
+template <typename T, int MP, bool F64>
 __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ x, const T *__restrict__ dt,
                                               cx<T> *__restrict__ Fbase, InvJobs<T> jobs) {
   typedef typename Mfma<T>::acc acc_t;
@@ -148,16 +152,21 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ x, const
   const int nlb = (L + 15) / 16;
   const int m = mb + r16;
   const int lb0 = max(max(mp0, mb), lmin) >> 4;
+  constexpr bool sums = sizeof(T) == 8 || F64;  // FP64 block sums (always for FP64 data)
   acc_t acc[MP][4];
-  double hs[MP][4][4];
+  double hs[sums ? MP : 1][4][4];
 #pragma unroll
   for (int j = 0; j < MP; ++j)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      acc[j][q] = Mfma<T>::zero();
+    for (int q = 0; q < 4; ++q) acc[j][q] = Mfma<T>::zero();
+  if constexpr (sums) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) hs[j][q][v] = 0.0;
-    }
+    for (int j = 0; j < MP; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) hs[j][q][v] = 0.0;
+  }
   for (int lb = lb0; lb < nlb; ++lb) {
     const int l0 = 16 * lb, nmp = 16 * (lb + 1);
     V4<T> d[MP], e[MP];
@@ -189,7 +198,7 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ x, const
         acc[j][3] = Mfma<T>::mma(e[j].v[s], dm * xm.y, acc[j][3]);
       }
     }
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4 && F64) {
       if ((lb - lb0) & 1) {  // flush the 32-ell block sums into FP64
 #pragma unroll
         for (int j = 0; j < MP; ++j)
@@ -213,8 +222,14 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ x, const
     for (int j = 0; j < MP; ++j) {
       const int mp = mp0 + j;
       if (mp >= L) continue;
-      const double rp = hs[j][0][v] + (double)acc[j][0][v], ip = hs[j][1][v] + (double)acc[j][1][v];
-      const double rm = hs[j][2][v] + (double)acc[j][2][v], im = hs[j][3][v] + (double)acc[j][3][v];
+      double rp = (double)acc[j][0][v], ip = (double)acc[j][1][v];
+      double rm = (double)acc[j][2][v], im = (double)acc[j][3][v];
+      if constexpr (sums) {
+        rp += hs[j][0][v];
+        ip += hs[j][1][v];
+        rm += hs[j][2][v];
+        im += hs[j][3][v];
+      }
       cx<T> *row = F + (size_t)mp * M;
       row[m] = mk<T>((T)rp, (T)ip);
       if (m > 0) row[-m] = mk<T>((T)rm, (T)im);
@@ -392,8 +407,14 @@ template void build_etab<float>(const double *, const ColDev *, int, const doubl
 template <typename T>
 void launch_inv_mfma(const void *x, const T *dt, void *Fbase, const InvJobs<T> &jobs, hipStream_t st) {
   if (jobs.n == 0 || jobs.nblk == 0) return;
-  hipLaunchKernelGGL((k_inv3<T, kInvMP>), dim3(jobs.nblk), dim3(256), 0, st, (const cx<T> *)x, dt, (cx<T> *)Fbase,
-                     jobs);
+  int Lmax = 0;
+  for (int k = 0; k < jobs.n; ++k) Lmax = std::max(Lmax, jobs.j[k].L);
+  if (sizeof(T) == 4 && Lmax <= kInvF32MaxL)
+    hipLaunchKernelGGL((k_inv3<T, kInvMP, false>), dim3(jobs.nblk), dim3(256), 0, st, (const cx<T> *)x, dt,
+                       (cx<T> *)Fbase, jobs);
+  else
+    hipLaunchKernelGGL((k_inv3<T, kInvMP, true>), dim3(jobs.nblk), dim3(256), 0, st, (const cx<T> *)x, dt,
+                       (cx<T> *)Fbase, jobs);
 }
 template void launch_inv_mfma<double>(const void *, const double *, void *, const InvJobs<double> &, hipStream_t);
 template void launch_inv_mfma<float>(const void *, const float *, void *, const InvJobs<float> &, hipStream_t);
