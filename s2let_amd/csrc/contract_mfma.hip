@@ -246,6 +246,7 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ x, const
 template <typename T>
 struct FwdTile {
   static constexpr int MC = sizeof(T) == 8 ? 4 : 8;
+  static constexpr int PD = sizeof(T) == 8 ? 1 : 4;  // m' records in flight per l-block
 };
 
 template <typename T>
@@ -322,16 +323,22 @@ __global__ __launch_bounds__(256) void k_fwd3(const SuperDev *__restrict__ sgs, 
         const T *dq = dt + dtile_base(lbi) + ((size_t)mbi * nmp + mp0) * 256 + 4 * lane;
         const T *eq = Ef + ((size_t)lbi * Lg + mp0) * 256 + 4 * lane;
         const int par = (l0 + k4 + mp0) & 1;
-        V4<T> dn{}, en{};
-        if (n > 0) {
-          dn = ld4(dq);
-          en = ld4(eq);
-        }
-        for (int mc = 0; mc < n; ++mc) {
-          const V4<T> dv = dn, ev = en;  // records of this m'; the next m' is loaded meanwhile
-          if (mc + 1 < n) {
-            dn = ld4(dq + (size_t)(mc + 1) * 256);
-            en = ld4(eq + (size_t)(mc + 1) * 256);
+        // the Delta / weight records of the next kFwdPD m' are in flight (ring of registers)
+        constexpr int kFwdPD = FwdTile<T>::PD;
+        V4<T> dr[kFwdPD], er[kFwdPD];
+#pragma unroll
+        for (int k = 0; k < kFwdPD; ++k)
+          if (k < n) {
+            dr[k] = ld4(dq + (size_t)k * 256);
+            er[k] = ld4(eq + (size_t)k * 256);
+          }
+#pragma unroll
+        for (int mc = 0; mc < MC; ++mc) {
+          if (mc >= n) break;
+          const V4<T> dv = dr[mc % kFwdPD], ev = er[mc % kFwdPD];
+          if (mc + kFwdPD < n) {
+            dr[mc % kFwdPD] = ld4(dq + (size_t)(mc + kFwdPD) * 256);
+            er[mc % kFwdPD] = ld4(eq + (size_t)(mc + kFwdPD) * 256);
           }
           acc_t h[4];
 #pragma unroll
