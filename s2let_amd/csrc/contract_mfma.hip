@@ -126,6 +126,32 @@ __global__ void k_etab(const double *__restrict__ dtab, const ColDev *__restrict
 }
 
 // ------------------------------------------------------------------ inverse
+// Harmonic coefficients in the B-operand layout of the inverse contraction: for l-block
+// lb and |m|-block mbi, lane (k4, r16) holds l = 16 lb + 4 s + k4, m = 16 mbi + r16 for s < 4:
+//   P[((lb nx + mbi) 64 + lane) 4 + s] = x[l, m],  Q[...] = (-1)^l x[l, -m]
+// (zero unless m <= l < Lx), nx = ceil(Lx / 16); Q follows P after nx^2 256 entries.
+size_t xtile_elems(int Lx) { return 2 * (size_t)((Lx + 15) / 16) * ((Lx + 15) / 16) * 256; }
+
+template <typename T>
+__global__ void k_xtile(const cx<T> *__restrict__ x, int Lx, cx<T> *__restrict__ xt) {
+  const int nx = (Lx + 15) / 16;
+  const size_t half = (size_t)nx * nx * 256;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < half; e += (size_t)gridDim.x * blockDim.x) {
+    const int s = (int)(e & 3), lane = (int)((e >> 2) & 63);
+    const size_t r = e >> 8;
+    const int mbi = (int)(r % nx), lb = (int)(r / nx);
+    const int l = 16 * lb + 4 * s + (lane >> 4), m = 16 * mbi + (lane & 15);
+    cx<T> p = mk<T>(0, 0), q = mk<T>(0, 0);
+    if (l < Lx && m <= l) {
+      p = x[(size_t)l * l + l + m];
+      q = x[(size_t)l * l + l - m];
+      if (l & 1) q = mk<T>(-q.x, -q.y);
+    }
+    xt[e] = p;
+    xt[half + e] = q;
+  }
+}
+
 // Block: 4 waves x 16 |m| (one m-block each), MP consecutive m'.  Grid: x = m-group
 // of 64 |m|, y = m' group.  F_c = Fbase + cols[c].out_off, layout [m'][m + L - 1].
 // x has Lx^2 entries (Lx >= L).
@@ -134,7 +160,7 @@ __global__ void k_etab(const double *__restrict__ dtab, const ColDev *__restrict
 // 1e-6 of FP64 and the 64 FP64 registers go)
 constexpr int kInvF32MaxL = 512;
 template <typename T, int MP, bool F64>
-__global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ x, const T *__restrict__ dt,
+__global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt, int nx, const T *__restrict__ dt,
                                               cx<T> *__restrict__ Fbase, InvJobs<T> jobs) {
   typedef typename Mfma<T>::acc acc_t;
   int jb = 0;
@@ -152,6 +178,7 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ x, const
   const int nlb = (L + 15) / 16;
   const int m = mb + r16;
   const int lb0 = max(max(mp0, mb), lmin) >> 4;
+  const size_t xhalf = (size_t)nx * nx * 256;
   constexpr bool sums = sizeof(T) == 8 || F64;  // FP64 block sums (always for FP64 data)
   acc_t acc[MP][4];
   double hs[sums ? MP : 1][4][4];
@@ -181,21 +208,23 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ x, const
         for (int s = 0; s < 4; ++s) d[j].v[s] = e[j].v[s] = (T)0;
       }
     }
+    // x[l, m] and (-1)^l x[l, -m] of this lane's four l (the sign (-1)^{m'} of the -m
+    // products is applied at the end); E vanishes for l >= L
+    const cx<T> *xq = xt + (((size_t)lb * nx + mbi) * 64 + lane) * 4;
+    const V4<T> p01 = ld4(&xq[0].x), p23 = ld4(&xq[2].x);
+    const V4<T> q01 = ld4(&xq[xhalf].x), q23 = ld4(&xq[xhalf + 2].x);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      const int l = l0 + 4 * s + k4;
-      cx<T> xp = mk<T>(0, 0), xm = mk<T>(0, 0);
-      if (l < L && m <= l) {
-        xp = x[(size_t)l * l + l + m];
-        xm = x[(size_t)l * l + l - m];
-      }
+      const V4<T> &pp = s < 2 ? p01 : p23, &qq = s < 2 ? q01 : q23;
+      const T xpr = pp.v[2 * (s & 1)], xpi = pp.v[2 * (s & 1) + 1];
+      const T xmr = qq.v[2 * (s & 1)], xmi = qq.v[2 * (s & 1) + 1];
 #pragma unroll
       for (int j = 0; j < MP; ++j) {
-        const T dp = d[j].v[s], dm = ((l + mp0 + j) & 1) ? -dp : dp;
-        acc[j][0] = Mfma<T>::mma(e[j].v[s], dp * xp.x, acc[j][0]);
-        acc[j][1] = Mfma<T>::mma(e[j].v[s], dp * xp.y, acc[j][1]);
-        acc[j][2] = Mfma<T>::mma(e[j].v[s], dm * xm.x, acc[j][2]);
-        acc[j][3] = Mfma<T>::mma(e[j].v[s], dm * xm.y, acc[j][3]);
+        const T dp = d[j].v[s];
+        acc[j][0] = Mfma<T>::mma(e[j].v[s], dp * xpr, acc[j][0]);
+        acc[j][1] = Mfma<T>::mma(e[j].v[s], dp * xpi, acc[j][1]);
+        acc[j][2] = Mfma<T>::mma(e[j].v[s], dp * xmr, acc[j][2]);
+        acc[j][3] = Mfma<T>::mma(e[j].v[s], dp * xmi, acc[j][3]);
       }
     }
     if constexpr (sizeof(T) == 4 && F64) {
@@ -222,13 +251,14 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ x, const
     for (int j = 0; j < MP; ++j) {
       const int mp = mp0 + j;
       if (mp >= L) continue;
+      const double sg = ((mp0 + j) & 1) ? -1.0 : 1.0;  // (-1)^{m'} of the -m products
       double rp = (double)acc[j][0][v], ip = (double)acc[j][1][v];
-      double rm = (double)acc[j][2][v], im = (double)acc[j][3][v];
+      double rm = sg * (double)acc[j][2][v], im = sg * (double)acc[j][3][v];
       if constexpr (sums) {
         rp += hs[j][0][v];
         ip += hs[j][1][v];
-        rm += hs[j][2][v];
-        im += hs[j][3][v];
+        rm += sg * hs[j][2][v];
+        im += sg * hs[j][3][v];
       }
       cx<T> *row = F + (size_t)mp * M;
       row[m] = mk<T>((T)rp, (T)ip);
@@ -412,19 +442,30 @@ template void build_etab<double>(const double *, const ColDev *, int, const doub
 template void build_etab<float>(const double *, const ColDev *, int, const double *, int, float *, float *, hipStream_t);
 
 template <typename T>
-void launch_inv_mfma(const void *x, const T *dt, void *Fbase, const InvJobs<T> &jobs, hipStream_t st) {
+void launch_xtile(const void *x, int Lx, void *xtile, hipStream_t st) {
+  const size_t half = xtile_elems(Lx) / 2;
+  const unsigned g = (unsigned)std::min<size_t>((half + 255) / 256, 65535 * 4);
+  hipLaunchKernelGGL((k_xtile<T>), dim3(g), dim3(256), 0, st, (const cx<T> *)x, Lx, (cx<T> *)xtile);
+}
+template void launch_xtile<double>(const void *, int, void *, hipStream_t);
+template void launch_xtile<float>(const void *, int, void *, hipStream_t);
+
+template <typename T>
+void launch_inv_mfma(const void *xtile, int Lx, const T *dt, void *Fbase, const InvJobs<T> &jobs, hipStream_t st) {
   if (jobs.n == 0 || jobs.nblk == 0) return;
   int Lmax = 0;
   for (int k = 0; k < jobs.n; ++k) Lmax = std::max(Lmax, jobs.j[k].L);
+  const int nx = (Lx + 15) / 16;
   if (sizeof(T) == 4 && Lmax <= kInvF32MaxL)
-    hipLaunchKernelGGL((k_inv3<T, kInvMP, false>), dim3(jobs.nblk), dim3(256), 0, st, (const cx<T> *)x, dt,
+    hipLaunchKernelGGL((k_inv3<T, kInvMP, false>), dim3(jobs.nblk), dim3(256), 0, st, (const cx<T> *)xtile, nx, dt,
                        (cx<T> *)Fbase, jobs);
   else
-    hipLaunchKernelGGL((k_inv3<T, kInvMP, true>), dim3(jobs.nblk), dim3(256), 0, st, (const cx<T> *)x, dt,
+    hipLaunchKernelGGL((k_inv3<T, kInvMP, true>), dim3(jobs.nblk), dim3(256), 0, st, (const cx<T> *)xtile, nx, dt,
                        (cx<T> *)Fbase, jobs);
 }
-template void launch_inv_mfma<double>(const void *, const double *, void *, const InvJobs<double> &, hipStream_t);
-template void launch_inv_mfma<float>(const void *, const float *, void *, const InvJobs<float> &, hipStream_t);
+template void launch_inv_mfma<double>(const void *, int, const double *, void *, const InvJobs<double> &,
+                                      hipStream_t);
+template void launch_inv_mfma<float>(const void *, int, const float *, void *, const InvJobs<float> &, hipStream_t);
 
 int fwd_splits_default(int L) { return std::max(1, std::min(kMaxSplits, L / 64)); }
 

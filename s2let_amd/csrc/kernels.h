@@ -114,8 +114,12 @@ struct InvJobs {
   InvJob<T> j[16];
 };
 constexpr int kInvMP = 2;  // m' per block of the inverse contraction
+// harmonic coefficients x (Lx^2) in the inverse contraction's operand layout (xtile_elems(Lx) entries)
+size_t xtile_elems(int Lx);
 template <typename T>
-void launch_inv_mfma(const void *x, const T *dt, void *Fbase, const InvJobs<T> &jobs, hipStream_t st);
+void launch_xtile(const void *x, int Lx, void *xtile, hipStream_t st);
+template <typename T>
+void launch_inv_mfma(const void *xtile, int Lx, const T *dt, void *Fbase, const InvJobs<T> &jobs, hipStream_t st);
 template <typename T>
 void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *const *Efs, const T *dt, int L,
                      const void *Jbase, void *y, void *parts, int splits, hipStream_t st);

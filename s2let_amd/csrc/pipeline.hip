@@ -211,9 +211,13 @@ template <typename T>
 void contract_inv_all(Plan &P, const void *x, const ColDev *cols, void *Fbase, hipStream_t st) {
   Context &C = Context::get();
   InvJobs<T> J{};
+  {
+    Stage s_("contract_inv", st);
+    launch_xtile<T>(x, P.p.L, P.xtile.p, st);
+  }
   auto flush = [&]() {
     Stage s_("contract_inv", st);
-    launch_inv_mfma<T>(x, Sel<T>::delta(C), Fbase, J, st);
+    launch_inv_mfma<T>(P.xtile.p, P.p.L, Sel<T>::delta(C), Fbase, J, st);
     J.n = 0;
     J.nblk = 0;
   };
@@ -440,7 +444,8 @@ void sht_inverse_t(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
     J.j[0] = InvJob<T>{(const T *)S.ei, col, 1, std::abs(S.spin), L, (L + 63) / 64, 0};
     J.nblk = J.j[0].gx * ((L + kInvMP - 1) / kInvMP);
     Stage s_("contract_inv", st);
-    launch_inv_mfma<T>(x, Sel<T>::delta(C), S.spec.p, J, st);
+    launch_xtile<T>(x, L, S.xtile.p, st);
+    launch_inv_mfma<T>(S.xtile.p, L, Sel<T>::delta(C), S.spec.p, J, st);
   }
   { Stage s_("beta_inv", st); theta_one<T>(L, col, 0, S.spec.p, S.work.p, S.ss, st); }
   { Stage s_("rows", st); rows_one<T, double, T>(L, S.work.p, f, 1, 0, +1, 1.0, S.ss, st); }
@@ -480,7 +485,8 @@ void sht_forward_adjoint_t(ShtPlan &S, const void *flm, void *f, hipStream_t st)
     J.j[0] = InvJob<T>{(const T *)S.ei, col, 1, std::abs(S.spin), L, (L + 63) / 64, 0};
     J.nblk = J.j[0].gx * ((L + kInvMP - 1) / kInvMP);
     Stage s_("contract_inv", st);
-    launch_inv_mfma<T>(x, Sel<T>::delta(C), S.spec.p, J, st);
+    launch_xtile<T>(x, L, S.xtile.p, st);
+    launch_inv_mfma<T>(S.xtile.p, L, Sel<T>::delta(C), S.spec.p, J, st);
   }
   { Stage s_("beta_fwd_adj", st); theta_one<T>(L, col, 3, S.spec.p, S.work.p, S.ss, st); }
   { Stage s_("rows", st); rows_one<T, double, T>(L, S.work.p, f, 1, 0, +1, kTwoPi / M, S.ss, st); }

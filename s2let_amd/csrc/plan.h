@@ -79,6 +79,7 @@ struct Plan {
   DevBuf spec, work;  // scratch (T = double or float per mode)
   DevBuf flm32;       // mixed mode: FP32 copy of the input harmonics
   DevBuf parts;       // forward-contraction partial sums over m' splits
+  DevBuf xtile;       // input harmonics in the inverse contraction's operand layout
   std::vector<size_t> gamma_off;  // per group: offsets of its 4 mixing matrices in d_gamma
   // staging for the host-pointer API
   DevBuf h_flm, h_wav, h_scal, h_map, h_tmp;
@@ -92,7 +93,7 @@ struct ShtPlan {
   int L, spin, mode, ss = 0;
   ColDev col;
   std::vector<double> bvals;
-  DevBuf d_col, d_b64, d_b32, spec, work, flm32, d_super, parts;
+  DevBuf d_col, d_b64, d_b32, spec, work, flm32, d_super, parts, xtile;
   DevBuf etab, d_efs;  // column weight tables (Ei, Ef) of the single column
   const void *ei = nullptr;
 };

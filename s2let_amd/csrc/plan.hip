@@ -542,6 +542,7 @@ void Context::finish_plan(Plan &pl, hipStream_t st) {
   pl.work.alloc(pl.spec_elems * esz);
   if (f32) pl.flm32.alloc((size_t)L * L * sizeof(cx<float>));
   pl.parts.alloc((size_t)kMaxSplits * L * L * sizeof(cx<double>));
+  pl.xtile.alloc(xtile_elems(L) * esz);
   S2H_CHECK_HIP(hipStreamSynchronize(st));
 }
 
@@ -591,6 +592,7 @@ ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st, int ss) {
   s.work.alloc(ne * esz);
   if (f32) s.flm32.alloc((size_t)L * L * sizeof(cx<float>));
   s.parts.alloc((size_t)kMaxSplits * L * L * sizeof(cx<double>));
+  s.xtile.alloc(xtile_elems(L) * esz);
   S2H_CHECK_HIP(hipStreamSynchronize(st));
   auto &ref = *S;
   shts_[key] = std::move(S);
