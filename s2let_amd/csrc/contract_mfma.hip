@@ -51,12 +51,19 @@ template <typename T>
 struct V4 {
   T v[4];
 };
+// Every operand these loads read lives in global memory.  The address-space casts
+// keep loads through pointers that were themselves read from memory (per-group
+// tables) off the flat path, whose completion every LDS wait would also cover.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const f32x4 gf32x4;
+typedef __attribute__((address_space(1))) const f64x2 gf64x2;
 __device__ inline V4<float> ld4(const float *p) {
-  const float4 a = *reinterpret_cast<const float4 *>(p);
+  const f32x4 a = *(gf32x4 *)p;
   return V4<float>{{a.x, a.y, a.z, a.w}};
 }
 __device__ inline V4<double> ld4(const double *p) {
-  const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+  const f64x2 a = ((gf64x2 *)p)[0], b = ((gf64x2 *)p)[1];
   return V4<double>{{a.x, a.y, b.x, b.y}};
 }
 
@@ -276,7 +283,7 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt, int 
 template <typename T>
 struct FwdTile {
   static constexpr int MC = sizeof(T) == 8 ? 4 : 8;
-  static constexpr int PD = sizeof(T) == 8 ? 1 : 4;  // m' records in flight per l-block
+  static constexpr int PD = sizeof(T) == 8 ? 2 : 4;  // m' records in flight per l-block
 };
 
 template <typename T>
@@ -318,19 +325,23 @@ __global__ __launch_bounds__(256) void k_fwd3(const SuperDev *__restrict__ sgs, 
     // sign (e / 16) % 2, column (e / 32) % 16 = tid / 32 + 8 (u % 2), m' = mp0 + e / 512)
     constexpr int NU = MC * 2;
     const int ml = threadIdx.x & 15, pm = (threadIdx.x >> 4) & 1, mm = mb + ml;
+    // element offsets of this thread's two columns at m' = 0 (invalid slots read column 0,
+    // m = 0, and are zeroed): every load is unconditional, so the wait counts stay exact
     long long coff[2];
+    bool cok[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int cc = (threadIdx.x >> 5) + 8 * h;
-      coff[h] = (cc < nc && mm < Lg) ? gc[cc].out_off + (Lg - 1) + (pm ? -mm : mm) : -1;
+      cok[h] = cc < nc && mm < Lg;
+      coff[h] = cok[h] ? gc[cc].out_off + (Lg - 1) + (pm ? -mm : mm) : gc[0].out_off + (Lg - 1);
     }
     cx<T> jr[NU];
     auto fetch = [&](int mp0) {
 #pragma unroll
       for (int u = 0; u < NU; ++u) {
         const int mpp = mp0 + (u >> 1);
-        const long long o = coff[u & 1];
-        jr[u] = (o >= 0 && mpp < Lg) ? Jbase[o + (long long)mpp * Mg] : mk<T>(0, 0);
+        const cx<T> v = Jbase[coff[u & 1] + (long long)min(mpp, Lg - 1) * Mg];
+        jr[u] = (cok[u & 1] && mpp < Lg) ? v : mk<T>(0, 0);
       }
     };
     fetch(mps);
@@ -343,33 +354,37 @@ __global__ __launch_bounds__(256) void k_fwd3(const SuperDev *__restrict__ sgs, 
         sJ[mc][cc][ml][2 * pm + 1] = jr[u].y;
       }
       __syncthreads();
-      if (mp0 + MC <= mpe) fetch(mp0 + MC);  // in flight during this chunk's products
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         const int lbi = lb[q], l0 = 16 * lbi;
-        if (q >= nblk || l0 >= Lg || lbi < mbi) continue;  // no l < Lg, or all |m| > l
         const int nmp = 16 * (lbi + 1);
         const int n = min(MC, min(l0 + 16, mpe + 1) - mp0);  // m' <= l0 + 15
+        // skip: no l < Lg, all |m| > l, or no m' <= l in this chunk
+        const bool run = q < nblk && l0 < Lg && lbi >= mbi && n > 0;
         const T *dq = dt + dtile_base(lbi) + ((size_t)mbi * nmp + mp0) * 256 + 4 * lane;
         const T *eq = Ef + ((size_t)lbi * Lg + mp0) * 256 + 4 * lane;
         const int par = (l0 + k4 + mp0) & 1;
-        // the Delta / weight records of the next kFwdPD m' are in flight (ring of registers)
+        // the Delta / weight records of the next kFwdPD m' are in flight (ring of registers;
+        // past the chunk's last m' the loads repeat it)
         constexpr int kFwdPD = FwdTile<T>::PD;
         V4<T> dr[kFwdPD], er[kFwdPD];
+        if (run) {
 #pragma unroll
-        for (int k = 0; k < kFwdPD; ++k)
-          if (k < n) {
-            dr[k] = ld4(dq + (size_t)k * 256);
-            er[k] = ld4(eq + (size_t)k * 256);
+          for (int k = 0; k < kFwdPD; ++k) {
+            const size_t o = (size_t)min(k, n - 1) * 256;
+            dr[k] = ld4(dq + o);
+            er[k] = ld4(eq + o);
           }
+        }
+        // the next chunk's J is issued behind the first records, so that waiting for
+        // those does not wait for it (loads complete in order); the last chunk's
+        // fetch is never stored
+        if (q == 0) fetch(mp0 + MC);
+        if (!run) continue;
 #pragma unroll
         for (int mc = 0; mc < MC; ++mc) {
           if (mc >= n) break;
           const V4<T> dv = dr[mc % kFwdPD], ev = er[mc % kFwdPD];
-          if (mc + kFwdPD < n) {
-            dr[mc % kFwdPD] = ld4(dq + (size_t)(mc + kFwdPD) * 256);
-            er[mc % kFwdPD] = ld4(eq + (size_t)(mc + kFwdPD) * 256);
-          }
           acc_t h[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) h[u] = Mfma<T>::zero();
@@ -387,6 +402,11 @@ __global__ __launch_bounds__(256) void k_fwd3(const SuperDev *__restrict__ sgs, 
             acc[q][1][v] += d * (double)h[1][v];
             acc[q][2][v] += dn * (double)h[2][v];
             acc[q][3][v] += dn * (double)h[3][v];
+          }
+          if (mc + kFwdPD < MC) {
+            const size_t o = (size_t)min(mc + kFwdPD, n - 1) * 256;
+            dr[mc % kFwdPD] = ld4(dq + o);
+            er[mc % kFwdPD] = ld4(eq + o);
           }
         }
       }

@@ -106,27 +106,28 @@ __global__ __launch_bounds__(256) void k_mix(MixJobs jobs) {
   // two sample points per thread and iteration (twice the loads in flight); the
   // f_wav side (FP64) is streamed with non-temporal loads / stores
   const size_t step = nbj * blockDim.x;
+  // every load of an iteration is unconditional (clamped point and column, the
+  // values of columns j >= nin scaled by zero: a select would let the compiler
+  // branch around the load), so that all are in flight before the first wait
   for (size_t p0 = b0 * blockDim.x + threadIdx.x; p0 < J.npts; p0 += 2 * step) {
-    const size_t pts[2] = {p0, p0 + step};
-    const bool ok1 = pts[1] < J.npts;
+    const size_t pts[2] = {p0, min(p0 + step, J.npts - 1)};
+    const bool ok1 = p0 + step < J.npts;
     cx<T> v[2][NMAX];
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
       for (int j = 0; j < NMAX; ++j) {
-        v[h][j] = mk<T>(0, 0);
-        if (j < nin && (h == 0 || ok1)) {
-          const cx<TI> *src = in + j * J.in_stride + pts[h];
-          TI ux, uy;
-          if constexpr (sizeof(TI) == 8) {
-            ux = __builtin_nontemporal_load(&src->x);
-            uy = __builtin_nontemporal_load(&src->y);
-          } else {
-            ux = src->x;
-            uy = src->y;
-          }
-          v[h][j] = mk<T>((T)ux, (T)uy);
+        const cx<TI> *src = in + min(j, nin - 1) * J.in_stride + pts[h];
+        TI ux, uy;
+        if constexpr (sizeof(TI) == 8) {
+          ux = __builtin_nontemporal_load(&src->x);
+          uy = __builtin_nontemporal_load(&src->y);
+        } else {
+          ux = src->x;
+          uy = src->y;
         }
+        const T z = (j < nin) ? (T)1 : (T)0;
+        v[h][j] = mk<T>((T)ux * z, (T)uy * z);
       }
     for (int k = 0; k < nout; ++k) {
       T ax[2] = {0, 0}, ay[2] = {0, 0};
