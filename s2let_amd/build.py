@@ -31,6 +31,7 @@ SOURCES = [
     ("kernels.hip", [HIPCC] + HIP_FLAGS),
     ("contract_mfma.hip", [HIPCC] + HIP_FLAGS),
     ("theta_mfma.hip", [HIPCC] + HIP_FLAGS),
+    ("theta_bf16.hip", [HIPCC] + HIP_FLAGS),
     ("rows_mfma.hip", [HIPCC] + HIP_FLAGS),
     ("plan.hip", [HIPCC] + HIP_FLAGS),
     ("pipeline.hip", [HIPCC] + HIP_FLAGS),

@@ -199,7 +199,7 @@ const char *s2let_hip_version(void) { return "s2let_amd 0.1 (gfx950)"; }
 
 int s2let_hip_selftest(void) {
   int bad = 0;
-  int rc = guarded([&] { bad = s2h::mfma_selftest(); });
+  int rc = guarded([&] { bad = s2h::mfma_selftest() | s2h::mfma_bf_selftest(); });
   return rc ? rc : bad;
 }
 

@@ -42,6 +42,7 @@ struct ThetaArgs {
   size_t plane;
   const ColDev *cols;
   const T *AT[2];
+  const unsigned short *AB[2];  // bf16 three-part tables (FP32 mode, theta_bf16.hip) or null
   int z_e[2], in_row[2], in_e[2], out_row[2], out_e[2];
 };
 
@@ -181,6 +182,11 @@ template <typename T>
 int theta_blocks(const ThetaArgs<T> &a, int rows);
 template <typename T>
 void launch_theta(const ThetaJobs<T> &jobs, hipStream_t st);
+size_t bf3_matrix_elems(int L, int ss);
+void build_theta_bf3(const double *th64, int L, int ss, unsigned short *out, hipStream_t st);
+bool theta_bf_eligible(int Lmax);
+void launch_theta_bf(const ThetaJobs<float> &jobs, hipStream_t st);
+int mfma_bf_selftest();
 template <typename TI, typename TO, typename T>
 void launch_rows(const RowJobs<T> &jobs, hipStream_t st);
 constexpr int kRowsItems = 8;  // first-product items a wave keeps in registers

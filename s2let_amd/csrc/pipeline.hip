@@ -136,6 +136,7 @@ ThetaArgs<T> theta_args(int L, const ColDev *cols, int ncol, int kind, int ss) {
   const T *th = Sel<T>::theta(lt);
   for (int c = 0; c < 2; ++c) {
     a.AT[c] = th + (size_t)(kind * 2 + c) * L * (L + ss);
+    a.AB[c] = lt.thbf.p ? lt.thbf.as<unsigned short>() + (size_t)(kind * 2 + c) * bf3_matrix_elems(L, ss) : nullptr;
     a.z_e[c] = 0;
     a.in_row[c] = a.out_row[c] = -1;
     a.in_e[c] = a.out_e[c] = 0;

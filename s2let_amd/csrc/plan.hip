@@ -230,6 +230,10 @@ const LTables &Context::ltab(int L, int ss) {
   if (t->ss) build_theta_tables_ss(L, dcs.as<double>(), dsn.as<double>(), t->th64.as<double>(), st);
   else build_theta_tables(L, dcs.as<double>(), dsn.as<double>(), t->th64.as<double>(), st);
   launch_to_f32(t->th64.as<double>(), t->th32.as<float>(), n, st);
+  if (theta_bf_eligible(L)) {
+    t->thbf.alloc(8 * bf3_matrix_elems(L, t->ss) * sizeof(unsigned short));
+    build_theta_bf3(t->th64.as<double>(), L, t->ss, t->thbf.as<unsigned short>(), st);
+  }
   S2H_CHECK_HIP(hipGetLastError());
   S2H_CHECK_HIP(hipStreamSynchronize(st));
   t->dM = &dft(M + t->ss);

@@ -375,7 +375,23 @@ template <typename T>
 void launch_theta(const ThetaJobs<T> &jobs, hipStream_t st) {
   if (jobs.n == 0 || jobs.nblk == 0) return;
   int Lmax = 0;
-  for (int k = 0; k < jobs.n; ++k) Lmax = std::max(Lmax, jobs.j[k].a.L);
+  bool bf = true;
+  for (int k = 0; k < jobs.n; ++k) {
+    Lmax = std::max(Lmax, jobs.j[k].a.L);
+    bf = bf && jobs.j[k].a.AB[0] && jobs.j[k].a.AB[1];
+  }
+  if constexpr (sizeof(T) == 4) {
+    // FP32 mode: the bf16 three-part products when every group has its tables
+    // (S2LET_THETA_BF=0 selects the FP32 products instead)
+    static const bool bf_on = [] {
+      const char *e = getenv("S2LET_THETA_BF");
+      return !(e && e[0] == '0');
+    }();
+    if (bf && bf_on && theta_bf_eligible(Lmax) && theta_rows<T>(Lmax) == 128) {
+      launch_theta_bf(jobs, st);
+      return;
+    }
+  }
   if (theta_rows<T>(Lmax) == 128)
     hipLaunchKernelGGL((k_theta<T, false, 4>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
   else
