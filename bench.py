@@ -25,6 +25,12 @@ CFG = dict(L=512, N=8, B=2.0, J_min=2, spin=0, upsample=0)
 HBM_PEAK = 8.0e12        # B/s
 FP64_PEAK = 78.6e12      # FLOP/s (dense)
 FP32_PEAK = 157.3e12
+BF16_PEAK = 2.5e15        # dense bf16 MFMA
+THETA_BF_MAX_L = 512      # the FP32-mode theta stage runs on the bf16 matrix cores up to here
+# roofs by bound: FP32 MFMA; the bf16 three-part products (six bf16 products per
+# FP32-equivalent product); HBM
+PEAKS = {"mfma": (FP32_PEAK / 1e12, "TFLOP/s"), "mfma_bf16x3": (BF16_PEAK / 6 / 1e12, "TFLOP/s"),
+         "hbm": (HBM_PEAK / 1e9, "GB/s")}
 
 
 def powerlaw_flm(L, seed):
@@ -60,8 +66,23 @@ def algorithmic_work(s2, p):
 STAGE_KERNEL = {
     "contract_inv": "k_inv3", "contract_fwd": "k_fwd3", "contract_fwd_sum": "k_sum_parts",
     "beta_inv": "k_theta", "beta_fwd": "k_theta", "beta_inv_adj": "k_theta", "beta_fwd_adj": "k_theta",
-    "rows": "k_rows", "gamma": "k_mix", "cast": "k_cast_cx",
+    "rows": "k_rows4", "gamma": "k_mix", "cast": "k_cast_cx",
 }
+
+
+def stage_kernel(name, mixed, L):
+    k = STAGE_KERNEL.get(name, name)
+    if k == "k_theta" and mixed and L <= THETA_BF_MAX_L:
+        return "k_theta_bf"
+    return k
+
+
+def dft_factors(M):
+    """M = M1 M2 with M1 the largest divisor <= sqrt(M) (the row stage's two-level DFT)."""
+    m1 = int(M ** 0.5)
+    while M % m1:
+        m1 -= 1
+    return m1, M // m1
 
 
 def column_groups(s2, p):
@@ -81,8 +102,10 @@ def kernel_work(s2, p, F, e):
     complex intermediate (8 mixed, 16 FP64); f_wav, f_scal and flm are complex128.
       k_inv3 / k_fwd3: the analysis / synthesis ell-sums, F/2 each (the path's
         algorithmic flop count);
-      k_theta: the theta stage as real matrix products, 2 (directions) x 4 L^2 M ncol;
-      k_rows:  each phi/alpha line read and written once;
+      k_theta: the theta stage as real matrix products, 2 (directions) x 4 L^2 M ncol
+        (k_theta_bf: the same products on the bf16 matrix cores, roof = bf16 peak / 6);
+      k_rows4: the two-level DFT of each phi/alpha line, M = M1 M2:
+        8 M1^2 M2 + 8 M2^2 M1 real flop (one product of 8 M^2 when M is prime);
       k_mix:   the column spectra and the f_wav slices read or written once;
       k_sum_parts: the m' split partials read once and flm written once;
       k_cast_cx: flm read (FP64) and written (FP32) once."""
@@ -93,14 +116,15 @@ def kernel_work(s2, p, F, e):
     mix = 0
     for Lg, nc, ng in g:
         M = 2 * Lg - 1
-        if ng == 0:
-            rows += 2 * Lg * M * (e + 16)
-        else:
-            rows += 2 * nc * Lg * M * (2 * e)
+        m1, m2 = dft_factors(M)
+        per_line = 8 * M * M if m1 == 1 else 8 * m1 * m1 * m2 + 8 * m2 * m2 * m1
+        rows += 2 * (1 if ng == 0 else nc) * Lg * per_line
+        if ng:
             mix += 2 * Lg * M * (nc * e + ng * 16)
     S = max(1, min(8, L // 64))
     work = {"k_inv3": ("mfma", F / 2), "k_fwd3": ("mfma", F / 2), "k_theta": ("mfma", theta),
-            "k_rows": ("hbm", rows), "k_mix": ("hbm", mix), "k_sum_parts": ("hbm", (S + 1) * L * L * 16)}
+            "k_theta_bf": ("mfma_bf16x3", theta), "k_rows4": ("mfma", rows), "k_mix": ("hbm", mix),
+            "k_sum_parts": ("hbm", (S + 1) * L * L * 16)}
     if e == 8:
         work["k_cast_cx"] = ("hbm", L * L * 24)
     return work
@@ -232,21 +256,20 @@ def main():
     # library's stream (each stage is one kernel function)
     per_k = {}
     for name, (cnt, ms) in st32.items():
-        kf = STAGE_KERNEL.get(name, name)
+        kf = stage_kernel(name, True, p.L)
         c0, m0 = per_k.get(kf, (0, 0.0))
         per_k[kf] = (c0 + cnt, m0 + ms)
     dk, (dcount, dms) = max(per_k.items(), key=lambda kv: kv[1][1])
     work = kernel_work(s2, p, F, 8)
     bound, w_step = work.get(dk, ("hbm", None))
     t_step = dms / K / 1e3  # s
-    if bound == "mfma":
-        peak, unit = FP32_PEAK / 1e12, "TFLOP/s"
-        achieved = None if w_step is None else w_step / t_step / 1e12
-    else:
-        peak, unit = HBM_PEAK / 1e9, "GB/s"
-        achieved = None if w_step is None else w_step / t_step / 1e9
+    peak, unit = PEAKS[bound]
+    achieved = None if w_step is None else w_step / t_step / (1e9 if bound == "hbm" else 1e12)
     traffic = pmc_traffic(os.path.join(ROOT, "profiles", "pmc_traffic_L512_N8.json")).get(dk)
-    roof = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+    roof = {"bound": "hbm" if bound == "hbm" else "mfma", "achieved": achieved, "peak": peak, "unit": unit,
+            "peak_basis": {"mfma": "FP32 dense MFMA 157.3 TFLOP/s",
+                           "mfma_bf16x3": "bf16 dense MFMA 2.5 PFLOP/s / 6 (three-part bf16 products per "
+                                          "FP32-equivalent product)", "hbm": "HBM3E 8 TB/s"}[bound],
             "frac": None if achieved is None else achieved / peak, "traffic": traffic,
             "kernel": dk, "launches_per_step": dcount / K, "avg_launch_ms": dms / dcount,
             "work_per_launch": None if w_step is None else w_step * K / dcount}
@@ -254,10 +277,9 @@ def main():
     for kf, (cnt, ms) in per_k.items():
         b, w = work.get(kf, ("hbm", None))
         t = ms / K / 1e3
-        rate = None if w is None else (w / t / 1e12 if b == "mfma" else w / t / 1e9)
+        rate = None if w is None else w / t / (1e9 if b == "hbm" else 1e12)
         kernels[kf] = {"ms_per_step": ms / K, "launches_per_step": cnt / K, "bound": b,
-                       "achieved": rate, "frac": None if rate is None else
-                       rate / (FP32_PEAK / 1e12 if b == "mfma" else HBM_PEAK / 1e9)}
+                       "achieved": rate, "frac": None if rate is None else rate / PEAKS[b][0]}
     out = {
         "metric": "analysis+synthesis maps/s at L=512,N=8: mixed-precision vs FP64, err<=1e-6",
         "value": maps_s,
@@ -269,7 +291,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-mfma+f64",
+        "dtype": "f32 (bf16x3 and f32 MFMA) + f64",
         "data": "synthetic: complex128 f_lm, C_l=(l+1)^-2, numpy default_rng(2)",
         "config": {"workload": "directional wavelet analysis+synthesis, one map per step", "L": cfg["L"],
                    "N": cfg["N"], "B": cfg["B"], "J_min": cfg["J_min"], "spin": cfg["spin"],

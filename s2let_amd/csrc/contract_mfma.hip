@@ -381,6 +381,15 @@ __global__ __launch_bounds__(256) void k_fwd3(const SuperDev *__restrict__ sgs, 
         // fetch is never stored
         if (q == 0) fetch(mp0 + MC);
         if (!run) continue;
+        // FP32: the chunk's Delta-weighted sums in FP32, added to the FP64 sums once per
+        // chunk (the chunks are the same MC-aligned m' ranges for every split of the m'
+        // range); FP64: straight into the FP64 sums
+        constexpr bool part = sizeof(T) == 4;
+        T pacc[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) pacc[u][v] = (T)0;
 #pragma unroll
         for (int mc = 0; mc < MC; ++mc) {
           if (mc >= n) break;
@@ -397,17 +406,30 @@ __global__ __launch_bounds__(256) void k_fwd3(const SuperDev *__restrict__ sgs, 
           const bool odd = (par + mc) & 1;
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const double d = (double)dv.v[v], dn = odd ? -d : d;
-            acc[q][0][v] += d * (double)h[0][v];
-            acc[q][1][v] += d * (double)h[1][v];
-            acc[q][2][v] += dn * (double)h[2][v];
-            acc[q][3][v] += dn * (double)h[3][v];
+            const T d = dv.v[v], dn = odd ? -d : d;
+            if constexpr (part) {
+              pacc[0][v] += d * h[0][v];
+              pacc[1][v] += d * h[1][v];
+              pacc[2][v] += dn * h[2][v];
+              pacc[3][v] += dn * h[3][v];
+            } else {
+              acc[q][0][v] += d * h[0][v];
+              acc[q][1][v] += d * h[1][v];
+              acc[q][2][v] += dn * h[2][v];
+              acc[q][3][v] += dn * h[3][v];
+            }
           }
           if (mc + kFwdPD < MC) {
             const size_t o = (size_t)min(mc + kFwdPD, n - 1) * 256;
             dr[mc % kFwdPD] = ld4(dq + o);
             er[mc % kFwdPD] = ld4(eq + o);
           }
+        }
+        if constexpr (part) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[q][u][v] += (double)pacc[u][v];
         }
       }
     }
