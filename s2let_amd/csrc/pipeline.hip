@@ -317,7 +317,7 @@ void rows_scal(Plan &P, const void *in, void *out, int in_c, int out_c, int sign
   J.j[0].out = out;
   J.j[0].blk0 = 0;
   J.nblk = (int)((J.j[0].a.nlines + J.j[0].a.lpb - 1) / J.j[0].a.lpb);
-  Stage s_("rows", st);
+  Stage s_("rows_scal", st);
   launch_rows<TI, TO, T>(J, st);
 }
 
