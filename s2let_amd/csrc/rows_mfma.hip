@@ -35,11 +35,13 @@ __device__ inline void divmod_small(int n, int d, float inv, int &q, int &r) {
 
 // one (row tile, up to two column tiles) item of a product: acc[u] += F_mt * S[tile u].
 // In FP32, up to kRowsKR K-steps the item's A fragments are all loaded first (one
-// latency per item); FP64 and longer reductions stream them four at a time.
+// latency per item) into av, which the caller keeps from item to item of the same
+// row tile (load = false); FP64 and longer reductions stream them four at a time.
 constexpr int kRowsKR = 20;
 template <typename T>
 __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const T *S, int cs, int t0, bool two,
-                                int lane, typename Mfma<T>::acc &a0, typename Mfma<T>::acc &a1) {
+                                int lane, T (&av)[kRowsKR], bool load, typename Mfma<T>::acc &a0,
+                                typename Mfma<T>::acc &a1) {
   const int r16 = lane & 15, k4 = lane >> 4;
   const T *ga = F + (size_t)mt * nks * 64 + lane;
   const T *b0 = S + (16 * t0 + r16) * cs + k4;
@@ -47,9 +49,10 @@ __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const 
   a0 = Mfma<T>::zero();
   a1 = Mfma<T>::zero();
   if (sizeof(T) == 4 && nks <= kRowsKR) {
-    T av[kRowsKR];
+    if (load) {
 #pragma unroll
-    for (int s = 0; s < kRowsKR; ++s) av[s] = (s < nks) ? ga[s * 64] : (T)0;
+      for (int s = 0; s < kRowsKR; ++s) av[s] = (s < nks) ? ga[s * 64] : (T)0;
+    }
 #pragma unroll
     for (int s = 0; s < kRowsKR; ++s) {
       if (s >= nks) break;
@@ -128,13 +131,16 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
     const float inmt = 1.0f / (float)nmt;
     const int ncol = nl * M2, nt = (ncol + 15) / 16, ntp = (nt + 1) / 2, nitems = nmt * ntp;
     acc_t r0[IMAX], r1[IMAX];
+    T av[kRowsKR];
+    int cur = -1;
 #pragma unroll
     for (int it = 0; it < IMAX; ++it) {
       const int item = wave + 4 * it;
       if (item >= nitems) break;
       int tp, mt;
       divmod_small(item, nmt, inmt, tp, mt);
-      dft_item<T>(F, nks, mt, S, d.CSA, 2 * tp, 2 * tp + 1 < nt, lane, r0[it], r1[it]);
+      dft_item<T>(F, nks, mt, S, d.CSA, 2 * tp, 2 * tp + 1 < nt, lane, av, mt != cur, r0[it], r1[it]);
+      cur = mt;
     }
     __syncthreads();  // every column of the first product has been read
     // twiddle and write the second product's columns (line, k1), rows 2 n2 + re/im
@@ -173,14 +179,22 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
   {  // second product and output
     const T *__restrict__ F = fwd ? d.F2f : d.F2b;
     const int nks = d.K2 / 4, nmt = d.R2 / 16;
-    const float inmt = 1.0f / (float)nmt;
     const int ncol = nl * M1, nt = (ncol + 15) / 16, ntp = (nt + 1) / 2, nitems = nmt * ntp;
     const T scale = a.scale;
-    for (int item = wave; item < nitems; item += 4) {
-      int tp, mt;
-      divmod_small(item, nmt, inmt, tp, mt);
+    // FP32: contiguous items per wave, row tile major, so that the A fragments kept in
+    // registers change once or twice; FP64 (streamed fragments): items round-robin
+    constexpr bool contig = sizeof(T) == 4;
+    const int per = (nitems + 3) / 4, i0 = contig ? wave * per : wave;
+    const int i1 = contig ? min(nitems, i0 + per) : nitems, istep = contig ? 1 : 4;
+    T av[kRowsKR];
+    int cur = -1;
+    for (int item = i0; item < i1; item += istep) {
+      int mt, tp;
+      if (contig) divmod_small(item, ntp, 1.0f / (float)ntp, mt, tp);
+      else divmod_small(item, nmt, 1.0f / (float)nmt, tp, mt);
       acc_t r0, r1;
-      dft_item<T>(F, nks, mt, S, d.CSC, 2 * tp, 2 * tp + 1 < nt, lane, r0, r1);
+      dft_item<T>(F, nks, mt, S, d.CSC, 2 * tp, 2 * tp + 1 < nt, lane, av, mt != cur, r0, r1);
+      cur = mt;
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int col = 16 * (2 * tp + u) + r16;
