@@ -93,7 +93,11 @@ __device__ inline void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
 
 __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs) {
   constexpr int RT = kBfRT, SJ = kBfSJ;
-  __shared__ __attribute__((aligned(16))) __bf16 sB[2][2][3][2][32][SJ];  // [buf][class][part][re/im][j][k]
+  // [buf][class][part][re/im][j][k]; class 1 is offset by 32 entries (64 bytes) so that
+  // the two classes of neighbouring lanes' 16-byte writes fall in different banks
+  constexpr int kCls = 3 * 2 * 32 * SJ + 32, kBuf = 2 * kCls;
+  __shared__ __attribute__((aligned(16))) __bf16 sB[2 * kBuf];
+  auto sbo = [&](int buf, int cl, int p, int ri, int j) { return buf * kBuf + cl * kCls + ((p * 2 + ri) * 32 + j) * SJ; };
   const int nb = jobs.nblk;
   const int lin = blockIdx.x, xx = lin & 7, kq8 = lin >> 3, q8 = nb >> 3, r8 = nb & 7;
   int w = xx * q8 + min(xx, r8) + kq8;
@@ -107,7 +111,8 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs) {
   const int nti = (R + 32 * RT - 1) / (32 * RT), ntm = (M + 63) / 64;
   const int ti = w % nti, rest = w / nti, tm = rest % ntm, c = rest / ntm;
   const int i0 = ti * 32 * RT, mi0 = tm * 64;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // wave-uniform indexes in scalar registers (scalar address arithmetic)
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int cls_lo = (mi0 - (L - 1)) & 1;
   const ColDev col = a.cols[c];
   const int par0 = (cls_lo + col.n) & 1;
@@ -131,20 +136,15 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs) {
     for (int qq = 0; qq < 4; ++qq) acc[r][qq] = f4{0.f, 0.f, 0.f, 0.f};
 
   // staging: thread (mi = mi0 + tid % 64, kq = tid / 64) moves k = k0 + 8 kq + e, e < 8;
-  // unconditional loads (clamped indexes), values outside the matrix scaled by zero
-  const int ml = tid & 63, kq = tid >> 6, mi = mi0 + ml;
+  // unconditional loads from clamped indexes: past the last sample row the table's
+  // columns are zero, and columns mi >= M are not written
+  const int ml = tid & 63, kq = wave, mi = mi0 + ml;
   const int scl = (ml & 1) ^ par0, sj = ml >> 1;
-  const float mok = mi < M ? 1.f : 0.f;
   const cx<float> *src = inc + min(mi, M - 1);
   cx<float> breg[8];
   auto load_b = [&](int k0) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int k = k0 + 8 * kq + e;
-      const cx<float> v = src[(size_t)min(k, K - 1) * M];
-      const float z = (k < K) ? mok : 0.f;
-      breg[e] = mk<float>(v.x * z, v.y * z);
-    }
+    for (int e = 0; e < 8; ++e) breg[e] = src[(size_t)min(k0 + 8 * kq + e, K - 1) * M];
   };
   bf16x8 ar[RT][3];
   auto load_a = [&](int r, int kc) {
@@ -175,8 +175,8 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs) {
       }
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
-        *reinterpret_cast<bf16x8 *>(&sB[buf][scl][p][0][sj][8 * kq]) = pr[p];
-        *reinterpret_cast<bf16x8 *>(&sB[buf][scl][p][1][sj][8 * kq]) = pi[p];
+        *reinterpret_cast<bf16x8 *>(&sB[sbo(buf, scl, p, 0, sj) + 8 * kq]) = pr[p];
+        *reinterpret_cast<bf16x8 *>(&sB[sbo(buf, scl, p, 1, sj) + 8 * kq]) = pi[p];
       }
     }
     __syncthreads();
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs) {
     for (int qq = 0; qq < 4; ++qq)
 #pragma unroll
       for (int p = 0; p < 3; ++p)
-        bq[qq][p] = *reinterpret_cast<const bf16x8 *>(&sB[buf][cls][p][qq >> 1][16 * (qq & 1) + r16][8 * kg]);
+        bq[qq][p] = *reinterpret_cast<const bf16x8 *>(&sB[sbo(buf, cls, p, qq >> 1, 16 * (qq & 1) + r16) + 8 * kg]);
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
 #pragma unroll
