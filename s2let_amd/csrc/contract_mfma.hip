@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt, int 
   const ColDev *__restrict__ cols = J.cols;
   const int nc = J.nc, lmin = J.lmin, L = J.L;
   const int b = blockIdx.x - J.blk0, bx = b % J.gx, by = b / J.gx;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int r16 = lane & 15, k4 = lane >> 4;
   const int mp0 = by * MP;
   const int mbi = bx * 4 + wave, mb = 16 * mbi;

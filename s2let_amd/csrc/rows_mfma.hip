@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
   T *S = reinterpret_cast<T *>(smem_raw);
   const size_t line0 = (size_t)(blockIdx.x - jobs.j[jb].blk0) * a.lpb;
   const int nl = (int)min((size_t)a.lpb, a.nlines - line0);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   const int r16 = lane & 15, k4 = lane >> 4;
   const bool fwd = a.sign < 0;
   const bool two_level = M1 > 1;

@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
   const int nti = (R + 32 * RT - 1) / (32 * RT), ntm = (M + 63) / 64;
   const int ti = w % nti, rest = w / nti, tm = rest % ntm, c = rest / ntm;
   const int i0 = ti * 32 * RT, mi0 = tm * 64;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   const int cls_lo = (mi0 - (L - 1)) & 1;  // parity of m at mi0 (class of ee = 0 is cls_lo ^ (n & 1))
   const ColDev col = a.cols[c];
   const int par0 = (cls_lo + col.n) & 1;
