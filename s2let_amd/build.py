@@ -20,9 +20,11 @@ LIB = os.path.join(HERE, "libs2let_amd.so")
 ARCH = os.environ.get("S2LET_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
+# matrix-core accumulators in ordinary vector registers (no copies out of the
+# accumulation registers before the vector units read a product)
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-Wno-unused-result"]
+             "-mllvm", "-amdgpu-mfma-vgpr-form", "-Wno-unused-result"]
 
 SOURCES = [
     ("tiling.cpp", ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]),
