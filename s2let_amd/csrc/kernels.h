@@ -50,10 +50,12 @@ struct ThetaArgs {
 // embeddings of the M1- and M2-point DFT matrices, R x K (R, K multiples of 16),
 // for exponent sign -1 (f) and +1 (b), stored as MFMA A fragments:
 // F[((mt K/4) + s) 64 + lane] = G[16 mt + lane % 16][4 s + lane / 16].
-// The lines of a block are staged in LDS as columns of the two products, with
-// the reduction index fastest: column stride CSA = K1 + 5 (first product) and
-// CSC = K2 + 5 (second).  perm[q] is the LDS offset, within a line, of the real
-// part of input sample q.  tw[k] = e^{-2 pi i k / M}.
+// The lines of a block are staged in LDS as columns of the two products.  FP64:
+// the reduction index fastest, column stride CSA = K1 + 5 (first product) and
+// CSC = K2 + 5 (second); FP32: K-step-major columns, reduction row k at
+// (k & 3) K/4 + (k >> 2) (four consecutive K steps of a lane in one 16-byte read),
+// strides K1 + 4 and K2 + 4.  perm[q] is the LDS offset, within a line, of the
+// real part of input sample q.  tw[k] = e^{-2 pi i k / M}.
 template <typename T>
 struct Dft2 {
   int M, M1, M2;

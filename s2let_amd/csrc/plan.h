@@ -29,7 +29,7 @@ struct DevBuf {
 struct DftTables {  // two-level DFT of one length M (see Dft2)
   Dft2<double> d64;
   Dft2<float> d32;
-  DevBuf g64, g32, tw64, tw32, perm;
+  DevBuf g64, g32, tw64, tw32, perm, perm32;
 };
 
 struct LTables {  // per band-limit L and sampling (MW / MWSS)
@@ -78,7 +78,6 @@ struct Plan {
   DevBuf d_cols_ana, d_cols_syn, d_b64, d_b32, d_gamma;
   DevBuf d_psi, d_phi;
   DevBuf spec, work;  // scratch (T = double or float per mode)
-  DevBuf flm32;       // mixed mode: FP32 copy of the input harmonics
   DevBuf parts;       // forward-contraction partial sums over m' splits
   DevBuf xtile;       // input harmonics in the inverse contraction's operand layout
   std::vector<size_t> gamma_off;  // per group: offsets of its 4 mixing matrices in d_gamma
@@ -94,7 +93,7 @@ struct ShtPlan {
   int L, spin, mode, ss = 0;
   ColDev col;
   std::vector<double> bvals;
-  DevBuf d_col, d_b64, d_b32, spec, work, flm32, d_super, parts, xtile;
+  DevBuf d_col, d_b64, d_b32, spec, work, d_super, parts, xtile;
   DevBuf etab, d_efs;  // column weight tables (Ei, Ef) of the single column
   const void *ei = nullptr;
 };

@@ -140,7 +140,10 @@ const DftTables &Context::dft(int n) {
   }
   auto t = std::make_unique<DftTables>();
   const int R1 = round_up(2 * m1, 16), K1 = round_up(2 * m1, 16), R2 = round_up(2 * m2, 16), K2 = round_up(2 * m2, 16);
-  const int CSA = K1 + 5, CSC = K2 + 5;  // odd strides: consecutive columns in consecutive LDS banks
+  // FP64: columns with the reduction index fastest, odd strides (consecutive columns in
+  // consecutive LDS banks); FP32: K-step-major columns (entry k at (k & 3) K/4 + (k >> 2):
+  // a lane's four consecutive K steps in one 16-byte read), strides K + 4
+  const int CSA = K1 + 5, CSC = K2 + 5, CSA32 = K1 + 4, CSC32 = K2 + 4;
   const size_t s1 = (size_t)R1 * K1, s2 = (size_t)R2 * K2;
   std::vector<double> g64, part64;
   std::vector<float> g32, part32;
@@ -170,16 +173,19 @@ const DftTables &Context::dft(int n) {
   }
   // LDS offset of sample q = M2 n1 + n2 within its line: column n2, reduction row 2 n1
   // (first product), or column 0, reduction row 2 q when M1 = 1 (second product only)
-  std::vector<int> perm(n);
+  std::vector<int> perm(n), perm32(n);
   for (int q = 0; q < n; ++q) perm[q] = (m1 > 1) ? (q % m2) * CSA + 2 * (q / m2) : 2 * q;
+  auto kpos = [](int k, int K) { return (k & 3) * (K / 4) + (k >> 2); };
+  for (int q = 0; q < n; ++q) perm32[q] = (m1 > 1) ? (q % m2) * CSA32 + kpos(2 * (q / m2), K1) : kpos(2 * q, K2);
   hipStream_t st = stream();
   upload(t->g64, g64, st);
   upload(t->g32, g32, st);
   upload(t->tw64, w, st);
   upload(t->tw32, wf, st);
   upload(t->perm, perm, st);
+  upload(t->perm32, perm32, st);
   S2H_CHECK_HIP(hipStreamSynchronize(st));
-  auto fill = [&](auto &d, auto *g, auto *tw) {
+  auto fill = [&](auto &d, auto *g, auto *tw, int csa, int csc, const int *pm) {
     d.M = n;
     d.M1 = m1;
     d.M2 = m2;
@@ -187,17 +193,17 @@ const DftTables &Context::dft(int n) {
     d.K1 = K1;
     d.R2 = R2;
     d.K2 = K2;
-    d.CSA = CSA;
-    d.CSC = CSC;
+    d.CSA = csa;
+    d.CSC = csc;
     d.F1f = g;
     d.F1b = g + s1;
     d.F2f = g + 2 * s1;
     d.F2b = g + 2 * s1 + s2;
     d.tw = tw;
-    d.perm = t->perm.as<int>();
+    d.perm = pm;
   };
-  fill(t->d64, t->g64.as<double>(), t->tw64.as<cx<double>>());
-  fill(t->d32, t->g32.as<float>(), t->tw32.as<cx<float>>());
+  fill(t->d64, t->g64.as<double>(), t->tw64.as<cx<double>>(), CSA, CSC, t->perm.as<int>());
+  fill(t->d32, t->g32.as<float>(), t->tw32.as<cx<float>>(), CSA32, CSC32, t->perm32.as<int>());
   auto &ref = *t;
   dft_[n] = std::move(t);
   return ref;
@@ -544,7 +550,6 @@ void Context::finish_plan(Plan &pl, hipStream_t st) {
   const size_t esz = f32 ? sizeof(cx<float>) : sizeof(cx<double>);
   pl.spec.alloc(pl.spec_elems * esz);
   pl.work.alloc(pl.spec_elems * esz);
-  if (f32) pl.flm32.alloc((size_t)L * L * sizeof(cx<float>));
   pl.parts.alloc((size_t)kMaxSplits * L * L * sizeof(cx<double>));
   pl.xtile.alloc(xtile_elems(L) * esz);
   S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -594,7 +599,6 @@ ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st, int ss) {
   const size_t ne = map_elems(L, ss);
   s.spec.alloc(ne * esz);
   s.work.alloc(ne * esz);
-  if (f32) s.flm32.alloc((size_t)L * L * sizeof(cx<float>));
   s.parts.alloc((size_t)kMaxSplits * L * L * sizeof(cx<double>));
   s.xtile.alloc(xtile_elems(L) * esz);
   S2H_CHECK_HIP(hipStreamSynchronize(st));

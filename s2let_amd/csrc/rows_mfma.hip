@@ -44,23 +44,50 @@ __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const 
                                 typename Mfma<T>::acc &a1) {
   const int r16 = lane & 15, k4 = lane >> 4;
   const T *ga = F + (size_t)mt * nks * 64 + lane;
-  const T *b0 = S + (16 * t0 + r16) * cs + k4;
-  const T *b1 = b0 + 16 * cs;
   a0 = Mfma<T>::zero();
   a1 = Mfma<T>::zero();
-  if (sizeof(T) == 4 && nks <= kRowsKR) {
-    if (load) {
+  if constexpr (sizeof(T) == 4) {
+    // K-step-major columns: the lane's K steps s at b[s], four per 16-byte read
+    const float *b0 = S + (16 * t0 + r16) * cs + k4 * nks;
+    const float *b1 = b0 + 16 * cs;
+    if (nks <= kRowsKR) {
+      if (load) {
 #pragma unroll
-      for (int s = 0; s < kRowsKR; ++s) av[s] = (s < nks) ? ga[s * 64] : (T)0;
+        for (int s = 0; s < kRowsKR; ++s) av[s] = (s < nks) ? ga[s * 64] : 0.f;
+      }
+#pragma unroll
+      for (int s = 0; s < kRowsKR; s += 4) {
+        if (s >= nks) break;
+        const float4 x0 = *reinterpret_cast<const float4 *>(b0 + s);
+        a0 = Mfma<T>::mma(av[s], x0.x, a0);
+        a0 = Mfma<T>::mma(av[s + 1], x0.y, a0);
+        a0 = Mfma<T>::mma(av[s + 2], x0.z, a0);
+        a0 = Mfma<T>::mma(av[s + 3], x0.w, a0);
+        if (two) {
+          const float4 x1 = *reinterpret_cast<const float4 *>(b1 + s);
+          a1 = Mfma<T>::mma(av[s], x1.x, a1);
+          a1 = Mfma<T>::mma(av[s + 1], x1.y, a1);
+          a1 = Mfma<T>::mma(av[s + 2], x1.z, a1);
+          a1 = Mfma<T>::mma(av[s + 3], x1.w, a1);
+        }
+      }
+      return;
     }
+    for (int s0 = 0; s0 < nks; s0 += 4) {  // nks is a multiple of 4
+      const float4 x0 = *reinterpret_cast<const float4 *>(b0 + s0);
+      const float4 x1 = two ? *reinterpret_cast<const float4 *>(b1 + s0) : x0;
+      const float xs0[4] = {x0.x, x0.y, x0.z, x0.w}, xs1[4] = {x1.x, x1.y, x1.z, x1.w};
 #pragma unroll
-    for (int s = 0; s < kRowsKR; ++s) {
-      if (s >= nks) break;
-      a0 = Mfma<T>::mma(av[s], b0[4 * s], a0);
-      if (two) a1 = Mfma<T>::mma(av[s], b1[4 * s], a1);
+      for (int q = 0; q < 4; ++q) {
+        const float avq = ga[(s0 + q) * 64];
+        a0 = Mfma<T>::mma(avq, xs0[q], a0);
+        if (two) a1 = Mfma<T>::mma(avq, xs1[q], a1);
+      }
     }
     return;
   }
+  const T *b0 = S + (16 * t0 + r16) * cs + k4;
+  const T *b1 = b0 + 16 * cs;
   for (int s0 = 0; s0 < nks; s0 += 4) {  // nks is a multiple of 4
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -96,6 +123,9 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
   const int r16 = lane & 15, k4 = lane >> 4;
   const bool fwd = a.sign < 0;
   const bool two_level = M1 > 1;
+  // FP32 columns are K-step-major: reduction row k at (k & 3) K/4 + (k >> 2)
+  constexpr bool KM = sizeof(T) == 4;
+  auto kpos = [](int k, int K) { return KM ? (k & 3) * (K >> 2) + (k >> 2) : k; };
 
   // ---- stage the lines: zero the padding, then scatter the samples
   {
@@ -103,7 +133,8 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
     const int P2 = 2 * (two_level ? M1 : M2);  // used reduction rows
     const int ncol = nl * cw, ncolp = (ncol + 15) & ~15;
     for (int c = tid; c < ncolp; c += 256)
-      for (int kk = (c < ncol) ? P2 : 0; kk < K; ++kk) S[c * cs + kk] = (T)0;
+      for (int kk = (c < ncol) ? P2 : 0; kk < K; ++kk) S[c * cs + kpos(kk, K)] = (T)0;
+    const int im = KM ? K / 4 : 1;  // offset of the imaginary part (reduction row 2n + 1)
     for (int k = 0; k < nl; ++k) {
       const cx<TI> *src = in + line_at((int)line0 + k, a.in_len);
       T *dst = S + (size_t)k * cw * cs;
@@ -119,7 +150,7 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
         const cx<TI> v = ok ? src[sidx] : mk<TI>(0, 0);
         const int o = d.perm[q];
         dst[o] = (T)v.x;
-        dst[o + 1] = (T)v.y;
+        dst[o + im] = (T)v.y;
       }
     }
   }
@@ -164,15 +195,15 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
           cx<T> w = d.tw[n2 * k1];  // e^{-2 pi i n2 k1 / M}
           if (!fwd) w.y = -w.y;
           const cx<T> z = cmul(mk<T>(r[2 * p], r[2 * p + 1]), w);
-          T *yo = S + (line * M1 + k1) * d.CSC + 2 * n2;
+          T *yo = S + (line * M1 + k1) * d.CSC + kpos(2 * n2, d.K2);
           yo[0] = z.x;
-          yo[1] = z.y;
+          yo[KM ? d.K2 / 4 : 1] = z.y;
         }
       }
     }
     const int ncolc = nl * M1, ncolcp = (ncolc + 15) & ~15;
     for (int c = tid; c < ncolcp; c += 256)
-      for (int kk = (c < ncolc) ? 2 * M2 : 0; kk < d.K2; ++kk) S[c * d.CSC + kk] = (T)0;
+      for (int kk = (c < ncolc) ? 2 * M2 : 0; kk < d.K2; ++kk) S[c * d.CSC + kpos(kk, d.K2)] = (T)0;
     __syncthreads();
   }
 
