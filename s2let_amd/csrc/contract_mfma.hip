@@ -139,8 +139,9 @@ __global__ void k_etab(const double *__restrict__ dtab, const ColDev *__restrict
 // (zero unless m <= l < Lx), nx = ceil(Lx / 16); Q follows P after nx^2 256 entries.
 size_t xtile_elems(int Lx) { return 2 * (size_t)((Lx + 15) / 16) * ((Lx + 15) / 16) * 256; }
 
+// (x is the caller's complex128 f_lm; the FP32 mode rounds it here)
 template <typename T>
-__global__ void k_xtile(const cx<T> *__restrict__ x, int Lx, cx<T> *__restrict__ xt) {
+__global__ void k_xtile(const cx<double> *__restrict__ x, int Lx, cx<T> *__restrict__ xt) {
   const int nx = (Lx + 15) / 16;
   const size_t half = (size_t)nx * nx * 256;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < half; e += (size_t)gridDim.x * blockDim.x) {
@@ -150,9 +151,10 @@ __global__ void k_xtile(const cx<T> *__restrict__ x, int Lx, cx<T> *__restrict__
     const int l = 16 * lb + 4 * s + (lane >> 4), m = 16 * mbi + (lane & 15);
     cx<T> p = mk<T>(0, 0), q = mk<T>(0, 0);
     if (l < Lx && m <= l) {
-      p = x[(size_t)l * l + l + m];
-      q = x[(size_t)l * l + l - m];
-      if (l & 1) q = mk<T>(-q.x, -q.y);
+      const cx<double> a = x[(size_t)l * l + l + m], b = x[(size_t)l * l + l - m];
+      const T sg = (l & 1) ? (T)-1 : (T)1;
+      p = mk<T>((T)a.x, (T)a.y);
+      q = mk<T>(sg * (T)b.x, sg * (T)b.y);
     }
     xt[e] = p;
     xt[half + e] = q;
@@ -487,7 +489,7 @@ template <typename T>
 void launch_xtile(const void *x, int Lx, void *xtile, hipStream_t st) {
   const size_t half = xtile_elems(Lx) / 2;
   const unsigned g = (unsigned)std::min<size_t>((half + 255) / 256, 65535 * 4);
-  hipLaunchKernelGGL((k_xtile<T>), dim3(g), dim3(256), 0, st, (const cx<T> *)x, Lx, (cx<T> *)xtile);
+  hipLaunchKernelGGL((k_xtile<T>), dim3(g), dim3(256), 0, st, (const cx<double> *)x, Lx, (cx<T> *)xtile);
 }
 template void launch_xtile<double>(const void *, int, void *, hipStream_t);
 template void launch_xtile<float>(const void *, int, void *, hipStream_t);

@@ -192,13 +192,6 @@ RowArgs<T> row_args(int L, size_t nlines, int in_c, int out_c, int sign, double 
   return a;
 }
 
-template <typename T>
-const void *harmonic_input(Plan &P, const void *flm, hipStream_t st) {
-  if (sizeof(T) == 8) return flm;
-  Stage s_("cast", st);
-  launch_cast<double, float>(flm, P.flm32.p, (size_t)P.p.L * P.p.L, st);
-  return P.flm32.p;
-}
 
 template <typename T>
 inline char *at(const DevBuf &b, size_t elems) {
@@ -361,7 +354,7 @@ void mix_groups(Plan &P, int which, bool to_wav, const void *src_wav, void *dst_
 
 template <typename T>
 void analysis_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
-  const void *x = harmonic_input<T>(P, flm, st);
+  const void *x = flm;  // complex128; k_xtile rounds it in the FP32 mode
   const ColDev *cols = P.d_cols_ana.as<ColDev>();
   contract_inv_all<T>(P, x, cols, P.spec.p, st);
   theta_groups<T>(P, cols, 0, P.spec, P.work, "beta_inv", st);
@@ -392,7 +385,7 @@ void analysis_adjoint_t(Plan &P, const void *f_wav, const void *f_scal, void *fl
 
 template <typename T>
 void synthesis_adjoint_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
-  const void *x = harmonic_input<T>(P, flm, st);
+  const void *x = flm;  // complex128; k_xtile rounds it in the FP32 mode
   const ColDev *cols = P.d_cols_syn.as<ColDev>();
   contract_inv_all<T>(P, x, cols, P.spec.p, st);
   theta_groups<T>(P, cols, 3, P.spec, P.work, "beta_fwd_adj", st);
@@ -426,12 +419,6 @@ void rows_one(int L, const void *in, void *out, int in_c, int out_c, int sign, d
   launch_rows<TI, TO, T>(J, st);
 }
 
-template <typename T>
-const void *sht_input(ShtPlan &S, const void *flm, hipStream_t st) {
-  if (sizeof(T) == 8) return flm;
-  launch_cast<double, float>(flm, S.flm32.p, (size_t)S.L * S.L, st);
-  return S.flm32.p;
-}
 
 template <typename T>
 void sht_inverse_t(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
@@ -439,7 +426,7 @@ void sht_inverse_t(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
   const int L = S.L;
   const ColDev *col = S.d_col.as<ColDev>();
   {
-    const void *x = sht_input<T>(S, flm, st);
+    const void *x = flm;  // complex128; k_xtile rounds it in the FP32 mode
     InvJobs<T> J{};
     J.n = 1;
     J.j[0] = InvJob<T>{(const T *)S.ei, col, 1, std::abs(S.spin), L, (L + 63) / 64, 0};
@@ -480,7 +467,7 @@ void sht_forward_adjoint_t(ShtPlan &S, const void *flm, void *f, hipStream_t st)
   const int L = S.L, M = 2 * L - 1 + S.ss;
   const ColDev *col = S.d_col.as<ColDev>();
   {
-    const void *x = sht_input<T>(S, flm, st);
+    const void *x = flm;  // complex128; k_xtile rounds it in the FP32 mode
     InvJobs<T> J{};
     J.n = 1;
     J.j[0] = InvJob<T>{(const T *)S.ei, col, 1, std::abs(S.spin), L, (L + 63) / 64, 0};
