@@ -66,7 +66,7 @@ def algorithmic_work(s2, p):
 STAGE_KERNEL = {
     "contract_inv": "k_inv3", "contract_fwd": "k_fwd3", "contract_fwd_sum": "k_sum_parts",
     "beta_inv": "k_theta", "beta_fwd": "k_theta", "beta_inv_adj": "k_theta", "beta_fwd_adj": "k_theta",
-    "rows": "k_rows4", "rows_scal": "k_rows4_scal", "gamma": "k_mix", "cast": "k_cast_cx",
+    "rows": "k_rows4", "gamma": "k_mix", "xtile": "k_xtile",
 }
 
 
@@ -104,34 +104,30 @@ def kernel_work(s2, p, F, e):
         algorithmic flop count);
       k_theta: the theta stage as real matrix products, 2 (directions) x 4 L^2 M ncol
         (k_theta_bf: the same products on the bf16 matrix cores, roof = bf16 peak / 6);
-      k_rows4: the two-level DFT of each alpha line of the wavelet columns, M = M1 M2:
-        8 M1^2 M2 + 8 M2^2 M1 real flop (one product of 8 M^2 when M is prime);
-        k_rows4_scal: the same for the phi lines of the scaling function;
+      k_rows4: the two-level DFT of each alpha line of the wavelet columns and phi line
+        of the scaling function, M = M1 M2: 8 M1^2 M2 + 8 M2^2 M1 real flop (one product
+        of 8 M^2 when M is prime);
       k_mix:   the column spectra and the f_wav slices read or written once;
       k_sum_parts: the m' split partials read once and flm written once;
-      k_cast_cx: flm read (FP64) and written (FP32) once."""
+      k_xtile: flm read (complex128) and its two operand tiles written once."""
     g = column_groups(s2, p)
     L = p.L
     theta = sum(2 * 4 * Lg * Lg * (2 * Lg - 1) * nc for Lg, nc, ng in g)
     rows = 0
-    rows_scal = 0
     mix = 0
     for Lg, nc, ng in g:
         M = 2 * Lg - 1
         m1, m2 = dft_factors(M)
         per_line = 8 * M * M if m1 == 1 else 8 * m1 * m1 * m2 + 8 * m2 * m2 * m1
+        rows += 2 * (nc if ng else 1) * Lg * per_line
         if ng:
-            rows += 2 * nc * Lg * per_line
             mix += 2 * Lg * M * (nc * e + ng * 16)
-        else:
-            rows_scal = 2 * Lg * per_line
     S = max(1, min(8, L // 64))
     work = {"k_inv3": ("mfma", F / 2), "k_fwd3": ("mfma", F / 2), "k_theta": ("mfma", theta),
-            "k_theta_bf": ("mfma_bf16x3", theta), "k_rows4": ("mfma", rows), "k_rows4_scal": ("mfma", rows_scal),
-            "k_mix": ("hbm", mix),
+            "k_theta_bf": ("mfma_bf16x3", theta), "k_rows4": ("mfma", rows), "k_mix": ("hbm", mix),
             "k_sum_parts": ("hbm", (S + 1) * L * L * 16)}
-    if e == 8:
-        work["k_cast_cx"] = ("hbm", L * L * 24)
+    nx = (L + 15) // 16
+    work["k_xtile"] = ("hbm", L * L * 16 + 2 * nx * nx * 256 * e)
     return work
 
 

@@ -154,6 +154,7 @@ struct RowJob {
   const void *in;
   void *out;
   int blk0;
+  int io64;  // FP32 kernels: bit 0 the input, bit 1 the output is complex128 (the caller's f_scal)
 };
 template <typename T>
 struct RowJobs {

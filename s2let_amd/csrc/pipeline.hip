@@ -206,7 +206,7 @@ void contract_inv_all(Plan &P, const void *x, const ColDev *cols, void *Fbase, h
   Context &C = Context::get();
   InvJobs<T> J{};
   {
-    Stage s_("contract_inv", st);
+    Stage s_("xtile", st);
     launch_xtile<T>(x, P.p.L, P.xtile.p, st);
   }
   auto flush = [&]() {
@@ -274,9 +274,12 @@ void theta_groups(Plan &P, const ColDev *cols, int kind, const DevBuf &in, DevBu
   }
 }
 
-// row DFTs of the wavelet groups (all lines of a group are contiguous, M each)
+// row DFTs of every group in grouped launches: the scaling-function group (one
+// spin-0 map; its caller-side f_scal is complex128, scal_in or scal_out) and the
+// wavelet groups (plan buffers; all lines of a group are contiguous, M each)
 template <typename T>
-void rows_groups(Plan &P, const DevBuf &in, DevBuf &out, int in_c, int out_c, int sign, bool scaled, hipStream_t st) {
+void rows_all(Plan &P, const DevBuf &in, DevBuf &out, const void *scal_in, void *scal_out, int in_c, int out_c,
+              int sign, bool scaled, hipStream_t st) {
   RowJobs<T> J{};
   auto flush = [&]() {
     Stage s_("rows", st);
@@ -285,33 +288,19 @@ void rows_groups(Plan &P, const DevBuf &in, DevBuf &out, int in_c, int out_c, in
     J.nblk = 0;
   };
   for (const Group &g : P.groups) {  // small band-limits (long single-level DFT blocks) first
-    if (g.ngamma == 0) continue;
+    const bool scal = g.ngamma == 0;
     const int M = 2 * g.L - 1 + P.ss;  // samples per line
     RowJob<T> &j = J.j[J.n++];
-    j.a = row_args<T>(g.L, (size_t)g.ncol * (g.L + P.ss), in_c, out_c, sign, scaled ? kTwoPi / M : 1.0, P.ss);
-    j.in = at<T>(in, g.spec_off);
-    j.out = at<T>(out, g.spec_off);
+    j.a = row_args<T>(g.L, (size_t)(scal ? 1 : g.ncol) * (g.L + P.ss), in_c, out_c, sign, scaled ? kTwoPi / M : 1.0,
+                      P.ss);
+    j.in = scal ? (scal_in ? scal_in : (const void *)at<T>(in, g.spec_off)) : at<T>(in, g.spec_off);
+    j.out = scal ? (scal_out ? scal_out : (void *)at<T>(out, g.spec_off)) : at<T>(out, g.spec_off);
+    j.io64 = scal ? ((scal_in ? 1 : 0) | (scal_out ? 2 : 0)) : 0;
     j.blk0 = J.nblk;
     J.nblk += (int)((j.a.nlines + j.a.lpb - 1) / j.a.lpb);
     if (J.n == kMaxJobs) flush();
   }
   if (J.n) flush();
-}
-
-// row DFT of the scaling-function group (one spin-0 map, FP64 on the caller's side)
-template <typename TI, typename TO, typename T>
-void rows_scal(Plan &P, const void *in, void *out, int in_c, int out_c, int sign, bool scaled, hipStream_t st) {
-  const Group &g = P.groups[0];
-  const int M = 2 * g.L - 1 + P.ss;
-  RowJobs<T> J{};
-  J.n = 1;
-  J.j[0].a = row_args<T>(g.L, (size_t)(g.L + P.ss), in_c, out_c, sign, scaled ? kTwoPi / M : 1.0, P.ss);
-  J.j[0].in = in;
-  J.j[0].out = out;
-  J.j[0].blk0 = 0;
-  J.nblk = (int)((J.j[0].a.nlines + J.j[0].a.lpb - 1) / J.j[0].a.lpb);
-  Stage s_("rows_scal", st);
-  launch_rows<TI, TO, T>(J, st);
 }
 
 // gamma mixing of every wavelet group between the plan's column spectra (cols)
@@ -358,8 +347,7 @@ void analysis_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t
   const ColDev *cols = P.d_cols_ana.as<ColDev>();
   contract_inv_all<T>(P, x, cols, P.spec.p, st);
   theta_groups<T>(P, cols, 0, P.spec, P.work, "beta_inv", st);
-  rows_scal<T, double, T>(P, at<T>(P.work, P.groups[0].spec_off), f_scal, 1, 0, +1, false, st);
-  rows_groups<T>(P, P.work, P.spec, 1, 0, +1, false, st);
+  rows_all<T>(P, P.work, P.spec, nullptr, f_scal, 1, 0, +1, false, st);
   mix_groups<T, double, T>(P, 0, true, nullptr, f_wav, P.spec, P.spec, st);
 }
 
@@ -367,8 +355,7 @@ template <typename T>
 void synthesis_t(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
   const ColDev *cols = P.d_cols_syn.as<ColDev>();
   mix_groups<double, T, T>(P, 2, false, f_wav, nullptr, P.work, P.work, st);
-  rows_scal<double, T, T>(P, f_scal, at<T>(P.spec, P.groups[0].spec_off), 0, 1, -1, true, st);
-  rows_groups<T>(P, P.work, P.spec, 0, 1, -1, true, st);
+  rows_all<T>(P, P.work, P.spec, f_scal, nullptr, 0, 1, -1, true, st);
   theta_groups<T>(P, cols, 2, P.spec, P.work, "beta_fwd", st);
   contract_fwd_all<T>(P, cols, P.work.p, flm, st);
 }
@@ -377,8 +364,7 @@ template <typename T>
 void analysis_adjoint_t(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
   const ColDev *cols = P.d_cols_ana.as<ColDev>();
   mix_groups<double, T, T>(P, 1, false, f_wav, nullptr, P.work, P.work, st);
-  rows_scal<double, T, T>(P, f_scal, at<T>(P.spec, P.groups[0].spec_off), 0, 1, -1, false, st);
-  rows_groups<T>(P, P.work, P.spec, 0, 1, -1, false, st);
+  rows_all<T>(P, P.work, P.spec, f_scal, nullptr, 0, 1, -1, false, st);
   theta_groups<T>(P, cols, 1, P.spec, P.work, "beta_inv_adj", st);
   contract_fwd_all<T>(P, cols, P.work.p, flm, st);
 }
@@ -389,8 +375,7 @@ void synthesis_adjoint_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hi
   const ColDev *cols = P.d_cols_syn.as<ColDev>();
   contract_inv_all<T>(P, x, cols, P.spec.p, st);
   theta_groups<T>(P, cols, 3, P.spec, P.work, "beta_fwd_adj", st);
-  rows_scal<T, double, T>(P, at<T>(P.work, P.groups[0].spec_off), f_scal, 1, 0, +1, true, st);
-  rows_groups<T>(P, P.work, P.spec, 1, 0, +1, true, st);
+  rows_all<T>(P, P.work, P.spec, nullptr, f_scal, 1, 0, +1, true, st);
   mix_groups<T, double, T>(P, 3, true, nullptr, f_wav, P.spec, P.spec, st);
 }
 

@@ -107,6 +107,8 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
   const RowArgs<T> &a = jobs.j[jb].a;
   const cx<TI> *__restrict__ in = static_cast<const cx<TI> *>(jobs.j[jb].in);
   cx<TO> *__restrict__ out = static_cast<cx<TO> *>(jobs.j[jb].out);
+  // a complex128 side of a job of an FP32 launch (uniform over the block)
+  const bool in64 = sizeof(TI) == 4 && (jobs.j[jb].io64 & 1), out64 = sizeof(TO) == 4 && (jobs.j[jb].io64 & 2);
   const Dft2<T> &d = a.d;
   const int M = d.M, M1 = d.M1, M2 = d.M2, half = a.L - 1, Mh = M - half;
   const float inv_rpp = 1.0f / (float)a.rows_pp;
@@ -147,7 +149,15 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
           else if (q >= Mh) sidx = q - Mh;
           else ok = false;
         }
-        const cx<TI> v = ok ? src[sidx] : mk<TI>(0, 0);
+        cx<TI> v = mk<TI>(0, 0);
+        if (ok) {
+          if (in64) {
+            const cx<double> w = reinterpret_cast<const cx<double> *>(in)[(src - in) + sidx];
+            v = mk<TI>((TI)w.x, (TI)w.y);
+          } else {
+            v = src[sidx];
+          }
+        }
         const int o = d.perm[q];
         dst[o] = (T)v.x;
         dst[o + im] = (T)v.y;
@@ -244,7 +254,10 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
             else if (o >= Mh) o -= Mh;
             else continue;
           }
-          dst[o] = mk<TO>((TO)(r[2 * p] * scale), (TO)(r[2 * p + 1] * scale));
+          if (out64)
+            reinterpret_cast<cx<double> *>(out)[(dst - out) + o] = mk<double>((double)(r[2 * p] * scale), (double)(r[2 * p + 1] * scale));
+          else
+            dst[o] = mk<TO>((TO)(r[2 * p] * scale), (TO)(r[2 * p + 1] * scale));
         }
       }
     }
