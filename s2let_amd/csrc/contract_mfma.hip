@@ -203,37 +203,46 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt, int 
 #pragma unroll
         for (int v = 0; v < 4; ++v) hs[j][q][v] = 0.0;
   }
-  for (int lb = lb0; lb < nlb; ++lb) {
-    const int l0 = 16 * lb, nmp = 16 * (lb + 1);
+  // operands of one l-block: the Delta and weight records of the MP m' (m' past the
+  // l-block's range read m' = mp0's and are scaled by zero) and the harmonics tile
+  struct Ops {
     V4<T> d[MP], e[MP];
+    V4<T> p01, p23, q01, q23;
+  };
+  auto load = [&](int lbr, Ops &o) {
+    const int lb = min(lbr, nlb - 1), nmp = 16 * (lb + 1);
 #pragma unroll
     for (int j = 0; j < MP; ++j) {
       const int mp = mp0 + j;
-      if (mp < nmp && mp < L) {
-        d[j] = ld4(dt + dtile_base(lb) + ((size_t)mbi * nmp + mp) * 256 + 4 * lane);
-        e[j] = ld4(Ei + (((size_t)mp * nlb + lb) * 64 + lane) * 4);
-      } else {
+      const bool ok = mp < nmp && mp < L;
+      const int mq = ok ? mp : mp0;  // mp0 < nmp for every l-block from lb0 on
+      o.d[j] = ld4(dt + dtile_base(lb) + ((size_t)mbi * nmp + mq) * 256 + 4 * lane);
+      o.e[j] = ld4(Ei + (((size_t)mq * nlb + lb) * 64 + lane) * 4);
+      const T z = ok ? (T)1 : (T)0;
 #pragma unroll
-        for (int s = 0; s < 4; ++s) d[j].v[s] = e[j].v[s] = (T)0;
-      }
+      for (int s = 0; s < 4; ++s) o.d[j].v[s] *= z;
     }
     // x[l, m] and (-1)^l x[l, -m] of this lane's four l (the sign (-1)^{m'} of the -m
     // products is applied at the end); E vanishes for l >= L
     const cx<T> *xq = xt + (((size_t)lb * nx + mbi) * 64 + lane) * 4;
-    const V4<T> p01 = ld4(&xq[0].x), p23 = ld4(&xq[2].x);
-    const V4<T> q01 = ld4(&xq[xhalf].x), q23 = ld4(&xq[xhalf + 2].x);
+    o.p01 = ld4(&xq[0].x);
+    o.p23 = ld4(&xq[2].x);
+    o.q01 = ld4(&xq[xhalf].x);
+    o.q23 = ld4(&xq[xhalf + 2].x);
+  };
+  auto compute = [&](int lb, const Ops &o) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      const V4<T> &pp = s < 2 ? p01 : p23, &qq = s < 2 ? q01 : q23;
+      const V4<T> &pp = s < 2 ? o.p01 : o.p23, &qq = s < 2 ? o.q01 : o.q23;
       const T xpr = pp.v[2 * (s & 1)], xpi = pp.v[2 * (s & 1) + 1];
       const T xmr = qq.v[2 * (s & 1)], xmi = qq.v[2 * (s & 1) + 1];
 #pragma unroll
       for (int j = 0; j < MP; ++j) {
-        const T dp = d[j].v[s];
-        acc[j][0] = Mfma<T>::mma(e[j].v[s], dp * xpr, acc[j][0]);
-        acc[j][1] = Mfma<T>::mma(e[j].v[s], dp * xpi, acc[j][1]);
-        acc[j][2] = Mfma<T>::mma(e[j].v[s], dp * xmr, acc[j][2]);
-        acc[j][3] = Mfma<T>::mma(e[j].v[s], dp * xmi, acc[j][3]);
+        const T dp = o.d[j].v[s];
+        acc[j][0] = Mfma<T>::mma(o.e[j].v[s], dp * xpr, acc[j][0]);
+        acc[j][1] = Mfma<T>::mma(o.e[j].v[s], dp * xpi, acc[j][1]);
+        acc[j][2] = Mfma<T>::mma(o.e[j].v[s], dp * xmr, acc[j][2]);
+        acc[j][3] = Mfma<T>::mma(o.e[j].v[s], dp * xmi, acc[j][3]);
       }
     }
     if constexpr (sizeof(T) == 4 && F64) {
@@ -247,6 +256,42 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt, int 
             acc[j][q] = Mfma<T>::zero();
           }
       }
+    }
+  };
+  if constexpr (sizeof(T) == 4 && !F64) {
+    // FP32 sums (occupancy to spare): two operand sets, the loads of l-block lb + 2 in
+    // flight during the products of lb + 1 (unconditional loads, clamped to the last
+    // l-block, so that the wait counts stay exact)
+    Ops o0, o1;
+    load(lb0, o0);
+    load(lb0 + 1, o1);
+    for (int lb = lb0; lb < nlb; lb += 2) {
+      compute(lb, o0);
+      load(lb + 2, o0);
+      if (lb + 1 < nlb) compute(lb + 1, o1);
+      load(lb + 3, o1);
+    }
+  } else {
+    for (int lb = lb0; lb < nlb; ++lb) {
+      const int nmp = 16 * (lb + 1);
+      Ops o;
+#pragma unroll
+      for (int j = 0; j < MP; ++j) {
+        const int mp = mp0 + j;
+        if (mp < nmp && mp < L) {
+          o.d[j] = ld4(dt + dtile_base(lb) + ((size_t)mbi * nmp + mp) * 256 + 4 * lane);
+          o.e[j] = ld4(Ei + (((size_t)mp * nlb + lb) * 64 + lane) * 4);
+        } else {
+#pragma unroll
+          for (int s = 0; s < 4; ++s) o.d[j].v[s] = o.e[j].v[s] = (T)0;
+        }
+      }
+      const cx<T> *xq = xt + (((size_t)lb * nx + mbi) * 64 + lane) * 4;
+      o.p01 = ld4(&xq[0].x);
+      o.p23 = ld4(&xq[2].x);
+      o.q01 = ld4(&xq[xhalf].x);
+      o.q23 = ld4(&xq[xhalf + 2].x);
+      compute(lb, o);
     }
   }
   if (m >= L) return;

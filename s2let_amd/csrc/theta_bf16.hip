@@ -162,7 +162,13 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs) {
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         cx<float> v = breg[e];
-        if (k0 + 8 * kq + e == (scl ? in_row1 : in_row0)) v = ipow_mul(scl ? in_e1 : in_e0, v);
+        // the twisted input row is row 0 or none (launch_theta_bf checks): only
+        // wave 0 of the first chunk holds it, as its e = 0 entry
+        if (e == 0 && k0 == 0 && kq == 0) {
+          const bool tw = (scl ? in_row1 : in_row0) == 0;
+          const cx<float> t = ipow_mul(scl ? in_e1 : in_e0, v);
+          v = tw ? t : v;
+        }
         __bf16 h, m, l;
         split3(v.x, h, m, l);
         pr[0][e] = h;
@@ -226,6 +232,13 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs) {
 }
 
 bool theta_bf_eligible(int Lmax) { return Lmax <= kThetaBfMaxL; }
+
+bool theta_bf_args_ok(const ThetaJobs<float> &jobs) {
+  for (int k = 0; k < jobs.n; ++k)
+    for (int c = 0; c < 2; ++c)
+      if (jobs.j[k].a.in_row[c] > 0) return false;
+  return true;
+}
 
 void launch_theta_bf(const ThetaJobs<float> &jobs, hipStream_t st) {
   if (jobs.n == 0 || jobs.nblk == 0) return;

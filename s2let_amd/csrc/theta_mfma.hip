@@ -387,7 +387,7 @@ void launch_theta(const ThetaJobs<T> &jobs, hipStream_t st) {
       const char *e = getenv("S2LET_THETA_BF");
       return !(e && e[0] == '0');
     }();
-    if (bf && bf_on && theta_bf_eligible(Lmax) && theta_rows<T>(Lmax) == 128) {
+    if (bf && bf_on && theta_bf_eligible(Lmax) && theta_rows<T>(Lmax) == 128 && theta_bf_args_ok(jobs)) {
       launch_theta_bf(jobs, st);
       return;
     }
