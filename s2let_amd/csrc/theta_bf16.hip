@@ -141,10 +141,12 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs) {
   const int ml = tid & 63, kq = wave, mi = mi0 + ml;
   const int scl = (ml & 1) ^ par0, sj = ml >> 1;
   const cx<float> *src = inc + min(mi, M - 1);
-  cx<float> breg[8];
-  auto load_b = [&](int k0) {
+  // the samples of the next two chunks are in flight (two register slots; the K loop
+  // is unrolled by two so that the slot of a chunk is known at compile time)
+  cx<float> b0[8], b1[8];
+  auto load_b = [&](cx<float>(&br)[8], int k0) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) breg[e] = src[(size_t)min(k0 + 8 * kq + e, K - 1) * M];
+    for (int e = 0; e < 8; ++e) br[e] = src[(size_t)min(k0 + 8 * kq + e, K - 1) * M];
   };
   bf16x8 ar[RT][3];
   auto load_a = [&](int r, int kc) {
@@ -152,16 +154,13 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs) {
 #pragma unroll
     for (int p = 0; p < 3; ++p) ar[r][p] = ldbf8(ab[r] + p * ps + (size_t)kk * 512);
   };
-  load_b(0);
-#pragma unroll
-  for (int r = 0; r < RT; ++r) load_a(r, 0);
-  for (int kc = 0; kc < nkc; ++kc) {
+  auto chunk = [&](int kc, cx<float>(&br)[8]) {
     const int k0 = 32 * kc, buf = kc & 1;
     {
       bf16x8 pr[3], pi[3];
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        cx<float> v = breg[e];
+        cx<float> v = br[e];
         // the twisted input row is row 0 or none (launch_theta_bf checks): only
         // wave 0 of the first chunk holds it, as its e = 0 entry
         if (e == 0 && k0 == 0 && kq == 0) {
@@ -186,7 +185,10 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs) {
       }
     }
     __syncthreads();
-    load_b(k0 + 32);  // the last chunk's look-ahead is never used
+    load_b(br, k0 + 64);  // past the last chunk the look-ahead is never used
+    // keep the look-ahead loads here: left to itself the scheduler sinks them to the
+    // end of the chunk, and the next chunk then waits for them in full
+    __builtin_amdgcn_sched_barrier(0);
     bf16x8 bq[4][3];
 #pragma unroll
     for (int qq = 0; qq < 4; ++qq)
@@ -206,8 +208,19 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs) {
         acc[r][qq] = mma_bf(ar[r][0], bq[qq][0], t);
       }
       load_a(r, kc + 1);
+      __builtin_amdgcn_sched_barrier(0);
     }
+  };
+  load_b(b0, 0);
+  load_b(b1, 32);
+#pragma unroll
+  for (int r = 0; r < RT; ++r) load_a(r, 0);
+  int kc = 0;
+  for (; kc + 1 < nkc; kc += 2) {
+    chunk(kc, b0);
+    chunk(kc + 1, b1);
   }
+  if (kc < nkc) chunk(kc, b0);
 
   const cx<double> ph = mk<double>(col.ph_re, col.ph_im);
   const int zc = a.z_e[cls], orow = a.out_row[cls], oe = a.out_e[cls];
