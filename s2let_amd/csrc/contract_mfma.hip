@@ -204,9 +204,11 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt, int 
         for (int v = 0; v < 4; ++v) hs[j][q][v] = 0.0;
   }
   // operands of one l-block: the Delta and weight records of the MP m' (m' past the
-  // l-block's range read m' = mp0's and are scaled by zero) and the harmonics tile
+  // l-block's range read m' = mp0's and are scaled by z = 0 where they are used: a
+  // product at the load would wait for the load) and the harmonics tile
   struct Ops {
     V4<T> d[MP], e[MP];
+    T z[MP];
     V4<T> p01, p23, q01, q23;
   };
   auto load = [&](int lbr, Ops &o) {
@@ -218,9 +220,7 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt, int 
       const int mq = ok ? mp : mp0;  // mp0 < nmp for every l-block from lb0 on
       o.d[j] = ld4(dt + dtile_base(lb) + ((size_t)mbi * nmp + mq) * 256 + 4 * lane);
       o.e[j] = ld4(Ei + (((size_t)mq * nlb + lb) * 64 + lane) * 4);
-      const T z = ok ? (T)1 : (T)0;
-#pragma unroll
-      for (int s = 0; s < 4; ++s) o.d[j].v[s] *= z;
+      o.z[j] = ok ? (T)1 : (T)0;
     }
     // x[l, m] and (-1)^l x[l, -m] of this lane's four l (the sign (-1)^{m'} of the -m
     // products is applied at the end); E vanishes for l >= L
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt, int 
       const T xmr = qq.v[2 * (s & 1)], xmi = qq.v[2 * (s & 1) + 1];
 #pragma unroll
       for (int j = 0; j < MP; ++j) {
-        const T dp = o.d[j].v[s];
+        const T dp = o.d[j].v[s] * o.z[j];
         acc[j][0] = Mfma<T>::mma(o.e[j].v[s], dp * xpr, acc[j][0]);
         acc[j][1] = Mfma<T>::mma(o.e[j].v[s], dp * xpi, acc[j][1]);
         acc[j][2] = Mfma<T>::mma(o.e[j].v[s], dp * xmr, acc[j][2]);
@@ -261,15 +261,25 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt, int 
   if constexpr (sizeof(T) == 4 && !F64) {
     // FP32 sums (occupancy to spare): two operand sets, the loads of l-block lb + 2 in
     // flight during the products of lb + 1 (unconditional loads, clamped to the last
-    // l-block, so that the wait counts stay exact)
-    Ops o0, o1;
-    load(lb0, o0);
-    load(lb0 + 1, o1);
-    for (int lb = lb0; lb < nlb; lb += 2) {
-      compute(lb, o0);
-      load(lb + 2, o0);
-      if (lb + 1 < nlb) compute(lb + 1, o1);
-      load(lb + 3, o1);
+    // l-block, so that the wait counts stay exact).  The scheduling barriers keep each
+    // set's loads together and in this order (the prologue's included: the wait at the
+    // top of the loop must hold for its first pass too).
+    if (lb0 < nlb) {
+      Ops o0, o1;
+      load(lb0, o0);
+      __builtin_amdgcn_sched_barrier(0);
+      load(lb0 + 1, o1);
+      __builtin_amdgcn_sched_barrier(0);
+      int lb = lb0;
+      do {
+        compute(lb, o0);
+        load(lb + 2, o0);
+        __builtin_amdgcn_sched_barrier(0);
+        if (lb + 1 < nlb) compute(lb + 1, o1);
+        load(lb + 3, o1);
+        __builtin_amdgcn_sched_barrier(0);
+        lb += 2;
+      } while (lb < nlb);
     }
   } else {
     for (int lb = lb0; lb < nlb; ++lb) {
@@ -278,6 +288,7 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt, int 
 #pragma unroll
       for (int j = 0; j < MP; ++j) {
         const int mp = mp0 + j;
+        o.z[j] = (T)1;
         if (mp < nmp && mp < L) {
           o.d[j] = ld4(dt + dtile_base(lb) + ((size_t)mbi * nmp + mp) * 256 + 4 * lane);
           o.e[j] = ld4(Ei + (((size_t)mp * nlb + lb) * 64 + lane) * 4);
