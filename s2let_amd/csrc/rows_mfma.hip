@@ -53,7 +53,7 @@ __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const 
     if (nks <= kRowsKR) {
       if (load) {
 #pragma unroll
-        for (int s = 0; s < kRowsKR; ++s) av[s] = (s < nks) ? ga[s * 64] : 0.f;
+        for (int s = 0; s < kRowsKR; ++s) av[s] = ga[min(s, nks - 1) * 64];  // s >= nks: not used
       }
 #pragma unroll
       for (int s = 0; s < kRowsKR; s += 4) {
@@ -73,29 +73,79 @@ __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const 
       }
       return;
     }
-    for (int s0 = 0; s0 < nks; s0 += 4) {  // nks is a multiple of 4
-      const float4 x0 = *reinterpret_cast<const float4 *>(b0 + s0);
-      const float4 x1 = two ? *reinterpret_cast<const float4 *>(b1 + s0) : x0;
-      const float xs0[4] = {x0.x, x0.y, x0.z, x0.w}, xs1[4] = {x1.x, x1.y, x1.z, x1.w};
+    // streamed fragments: the next four K steps' loads are issued before this step's
+    // products, into the other of two register sets (no copies: a copy would wait for
+    // the load); the last group's look-ahead repeats it.  One column tile or two, each
+    // without branches between the products.
+    auto stream = [&](auto two_c) {
+      constexpr bool TWO = decltype(two_c)::value;
+      auto load = [&](int s, float (&w)[4]) {
+        const int sc = min(s, nks - 4);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float avq = ga[(s0 + q) * 64];
-        a0 = Mfma<T>::mma(avq, xs0[q], a0);
-        if (two) a1 = Mfma<T>::mma(avq, xs1[q], a1);
+        for (int q = 0; q < 4; ++q) w[q] = ga[(sc + q) * 64];
+      };
+      auto group = [&](int s0, const float (&w)[4]) {
+        const float4 x0 = *reinterpret_cast<const float4 *>(b0 + s0);
+        a0 = Mfma<T>::mma(w[0], x0.x, a0);
+        a0 = Mfma<T>::mma(w[1], x0.y, a0);
+        a0 = Mfma<T>::mma(w[2], x0.z, a0);
+        a0 = Mfma<T>::mma(w[3], x0.w, a0);
+        if constexpr (TWO) {
+          const float4 x1 = *reinterpret_cast<const float4 *>(b1 + s0);
+          a1 = Mfma<T>::mma(w[0], x1.x, a1);
+          a1 = Mfma<T>::mma(w[1], x1.y, a1);
+          a1 = Mfma<T>::mma(w[2], x1.z, a1);
+          a1 = Mfma<T>::mma(w[3], x1.w, a1);
+        }
+      };
+      float wa[4], wb[4];
+      load(0, wa);
+      int s0 = 0;
+      for (; s0 + 8 <= nks; s0 += 8) {  // no branch between a load and its use
+        load(s0 + 4, wb);
+        __builtin_amdgcn_sched_barrier(0);
+        group(s0, wa);
+        load(s0 + 8, wa);
+        __builtin_amdgcn_sched_barrier(0);
+        group(s0 + 4, wb);
       }
-    }
+      if (s0 < nks) group(s0, wa);  // nks is a multiple of 4: at most one group left
+    };
+    if (two) stream(std::true_type{});
+    else stream(std::false_type{});
     return;
   }
   const T *b0 = S + (16 * t0 + r16) * cs + k4;
   const T *b1 = b0 + 16 * cs;
-  for (int s0 = 0; s0 < nks; s0 += 4) {  // nks is a multiple of 4
+  auto stream = [&](auto two_c) {
+    constexpr bool TWO = decltype(two_c)::value;
+    auto load = [&](int s, T (&w)[4]) {
+      const int sc = min(s, nks - 4);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const T av = ga[(s0 + q) * 64];
-      a0 = Mfma<T>::mma(av, b0[4 * (s0 + q)], a0);
-      if (two) a1 = Mfma<T>::mma(av, b1[4 * (s0 + q)], a1);
+      for (int q = 0; q < 4; ++q) w[q] = ga[(sc + q) * 64];
+    };
+    auto group = [&](int s0, const T (&w)[4]) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        a0 = Mfma<T>::mma(w[q], b0[4 * (s0 + q)], a0);
+        if constexpr (TWO) a1 = Mfma<T>::mma(w[q], b1[4 * (s0 + q)], a1);
+      }
+    };
+    T wa[4], wb[4];
+    load(0, wa);
+    int s0 = 0;
+    for (; s0 + 8 <= nks; s0 += 8) {  // no branch between a load and its use
+      load(s0 + 4, wb);
+      __builtin_amdgcn_sched_barrier(0);
+      group(s0, wa);
+      load(s0 + 8, wa);
+      __builtin_amdgcn_sched_barrier(0);
+      group(s0 + 4, wb);
     }
-  }
+    if (s0 < nks) group(s0, wa);  // nks is a multiple of 4: at most one group left
+  };
+  if (two) stream(std::true_type{});
+  else stream(std::false_type{});
 }
 
 template <typename TI, typename TO, typename T, int IMAX>
@@ -137,30 +187,61 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
     for (int c = tid; c < ncolp; c += 256)
       for (int kk = (c < ncol) ? P2 : 0; kk < K; ++kk) S[c * cs + kpos(kk, K)] = (T)0;
     const int im = KM ? K / 4 : 1;  // offset of the imaginary part (reduction row 2n + 1)
-    for (int k = 0; k < nl; ++k) {
-      const cx<TI> *src = in + line_at((int)line0 + k, a.in_len);
-      T *dst = S + (size_t)k * cw * cs;
-      for (int q = tid; q < M; q += 256) {
-        // DFT bin q <- centred m = q (q <= L-1) or q - M (q >= M - (L-1)); MWSS bin L is zero
-        int sidx = q;
-        bool ok = true;
-        if (a.in_centred) {
-          if (q <= half) sidx = q + half;
-          else if (q >= Mh) sidx = q - Mh;
-          else ok = false;
+    // two lines x 4 samples per thread and pass: every load (sample and LDS position) is
+    // issued before the first is used, from clamped indexes (a slot past the line or a
+    // zero bin reads entry 0 and is zeroed; the second line of an odd tail repeats the
+    // first and is not written)
+    constexpr int SU = 4;
+    for (int k0 = 0; k0 < nl; k0 += 2) {
+      size_t base[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) base[h] = line_at((int)line0 + min(k0 + h, nl - 1), a.in_len);
+      for (int q0 = 0; q0 < M; q0 += SU * 256) {
+        int sidx[SU], pos[SU];
+        bool ok[SU];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+          // DFT bin q <- centred m = q (q <= L-1) or q - M (q >= M - (L-1)); MWSS bin L is zero
+          const int q = q0 + 256 * u + tid;
+          int si = q;
+          bool okq = q < M;
+          if (a.in_centred) {
+            if (q <= half) si = q + half;
+            else if (q >= Mh) si = q - Mh;
+            else okq = false;
+          }
+          ok[u] = okq;
+          sidx[u] = okq ? si : 0;
+          pos[u] = d.perm[min(q, M - 1)];
         }
-        cx<TI> v = mk<TI>(0, 0);
-        if (ok) {
-          if (in64) {
-            const cx<double> w = reinterpret_cast<const cx<double> *>(in)[(src - in) + sidx];
-            v = mk<TI>((TI)w.x, (TI)w.y);
-          } else {
-            v = src[sidx];
+        cx<TI> v[2][SU];
+        if (in64) {
+          const cx<double> *in2 = reinterpret_cast<const cx<double> *>(in);
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+              const cx<double> w = in2[base[h] + sidx[u]];
+              v[h][u] = mk<TI>((TI)w.x, (TI)w.y);
+            }
+        } else {
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int u = 0; u < SU; ++u) v[h][u] = in[base[h] + sidx[u]];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          if (k0 + h >= nl) break;
+          T *dst = S + (size_t)(k0 + h) * cw * cs;
+#pragma unroll
+          for (int u = 0; u < SU; ++u) {
+            if (q0 + 256 * u + tid >= M) break;
+            const int o = pos[u];
+            dst[o] = ok[u] ? (T)v[h][u].x : (T)0;
+            dst[o + im] = ok[u] ? (T)v[h][u].y : (T)0;
           }
         }
-        const int o = d.perm[q];
-        dst[o] = (T)v.x;
-        dst[o + im] = (T)v.y;
       }
     }
   }
@@ -184,11 +265,36 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
       cur = mt;
     }
     __syncthreads();  // every column of the first product has been read
-    // twiddle and write the second product's columns (line, k1), rows 2 n2 + re/im
+    // twiddle and write the second product's columns (line, k1), rows 2 n2 + re/im.
+    // The four twiddles of an item are loaded together, those of the next item while
+    // this one is written (clamped: a slot without a column reads tw[0] and is unused).
+    auto tw_load = [&](int it, cx<T> (&w)[4]) {
+      const int item = min(wave + 4 * it, nitems - 1);
+      int tp, mt;
+      divmod_small(item, nmt, inmt, tp, mt);
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int col = 16 * (2 * tp + u) + r16;
+        int line, n2;
+        divmod_small(col, M2, 1.0f / (float)M2, line, n2);
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          const int k1 = 8 * mt + 2 * k4 + p;
+          w[2 * u + p] = d.tw[(col < ncol && k1 < M1) ? n2 * k1 : 0];  // e^{-2 pi i n2 k1 / M}
+        }
+      }
+    };
+    cx<T> wn[4];
+    tw_load(0, wn);
 #pragma unroll
     for (int it = 0; it < IMAX; ++it) {
       const int item = wave + 4 * it;
       if (item >= nitems) break;
+      cx<T> wc[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) wc[e] = wn[e];
+      if (it + 1 < IMAX) tw_load(it + 1, wn);
+      __builtin_amdgcn_sched_barrier(0);  // the next item's loads stay ahead of this one's use
       int tp, mt;
       divmod_small(item, nmt, inmt, tp, mt);
 #pragma unroll
@@ -202,7 +308,7 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
         for (int p = 0; p < 2; ++p) {
           const int k1 = 8 * mt + 2 * k4 + p;
           if (k1 >= M1) continue;
-          cx<T> w = d.tw[n2 * k1];  // e^{-2 pi i n2 k1 / M}
+          cx<T> w = wc[2 * u + p];
           if (!fwd) w.y = -w.y;
           const cx<T> z = cmul(mk<T>(r[2 * p], r[2 * p + 1]), w);
           T *yo = S + (line * M1 + k1) * d.CSC + kpos(2 * n2, d.K2);
