@@ -86,8 +86,6 @@ struct RowArgs {
 void launch_risbo_half(const double *in, double *out, int n, int rmax, int symrow, int lp, const double *sq,
                        double *table, hipStream_t st);
 void launch_to_f32(const double *a, float *b, size_t n, hipStream_t st);
-template <typename A, typename B>
-void launch_cast(const void *a, void *b, size_t n, hipStream_t st);
 void launch_add_cx(void *a, const void *b, size_t n, hipStream_t st);  // complex128 a += b
 void launch_hardthreshold(double *g, size_t n, double thr, hipStream_t st);
 void launch_re2cx(const double *a, void *b, size_t n, hipStream_t st);  // complex128 b = a + 0i

@@ -57,12 +57,6 @@ __global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, si
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = (float)a[i];
 }
 
-template <typename A, typename B>
-__global__ void k_cast_cx(const cx<A> *__restrict__ a, cx<B> *__restrict__ b, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    b[i] = mk<B>((B)a[i].x, (B)a[i].y);
-}
-
 // ------------------------------------------------------------------ real signals
 __global__ void k_re2cx(const double *__restrict__ a, cx<double> *__restrict__ b, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -187,13 +181,6 @@ void launch_re2cx(const double *a, void *b, size_t n, hipStream_t st) {
 void launch_cx2re(const void *a, double *b, size_t n, hipStream_t st) {
   if (n) hipLaunchKernelGGL(k_cx2re, dim3(grid1d(n)), dim3(256), 0, st, (const cx<double> *)a, b, n);
 }
-
-template <typename A, typename B>
-void launch_cast(const void *a, void *b, size_t n, hipStream_t st) {
-  hipLaunchKernelGGL((k_cast_cx<A, B>), dim3(grid1d(n)), dim3(256), 0, st, (const cx<A> *)a, (cx<B> *)b, n);
-}
-template void launch_cast<double, float>(const void *, void *, size_t, hipStream_t);
-template void launch_cast<float, double>(const void *, void *, size_t, hipStream_t);
 
 template <typename TI, typename TO, typename T>
 void launch_mix(const MixJobs &jobs, hipStream_t st) {
