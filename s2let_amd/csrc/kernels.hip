@@ -98,7 +98,9 @@ __global__ __launch_bounds__(256) void k_mix(MixJobs jobs) {
   __syncthreads();
   const size_t b0 = blockIdx.x - J.blk0, nbj = (jb + 1 < jobs.n ? jobs.j[jb + 1].blk0 : jobs.nblk) - J.blk0;
   // two sample points per thread and iteration (twice the loads in flight); the
-  // f_wav side (FP64) is streamed with non-temporal loads / stores
+  // complex128 f_wav side is read with non-temporal loads, and written with
+  // non-temporal stores when the mixing is FP32 (plain stores measured faster when it
+  // is FP64)
   const size_t step = nbj * blockDim.x;
   // every load of an iteration is unconditional (clamped point and column, the
   // values of columns j >= nin scaled by zero: a select would let the compiler
@@ -140,7 +142,7 @@ __global__ __launch_bounds__(256) void k_mix(MixJobs jobs) {
       for (int h = 0; h < 2; ++h) {
         if (h == 1 && !ok1) break;
         cx<TO> *dst = out + k * J.out_stride + pts[h];
-        if constexpr (sizeof(TO) == 8) {
+        if constexpr (sizeof(TO) == 8 && sizeof(T) == 4) {
           __builtin_nontemporal_store((TO)ax[h], &dst->x);
           __builtin_nontemporal_store((TO)ay[h], &dst->y);
         } else {
