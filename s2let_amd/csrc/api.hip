@@ -451,8 +451,8 @@ int s2let_hip_autotune(const s2let_parameters_t *p, int mode, void *stream) {
       Tune t = C.tune(L, esz);
       float best = 1e30f;
       int bs = 0;
-      for (int s : {1, 2, 4, 8}) {
-        if (s > kMaxSplits || (s > 1 && L < 32 * s)) continue;
+      for (int s : {1, 2, 4, 8, 16}) {
+        if (s > fwd_splits_max(L) || (s > 1 && L < 32 * s)) continue;
         t.splits = s;
         C.set_tune(L, esz, t);
         const float ms = time_it([&] { run_synthesis(P, fw.p, fs.p, flm.p, st); });  // the contraction's launch

@@ -567,7 +567,10 @@ template void launch_inv_mfma<double>(const void *, int, const double *, void *,
                                       hipStream_t);
 template void launch_inv_mfma<float>(const void *, int, const float *, void *, const InvJobs<float> &, hipStream_t);
 
-int fwd_splits_default(int L) { return std::max(1, std::min(kMaxSplits, L / 64)); }
+int fwd_splits_default(int L) { return std::max(1, std::min(8, L / 64)); }
+// 16 splits up to L = 1024 (shorter m' ranges per block: the longest waves, those of the
+// first |m| block, set the launch's time), 8 above (partials of 8 L^2 complex128)
+int fwd_splits_max(int L) { return L <= 1024 ? kMaxSplits : 8; }
 
 template <typename T>
 void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *const *Efs, const T *dt, int L,

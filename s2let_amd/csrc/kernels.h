@@ -124,9 +124,11 @@ void launch_inv_mfma(const void *xtile, int Lx, const T *dt, void *Fbase, const 
 template <typename T>
 void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *const *Efs, const T *dt, int L,
                      const void *Jbase, void *y, void *parts, int splits, hipStream_t st);
-// m' splits of the forward contraction (partials: splits * L^2 complex128, at most kMaxSplits)
-constexpr int kMaxSplits = 8;
+// m' splits of the forward contraction (partials: splits * L^2 complex128, at most
+// fwd_splits_max(L) <= kMaxSplits)
+constexpr int kMaxSplits = 16;
 int fwd_splits_default(int L);
+int fwd_splits_max(int L);
 void launch_sum_parts(int L, int splits, const void *parts, void *y, hipStream_t st);  // y = sum of the partials
 int mfma_selftest();
 

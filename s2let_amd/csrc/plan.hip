@@ -550,7 +550,7 @@ void Context::finish_plan(Plan &pl, hipStream_t st) {
   const size_t esz = f32 ? sizeof(cx<float>) : sizeof(cx<double>);
   pl.spec.alloc(pl.spec_elems * esz);
   pl.work.alloc(pl.spec_elems * esz);
-  pl.parts.alloc((size_t)kMaxSplits * L * L * sizeof(cx<double>));
+  pl.parts.alloc((size_t)fwd_splits_max(L) * L * L * sizeof(cx<double>));
   pl.xtile.alloc(xtile_elems(L) * esz);
   S2H_CHECK_HIP(hipStreamSynchronize(st));
 }
@@ -599,7 +599,7 @@ ShtPlan &Context::sht(int L, int spin, int mode, hipStream_t st, int ss) {
   const size_t ne = map_elems(L, ss);
   s.spec.alloc(ne * esz);
   s.work.alloc(ne * esz);
-  s.parts.alloc((size_t)kMaxSplits * L * L * sizeof(cx<double>));
+  s.parts.alloc((size_t)fwd_splits_max(L) * L * L * sizeof(cx<double>));
   s.xtile.alloc(xtile_elems(L) * esz);
   S2H_CHECK_HIP(hipStreamSynchronize(st));
   auto &ref = *S;
