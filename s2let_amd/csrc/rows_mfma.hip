@@ -36,7 +36,8 @@ __device__ inline void divmod_small(int n, int d, float inv, int &q, int &r) {
 // one (row tile, up to two column tiles) item of a product: acc[u] += F_mt * S[tile u].
 // In FP32, up to kRowsKR K-steps the item's A fragments are all loaded first (one
 // latency per item) into av, which the caller keeps from item to item of the same
-// row tile (load = false); FP64 and longer reductions stream them four at a time.
+// row tile (load = false); FP64 and longer reductions stream them four K steps at a
+// time, the next four in flight.
 constexpr int kRowsKR = 20;
 template <typename T>
 __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const T *S, int cs, int t0, bool two,
