@@ -438,7 +438,7 @@ int s2let_hip_autotune(const s2let_parameters_t *p, int mode, void *stream) {
     auto time_it = [&](auto &&fn) {
       fn();  // warm-up
       S2H_CHECK_HIP(hipEventRecord(e0, st));
-      for (int r = 0; r < 3; ++r) fn();
+      for (int r = 0; r < 8; ++r) fn();  // enough runs that choices a few % apart are told apart
       S2H_CHECK_HIP(hipEventRecord(e1, st));
       S2H_CHECK_HIP(hipEventSynchronize(e1));
       float ms = 0;
@@ -471,7 +471,7 @@ int s2let_hip_autotune(const s2let_parameters_t *p, int mode, void *stream) {
         Tune tg = C.tune(Lg, esz);
         float bt = 1e30f;
         int bd = 4;
-        for (int dv : {2, 4, 8}) {
+        for (int dv : {2, 3, 4, 6, 8}) {
           tg.rows_div = dv;
           C.set_tune(Lg, esz, tg);
           const float ms = time_it([&] { run_analysis(P, flm.p, fw.p, fs.p, st); });
