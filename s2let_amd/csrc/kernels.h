@@ -196,7 +196,12 @@ void launch_rows(const RowJobs<T> &jobs, hipStream_t st);
 constexpr int kRowsItems = 8;  // first-product items a wave keeps in registers
 int rows_items_per_wave(int M1, int M2, int R1, int lpb);
 size_t rows_lds_bytes(int M1, int M2, int CSA, int CSC, int lpb, size_t esz);
-constexpr int kMixBlocks = 256;  // blocks per mixing job
+// blocks of a mixing job over npts sample points: one pass of two points per thread
+// (256 threads; enough waves in flight to keep HBM busy), at most kMixMaxBlocks
+constexpr int kMixMaxBlocks = 4096;
+inline int mix_blocks(size_t npts) {
+  return (int)std::max<size_t>(1, std::min<size_t>(kMixMaxBlocks, (npts + 511) / 512));
+}
 template <typename TI, typename TO, typename T>
 void launch_mix(const MixJobs &jobs, hipStream_t st);
 

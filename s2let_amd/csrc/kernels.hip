@@ -83,7 +83,7 @@ __global__ void k_add_cx(cx<double> *__restrict__ a, const cx<double> *__restric
 
 // ------------------------------------------------------------------ gamma mixing
 // per job: out_k[pt] = sum_j W[k][j] in_j[pt] (complex, nout x nin matrix W, nin <= NMAX),
-// kMixBlocks blocks per job
+// mix_blocks(npts) blocks per job
 template <typename TI, typename TO, typename T, int NMAX>
 __global__ __launch_bounds__(256) void k_mix(MixJobs jobs) {
   __shared__ cx<T> sW[32 * 32];

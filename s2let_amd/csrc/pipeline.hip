@@ -335,7 +335,7 @@ void mix_groups(Plan &P, int which, bool to_wav, const void *src_wav, void *dst_
     }
     j.in_stride = j.out_stride = j.npts = LM;
     j.blk0 = J.nblk;
-    J.nblk += kMixBlocks;
+    J.nblk += mix_blocks(LM);
     if (J.n == kMaxJobs) flush();
   }
   if (J.n) flush();
