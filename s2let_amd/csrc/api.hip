@@ -5,6 +5,7 @@
 #include <cstring>
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <sstream>
 
 #include "plan.h"
@@ -90,18 +91,30 @@ struct HostIO {
 
 // ---- batch as a HIP graph
 
-void run_op(Plan &P, int op, void *oa, void *ob, const void *ia, const void *ib, hipStream_t st) {
+void run_op(Plan &P, const Scratch &X, int op, void *oa, void *ob, const void *ia, const void *ib, hipStream_t st) {
   switch (op) {
-    case S2LET_HIP_OP_ANALYSIS_LM2WAV: run_analysis(P, ia, oa, ob, st); break;
-    case S2LET_HIP_OP_SYNTHESIS_WAV2LM: run_synthesis(P, ia, ib, oa, st); break;
-    case S2LET_HIP_OP_ANALYSIS_ADJOINT_WAV2LM: run_analysis_adjoint(P, ia, ib, oa, st); break;
-    case S2LET_HIP_OP_SYNTHESIS_ADJOINT_LM2WAV: run_synthesis_adjoint(P, ia, oa, ob, st); break;
+    case S2LET_HIP_OP_ANALYSIS_LM2WAV: run_analysis(P, X, ia, oa, ob, st); break;
+    case S2LET_HIP_OP_SYNTHESIS_WAV2LM: run_synthesis(P, X, ia, ib, oa, st); break;
+    case S2LET_HIP_OP_ANALYSIS_ADJOINT_WAV2LM: run_analysis_adjoint(P, X, ia, ib, oa, st); break;
+    case S2LET_HIP_OP_SYNTHESIS_ADJOINT_LM2WAV: run_synthesis_adjoint(P, X, ia, oa, ob, st); break;
     default: set_error(-1, "unknown op"); throw ArgFailure();
   }
 }
 
+// Captured batches.  A graph holds the device pointers of everything its launches
+// read (tables, plan scratch, the caller's buffers), so an entry is valid only for the
+// context generation it was captured in (tables reallocated or plans freed since make
+// it stale) and for the same caller pointers, which are part of its key.  At most
+// kMaxGraphs entries are kept, the least recently used evicted first.
+struct GraphEntry {
+  hipGraphExec_t exec = nullptr;
+  uint64_t last_use = 0;
+  std::vector<Plan *> plans;  // distinct plans whose scratch the graph uses
+};
 struct GraphCache {
-  std::map<std::string, hipGraphExec_t> exec;
+  static constexpr size_t kMaxGraphs = 64;
+  std::map<std::string, GraphEntry> exec;
+  uint64_t gen = 0, clock = 0;
   std::vector<hipStream_t> streams;  // [0]: capture / launch stream, then one per branch
   std::vector<hipEvent_t> events;    // [0], [1]: fences with the caller's stream, then fork / joins
   hipStream_t stream(size_t i) {
@@ -121,21 +134,32 @@ struct GraphCache {
     return events[i];
   }
   void clear() {
-    for (auto &kv : exec) (void)hipGraphExecDestroy(kv.second);
+    if (!exec.empty() && !streams.empty()) (void)hipStreamSynchronize(streams[0]);  // replays in flight
+    for (auto &kv : exec) (void)hipGraphExecDestroy(kv.second.exec);
     exec.clear();
+  }
+  // drops every entry captured in an older context generation
+  void validate(uint64_t g) {
+    if (g != gen) {
+      clear();
+      gen = g;
+    }
+  }
+  void evict_to(size_t n) {
+    while (exec.size() > n) {
+      auto victim = exec.begin();
+      for (auto i = exec.begin(); i != exec.end(); ++i)
+        if (i->second.last_use < victim->second.last_use) victim = i;
+      (void)hipStreamSynchronize(stream(0));
+      (void)hipGraphExecDestroy(victim->second.exec);
+      exec.erase(victim);
+    }
   }
 };
 GraphCache &graphs() {
   static GraphCache *g = new GraphCache();  // lives until process exit
   return *g;
 }
-
-std::string param_key(const s2let_parameters_t *p) {
-  char b[160];
-  snprintf(b, sizeof b, "%d,%d,%.17g,%d,%d,%d,%d", p->L, p->N, p->B, p->J_min, p->spin, p->original_spin, p->upsample);
-  return b;
-}
-
 
 // ---- real signals (s2let_analysis.c:371-523, s2let_synthesis.c:386-540,
 // s2let_analysis_adjoint.c:402-556, s2let_synthesis_adjoint.c:380-532).  The
@@ -166,23 +190,23 @@ struct RealWork {
 };
 
 // device-pointer real transforms on plan P (f_wav, f_scal, maps real; flm complex128)
-void real_analysis(Plan &P, RealWork &w, const void *flm, double *f_wav, double *f_scal, hipStream_t st) {
-  run_analysis(P, flm, P.r_wav.p, P.r_scal.p, st);
+void real_analysis(Plan &P, const Scratch &X, RealWork &w, const void *flm, double *f_wav, double *f_scal, hipStream_t st) {
+  run_analysis(P, X, flm, P.r_wav.p, P.r_scal.p, st);
   launch_cx2re(P.r_wav.p, f_wav, w.nwav, st);
   launch_cx2re(P.r_scal.p, f_scal, w.nscal, st);
 }
-void real_synthesis(Plan &P, RealWork &w, const double *f_wav, const double *f_scal, void *flm, hipStream_t st) {
+void real_synthesis(Plan &P, const Scratch &X, RealWork &w, const double *f_wav, const double *f_scal, void *flm, hipStream_t st) {
   launch_re2cx(f_wav, P.r_wav.p, w.nwav, st);
   launch_re2cx(f_scal, P.r_scal.p, w.nscal, st);
-  run_synthesis(P, P.r_wav.p, P.r_scal.p, flm, st);
+  run_synthesis(P, X, P.r_wav.p, P.r_scal.p, flm, st);
 }
-void real_analysis_adjoint(Plan &P, RealWork &w, const double *f_wav, const double *f_scal, void *flm, hipStream_t st) {
+void real_analysis_adjoint(Plan &P, const Scratch &X, RealWork &w, const double *f_wav, const double *f_scal, void *flm, hipStream_t st) {
   launch_re2cx(f_wav, P.r_wav.p, w.nwav, st);
   launch_re2cx(f_scal, P.r_scal.p, w.nscal, st);
-  run_analysis_adjoint(P, P.r_wav.p, P.r_scal.p, flm, st);
+  run_analysis_adjoint(P, X, P.r_wav.p, P.r_scal.p, flm, st);
 }
-void real_synthesis_adjoint(Plan &P, RealWork &w, const void *flm, double *f_wav, double *f_scal, hipStream_t st) {
-  run_synthesis_adjoint(P, flm, P.r_wav.p, P.r_scal.p, st);
+void real_synthesis_adjoint(Plan &P, const Scratch &X, RealWork &w, const void *flm, double *f_wav, double *f_scal, hipStream_t st) {
+  run_synthesis_adjoint(P, X, flm, P.r_wav.p, P.r_scal.p, st);
   launch_cx2re(P.r_wav.p, f_wav, w.nwav, st);
   launch_cx2re(P.r_scal.p, f_scal, w.nscal, st);
 }
@@ -239,13 +263,15 @@ void s2let_hip_release(void) {
   Context &C = Context::get();                    \
   std::lock_guard<std::mutex> lk(C.mu);           \
   hipStream_t st = (hipStream_t)stream;           \
-  Plan &P = C.plan(p, mode, st);
+  Plan &P = C.plan(p, mode, st);                  \
+  const Scratch X = C.scratch(P);                 \
+  PlanUse use_(P.sync, st);
 
 int s2let_hip_analysis_lm2wav(void *f_wav, void *f_scal, const void *flm, const s2let_parameters_t *p, int mode,
                               void *stream) {
   return guarded([&] {
     S2H_DEV_PLAN
-    run_analysis(P, flm, f_wav, f_scal, st);
+    run_analysis(P, X, flm, f_wav, f_scal, st);
     S2H_CHECK_HIP(hipGetLastError());
   });
 }
@@ -253,7 +279,7 @@ int s2let_hip_synthesis_wav2lm(void *flm, const void *f_wav, const void *f_scal,
                                void *stream) {
   return guarded([&] {
     S2H_DEV_PLAN
-    run_synthesis(P, f_wav, f_scal, flm, st);
+    run_synthesis(P, X, f_wav, f_scal, flm, st);
     S2H_CHECK_HIP(hipGetLastError());
   });
 }
@@ -261,7 +287,7 @@ int s2let_hip_analysis_adjoint_wav2lm(void *flm, const void *f_wav, const void *
                                       int mode, void *stream) {
   return guarded([&] {
     S2H_DEV_PLAN
-    run_analysis_adjoint(P, f_wav, f_scal, flm, st);
+    run_analysis_adjoint(P, X, f_wav, f_scal, flm, st);
     S2H_CHECK_HIP(hipGetLastError());
   });
 }
@@ -269,7 +295,7 @@ int s2let_hip_synthesis_adjoint_lm2wav(void *f_wav, void *f_scal, const void *fl
                                        int mode, void *stream) {
   return guarded([&] {
     S2H_DEV_PLAN
-    run_synthesis_adjoint(P, flm, f_wav, f_scal, st);
+    run_synthesis_adjoint(P, X, flm, f_wav, f_scal, st);
     S2H_CHECK_HIP(hipGetLastError());
   });
 }
@@ -280,7 +306,7 @@ int s2let_hip_analysis_lm2wav_real(double *f_wav, double *f_scal, const void *fl
   return guarded([&] {
     S2H_DEV_PLAN
     RealWork w(P);
-    real_analysis(P, w, flm, f_wav, f_scal, st);
+    real_analysis(P, X, w, flm, f_wav, f_scal, st);
     S2H_CHECK_HIP(hipGetLastError());
   });
 }
@@ -289,7 +315,7 @@ int s2let_hip_synthesis_wav2lm_real(void *flm, const double *f_wav, const double
   return guarded([&] {
     S2H_DEV_PLAN
     RealWork w(P);
-    real_synthesis(P, w, f_wav, f_scal, flm, st);
+    real_synthesis(P, X, w, f_wav, f_scal, flm, st);
     S2H_CHECK_HIP(hipGetLastError());
   });
 }
@@ -298,7 +324,7 @@ int s2let_hip_analysis_adjoint_wav2lm_real(void *flm, const double *f_wav, const
   return guarded([&] {
     S2H_DEV_PLAN
     RealWork w(P);
-    real_analysis_adjoint(P, w, f_wav, f_scal, flm, st);
+    real_analysis_adjoint(P, X, w, f_wav, f_scal, flm, st);
     S2H_CHECK_HIP(hipGetLastError());
   });
 }
@@ -307,7 +333,7 @@ int s2let_hip_synthesis_adjoint_lm2wav_real(double *f_wav, double *f_scal, const
   return guarded([&] {
     S2H_DEV_PLAN
     RealWork w(P);
-    real_synthesis_adjoint(P, w, flm, f_wav, f_scal, st);
+    real_synthesis_adjoint(P, X, w, flm, f_wav, f_scal, st);
     S2H_CHECK_HIP(hipGetLastError());
   });
 }
@@ -316,7 +342,8 @@ int s2let_hip_synthesis_adjoint_lm2wav_real(double *f_wav, double *f_scal, const
   Context &C = Context::get();          \
   std::lock_guard<std::mutex> lk(C.mu); \
   hipStream_t st = (hipStream_t)stream; \
-  ShtPlan &S = C.sht(L, spin, mode, st);
+  ShtPlan &S = C.sht(L, spin, mode, st); \
+  PlanUse use_(S.sync, st);
 
 int s2let_hip_mw_alm2map(void *f, const void *flm, int L, int spin, int mode, void *stream) {
   return guarded([&] {
@@ -359,24 +386,32 @@ int s2let_hip_batch_graph(int op, int nmaps, void *const *out_a, void *const *ou
     // graphs are captured on and launched from a library stream fenced with the
     // caller's stream (which may be the null stream, where capture is not allowed)
     hipStream_t user = (hipStream_t)stream;
-    std::ostringstream key;
-    key << op << ';' << mode << ';' << nmaps;
-    std::map<std::string, std::vector<int>> branch;  // parameter set -> maps, in order
-    std::vector<std::string> order;
+    // plans first (building one may reallocate tables and so move the generation), and
+    // their scratch, both outside any capture
+    std::vector<Plan *> plans(nmaps);
+    std::vector<Scratch> xs(nmaps);
     for (int k = 0; k < nmaps; ++k) {
       require_mw(params + k, "s2let_hip_batch_graph");
-      const std::string pk = param_key(params + k);
+      plans[k] = &C.plan(params + k, mode, C.stream());
+    }
+    for (int k = 0; k < nmaps; ++k) xs[k] = C.scratch(*plans[k]);
+    GraphCache &G = graphs();
+    G.validate(C.generation());
+    std::ostringstream key;
+    key << op << ';' << mode << ';' << nmaps;
+    std::map<std::string, std::vector<int>> branch;  // plan -> maps, in order
+    std::vector<std::string> order;
+    for (int k = 0; k < nmaps; ++k) {
+      const std::string pk = plan_key(params + k, mode);
       key << ';' << pk << '@' << out_a[k] << ',' << (out_b ? out_b[k] : nullptr) << ',' << in_a[k] << ','
           << (in_b ? in_b[k] : nullptr);
       if (!branch.count(pk)) order.push_back(pk);
       branch[pk].push_back(k);
     }
-    GraphCache &G = graphs();
     hipStream_t origin = G.stream(0);
     auto it = G.exec.find(key.str());
     if (it == G.exec.end()) {
-      std::vector<Plan *> plans(nmaps);
-      for (int k = 0; k < nmaps; ++k) plans[k] = &C.plan(params + k, mode, C.stream());  // built outside the capture
+      G.evict_to(GraphCache::kMaxGraphs - 1);
       const bool was = timing_pause();
       hipGraph_t graph = nullptr;
       try {
@@ -387,7 +422,8 @@ int s2let_hip_batch_graph(int op, int nmaps, void *const *out_a, void *const *ou
           hipStream_t sb = G.stream(b + 1);
           S2H_CHECK_HIP(hipStreamWaitEvent(sb, fork, 0));
           for (int k : branch[order[b]])
-            run_op(*plans[k], op, out_a[k], out_b ? out_b[k] : nullptr, in_a[k], in_b ? in_b[k] : nullptr, sb);
+            run_op(*plans[k], xs[k], op, out_a[k], out_b ? out_b[k] : nullptr, in_a[k], in_b ? in_b[k] : nullptr,
+                   sb);
           hipEvent_t join = G.event(b + 3);
           S2H_CHECK_HIP(hipEventRecord(join, sb));
           S2H_CHECK_HIP(hipStreamWaitEvent(origin, join, 0));
@@ -401,15 +437,20 @@ int s2let_hip_batch_graph(int op, int nmaps, void *const *out_a, void *const *ou
         throw;
       }
       timing_resume(was);
-      hipGraphExec_t ex = nullptr;
-      const hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+      GraphEntry e;
+      const hipError_t err = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
-      S2H_CHECK_HIP(e);
-      it = G.exec.emplace(key.str(), ex).first;
+      S2H_CHECK_HIP(err);
+      for (Plan *q : plans)
+        if (std::find(e.plans.begin(), e.plans.end(), q) == e.plans.end()) e.plans.push_back(q);
+      it = G.exec.emplace(key.str(), std::move(e)).first;
     }
+    it->second.last_use = ++G.clock;
     S2H_CHECK_HIP(hipEventRecord(G.event(0), user));
     S2H_CHECK_HIP(hipStreamWaitEvent(origin, G.event(0), 0));
-    S2H_CHECK_HIP(hipGraphLaunch(it->second, origin));
+    for (Plan *q : it->second.plans) sync_acquire(q->sync, origin);  // other streams' use of the scratch
+    S2H_CHECK_HIP(hipGraphLaunch(it->second.exec, origin));
+    for (Plan *q : it->second.plans) sync_release(q->sync, origin);
     S2H_CHECK_HIP(hipEventRecord(G.event(1), origin));
     S2H_CHECK_HIP(hipStreamWaitEvent(user, G.event(1), 0));
   });
@@ -422,6 +463,8 @@ int s2let_hip_autotune(const s2let_parameters_t *p, int mode, void *stream) {
     std::lock_guard<std::mutex> lk(C.mu);
     hipStream_t st = (hipStream_t)stream;
     Plan &P = C.plan(p, mode, C.stream());
+    const Scratch X = C.scratch(P);
+    PlanUse use_(P.sync, st);
     const int L = p->L;
     const size_t nlm = (size_t)L * L, nw = s2let_n_wav(p), ns = s2let_n_scal(p);
     DevBuf flm, fw, fs;
@@ -455,7 +498,7 @@ int s2let_hip_autotune(const s2let_parameters_t *p, int mode, void *stream) {
         if (s > fwd_splits_max(L) || (s > 1 && L < 32 * s)) continue;
         t.splits = s;
         C.set_tune(L, esz, t);
-        const float ms = time_it([&] { run_synthesis(P, fw.p, fs.p, flm.p, st); });  // the contraction's launch
+        const float ms = time_it([&] { run_synthesis(P, X, fw.p, fs.p, flm.p, st); });  // the contraction's launch
         if (ms < best) {
           best = ms;
           bs = s;
@@ -474,7 +517,7 @@ int s2let_hip_autotune(const s2let_parameters_t *p, int mode, void *stream) {
         for (int dv : {2, 3, 4, 6, 8}) {
           tg.rows_div = dv;
           C.set_tune(Lg, esz, tg);
-          const float ms = time_it([&] { run_analysis(P, flm.p, fw.p, fs.p, st); });
+          const float ms = time_it([&] { run_analysis(P, X, flm.p, fw.p, fs.p, st); });
           if (ms < bt) {
             bt = ms;
             bd = dv;
@@ -519,6 +562,8 @@ int s2let_hip_batch(int op, int nmaps, void *const *out_a, void *const *out_b, c
   std::lock_guard<std::mutex> lk(C.mu); \
   hipStream_t st = C.stream();          \
   Plan &P = C.plan(p, S2LET_HIP_FP64, st); \
+  const Scratch X = C.scratch(P);       \
+  PlanUse use_(P.sync, st);             \
   HostIO io(P);
 
 void s2let_analysis_lm2wav(std::complex<double> *f_wav, std::complex<double> *f_scal, const std::complex<double> *flm,
@@ -527,7 +572,7 @@ void s2let_analysis_lm2wav(std::complex<double> *f_wav, std::complex<double> *f_
   host_call(__func__, [&] {
     S2H_HOST_PLAN
     h2d(P.h_flm.p, flm, io.nlm * CB, st);
-    run_analysis(P, P.h_flm.p, P.h_wav.p, P.h_scal.p, st);
+    run_analysis(P, X, P.h_flm.p, P.h_wav.p, P.h_scal.p, st);
     d2h(f_wav, P.h_wav.p, io.nwav * CB, st);
     d2h(f_scal, P.h_scal.p, io.nscal * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -541,7 +586,7 @@ void s2let_synthesis_wav2lm(std::complex<double> *flm, const std::complex<double
     S2H_HOST_PLAN
     h2d(P.h_wav.p, f_wav, io.nwav * CB, st);
     h2d(P.h_scal.p, f_scal, io.nscal * CB, st);
-    run_synthesis(P, P.h_wav.p, P.h_scal.p, P.h_flm.p, st);
+    run_synthesis(P, X, P.h_wav.p, P.h_scal.p, P.h_flm.p, st);
     d2h(flm, P.h_flm.p, io.nlm * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
   });
@@ -554,7 +599,7 @@ void s2let_analysis_adjoint_wav2lm(std::complex<double> *flm, const std::complex
     S2H_HOST_PLAN
     h2d(P.h_wav.p, f_wav, io.nwav * CB, st);
     h2d(P.h_scal.p, f_scal, io.nscal * CB, st);
-    run_analysis_adjoint(P, P.h_wav.p, P.h_scal.p, P.h_flm.p, st);
+    run_analysis_adjoint(P, X, P.h_wav.p, P.h_scal.p, P.h_flm.p, st);
     d2h(flm, P.h_flm.p, io.nlm * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
   });
@@ -566,7 +611,7 @@ void s2let_synthesis_adjoint_lm2wav(std::complex<double> *f_wav, std::complex<do
   host_call(__func__, [&] {
     S2H_HOST_PLAN
     h2d(P.h_flm.p, flm, io.nlm * CB, st);
-    run_synthesis_adjoint(P, P.h_flm.p, P.h_wav.p, P.h_scal.p, st);
+    run_synthesis_adjoint(P, X, P.h_flm.p, P.h_wav.p, P.h_scal.p, st);
     d2h(f_wav, P.h_wav.p, io.nwav * CB, st);
     d2h(f_scal, P.h_scal.p, io.nscal * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -579,9 +624,10 @@ void s2let_analysis_px2wav(std::complex<double> *f_wav, std::complex<double> *f_
   host_call(__func__, [&] {
     S2H_HOST_PLAN
     ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st, P.ss);
+    PlanUse suse_(S.sync, st);
     h2d(P.h_map.p, f, io.npx * CB, st);
     run_sht_forward(S, P.h_map.p, P.h_tmp.p, st);
-    run_analysis(P, P.h_tmp.p, P.h_wav.p, P.h_scal.p, st);
+    run_analysis(P, X, P.h_tmp.p, P.h_wav.p, P.h_scal.p, st);
     d2h(f_wav, P.h_wav.p, io.nwav * CB, st);
     d2h(f_scal, P.h_scal.p, io.nscal * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -594,9 +640,10 @@ void s2let_synthesis_wav2px(std::complex<double> *f, const std::complex<double> 
   host_call(__func__, [&] {
     S2H_HOST_PLAN
     ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st, P.ss);
+    PlanUse suse_(S.sync, st);
     h2d(P.h_wav.p, f_wav, io.nwav * CB, st);
     h2d(P.h_scal.p, f_scal, io.nscal * CB, st);
-    run_synthesis(P, P.h_wav.p, P.h_scal.p, P.h_tmp.p, st);
+    run_synthesis(P, X, P.h_wav.p, P.h_scal.p, P.h_tmp.p, st);
     run_sht_inverse(S, P.h_tmp.p, P.h_map.p, st);
     d2h(f, P.h_map.p, io.npx * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -609,9 +656,10 @@ void s2let_analysis_adjoint_wav2px(std::complex<double> *f, const std::complex<d
   host_call(__func__, [&] {
     S2H_HOST_PLAN
     ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st, P.ss);
+    PlanUse suse_(S.sync, st);
     h2d(P.h_wav.p, f_wav, io.nwav * CB, st);
     h2d(P.h_scal.p, f_scal, io.nscal * CB, st);
-    run_analysis_adjoint(P, P.h_wav.p, P.h_scal.p, P.h_tmp.p, st);
+    run_analysis_adjoint(P, X, P.h_wav.p, P.h_scal.p, P.h_tmp.p, st);
     run_sht_forward_adjoint(S, P.h_tmp.p, P.h_map.p, st);
     d2h(f, P.h_map.p, io.npx * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -624,9 +672,10 @@ void s2let_synthesis_adjoint_px2wav(std::complex<double> *f_wav, std::complex<do
   host_call(__func__, [&] {
     S2H_HOST_PLAN
     ShtPlan &S = C.sht(p->L, p->spin, S2LET_HIP_FP64, st, P.ss);
+    PlanUse suse_(S.sync, st);
     h2d(P.h_map.p, f, io.npx * CB, st);
     run_sht_inverse_adjoint(S, P.h_map.p, P.h_tmp.p, st);
-    run_synthesis_adjoint(P, P.h_tmp.p, P.h_wav.p, P.h_scal.p, st);
+    run_synthesis_adjoint(P, X, P.h_tmp.p, P.h_wav.p, P.h_scal.p, st);
     d2h(f_wav, P.h_wav.p, io.nwav * CB, st);
     d2h(f_scal, P.h_scal.p, io.nscal * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -639,6 +688,7 @@ void s2let_mw_alm2map(std::complex<double> *f, const std::complex<double> *flm, 
     std::lock_guard<std::mutex> lk(C.mu);
     hipStream_t st = C.stream();
     ShtPlan &S = C.sht(L, spin, S2LET_HIP_FP64, st);
+    PlanUse suse_(S.sync, st);
     DevBuf a, b;
     a.alloc((size_t)L * L * CB);
     b.alloc((size_t)L * (2 * L - 1) * CB);
@@ -655,6 +705,7 @@ void s2let_mw_map2alm(std::complex<double> *flm, const std::complex<double> *f, 
     std::lock_guard<std::mutex> lk(C.mu);
     hipStream_t st = C.stream();
     ShtPlan &S = C.sht(L, spin, S2LET_HIP_FP64, st);
+    PlanUse suse_(S.sync, st);
     DevBuf a, b;
     a.alloc((size_t)L * (2 * L - 1) * CB);
     b.alloc((size_t)L * L * CB);
@@ -792,6 +843,8 @@ static Plan &manual_plan(Context &C, const std::complex<double> *wav_lm, const d
   std::lock_guard<std::mutex> lk(C.mu);                                                                  \
   hipStream_t st = C.stream();                                                                           \
   Plan &P = manual_plan(C, wav_lm, scal_l, scal_bandlimit, wav_bandlimits, J, L, spin, N, conj_psi, st); \
+  const Scratch X = C.scratch(P);                                                                        \
+  PlanUse use_(P.sync, st);                                                                              \
   HostIO io(P);
 
 void s2let_analysis_lm2wav_manual(std::complex<double> *f_wav, std::complex<double> *f_scal,
@@ -801,7 +854,7 @@ void s2let_analysis_lm2wav_manual(std::complex<double> *f_wav, std::complex<doub
   host_call(__func__, [&] {
     S2H_MANUAL_PLAN(false)
     h2d(P.h_flm.p, flm, io.nlm * CB, st);
-    run_analysis(P, P.h_flm.p, P.h_wav.p, P.h_scal.p, st);
+    run_analysis(P, X, P.h_flm.p, P.h_wav.p, P.h_scal.p, st);
     d2h(f_wav, P.h_wav.p, io.nwav * CB, st);
     d2h(f_scal, P.h_scal.p, io.nscal * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -817,7 +870,7 @@ void s2let_synthesis_wav2lm_manual(std::complex<double> *flm, const std::complex
     h2d(P.h_wav.p, f_wav, io.nwav * CB, st);
     h2d(P.h_scal.p, f_scal, io.nscal * CB, st);
     h2d(P.h_tmp.p, flm, io.nlm * CB, st);
-    run_synthesis(P, P.h_wav.p, P.h_scal.p, P.h_flm.p, st);
+    run_synthesis(P, X, P.h_wav.p, P.h_scal.p, P.h_flm.p, st);
     launch_add_cx(P.h_tmp.p, P.h_flm.p, io.nlm, st);  // flm += (s2let_synthesis.c:250-284)
     d2h(flm, P.h_tmp.p, io.nlm * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -833,7 +886,7 @@ void s2let_analysis_adjoint_wav2lm_manual(std::complex<double> *flm, const std::
     h2d(P.h_wav.p, f_wav, io.nwav * CB, st);
     h2d(P.h_scal.p, f_scal, io.nscal * CB, st);
     h2d(P.h_tmp.p, flm, io.nlm * CB, st);
-    run_analysis_adjoint(P, P.h_wav.p, P.h_scal.p, P.h_flm.p, st);
+    run_analysis_adjoint(P, X, P.h_wav.p, P.h_scal.p, P.h_flm.p, st);
     launch_add_cx(P.h_tmp.p, P.h_flm.p, io.nlm, st);  // flm += (s2let_analysis_adjoint.c:251-287)
     d2h(flm, P.h_tmp.p, io.nlm * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -847,7 +900,7 @@ void s2let_synthesis_adjoint_lm2wav_manual(std::complex<double> *f_wav, std::com
   host_call(__func__, [&] {
     S2H_MANUAL_PLAN(true)
     h2d(P.h_flm.p, flm, io.nlm * CB, st);
-    run_synthesis_adjoint(P, P.h_flm.p, P.h_wav.p, P.h_scal.p, st);
+    run_synthesis_adjoint(P, X, P.h_flm.p, P.h_wav.p, P.h_scal.p, st);
     d2h(f_wav, P.h_wav.p, io.nwav * CB, st);
     d2h(f_scal, P.h_scal.p, io.nscal * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -865,7 +918,7 @@ void s2let_analysis_lm2wav_real(double *f_wav, double *f_scal, const std::comple
     S2H_HOST_PLAN
     RealWork w(P);
     h2d(P.h_flm.p, flm, io.nlm * CB, st);
-    real_analysis(P, w, P.h_flm.p, P.r_real.as<double>(), P.r_real2.as<double>(), st);
+    real_analysis(P, X, w, P.h_flm.p, P.r_real.as<double>(), P.r_real2.as<double>(), st);
     d2h(f_wav, P.r_real.p, w.nwav * DB, st);
     d2h(f_scal, P.r_real2.p, w.nscal * DB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -880,7 +933,7 @@ void s2let_synthesis_wav2lm_real(std::complex<double> *flm, const double *f_wav,
     RealWork w(P);
     h2d(P.r_real.p, f_wav, w.nwav * DB, st);
     h2d(P.r_real2.p, f_scal, w.nscal * DB, st);
-    real_synthesis(P, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_flm.p, st);
+    real_synthesis(P, X, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_flm.p, st);
     d2h(flm, P.h_flm.p, io.nlm * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
   });
@@ -892,10 +945,11 @@ void s2let_analysis_px2wav_real(double *f_wav, double *f_scal, const double *f, 
     S2H_HOST_PLAN
     RealWork w(P);
     ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st, P.ss);  // real maps: spin 0
+    PlanUse suse_(S.sync, st);
     h2d(P.r_real.p, f, w.npx * DB, st);
     launch_re2cx(P.r_real.as<double>(), P.r_map.p, w.npx, st);
     run_sht_forward(S, P.r_map.p, P.h_tmp.p, st);
-    real_analysis(P, w, P.h_tmp.p, P.r_real.as<double>(), P.r_real2.as<double>(), st);
+    real_analysis(P, X, w, P.h_tmp.p, P.r_real.as<double>(), P.r_real2.as<double>(), st);
     d2h(f_wav, P.r_real.p, w.nwav * DB, st);
     d2h(f_scal, P.r_real2.p, w.nscal * DB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -908,9 +962,10 @@ void s2let_synthesis_wav2px_real(double *f, const double *f_wav, const double *f
     S2H_HOST_PLAN
     RealWork w(P);
     ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st, P.ss);
+    PlanUse suse_(S.sync, st);
     h2d(P.r_real.p, f_wav, w.nwav * DB, st);
     h2d(P.r_real2.p, f_scal, w.nscal * DB, st);
-    real_synthesis(P, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_tmp.p, st);
+    real_synthesis(P, X, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_tmp.p, st);
     run_sht_inverse(S, P.h_tmp.p, P.r_map.p, st);
     launch_cx2re(P.r_map.p, P.r_real.as<double>(), w.npx, st);
     d2h(f, P.r_real.p, w.npx * DB, st);
@@ -926,7 +981,7 @@ void s2let_analysis_adjoint_wav2lm_real(std::complex<double> *flm, const double 
     RealWork w(P);
     h2d(P.r_real.p, f_wav, w.nwav * DB, st);
     h2d(P.r_real2.p, f_scal, w.nscal * DB, st);
-    real_analysis_adjoint(P, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_flm.p, st);
+    real_analysis_adjoint(P, X, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_flm.p, st);
     d2h(flm, P.h_flm.p, io.nlm * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
   });
@@ -939,9 +994,10 @@ void s2let_analysis_adjoint_wav2px_real(double *f, const double *f_wav, const do
     S2H_HOST_PLAN
     RealWork w(P);
     ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st, P.ss);
+    PlanUse suse_(S.sync, st);
     h2d(P.r_real.p, f_wav, w.nwav * DB, st);
     h2d(P.r_real2.p, f_scal, w.nscal * DB, st);
-    real_analysis_adjoint(P, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_tmp.p, st);
+    real_analysis_adjoint(P, X, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_tmp.p, st);
     run_sht_forward_adjoint(S, P.h_tmp.p, P.r_map.p, st);
     launch_cx2re(P.r_map.p, P.r_real.as<double>(), w.npx, st);
     d2h(f, P.r_real.p, w.npx * DB, st);
@@ -956,7 +1012,7 @@ void s2let_synthesis_adjoint_lm2wav_real(double *f_wav, double *f_scal, const st
     S2H_HOST_PLAN
     RealWork w(P);
     h2d(P.h_flm.p, flm, io.nlm * CB, st);
-    real_synthesis_adjoint(P, w, P.h_flm.p, P.r_real.as<double>(), P.r_real2.as<double>(), st);
+    real_synthesis_adjoint(P, X, w, P.h_flm.p, P.r_real.as<double>(), P.r_real2.as<double>(), st);
     d2h(f_wav, P.r_real.p, w.nwav * DB, st);
     d2h(f_scal, P.r_real2.p, w.nscal * DB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -970,10 +1026,11 @@ void s2let_synthesis_adjoint_px2wav_real(double *f_wav, double *f_scal, const do
     S2H_HOST_PLAN
     RealWork w(P);
     ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st, P.ss);
+    PlanUse suse_(S.sync, st);
     h2d(P.r_real.p, f, w.npx * DB, st);
     launch_re2cx(P.r_real.as<double>(), P.r_map.p, w.npx, st);
     run_sht_inverse_adjoint(S, P.r_map.p, P.h_tmp.p, st);
-    real_synthesis_adjoint(P, w, P.h_tmp.p, P.r_real.as<double>(), P.r_real2.as<double>(), st);
+    real_synthesis_adjoint(P, X, w, P.h_tmp.p, P.r_real.as<double>(), P.r_real2.as<double>(), st);
     d2h(f_wav, P.r_real.p, w.nwav * DB, st);
     d2h(f_scal, P.r_real2.p, w.nscal * DB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
@@ -986,6 +1043,7 @@ void s2let_mw_alm2map_real(double *f, const std::complex<double> *flm, int L) {
     std::lock_guard<std::mutex> lk(C.mu);
     hipStream_t st = C.stream();
     ShtPlan &S = C.sht(L, 0, S2LET_HIP_FP64, st);
+    PlanUse suse_(S.sync, st);
     const size_t nlm = (size_t)L * L, npx = (size_t)L * (2 * L - 1);
     DevBuf a, b, r;
     a.alloc(nlm * CB);
@@ -1005,6 +1063,7 @@ void s2let_mw_map2alm_real(std::complex<double> *flm, const double *f, int L) {
     std::lock_guard<std::mutex> lk(C.mu);
     hipStream_t st = C.stream();
     ShtPlan &S = C.sht(L, 0, S2LET_HIP_FP64, st);
+    PlanUse suse_(S.sync, st);
     const size_t nlm = (size_t)L * L, npx = (size_t)L * (2 * L - 1);
     DevBuf a, b, r;
     a.alloc(nlm * CB);

@@ -194,24 +194,24 @@ RowArgs<T> row_args(int L, size_t nlines, int in_c, int out_c, int sign, double 
 
 
 template <typename T>
-inline char *at(const DevBuf &b, size_t elems) {
-  return (char *)b.p + elems * sizeof(cx<T>);
+inline char *at(void *b, size_t elems) {
+  return (char *)b + elems * sizeof(cx<T>);
 }
 inline char *at64(const void *p, size_t elems) { return (char *)p + elems * sizeof(cx<double>); }
 
 const double kTwoPi = 6.28318530717958647692;
 
 template <typename T>
-void contract_inv_all(Plan &P, const void *x, const ColDev *cols, void *Fbase, hipStream_t st) {
+void contract_inv_all(Plan &P, const Scratch &X, const void *x, const ColDev *cols, void *Fbase, hipStream_t st) {
   Context &C = Context::get();
   InvJobs<T> J{};
   {
     Stage s_("xtile", st);
-    launch_xtile<T>(x, P.p.L, P.xtile.p, st);
+    launch_xtile<T>(x, P.p.L, X.xtile, st);
   }
   auto flush = [&]() {
     Stage s_("contract_inv", st);
-    launch_inv_mfma<T>(P.xtile.p, P.p.L, Sel<T>::delta(C), Fbase, J, st);
+    launch_inv_mfma<T>(X.xtile, P.p.L, Sel<T>::delta(C), Fbase, J, st);
     J.n = 0;
     J.nblk = 0;
   };
@@ -232,36 +232,34 @@ void contract_inv_all(Plan &P, const void *x, const ColDev *cols, void *Fbase, h
 }
 
 template <typename T>
-void contract_fwd_all(Plan &P, const ColDev *cols, const void *Jbase, void *flm, hipStream_t st) {
+void contract_fwd_all(Plan &P, const Scratch &X, const ColDev *cols, const void *Jbase, void *flm, hipStream_t st) {
   Context &C = Context::get();
   {
     Stage s_("contract_fwd", st);
     launch_fwd_mfma<T>(P.d_supers.as<SuperDev>(), (int)P.supers.size(), cols, P.d_efs.as<const T *>(),
-                       Sel<T>::delta(C), P.p.L, Jbase, flm, P.parts.p, C.splits(P.p.L, (int)sizeof(T)), st);
+                       Sel<T>::delta(C), P.p.L, Jbase, flm, X.parts, C.splits(P.p.L, (int)sizeof(T)), st);
   }
   if (C.splits(P.p.L, (int)sizeof(T)) > 1) {
     Stage s_("contract_fwd_sum", st);
-    launch_sum_parts(P.p.L, C.splits(P.p.L, (int)sizeof(T)), P.parts.p, flm, st);
+    launch_sum_parts(P.p.L, C.splits(P.p.L, (int)sizeof(T)), X.parts, flm, st);
   }
 }
 
 // ---- grouped launches over the plan's column groups
 // theta stage of every group: in/out = plan buffers at each group's spectra offset
 template <typename T>
-void theta_groups(Plan &P, const ColDev *cols, int kind, const DevBuf &in, DevBuf &out, const char *name,
-                  hipStream_t st) {
-  // job lists of at most kMaxJobs groups, largest band-limit first; the rows per
-  // block of a launch follow its largest band-limit
-  std::vector<size_t> order;
-  for (size_t gi = P.groups.size(); gi-- > 0;) order.push_back(gi);
-  for (size_t k0 = 0; k0 < order.size(); k0 += kMaxJobs) {
-    const size_t k1 = std::min(order.size(), k0 + kMaxJobs);
+void theta_groups(Plan &P, const ColDev *cols, int kind, void *in, void *out, const char *name, hipStream_t st) {
+  // job lists of at most kMaxJobs groups, largest band-limit first (k-th job: group
+  // ng - 1 - k); the rows per block of a launch follow its largest band-limit
+  const size_t ng = P.groups.size();
+  for (size_t k0 = 0; k0 < ng; k0 += kMaxJobs) {
+    const size_t k1 = std::min(ng, k0 + kMaxJobs);
     int Lmax = 0;
-    for (size_t k = k0; k < k1; ++k) Lmax = std::max(Lmax, P.groups[order[k]].L);
+    for (size_t k = k0; k < k1; ++k) Lmax = std::max(Lmax, P.groups[ng - 1 - k].L);
     const int rows = theta_rows<T>(Lmax);
     ThetaJobs<T> J{};
     for (size_t k = k0; k < k1; ++k) {
-      const Group &g = P.groups[order[k]];
+      const Group &g = P.groups[ng - 1 - k];
       ThetaJob<T> &j = J.j[J.n++];
       j.a = theta_args<T>(g.L, cols + g.col0, g.ncol, kind, P.ss);
       j.in = at<T>(in, g.spec_off);
@@ -278,8 +276,8 @@ void theta_groups(Plan &P, const ColDev *cols, int kind, const DevBuf &in, DevBu
 // spin-0 map; its caller-side f_scal is complex128, scal_in or scal_out) and the
 // wavelet groups (plan buffers; all lines of a group are contiguous, M each)
 template <typename T>
-void rows_all(Plan &P, const DevBuf &in, DevBuf &out, const void *scal_in, void *scal_out, int in_c, int out_c,
-              int sign, bool scaled, hipStream_t st) {
+void rows_all(Plan &P, void *in, void *out, const void *scal_in, void *scal_out, int in_c, int out_c, int sign,
+              bool scaled, hipStream_t st) {
   RowJobs<T> J{};
   auto flush = [&]() {
     Stage s_("rows", st);
@@ -307,8 +305,8 @@ void rows_all(Plan &P, const DevBuf &in, DevBuf &out, const void *scal_in, void 
 // and the samples of f_wav; which = 0 inverse, 1 inverse adjoint, 2 forward,
 // 3 forward adjoint (the mixing matrices of the plan)
 template <typename TI, typename TO, typename T>
-void mix_groups(Plan &P, int which, bool to_wav, const void *src_wav, void *dst_wav, const DevBuf &cols,
-                DevBuf &cols_out, hipStream_t st) {
+void mix_groups(Plan &P, int which, bool to_wav, const void *src_wav, void *dst_wav, void *cols, void *cols_out,
+                hipStream_t st) {
   MixJobs J{};
   auto flush = [&]() {
     Stage s_("gamma", st);
@@ -342,41 +340,41 @@ void mix_groups(Plan &P, int which, bool to_wav, const void *src_wav, void *dst_
 }
 
 template <typename T>
-void analysis_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
+void analysis_t(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
   const void *x = flm;  // complex128; k_xtile rounds it in the FP32 mode
   const ColDev *cols = P.d_cols_ana.as<ColDev>();
-  contract_inv_all<T>(P, x, cols, P.spec.p, st);
-  theta_groups<T>(P, cols, 0, P.spec, P.work, "beta_inv", st);
-  rows_all<T>(P, P.work, P.spec, nullptr, f_scal, 1, 0, +1, false, st);
-  mix_groups<T, double, T>(P, 0, true, nullptr, f_wav, P.spec, P.spec, st);
+  contract_inv_all<T>(P, X, x, cols, X.spec, st);
+  theta_groups<T>(P, cols, 0, X.spec, X.work, "beta_inv", st);
+  rows_all<T>(P, X.work, X.spec, nullptr, f_scal, 1, 0, +1, false, st);
+  mix_groups<T, double, T>(P, 0, true, nullptr, f_wav, X.spec, X.spec, st);
 }
 
 template <typename T>
-void synthesis_t(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
+void synthesis_t(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
   const ColDev *cols = P.d_cols_syn.as<ColDev>();
-  mix_groups<double, T, T>(P, 2, false, f_wav, nullptr, P.work, P.work, st);
-  rows_all<T>(P, P.work, P.spec, f_scal, nullptr, 0, 1, -1, true, st);
-  theta_groups<T>(P, cols, 2, P.spec, P.work, "beta_fwd", st);
-  contract_fwd_all<T>(P, cols, P.work.p, flm, st);
+  mix_groups<double, T, T>(P, 2, false, f_wav, nullptr, X.work, X.work, st);
+  rows_all<T>(P, X.work, X.spec, f_scal, nullptr, 0, 1, -1, true, st);
+  theta_groups<T>(P, cols, 2, X.spec, X.work, "beta_fwd", st);
+  contract_fwd_all<T>(P, X, cols, X.work, flm, st);
 }
 
 template <typename T>
-void analysis_adjoint_t(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
+void analysis_adjoint_t(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
   const ColDev *cols = P.d_cols_ana.as<ColDev>();
-  mix_groups<double, T, T>(P, 1, false, f_wav, nullptr, P.work, P.work, st);
-  rows_all<T>(P, P.work, P.spec, f_scal, nullptr, 0, 1, -1, false, st);
-  theta_groups<T>(P, cols, 1, P.spec, P.work, "beta_inv_adj", st);
-  contract_fwd_all<T>(P, cols, P.work.p, flm, st);
+  mix_groups<double, T, T>(P, 1, false, f_wav, nullptr, X.work, X.work, st);
+  rows_all<T>(P, X.work, X.spec, f_scal, nullptr, 0, 1, -1, false, st);
+  theta_groups<T>(P, cols, 1, X.spec, X.work, "beta_inv_adj", st);
+  contract_fwd_all<T>(P, X, cols, X.work, flm, st);
 }
 
 template <typename T>
-void synthesis_adjoint_t(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
+void synthesis_adjoint_t(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
   const void *x = flm;  // complex128; k_xtile rounds it in the FP32 mode
   const ColDev *cols = P.d_cols_syn.as<ColDev>();
-  contract_inv_all<T>(P, x, cols, P.spec.p, st);
-  theta_groups<T>(P, cols, 3, P.spec, P.work, "beta_fwd_adj", st);
-  rows_all<T>(P, P.work, P.spec, nullptr, f_scal, 1, 0, +1, true, st);
-  mix_groups<T, double, T>(P, 3, true, nullptr, f_wav, P.spec, P.spec, st);
+  contract_inv_all<T>(P, X, x, cols, X.spec, st);
+  theta_groups<T>(P, cols, 3, X.spec, X.work, "beta_fwd_adj", st);
+  rows_all<T>(P, X.work, X.spec, nullptr, f_scal, 1, 0, +1, true, st);
+  mix_groups<T, double, T>(P, 3, true, nullptr, f_wav, X.spec, X.spec, st);
 }
 
 // ---- single spin-s transform at L (one column)
@@ -475,17 +473,18 @@ void sht_forward_adjoint_t(ShtPlan &S, const void *flm, void *f, hipStream_t st)
       fn<double>(__VA_ARGS__);      \
   } while (0)
 
-void run_analysis(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
-  S2H_DISPATCH(P.mode, analysis_t, P, flm, f_wav, f_scal, st);
+void run_analysis(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
+  S2H_DISPATCH(P.mode, analysis_t, P, X, flm, f_wav, f_scal, st);
 }
-void run_synthesis(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
-  S2H_DISPATCH(P.mode, synthesis_t, P, f_wav, f_scal, flm, st);
+void run_synthesis(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
+  S2H_DISPATCH(P.mode, synthesis_t, P, X, f_wav, f_scal, flm, st);
 }
-void run_analysis_adjoint(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
-  S2H_DISPATCH(P.mode, analysis_adjoint_t, P, f_wav, f_scal, flm, st);
+void run_analysis_adjoint(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm,
+                          hipStream_t st) {
+  S2H_DISPATCH(P.mode, analysis_adjoint_t, P, X, f_wav, f_scal, flm, st);
 }
-void run_synthesis_adjoint(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
-  S2H_DISPATCH(P.mode, synthesis_adjoint_t, P, flm, f_wav, f_scal, st);
+void run_synthesis_adjoint(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
+  S2H_DISPATCH(P.mode, synthesis_adjoint_t, P, X, flm, f_wav, f_scal, st);
 }
 void run_sht_inverse(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
   S2H_DISPATCH(S.mode, sht_inverse_t, S, flm, f, st);

@@ -55,6 +55,37 @@ struct Group {
   int scale;        // j, or -1 for the scaling function / plain SHT
 };
 
+// Scratch of one transform call: the two spectra buffers (spec_elems complex entries each,
+// in the precision of the mode), the forward contraction's split partials
+// (fwd_splits_max(L) L^2 complex128) and the inverse contraction's operand tiles
+// (xtile_elems(L) entries).  A plan owns one set (allocated on its first use by an
+// allocating entry point); the allocation-free entry point passes its own.
+struct Scratch {
+  void *spec = nullptr, *work = nullptr, *parts = nullptr, *xtile = nullptr;
+};
+
+// Orders calls that share a plan's scratch across streams: every call on a stream
+// other than the previous one waits for the previous call's end event.
+struct PlanSync {
+  hipEvent_t done = nullptr;
+  hipStream_t last = nullptr;
+  bool used = false;
+  PlanSync() = default;
+  PlanSync(const PlanSync &) = delete;
+  PlanSync &operator=(const PlanSync &) = delete;
+  ~PlanSync() {
+    if (done) (void)hipEventDestroy(done);
+  }
+};
+void sync_acquire(PlanSync &s, hipStream_t st);  // no-ops on a capturing stream
+void sync_release(PlanSync &s, hipStream_t st);
+struct PlanUse {  // acquire for the lifetime of one entry-point call
+  PlanSync &s;
+  hipStream_t st;
+  PlanUse(PlanSync &sy, hipStream_t stream) : s(sy), st(stream) { sync_acquire(s, st); }
+  ~PlanUse() { sync_release(s, st); }
+};
+
 struct Plan {
   s2let_parameters_t p;
   int mode;    // S2LET_HIP_FP64 / S2LET_HIP_MIXED
@@ -80,6 +111,10 @@ struct Plan {
   DevBuf spec, work;  // scratch (T = double or float per mode)
   DevBuf parts;       // forward-contraction partial sums over m' splits
   DevBuf xtile;       // input harmonics in the inverse contraction's operand layout
+  bool manual = false;
+  uint64_t last_use = 0;  // plan-cache clock (manual plans are evicted least recently used first)
+  PlanSync sync;
+  size_t xtile_elems_ = 0, parts_elems = 0;  // sizes of the scratch parts (entries)
   std::vector<size_t> gamma_off;  // per group: offsets of its 4 mixing matrices in d_gamma
   // staging for the host-pointer API
   DevBuf h_flm, h_wav, h_scal, h_map, h_tmp;
@@ -96,6 +131,7 @@ struct ShtPlan {
   DevBuf d_col, d_b64, d_b32, spec, work, d_super, parts, xtile;
   DevBuf etab, d_efs;  // column weight tables (Ei, Ef) of the single column
   const void *ei = nullptr;
+  PlanSync sync;
 };
 
 // Launch-shape choices per band-limit (defaults, or the autotuner's picks).
@@ -124,7 +160,14 @@ class Context {
   const DftTables &dft(int n);
   const LTables &ltab(int L, int ss = 0);
   Plan &plan(const s2let_parameters_t *p, int mode, hipStream_t st);
+  Plan *find_plan(const s2let_parameters_t *p, int mode);  // cached plan or null (builds nothing)
   Plan &plan_manual(const ManualTiling &m, int mode, hipStream_t st);
+  Scratch scratch(Plan &pl);  // the plan's own scratch, allocated on first use
+  static size_t scratch_bytes(const Plan &pl);  // bytes of one Scratch set of this plan (256-aligned parts)
+  // Incremented whenever a table that launches read through a pointer (Delta tables,
+  // their tiled copies) is reallocated, and when plans are freed: anything that keeps
+  // such pointers (captured graphs) is stale once the generation moves.
+  uint64_t generation() const { return gen_; }
   ShtPlan &sht(int L, int spin, int mode, hipStream_t st, int ss = 0);
   size_t lds_bytes() const { return lds_bytes_; }
   // launch shapes per (band-limit, precision of the intermediate data: 4 or 8 bytes)
@@ -147,6 +190,7 @@ class Context {
   int dev_ = 0;
   size_t lds_bytes_ = 65536;
   int dl_L_ = 0, t64_L_ = 0, t32_L_ = 0;
+  uint64_t gen_ = 1, clock_ = 0;
   DevBuf dtab64_, tile64_, tile32_;
   std::map<int, std::unique_ptr<DftTables>> dft_;
   std::map<int, std::unique_ptr<LTables>> lt_;
@@ -161,10 +205,12 @@ void timing_enable(bool on);
 bool timing_pause();  // stops recording without clearing; returns the previous state
 void timing_resume(bool was_on);
 std::string timing_report();
-void run_analysis(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st);
-void run_synthesis(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st);
-void run_analysis_adjoint(Plan &P, const void *f_wav, const void *f_scal, void *flm, hipStream_t st);
-void run_synthesis_adjoint(Plan &P, const void *flm, void *f_wav, void *f_scal, hipStream_t st);
+void run_analysis(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st);
+void run_synthesis(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm, hipStream_t st);
+void run_analysis_adjoint(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm,
+                          hipStream_t st);
+void run_synthesis_adjoint(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st);
+std::string plan_key(const s2let_parameters_t *p, int mode);
 void run_sht_inverse(ShtPlan &S, const void *flm, void *f, hipStream_t st);
 void run_sht_forward(ShtPlan &S, const void *f, void *flm, hipStream_t st);
 void run_sht_inverse_adjoint(ShtPlan &S, const void *f, void *flm, hipStream_t st);

@@ -49,7 +49,28 @@ hipStream_t Context::stream() {
   return stream_;
 }
 
+void sync_acquire(PlanSync &s, hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  S2H_CHECK_HIP(hipStreamIsCapturing(st, &cs));
+  if (cs != hipStreamCaptureStatusNone) return;  // ordered inside the capturing graph
+  if (s.used && s.last != st) S2H_CHECK_HIP(hipStreamWaitEvent(st, s.done, 0));
+}
+
+void sync_release(PlanSync &s, hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+  if (!s.done && hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+    s.done = nullptr;
+    return;
+  }
+  if (hipEventRecord(s.done, st) == hipSuccess) {
+    s.last = st;
+    s.used = true;
+  }
+}
+
 void Context::release() {
+  ++gen_;
   plans_.clear();
   shts_.clear();
   lt_.clear();
@@ -71,6 +92,7 @@ static inline size_t qsize(int L) { return (size_t)L * (L + 1) * (2 * L + 1) / 6
 void Context::ensure_delta(int L, bool need_f32, hipStream_t st) {
   if (L > dl_L_) {
     const int Lb = L;
+    ++gen_;
     dtab64_.alloc(qsize(Lb) * sizeof(double));
     DevBuf a, b, sq;
     const size_t plane = (size_t)(2 * Lb + 2) * (2 * Lb + 2);
@@ -94,6 +116,7 @@ void Context::ensure_delta(int L, bool need_f32, hipStream_t st) {
   int &tl = need_f32 ? t32_L_ : t64_L_;
   if (tl < dl_L_) {
     DevBuf &t = need_f32 ? tile32_ : tile64_;
+    ++gen_;
     t.alloc(dtile_elems(dl_L_) * (need_f32 ? sizeof(float) : sizeof(double)));
     if (need_f32) build_dtile<float>(dtab64_.as<double>(), dl_L_, t.as<float>(), st);
     else build_dtile<double>(dtab64_.as<double>(), dl_L_, t.as<double>(), st);
@@ -278,7 +301,7 @@ static inline void ipow(int k, double &re, double &im) {
   im = (k == 1) ? 1 : (k == 3 ? -1 : 0);
 }
 
-static std::string plan_key(const s2let_parameters_t *p, int mode) {
+std::string plan_key(const s2let_parameters_t *p, int mode) {
   std::ostringstream o;
   o.precision(17);
   o << p->L << ',' << p->N << ',' << p->B << ',' << p->J_min << ',' << p->spin << ',' << p->original_spin << ','
@@ -333,10 +356,20 @@ struct Columns {
 
 }  // namespace
 
+Plan *Context::find_plan(const s2let_parameters_t *p, int mode) {
+  auto it = plans_.find(plan_key(p, mode));
+  if (it == plans_.end()) return nullptr;
+  it->second->last_use = ++clock_;
+  return it->second.get();
+}
+
 Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
   const std::string key = plan_key(p, mode);
   auto it = plans_.find(key);
-  if (it != plans_.end()) return *it->second;
+  if (it != plans_.end()) {
+    it->second->last_use = ++clock_;
+    return *it->second;
+  }
   if (p->L < 1 || p->N < 1 || p->B <= 1.0 || p->N > 16 || std::abs(p->spin) >= p->L) {
     set_error(-1, "unsupported parameters (need L >= 1, 1 <= N <= 16, B > 1, |spin| < L)");
     throw ArgFailure();
@@ -382,6 +415,7 @@ Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
   }
   cb.finish();
   finish_plan(pl, st);
+  pl.last_use = ++clock_;
   auto &ref = *P;
   plans_[key] = std::move(P);
   return ref;
@@ -417,9 +451,27 @@ Plan &Context::plan_manual(const ManualTiling &m, int mode, hipStream_t st) {
   o << ',' << h;
   const std::string key = o.str();
   auto it = plans_.find(key);
-  if (it != plans_.end()) return *it->second;
+  if (it != plans_.end()) {
+    it->second->last_use = ++clock_;
+    return *it->second;
+  }
+  // a caller looping over many tilings would otherwise keep one plan (tables and
+  // scratch) per tiling: keep at most kMaxManualPlans, dropping the least recently used
+  constexpr int kMaxManualPlans = 8;
+  int nman = 0;
+  for (auto &kv : plans_) nman += kv.second->manual;
+  while (nman >= kMaxManualPlans) {
+    auto victim = plans_.end();
+    for (auto i = plans_.begin(); i != plans_.end(); ++i)
+      if (i->second->manual && (victim == plans_.end() || i->second->last_use < victim->second->last_use)) victim = i;
+    S2H_CHECK_HIP(hipDeviceSynchronize());  // its scratch may still be in use by queued work
+    ++gen_;                                  // graphs may hold its pointers
+    plans_.erase(victim);
+    --nman;
+  }
   auto P = std::make_unique<Plan>();
   Plan &pl = *P;
+  pl.manual = true;
   pl.p = s2let_parameters_t{};
   pl.p.L = L;
   pl.p.N = N;
@@ -466,9 +518,32 @@ Plan &Context::plan_manual(const ManualTiling &m, int mode, hipStream_t st) {
   }
   cb.finish();
   finish_plan(pl, st);
+  pl.last_use = ++clock_;
   auto &ref = *P;
   plans_[key] = std::move(P);
   return ref;
+}
+
+static inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t Context::scratch_bytes(const Plan &pl) {
+  const size_t esz = pl.mode == S2LET_HIP_MIXED ? sizeof(cx<float>) : sizeof(cx<double>);
+  return 2 * align256(pl.spec_elems * esz) + align256(pl.parts_elems * sizeof(cx<double>)) +
+         align256(pl.xtile_elems_ * esz);
+}
+
+Scratch Context::scratch(Plan &pl) {
+  const size_t esz = pl.mode == S2LET_HIP_MIXED ? sizeof(cx<float>) : sizeof(cx<double>);
+  pl.spec.alloc(pl.spec_elems * esz);
+  pl.work.alloc(pl.spec_elems * esz);
+  pl.parts.alloc(pl.parts_elems * sizeof(cx<double>));
+  pl.xtile.alloc(pl.xtile_elems_ * esz);
+  Scratch x;
+  x.spec = pl.spec.p;
+  x.work = pl.work.p;
+  x.parts = pl.parts.p;
+  x.xtile = pl.xtile.p;
+  return x;
 }
 
 void Context::finish_plan(Plan &pl, hipStream_t st) {
@@ -547,11 +622,9 @@ void Context::finish_plan(Plan &pl, hipStream_t st) {
   for (size_t i = 0; i < pl.psi.size(); ++i) psi_dev[i] = mk<double>(pl.psi[i].real(), pl.psi[i].imag());
   upload(pl.d_psi, psi_dev, st);
   upload(pl.d_phi, pl.phi, st);
-  const size_t esz = f32 ? sizeof(cx<float>) : sizeof(cx<double>);
-  pl.spec.alloc(pl.spec_elems * esz);
-  pl.work.alloc(pl.spec_elems * esz);
-  pl.parts.alloc((size_t)fwd_splits_max(L) * L * L * sizeof(cx<double>));
-  pl.xtile.alloc(xtile_elems(L) * esz);
+  // scratch is allocated on the plan's first use by an allocating entry point (scratch())
+  pl.parts_elems = (size_t)fwd_splits_max(L) * L * L;
+  pl.xtile_elems_ = xtile_elems(L);
   S2H_CHECK_HIP(hipStreamSynchronize(st));
 }
 
