@@ -372,6 +372,20 @@ int s2let_hip_batch(int op, int nmaps, void *const *out_a, void *const *out_b, c
 int s2let_hip_batch_graph(int op, int nmaps, void *const *out_a, void *const *out_b, const void *const *in_a,
                           const void *const *in_b, const s2let_parameters_t *params, int mode, void *stream);
 
+/* Allocation-free analysis (no counterpart in the reference; for callers that
+ * cannot allocate device memory: code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results as
+ * s2let_hip_analysis_lm2wav, but it builds nothing and allocates nothing: the
+ * plan for (parameters, mode) must exist already (s2let_hip_prepare, or any
+ * earlier call with the same parameters and mode; otherwise it returns -4), and
+ * its scratch is a static device array of 96 MiB per GPU, held until the process
+ * ends (parameter sets that need more return -5 before launching anything; e.g.
+ * L=256, N=8 fits in both modes, L=512, N=8 does not).  Every call on a GPU
+ * shares that array: calls that overlap on one GPU (different streams, or host
+ * threads) race and must be ordered by the caller. */
+int s2let_hip_analysis_lm2wav_noalloc(void *f_wav, void *f_scal, const void *flm,
+                                      const s2let_parameters_t *parameters, int mode, void *stream);
+
 /* Autotuner: times candidate launch shapes for the band-limits of this plan
  * (forward-contraction m' splits; LDS share of the row-DFT blocks) on
  * `stream` and keeps the fastest for every later call on those band-limits

@@ -131,6 +131,7 @@ _SIGS["s2let_hip_batch"] = (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [ctypes
                             [ctypes.c_int, ctypes.c_void_p])
 _SIGS["s2let_hip_batch_graph"] = _SIGS["s2let_hip_batch"]
 _SIGS["s2let_hip_autotune"] = (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p])
+_SIGS["s2let_hip_analysis_lm2wav_noalloc"] = _SIGS["s2let_hip_analysis_lm2wav"]
 OP_ANALYSIS_LM2WAV, OP_SYNTHESIS_WAV2LM, OP_ANALYSIS_ADJOINT_WAV2LM, OP_SYNTHESIS_ADJOINT_LM2WAV = 0, 1, 2, 3
 
 

@@ -300,6 +300,78 @@ int s2let_hip_synthesis_adjoint_lm2wav(void *f_wav, void *f_scal, const void *fl
   });
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free analysis
+// Scratch of s2let_hip_analysis_lm2wav_noalloc: one array per device, held until the
+// process ends and shared by every call on that device.  Host code never uses the
+// array's name as an address: its device address is resolved once per device with
+// hipGetSymbolAddress and kept in g_noalloc_ptr.
+constexpr size_t kNoAllocScratchBytes = (size_t)96 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kNoAllocMaxDevices];
+
+// the array's address on the current device (resolved on this entry point's first call there)
+void *noalloc_scratch() {
+  int dev = 0;
+  S2H_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) {
+    set_error(-6, "s2let_hip_analysis_lm2wav_noalloc: device index out of range");
+    throw ArgFailure();
+  }
+  std::lock_guard<std::mutex> g(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void *ptr = nullptr;
+    S2H_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_noalloc_ptr[dev] = ptr;
+  }
+  return g_noalloc_ptr[dev];
+}
+}  // namespace
+
+static size_t noalloc_analysis_bytes(const Plan &P) {
+  const size_t esz = P.mode == S2LET_HIP_MIXED ? sizeof(cx<float>) : sizeof(cx<double>);
+  auto a = [](size_t b) { return (b + 255) / 256 * 256; };
+  return 2 * a(P.spec_elems * esz) + a(P.xtile_elems_ * esz);
+}
+
+extern "C" int s2let_hip_analysis_lm2wav_noalloc(void *f_wav, void *f_scal, const void *flm,
+                                                 const s2let_parameters_t *p, int mode, void *stream) {
+  return guarded([&] {
+    Context &C = Context::get();
+    std::lock_guard<std::mutex> lk(C.mu);
+    hipStream_t st = (hipStream_t)stream;
+    Plan *P = C.find_plan(p, mode);
+    if (!P) {
+      set_error(-4, "s2let_hip_analysis_lm2wav_noalloc: no plan for these parameters and mode; call "
+                    "s2let_hip_prepare first (this entry point builds nothing)");
+      throw ArgFailure();
+    }
+    const size_t need = noalloc_analysis_bytes(*P);
+    if (need > kNoAllocScratchBytes) {
+      set_error(-5, "s2let_hip_analysis_lm2wav_noalloc: these parameters need " + std::to_string(need) +
+                        " bytes of scratch, above the maximum of " + std::to_string(kNoAllocScratchBytes) +
+                        " (use s2let_hip_analysis_lm2wav)");
+      throw ArgFailure();
+    }
+    const size_t esz = P->mode == S2LET_HIP_MIXED ? sizeof(cx<float>) : sizeof(cx<double>);
+    const size_t part = (P->spec_elems * esz + 255) / 256 * 256;
+    char *base = (char *)noalloc_scratch();
+    Scratch X;
+    X.spec = base;
+    X.work = base + part;
+    X.xtile = base + 2 * part;
+    run_analysis(*P, X, flm, f_wav, f_scal, st);  // every stage writes all it later reads
+    S2H_CHECK_HIP(hipGetLastError());
+  });
+}
+
+extern "C" {
+
 // ---------------------------------------------------------------- real signals (device pointers)
 int s2let_hip_analysis_lm2wav_real(double *f_wav, double *f_scal, const void *flm, const s2let_parameters_t *p,
                                    int mode, void *stream) {
