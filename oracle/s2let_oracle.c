@@ -13,6 +13,18 @@
  *              convolution with the exact weights I(p) = int_0^pi e^{ip t} sin t dt,
  *              then the Delta ell-sum;
  *   adjoints : exact conjugate transposes of the steps above, in reverse.
+ *
+ * How each part is pinned (tests/test_oracle_*.py): tiling against the
+ * reference's kappas_*.mat fixtures; Delta against Wigner's sum in exact
+ * rationals; inverse SSHT against scipy / Wigner's explicit d; inverse SO(3)
+ * against the direct sum of its definition; forward SSHT and SO(3), MW and MWSS,
+ * on samples that are not band-limited, against the exact integral of the
+ * trigonometric interpolant of the theta-extended samples (MWSS: Nyquist
+ * frequencies left out, this file's stated definition); adjoints by dot-product
+ * identities; drivers by the reference's round-trip / adjoint / axisym suites.
+ * Parity unpinned: the within-block order of the compact flmn storage
+ * (or_elmn2ind, derived from SO3's size formula; the reference holds no vector
+ * that fixes it) and the MWSS Nyquist convention against SSHT itself.
  */
 #include "s2let_oracle.h"
 
