@@ -311,6 +311,24 @@ void s2let_fits_mwss_write_spin_maps(char *filename, double *fQ, double *fU, int
 void s2let_fits_mw_read_spin_maps(double *fQ, double *fU, char *filename, int L);    /* s2let_fits.c:398 */
 void s2let_fits_mwss_read_spin_maps(double *fQ, double *fU, char *filename, int L);  /* s2let_fits.c:471 */
 
+/* ---- test inputs, power and diagnostics (s2let_lm.h, s2let_mw.h, s2let_math.h).
+ * ran2_dp takes its seed by value like the reference's: a seed <= 0 re-initialises the
+ * generator's process-wide state, a positive seed does not, so a sequence depends on
+ * the calls made before it.  maxerr keeps the reference's int abs() of each difference
+ * (its integer part). ---- */
+double s2let_lm_power(S2LET_COMPLEX(double) *flm, int L);                         /* s2let_lm.c:17 */
+void s2let_lm_random_flm(S2LET_COMPLEX(double) *flm, int L, int spin, int seed);  /* s2let_lm.c:35 */
+void s2let_lm_random_flm_real(S2LET_COMPLEX(double) *flm, int L, int seed);       /* s2let_lm.c:52 */
+double s2let_mw_power(S2LET_COMPLEX(double) *f, int L);                           /* s2let_mw.c:75 */
+double s2let_mw_power_real(double *f, int L);                                     /* s2let_mw.c:87 */
+double ran2_dp(int idum);                                                         /* s2let_math.c:97 */
+double maxerr_cplx(S2LET_COMPLEX(double) *a, S2LET_COMPLEX(double) *b, int size); /* s2let_math.c:139 */
+double maxerr(double *a, double *b, int size);                                    /* s2let_math.c:154 */
+unsigned long binomial_coefficient(int n, int k, int exact);                      /* s2let_math.c:241 */
+double s2let_math_kappa0_quadtrap_s2dw(double a, double b, int n, double B);      /* s2let_math.c:53 */
+double s2let_math_kappa0_quadtrap_needlet(double a, double b, int n);             /* s2let_math.c:75 */
+double s2let_math_spline_scalingfct(double x, double y);                          /* s2let_math.c:45 */
+
 /* ---- MW spherical harmonic transforms used by pys2let (s2let_mw.c:21-70) ---- */
 void s2let_mw_alm2map(S2LET_COMPLEX(double) *f, const S2LET_COMPLEX(double) *flm, int L, int spin);
 void s2let_mw_map2alm(S2LET_COMPLEX(double) *flm, const S2LET_COMPLEX(double) *f, int L, int spin);

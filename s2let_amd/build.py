@@ -29,6 +29,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
 SOURCES = [
     ("tiling.cpp", ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]),
     ("fits.cpp", ["g++", "-O2", "-std=c++17", "-fPIC"]),
+    ("lm.cpp", ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]),
     ("multiply.hip", [HIPCC] + HIP_FLAGS + ["-ffp-contract=off"]),
     ("kernels.hip", [HIPCC] + HIP_FLAGS),
     ("contract_mfma.hip", [HIPCC] + HIP_FLAGS),

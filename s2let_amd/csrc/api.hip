@@ -788,6 +788,18 @@ void s2let_mw_map2alm(std::complex<double> *flm, const std::complex<double> *f, 
   });
 }
 
+// s2let_mw.c:75-91: power of an MW map = power of its harmonic coefficients (spin 0)
+double s2let_mw_power(std::complex<double> *f, int L) {
+  std::vector<std::complex<double>> flm((size_t)L * L);
+  s2let_mw_map2alm(flm.data(), f, L, 0);
+  return s2let_lm_power(flm.data(), L);
+}
+double s2let_mw_power_real(double *f, int L) {
+  std::vector<std::complex<double>> flm((size_t)L * L);
+  s2let_mw_map2alm_real(flm.data(), f, L);
+  return s2let_lm_power(flm.data(), L);
+}
+
 // ---------------------------------------------------------------- harmonic multiplies (lmn API)
 // Buffers are caller-provided host arrays; the multiplies run as GPU kernels
 // over the per-scale compact blocks.

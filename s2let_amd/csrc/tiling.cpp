@@ -120,6 +120,21 @@ void phi2_table(std::vector<double> &phi2, const s2let_parameters_t *p) {
     }
 }
 
+// ln(n!) by Lanczos's approximation of the gamma function, with the six-term series of
+// Numerical Recipes' gammln (the reference's approximate binomial, s2let_math.c:196-231)
+double log_factorial_lanczos(int n) {
+  static const double coef[6] = {76.18009172947146, -86.50532032941677, 24.01409824083091,
+                                 -1.231739572450155, 0.1208650973866179e-2, -0.5395239384953e-5};
+  const double x = n + 1.0;
+  const double head = x + 5.5 - (x + 0.5) * log(x + 5.5);
+  double series = 1.000000000190015, y = x;
+  for (int i = 0; i < 6; ++i) {
+    y += 1.0;
+    series = series + coef[i] / y;
+  }
+  return -head + log(2.5066282746310005 * series / x);
+}
+
 double spin_norm(int el, int spin) {
   double factor = 1;
   for (int s = -(int)dabs(spin) + 1; s <= (int)dabs(spin); ++s) factor *= el + s;
@@ -395,6 +410,18 @@ void s2let_transform_axisym_allocate_mw_f_wav_multires_real(double **f_wav, doub
                                                             const s2let_parameters_t *p) {
   *f_wav = (double *)calloc(axisym_wav_px(p, true), sizeof(double));
   *f_scal = (double *)calloc(axisym_scal_px(p, true), sizeof(double));
+}
+
+// ---- s2let_math.h (s2let_math.c:15-268): the tiling's building blocks, and the binomial
+double s2let_math_kappa0_quadtrap_s2dw(double a, double b, int n, double B) {
+  return trapezoid(a, b, n, [B](double k) { return s2dw_profile(k, B); });
+}
+double s2let_math_kappa0_quadtrap_needlet(double a, double b, int n) { return trapezoid(a, b, n, needlet_profile); }
+double s2let_math_spline_scalingfct(double x, double y) { return 1.5 * cubic_bspline(2.0 * x / y); }
+unsigned long binomial_coefficient(int n, int k, int exact) {
+  if (exact) return choose_exact(n, k);
+  return (unsigned long)floor(0.5 + exp(log_factorial_lanczos(n) - log_factorial_lanczos(k) -
+                                        log_factorial_lanczos(n - k)));
 }
 
 void s2let_allocate_mwss(std::complex<double> **f, int L) {
