@@ -25,12 +25,13 @@ CFG = dict(L=512, N=8, B=2.0, J_min=2, spin=0, upsample=0)
 HBM_PEAK = 8.0e12        # B/s
 FP64_PEAK = 78.6e12      # FLOP/s (dense)
 FP32_PEAK = 157.3e12
-BF16_PEAK = 2.5e15        # dense bf16 MFMA
 THETA_BF_MAX_L = 512      # the FP32-mode theta stage runs on the bf16 matrix cores up to here
-# roofs by bound: FP32 MFMA; the bf16 three-part products (six bf16 products per
-# FP32-equivalent product); HBM
-PEAKS = {"mfma": (FP32_PEAK / 1e12, "TFLOP/s"), "mfma_bf16x3": (BF16_PEAK / 6 / 1e12, "TFLOP/s"),
+# roofs by bound: the ell-contractions on the MFMA roof of their precision, the
+# streaming stages (DFTs, theta products, mixing) on HBM
+PEAKS = {"mfma_fp32": (FP32_PEAK / 1e12, "TFLOP/s"), "mfma_fp64": (FP64_PEAK / 1e12, "TFLOP/s"),
          "hbm": (HBM_PEAK / 1e9, "GB/s")}
+PEAK_BASIS = {"mfma_fp32": "FP32 dense MFMA 157.3 TFLOP/s", "mfma_fp64": "FP64 dense MFMA 78.6 TFLOP/s",
+              "hbm": "HBM3E 8 TB/s"}
 
 
 def powerlaw_flm(L, seed):
@@ -77,58 +78,33 @@ def stage_kernel(name, mixed, L):
     return k
 
 
-def dft_factors(M):
-    """M = M1 M2 with M1 the largest divisor <= sqrt(M) (the row stage's two-level DFT)."""
-    m1 = int(M ** 0.5)
-    while M % m1:
-        m1 -= 1
-    return m1, M // m1
-
-
-def column_groups(s2, p):
-    """(L, columns, gamma samples) of the scaling function and of every scale."""
-    out = [(min(s2.bandlimit(p.J_min - 1, p), p.L), 1, 0)]
-    for j in range(p.J_min, s2.j_max(p) + 1):
-        Lj = min(s2.bandlimit(j, p), p.L)
-        Nj = min(p.N, Lj)
-        Nj += (Nj + p.N) % 2
-        out.append((Lj, Nj, 2 * Nj - 1))
-    return out
-
-
-def kernel_work(s2, p, F, e):
-    """Algorithmic work per step (one analysis + one synthesis) of each kernel
-    function, with its roof: ("mfma", flop) or ("hbm", bytes).  e = bytes of one
-    complex intermediate (8 mixed, 16 FP64); f_wav, f_scal and flm are complex128.
-      k_inv3 / k_fwd3: the analysis / synthesis ell-sums, F/2 each (the path's
-        algorithmic flop count);
-      k_theta: the theta stage as real matrix products, 2 (directions) x 4 L^2 M ncol
-        (k_theta_bf: the same products on the bf16 matrix cores, roof = bf16 peak / 6);
-      k_rows4: the two-level DFT of each alpha line of the wavelet columns and phi line
-        of the scaling function, M = M1 M2: 8 M1^2 M2 + 8 M2^2 M1 real flop (one product
-        of 8 M^2 when M is prime);
-      k_mix:   the column spectra and the f_wav slices read or written once;
-      k_sum_parts: the m' split partials read once and flm written once;
-      k_xtile: flm read (complex128) and its two operand tiles written once."""
-    g = column_groups(s2, p)
-    L = p.L
-    theta = sum(2 * 4 * Lg * Lg * (2 * Lg - 1) * nc for Lg, nc, ng in g)
-    rows = 0
-    mix = 0
-    for Lg, nc, ng in g:
-        M = 2 * Lg - 1
-        m1, m2 = dft_factors(M)
-        per_line = 8 * M * M if m1 == 1 else 8 * m1 * m1 * m2 + 8 * m2 * m2 * m1
-        rows += 2 * (nc if ng else 1) * Lg * per_line
-        if ng:
-            mix += 2 * Lg * M * (nc * e + ng * 16)
-    S = max(1, min(8, L // 64))
-    work = {"k_inv3": ("mfma", F / 2), "k_fwd3": ("mfma", F / 2), "k_theta": ("mfma", theta),
-            "k_theta_bf": ("mfma_bf16x3", theta), "k_rows4": ("mfma", rows), "k_mix": ("hbm", mix),
-            "k_sum_parts": ("hbm", (S + 1) * L * L * 16)}
+def kernel_work(info, F, mixed):
+    """Algorithmic work per step (one analysis + one synthesis) of each kernel function
+    and the roof it is scored on, from the plan's layout (s2let_hip_plan_info):
+    the ell-contractions (k_inv3, k_fwd3) are dense contractions, F/2 each on the MFMA
+    roof of the mode's precision; every other stage is a streaming pass scored in HBM
+    bytes (its FFT / matrix-product flops are overhead):
+      k_theta(_bf): the column spectra read and written once per direction;
+      k_rows4:      the same for the alpha / phi lines (the scaling function's map is
+                    complex128 on the caller's side);
+      k_mix:        the column spectra and the f_wav slices, once per direction;
+      k_sum_parts:  the m' split partials read once and flm written once;
+      k_xtile:      flm read (complex128) and its two operand tiles written once."""
+    e = info["complex_bytes"]
+    groups = info["groups"]
+    spec = info["spec_elems"]
+    L = info["L"]
+    scal_plane = groups[0][0] * (2 * groups[0][0] - 1)
+    theta = 2 * 2 * spec * e
+    rows = 2 * (2 * (spec - scal_plane) * e + scal_plane * (e + 16))
+    mix = sum(2 * Lg * (2 * Lg - 1) * (nc * e + ng * 16) for Lg, nc, ng in groups if ng)
+    S = info["splits"]
     nx = (L + 15) // 16
-    work["k_xtile"] = ("hbm", L * L * 16 + 2 * nx * nx * 256 * e)
-    return work
+    peak_mfma = "mfma_fp32" if mixed else "mfma_fp64"
+    return {"k_inv3": (peak_mfma, F / 2), "k_fwd3": (peak_mfma, F / 2), "k_theta": ("hbm", theta),
+            "k_theta_bf": ("hbm", theta), "k_rows4": ("hbm", rows), "k_mix": ("hbm", mix),
+            "k_sum_parts": ("hbm", (S + 1) * L * L * 16 if S > 1 else 0),
+            "k_xtile": ("hbm", L * L * 16 + 2 * nx * nx * 256 * e)}
 
 
 def pmc_traffic(path):
@@ -152,20 +128,96 @@ def timed(fn, steps, warmup, torch):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(p_cfg, flm):
-    """The oracle (FP64 C restatement of the reference algorithm, OpenMP) on one
-    map of the same workload: analysis + synthesis."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(p_cfg, flm, reps):
+    """The oracle (FP64 C restatement of the reference algorithm) on the host cores:
+    all the cores this process may use (OpenMP) on one map of the workload, the
+    median of `reps` runs, with and without the per-call tiling generation (the
+    reference regenerates the tiling in every lm2wav / wav2lm call); and one thread,
+    as the reference runs, on a bounded sample: one map at L=256 with the same N, B,
+    J_min (an eighth of the L=512 map's ell-sum work), to keep the run short."""
     from oracle import oracle as O
-    threads = int(os.environ.get("S2LET_CPU_THREADS", "0")) or min(16, len(os.sched_getaffinity(0)))
-    O.set_threads(threads)
+    affinity = len(os.sched_getaffinity(0))
+    cores = int(os.environ.get("S2LET_CPU_THREADS", "0")) or int(os.environ.get("OMP_NUM_THREADS", "0")) or affinity
     q = O.params(p_cfg["L"], p_cfg["N"], p_cfg["B"], p_cfg["J_min"], p_cfg["spin"], p_cfg["upsample"])
+
+    def one_map(qq, x):
+        t0 = time.perf_counter()
+        fw, fs = O.analysis_lm2wav(x, qq)
+        O.synthesis_wav2lm(fw, fs, qq)
+        return time.perf_counter() - t0
+
+    O.set_threads(cores)
+    t_tiling = sorted(_timeit(lambda: O.tiling_wavelet(q)) for _ in range(5))[2]
+    ts = sorted(one_map(q, flm) for _ in range(max(1, reps)))
+    t_all = ts[len(ts) // 2]
+    O.set_threads(1)
+    L1 = min(256, p_cfg["L"])
+    q1 = O.params(L1, p_cfg["N"], p_cfg["B"], p_cfg["J_min"], p_cfg["spin"], p_cfg["upsample"])
+    t_one = one_map(q1, powerlaw_flm(L1, 2))
+    O.set_threads(cores)
+    return {"value": 1.0 / t_all, "unit": "maps/s", "cores": cores, "kind": "port",
+            "sample": "median of %d map(s) (analysis+synthesis) at L=%d N=%d J_min=%d, CPU FP64 oracle, %d OpenMP "
+                      "threads, %.1f s per map" % (len(ts), p_cfg["L"], p_cfg["N"], p_cfg["J_min"], cores, t_all),
+            "cpu_model": cpu_model(), "cores_available": affinity,
+            "excluding_tiling": {"value": 1.0 / max(1e-9, t_all - 2 * t_tiling), "unit": "maps/s",
+                                 "tiling_s_per_call": t_tiling},
+            "single_thread": {"value": 1.0 / t_one, "unit": "maps/s", "cores": 1,
+                              "sample": "1 map at L=%d N=%d J_min=%d, one thread, %.1f s" % (
+                                  L1, p_cfg["N"], p_cfg["J_min"], t_one)}}
+
+
+def _timeit(fn):
     t0 = time.perf_counter()
-    fw, fs = O.analysis_lm2wav(flm, q)
-    O.synthesis_wav2lm(fw, fs, q)
-    dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "maps/s", "cores": threads, "kind": "port",
-            "sample": "1 map (analysis+synthesis) at L=%d N=%d J_min=%d, CPU FP64 oracle, %d OpenMP threads, %.1f s"
-                      % (p_cfg["L"], p_cfg["N"], p_cfg["J_min"], threads, dt)}
+    fn()
+    return time.perf_counter() - t0
+
+
+def c3_batch(s2, ops, torch, steps, warmup):
+    """The uneven batch: 256 maps, L in {64,128,256,512}, N in {1,4,8}, spin in {0,2},
+    B=2, J_min=1, f_lm with C_l = (l+1)^-2, drawn with numpy default_rng(3); one step =
+    the analysis batch then the synthesis batch, each one cached HIP graph."""
+    rng = np.random.default_rng(3)
+    shapes = [(int(rng.choice([64, 128, 256, 512])), int(rng.choice([1, 4, 8])), int(rng.choice([0, 2])))
+              for _ in range(256)]
+    flms = []
+    for L, _, spin in shapes:
+        ell = np.repeat(np.arange(L), 2 * np.arange(L) + 1)
+        f = (rng.standard_normal(L * L) + 1j * rng.standard_normal(L * L)) / np.sqrt(2) / (ell + 1.0)
+        f[:spin * spin] = 0
+        flms.append(f)
+    ps = [s2.parameters(L, N, 2.0, 1, sp) for L, N, sp in shapes]
+    x = [torch.from_numpy(f).cuda() for f in flms]
+    out = {"maps": len(ps)}
+    F_total = 0
+    for p in ps:
+        F_total += algorithmic_work(s2, p)[0]
+    for mode, name in ((s2.FP64, "fp64"), (s2.MIXED, "mixed")):
+        fw = [torch.zeros(s2.n_wav(p), dtype=torch.complex128, device="cuda") for p in ps]
+        fs = [torch.zeros(s2.n_scal(p), dtype=torch.complex128, device="cuda") for p in ps]
+        rec = [torch.zeros_like(t) for t in x]
+
+        def step():
+            ops.batch(0, fw, fs, x, None, ps, mode, graph=True)
+            ops.batch(1, rec, None, fw, fs, ps, mode, graph=True)
+
+        dt = timed(step, steps, warmup, torch)
+        worst = max(float((rec[k] - x[k]).abs().max() / x[k].abs().max()) for k in range(len(ps)))
+        out[name] = {"value": steps * len(ps) / dt, "unit": "maps/s", "ms_per_batch": 1e3 * dt / steps,
+                     "max_reconstruction_rel_err": worst}
+        del fw, fs, rec
+    out["flop_per_batch"] = F_total
+    return out
 
 
 def main():
@@ -174,6 +226,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 256-map uneven batch")
+    ap.add_argument("--cpu-reps", type=int, default=1, help="CPU baseline maps (median)")
     ap.add_argument("--L", type=int, default=CFG["L"])
     ap.add_argument("--N", type=int, default=CFG["N"])
     ap.add_argument("--J_min", type=int, default=CFG["J_min"])
@@ -252,35 +306,48 @@ def main():
     dt64, st64 = res[s2.FP64]
     K = args.steps
     maps_s = K / dt32
-    # dominant kernel of the headline (mixed) mode: the kernel function with the
-    # largest time per step, from the per-stage HIP events recorded on the
-    # library's stream (each stage is one kernel function)
-    per_k = {}
-    for name, (cnt, ms) in st32.items():
-        kf = stage_kernel(name, True, p.L)
-        c0, m0 = per_k.get(kf, (0, 0.0))
-        per_k[kf] = (c0 + cnt, m0 + ms)
-    dk, (dcount, dms) = max(per_k.items(), key=lambda kv: kv[1][1])
-    work = kernel_work(s2, p, F, 8)
-    bound, w_step = work.get(dk, ("hbm", None))
-    t_step = dms / K / 1e3  # s
-    peak, unit = PEAKS[bound]
-    achieved = None if w_step is None else w_step / t_step / (1e9 if bound == "hbm" else 1e12)
+
+    def kernel_table(stages, mixed):
+        """per kernel function: launches and ms per step (the library's per-stage HIP
+        events on the stream the kernels run on), its algorithmic work and roof"""
+        info = s2.plan_info(p, s2.MIXED if mixed else s2.FP64)
+        work = kernel_work(info, F, mixed)
+        per_k = {}
+        for name, (cnt, ms) in stages.items():
+            kf = stage_kernel(name, mixed, p.L)
+            c0, m0 = per_k.get(kf, (0, 0.0))
+            per_k[kf] = (c0 + cnt, m0 + ms)
+        table = {}
+        for kf, (cnt, ms) in per_k.items():
+            b, w = work.get(kf, ("hbm", None))
+            t = ms / K / 1e3
+            rate = None if w is None else w / t / (1e9 if b == "hbm" else 1e12)
+            table[kf] = {"ms_per_step": ms / K, "launches_per_step": cnt / K, "bound": b,
+                         "work_per_launch": None if w is None else w * K / cnt,
+                         "unit": "B" if b == "hbm" else "flop",
+                         "achieved": rate, "frac": None if rate is None else rate / PEAKS[b][0]}
+        return table, info
+
+    kernels32, info32 = kernel_table(st32, True)
+    kernels64, _ = kernel_table(st64, False)
+    # dominant kernel of the headline (mixed) mode: the largest time per step
+    dk = max(kernels32, key=lambda k: kernels32[k]["ms_per_step"])
+    d = kernels32[dk]
+    peak, unit = PEAKS[d["bound"]]
     traffic = pmc_traffic(os.path.join(ROOT, "profiles", "pmc_traffic_L512_N8.json")).get(dk)
-    roof = {"bound": "hbm" if bound == "hbm" else "mfma", "achieved": achieved, "peak": peak, "unit": unit,
-            "peak_basis": {"mfma": "FP32 dense MFMA 157.3 TFLOP/s",
-                           "mfma_bf16x3": "bf16 dense MFMA 2.5 PFLOP/s / 6 (three-part bf16 products per "
-                                          "FP32-equivalent product)", "hbm": "HBM3E 8 TB/s"}[bound],
-            "frac": None if achieved is None else achieved / peak, "traffic": traffic,
-            "kernel": dk, "launches_per_step": dcount / K, "avg_launch_ms": dms / dcount,
-            "work_per_launch": None if w_step is None else w_step * K / dcount}
-    kernels = {}
-    for kf, (cnt, ms) in per_k.items():
-        b, w = work.get(kf, ("hbm", None))
-        t = ms / K / 1e3
-        rate = None if w is None else w / t / (1e9 if b == "hbm" else 1e12)
-        kernels[kf] = {"ms_per_step": ms / K, "launches_per_step": cnt / K, "bound": b,
-                       "achieved": rate, "frac": None if rate is None else rate / PEAKS[b][0]}
+    # path roofline: the time the map would take at the binding roof (F on FP32 MFMA, Q on HBM)
+    t32 = dt32 / K
+    t64 = dt64 / K
+    bound32 = max(F / FP32_PEAK, Q / HBM_PEAK)
+    bound64 = max(F / FP64_PEAK, Q / HBM_PEAK)
+    roof = {"bound": "hbm" if d["bound"] == "hbm" else "mfma", "achieved": d["achieved"], "peak": peak,
+            "unit": unit, "peak_basis": PEAK_BASIS[d["bound"]], "frac": d["frac"], "traffic": traffic,
+            "kernel": dk, "launches_per_step": d["launches_per_step"],
+            "avg_launch_ms": d["ms_per_step"] / d["launches_per_step"], "work_per_launch": d["work_per_launch"],
+            "path": {"mixed": {"bound_us": 1e6 * bound32, "binding": "hbm" if Q / HBM_PEAK > F / FP32_PEAK
+                               else "mfma_fp32", "measured_us": 1e6 * t32, "frac": bound32 / t32},
+                     "fp64": {"bound_us": 1e6 * bound64, "binding": "hbm" if Q / HBM_PEAK > F / FP64_PEAK
+                              else "mfma_fp64", "measured_us": 1e6 * t64, "frac": bound64 / t64}}}
     out = {
         "metric": "analysis+synthesis maps/s at L=512,N=8: mixed-precision vs FP64, err<=1e-6",
         "value": maps_s,
@@ -304,10 +371,14 @@ def main():
                                "fp64": {k: v[1] / K for k, v in st64.items()}},
         "algorithmic": {"flop_per_map": F, "bytes_per_map": Q},
         "roofline": roof,
-        "kernels_mixed": kernels,
+        "kernels_mixed": kernels32,
+        "kernels_fp64": kernels64,
+        "plan_mixed": {"splits": info32["splits"], "spec_elems": info32["spec_elems"]},
     }
+    if not args.no_c3:
+        out["c3_batch"] = c3_batch(s2, ops, torch, max(2, args.steps // 4), 1)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg, flm_h)
+        out["cpu_baseline"] = cpu_baseline(cfg, flm_h, args.cpu_reps)
     print(json.dumps(out))
 
 

@@ -411,6 +411,14 @@ int s2let_hip_analysis_lm2wav_noalloc(void *f_wav, void *f_scal, const void *flm
  * 0 on success. */
 int s2let_hip_autotune(const s2let_parameters_t *parameters, int mode, void *stream);
 
+/* Describes the plan for these parameters and mode (building it if needed) as a
+ * JSON object: band-limit, bytes per intermediate complex value, the forward
+ * contraction's m' split count in use, the scratch spectra size, the column groups
+ * [L, columns, gamma samples] (scaling function first), super-group count and
+ * scratch bytes.  Returns the text's length (the text is cut to len - 1 bytes) or
+ * < 0 on error. */
+int s2let_hip_plan_info(const s2let_parameters_t *parameters, int mode, char *buf, int len);
+
 /* Builds (or fetches) the cached plan for these parameters on the current
  * device; returns 0 on success.  Plans hold the tiling, Delta tables and DFT
  * twiddles; they are built on first use by the calls above as well. */

@@ -132,6 +132,7 @@ _SIGS["s2let_hip_batch"] = (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [ctypes
 _SIGS["s2let_hip_batch_graph"] = _SIGS["s2let_hip_batch"]
 _SIGS["s2let_hip_autotune"] = (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p])
 _SIGS["s2let_hip_analysis_lm2wav_noalloc"] = _SIGS["s2let_hip_analysis_lm2wav"]
+_SIGS["s2let_hip_plan_info"] = (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int])
 OP_ANALYSIS_LM2WAV, OP_SYNTHESIS_WAV2LM, OP_ANALYSIS_ADJOINT_WAV2LM, OP_SYNTHESIS_ADJOINT_LM2WAV = 0, 1, 2, 3
 
 
@@ -657,6 +658,16 @@ def timing_report():
     if n < 0:
         check(n)
     return {k: (v[0], v[1]) for k, v in json.loads(buf.value.decode()).items()}
+
+
+def plan_info(p, mode=FP64):
+    """The plan's layout and launch choices (s2let_hip_plan_info) as a dict."""
+    import json
+    buf = ctypes.create_string_buffer(1 << 16)
+    n = lib().s2let_hip_plan_info(_ref(p), int(mode), buf, len(buf))
+    if n < 0:
+        check(n)
+    return json.loads(buf.value.decode())
 
 
 def prepare(p, mode=FP64):

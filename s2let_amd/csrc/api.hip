@@ -611,6 +611,30 @@ int s2let_hip_autotune(const s2let_parameters_t *p, int mode, void *stream) {
   });
 }
 
+int s2let_hip_plan_info(const s2let_parameters_t *p, int mode, char *buf, int len) {
+  std::string r;
+  int rc = guarded([&] {
+    Context &C = Context::get();
+    std::lock_guard<std::mutex> lk(C.mu);
+    Plan &P = C.plan(p, mode, C.stream());
+    const int esz = mode == S2LET_HIP_MIXED ? 4 : 8;
+    std::ostringstream o;
+    o << "{\"L\": " << P.p.L << ", \"mode\": " << mode << ", \"complex_bytes\": " << 2 * esz
+      << ", \"splits\": " << C.splits(P.p.L, esz) << ", \"spec_elems\": " << P.spec_elems
+      << ", \"n_wav_used\": " << P.n_wav_used << ", \"groups\": [";
+    for (size_t k = 0; k < P.groups.size(); ++k)
+      o << (k ? ", " : "") << "[" << P.groups[k].L << ", " << P.groups[k].ncol << ", " << P.groups[k].ngamma << "]";
+    o << "], \"super_groups\": " << P.supers.size() << ", \"scratch_bytes\": " << Context::scratch_bytes(P) << "}";
+    r = o.str();
+  });
+  if (rc) return rc;
+  if (buf && len > 0) {
+    strncpy(buf, r.c_str(), (size_t)len - 1);
+    buf[len - 1] = 0;
+  }
+  return (int)r.size();
+}
+
 int s2let_hip_batch(int op, int nmaps, void *const *out_a, void *const *out_b, const void *const *in_a,
                     const void *const *in_b, const s2let_parameters_t *params, int mode, void *stream) {
   for (int k = 0; k < nmaps; ++k) {
