@@ -333,45 +333,53 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt, int 
 }
 
 // ------------------------------------------------------------------ forward
-// Block: 4 waves; wave w takes the l-block pair (p, nb-1-p), p = 4 blockIdx.y + w,
-// for the |m| block 16 blockIdx.x, over the m' range of split blockIdx.z.  The
-// J tiles of MC consecutive m' (16 columns x 16 |m| x {+m re, +m im, -m re,
-// -m im}) are staged in LDS.  Partial sums over the m' split go to
-// ypart + z L^2 (reduced by k_sum_parts).
+// Block: 8 waves for the |m| block 16 blockIdx.x and the m' range [mps, mpe] of split
+// blockIdx.z.  Only l-blocks lb >= lb_lo = max(|m| block, mps / 16) have m' <= l in the
+// split; wave w of block y takes lb = lb_lo + 8 y + w, so that the waves of a block have
+// (nearly) the same m' range, and blocks past the last l-block exit.  The J tiles of MC
+// consecutive m' (16 columns x 16 |m| x {+m re, +m im, -m re, -m im}) are staged in LDS
+// once per block and read by all eight waves.  One l-block per wave keeps the FP64 sums
+// at 32 registers.  Partial sums of split z go to ypart + z L^2 for the l-blocks from
+// lb_lo on; k_sum_parts adds, for each l, the splits that start at or below l's l-block
+// end (fwd_split_covers).
 template <typename T>
 struct FwdTile {
-  static constexpr int MC = sizeof(T) == 8 ? 4 : 8;
-  static constexpr int PD = sizeof(T) == 8 ? 2 : 4;  // m' records in flight per l-block
+  static constexpr int MC = sizeof(T) == 8 ? 4 : 8;  // m' per staged chunk
+  static constexpr int PD = sizeof(T) == 8 ? 1 : 2;  // m' records in flight (FP64: one, for registers)
+  static constexpr int WAVES = 8;
 };
 
+// split z (m' from z * chunk) holds partial sums for row l iff it starts at or below the
+// last m' of l's l-block
+__host__ __device__ inline bool fwd_split_covers(int z, int chunk, int l) { return z * chunk <= 16 * (l / 16) + 15; }
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd3(const SuperDev *__restrict__ sgs, int nsg, const ColDev *__restrict__ cols,
+__global__ __launch_bounds__(512) void k_fwd3(const SuperDev *__restrict__ sgs, int nsg, const ColDev *__restrict__ cols,
                                               const T *const *__restrict__ Efs, const T *__restrict__ dt, int L,
                                               const cx<T> *__restrict__ Jbase, cx<double> *__restrict__ ypart,
                                               int mpchunk) {
-  constexpr int MC = FwdTile<T>::MC;
+  constexpr int MC = FwdTile<T>::MC, PD = FwdTile<T>::PD;
   __shared__ T sJ[MC][16][16][4];  // [m'][column][|m|][variant]
   typedef typename Mfma<T>::acc acc_t;
   cx<double> *__restrict__ y = ypart + (size_t)blockIdx.z * L * L;
   const int mps = blockIdx.z * mpchunk;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   const int r16 = lane & 15, k4 = lane >> 4;
   const int mbi = blockIdx.x, mb = 16 * mbi;
   const int nb = (L + 15) / 16;
-  const int npairs = (nb + 1) / 2;
-  const int pair = blockIdx.y * 4 + wave;
-  const bool active = pair < npairs;
-  int lb[2];
-  lb[0] = pair;
-  lb[1] = nb - 1 - pair;
-  const int nblk = active ? ((lb[1] != lb[0]) ? 2 : 1) : 0;
-  double acc[2][4][4];
+  const int lb_lo = max(mbi, mps / 16);
+  if (lb_lo + 8 * (int)blockIdx.y >= nb || mps >= L) return;  // uniform: no l-block of this block
+  const int lbw = lb_lo + 8 * blockIdx.y + wave;
+  const bool active = lbw < nb;
+  const int l0 = 16 * lbw;
+  double acc[4][4];
 #pragma unroll
-  for (int q = 0; q < 2; ++q)
+  for (int u = 0; u < 4; ++u)
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc[q][u][v] = 0.0;
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+  // J staging: thread tid moves |m| = mb + tid % 16, sign (tid / 16) % 2, column tid / 32,
+  // for the MC m' of a chunk
+  const int ml = tid & 15, pm = (tid >> 4) & 1, cc = tid >> 5, mm = mb + ml;
   for (int g = 0; g < nsg; ++g) {
     const SuperDev S = sgs[g];
     const int Lg = S.L, Mg = 2 * Lg - 1, nc = S.ncol;
@@ -379,139 +387,126 @@ __global__ __launch_bounds__(256) void k_fwd3(const SuperDev *__restrict__ sgs, 
     const ColDev *gc = cols + S.col0;
     const T *__restrict__ Ef = Efs[g];
     const int mpe = min(Lg - 1, mps + mpchunk - 1);
-    // J staging: thread tid moves elements e = tid + 256 u of a chunk (|m| = mb + e % 16,
-    // sign (e / 16) % 2, column (e / 32) % 16 = tid / 32 + 8 (u % 2), m' = mp0 + e / 512)
-    constexpr int NU = MC * 2;
-    const int ml = threadIdx.x & 15, pm = (threadIdx.x >> 4) & 1, mm = mb + ml;
-    // element offsets of this thread's two columns at m' = 0 (invalid slots read column 0,
-    // m = 0, and are zeroed): every load is unconditional, so the wait counts stay exact
-    long long coff[2];
-    bool cok[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int cc = (threadIdx.x >> 5) + 8 * h;
-      cok[h] = cc < nc && mm < Lg;
-      coff[h] = cok[h] ? gc[cc].out_off + (Lg - 1) + (pm ? -mm : mm) : gc[0].out_off + (Lg - 1);
-    }
-    cx<T> jr[NU];
+    // this thread's element offset at m' = 0 (an invalid slot reads column 0, m = 0, and
+    // is zeroed): every load is unconditional, so the wait counts stay exact
+    const bool cok = cc < nc && mm < Lg;
+    const long long coff = cok ? gc[cc].out_off + (Lg - 1) + (pm ? -mm : mm) : gc[0].out_off + (Lg - 1);
+    cx<T> jr[MC];
     auto fetch = [&](int mp0) {
 #pragma unroll
-      for (int u = 0; u < NU; ++u) {
-        const int mpp = mp0 + (u >> 1);
-        const cx<T> v = Jbase[coff[u & 1] + (long long)min(mpp, Lg - 1) * Mg];
-        jr[u] = (cok[u & 1] && mpp < Lg) ? v : mk<T>(0, 0);
+      for (int u = 0; u < MC; ++u) {
+        const int mpp = mp0 + u;
+        const cx<T> v = Jbase[coff + (long long)min(mpp, Lg - 1) * Mg];
+        jr[u] = (cok && mpp < Lg) ? v : mk<T>(0, 0);
       }
     };
+    const bool run_g = active && l0 < Lg;
+    const int nmp = 16 * (lbw + 1);
     fetch(mps);
     for (int mp0 = mps; mp0 <= mpe; mp0 += MC) {
       __syncthreads();
 #pragma unroll
-      for (int u = 0; u < NU; ++u) {
-        const int cc = (threadIdx.x >> 5) + 8 * (u & 1), mc = u >> 1;
-        sJ[mc][cc][ml][2 * pm] = jr[u].x;
-        sJ[mc][cc][ml][2 * pm + 1] = jr[u].y;
+      for (int u = 0; u < MC; ++u) {
+        sJ[u][cc][ml][2 * pm] = jr[u].x;
+        sJ[u][cc][ml][2 * pm + 1] = jr[u].y;
       }
       __syncthreads();
+      // m' <= l0 + 15 and within the split
+      const int n = min(MC, min(l0 + 16, mpe + 1) - mp0);
+      const bool run = run_g && n > 0;
+      const T *dq = dt + dtile_base(lbw) + ((size_t)mbi * nmp + mp0) * 256 + 4 * lane;
+      const T *eq = Ef + ((size_t)lbw * Lg + mp0) * 256 + 4 * lane;
+      // the Delta / weight records of the next PD m' are in flight (ring of registers;
+      // past the chunk's last m' the loads repeat it)
+      V4<T> dr[PD], er[PD];
+      if (run) {
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int lbi = lb[q], l0 = 16 * lbi;
-        const int nmp = 16 * (lbi + 1);
-        const int n = min(MC, min(l0 + 16, mpe + 1) - mp0);  // m' <= l0 + 15
-        // skip: no l < Lg, all |m| > l, or no m' <= l in this chunk
-        const bool run = q < nblk && l0 < Lg && lbi >= mbi && n > 0;
-        const T *dq = dt + dtile_base(lbi) + ((size_t)mbi * nmp + mp0) * 256 + 4 * lane;
-        const T *eq = Ef + ((size_t)lbi * Lg + mp0) * 256 + 4 * lane;
-        const int par = (l0 + k4 + mp0) & 1;
-        // the Delta / weight records of the next kFwdPD m' are in flight (ring of registers;
-        // past the chunk's last m' the loads repeat it)
-        constexpr int kFwdPD = FwdTile<T>::PD;
-        V4<T> dr[kFwdPD], er[kFwdPD];
-        if (run) {
+        for (int k = 0; k < PD; ++k) {
+          const size_t o = (size_t)min(k, n - 1) * 256;
+          dr[k] = ld4(dq + o);
+          er[k] = ld4(eq + o);
+        }
+      }
+      // the next chunk's J is issued behind the first records, so that waiting for those
+      // does not wait for it (loads complete in order); the last chunk's fetch is never stored
+      fetch(mp0 + MC);
+      if (!run) continue;
+      const int par = (l0 + k4 + mp0) & 1;
+      // FP32: the chunk's Delta-weighted sums in FP32, added to the FP64 sums once per chunk
+      // (the chunks are the same MC-aligned m' ranges for every split); FP64: straight into
+      // the FP64 sums
+      constexpr bool part = sizeof(T) == 4;
+      T pacc[part ? 4 : 1][4];
 #pragma unroll
-          for (int k = 0; k < kFwdPD; ++k) {
-            const size_t o = (size_t)min(k, n - 1) * 256;
-            dr[k] = ld4(dq + o);
-            er[k] = ld4(eq + o);
+      for (int u = 0; u < (part ? 4 : 1); ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) pacc[u][v] = (T)0;
+#pragma unroll
+      for (int mc = 0; mc < MC; ++mc) {
+        if (mc >= n) break;
+        const V4<T> dv = dr[mc % PD], ev = er[mc % PD];
+        acc_t h[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) h[u] = Mfma<T>::zero();
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const T *bj = &sJ[mc][4 * s + k4][r16][0];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) h[u] = Mfma<T>::mma(ev.v[s], bj[u], h[u]);
+        }
+        if (mc + PD < MC) {
+          const size_t o = (size_t)min(mc + PD, n - 1) * 256;
+          dr[mc % PD] = ld4(dq + o);
+          er[mc % PD] = ld4(eq + o);
+        }
+        const bool odd = (par + mc) & 1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const T d = dv.v[v], dn = odd ? -d : d;
+          if constexpr (part) {
+            pacc[0][v] += d * h[0][v];
+            pacc[1][v] += d * h[1][v];
+            pacc[2][v] += dn * h[2][v];
+            pacc[3][v] += dn * h[3][v];
+          } else {
+            acc[0][v] += d * h[0][v];
+            acc[1][v] += d * h[1][v];
+            acc[2][v] += dn * h[2][v];
+            acc[3][v] += dn * h[3][v];
           }
         }
-        // the next chunk's J is issued behind the first records, so that waiting for
-        // those does not wait for it (loads complete in order); the last chunk's
-        // fetch is never stored
-        if (q == 0) fetch(mp0 + MC);
-        if (!run) continue;
-        // FP32: the chunk's Delta-weighted sums in FP32, added to the FP64 sums once per
-        // chunk (the chunks are the same MC-aligned m' ranges for every split of the m'
-        // range); FP64: straight into the FP64 sums
-        constexpr bool part = sizeof(T) == 4;
-        T pacc[4][4];
+      }
+      if constexpr (part) {
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) pacc[u][v] = (T)0;
-#pragma unroll
-        for (int mc = 0; mc < MC; ++mc) {
-          if (mc >= n) break;
-          const V4<T> dv = dr[mc % kFwdPD], ev = er[mc % kFwdPD];
-          acc_t h[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) h[u] = Mfma<T>::zero();
-#pragma unroll
-          for (int s = 0; s < 4; ++s) {
-            const T *bj = &sJ[mc][4 * s + k4][r16][0];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) h[u] = Mfma<T>::mma(ev.v[s], bj[u], h[u]);
-          }
-          const bool odd = (par + mc) & 1;
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const T d = dv.v[v], dn = odd ? -d : d;
-            if constexpr (part) {
-              pacc[0][v] += d * h[0][v];
-              pacc[1][v] += d * h[1][v];
-              pacc[2][v] += dn * h[2][v];
-              pacc[3][v] += dn * h[3][v];
-            } else {
-              acc[q][0][v] += d * h[0][v];
-              acc[q][1][v] += d * h[1][v];
-              acc[q][2][v] += dn * h[2][v];
-              acc[q][3][v] += dn * h[3][v];
-            }
-          }
-          if (mc + kFwdPD < MC) {
-            const size_t o = (size_t)min(mc + kFwdPD, n - 1) * 256;
-            dr[mc % kFwdPD] = ld4(dq + o);
-            er[mc % kFwdPD] = ld4(eq + o);
-          }
-        }
-        if constexpr (part) {
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[q][u][v] += (double)pacc[u][v];
-        }
+          for (int v = 0; v < 4; ++v) acc[u][v] += (double)pacc[u][v];
       }
     }
   }
-  // write: lane holds l = 16 lb[q] + k4 + 4 v, |m| = mb + r16
-  const int mm = mb + r16;
+  // write: lane holds l = l0 + k4 + 4 v, |m| = mb + r16 (every (l, m <= l) that split z
+  // covers is written by exactly one of its waves)
+  if (!active) return;
+  const int mo = mb + r16;
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    if (q >= nblk) continue;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int l = 16 * lb[q] + k4 + 4 * v;
-      if (l >= L || mm > l) continue;
-      cx<double> *yl = y + (size_t)l * l + l;
-      yl[mm] = mk<double>(acc[q][0][v], acc[q][1][v]);
-      if (mm > 0) yl[-mm] = mk<double>(acc[q][2][v], acc[q][3][v]);
-    }
+  for (int v = 0; v < 4; ++v) {
+    const int l = l0 + k4 + 4 * v;
+    if (l >= L || mo > l) continue;
+    cx<double> *yl = y + (size_t)l * l + l;
+    yl[mo] = mk<double>(acc[0][v], acc[1][v]);
+    if (mo > 0) yl[-mo] = mk<double>(acc[2][v], acc[3][v]);
   }
 }
 
-__global__ void k_sum_parts(const cx<double> *__restrict__ part, int nsplit, size_t n, cx<double> *__restrict__ y) {
+__global__ void k_sum_parts(const cx<double> *__restrict__ part, int nsplit, int chunk, size_t n,
+                            cx<double> *__restrict__ y) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    int l = (int)sqrtf((float)i);  // i = l^2 + l + m
+    while ((size_t)(l + 1) * (l + 1) <= i) ++l;
+    while ((size_t)l * l > i) --l;
     double a = 0, b = 0;
     for (int s = 0; s < nsplit; ++s) {
+      if (!fwd_split_covers(s, chunk, l)) break;
       const cx<double> v = part[(size_t)s * n + i];
       a += v.x;
       b += v.y;
@@ -572,26 +567,30 @@ int fwd_splits_default(int L) { return std::max(1, std::min(8, L / 64)); }
 // first |m| block, set the launch's time), 8 above (partials of 8 L^2 complex128)
 int fwd_splits_max(int L) { return L <= 1024 ? kMaxSplits : 8; }
 
+int fwd_chunk(int L, int S, int esz) {
+  const int MC = esz == 8 ? FwdTile<double>::MC : FwdTile<float>::MC;
+  const int chunk = (L + S - 1) / S;
+  return (chunk + MC - 1) / MC * MC;
+}
+
 template <typename T>
 void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *const *Efs, const T *dt, int L,
                      const void *Jbase, void *y, void *parts, int S, hipStream_t st) {
   const int nb = (L + 15) / 16;
-  const int npairs = (nb + 1) / 2;
-  constexpr int MC = FwdTile<T>::MC;
-  int chunk = (L + S - 1) / S;
-  chunk = (chunk + MC - 1) / MC * MC;
-  dim3 grid(nb, (npairs + 3) / 4, S);
+  const int chunk = fwd_chunk(L, S, (int)sizeof(T));
+  dim3 grid(nb, (nb + FwdTile<T>::WAVES - 1) / FwdTile<T>::WAVES, S);
   cx<double> *out = (S == 1) ? (cx<double> *)y : (cx<double> *)parts;
-  hipLaunchKernelGGL((k_fwd3<T>), grid, dim3(256), 0, st, sgs, nsg, cols, Efs, dt, L, (const cx<T> *)Jbase, out, chunk);
+  hipLaunchKernelGGL((k_fwd3<T>), grid, dim3(64 * FwdTile<T>::WAVES), 0, st, sgs, nsg, cols, Efs, dt, L,
+                     (const cx<T> *)Jbase, out, chunk);
 }
 template void launch_fwd_mfma<double>(const SuperDev *, int, const ColDev *, const double *const *, const double *, int, const void *, void *, void *, int, hipStream_t);
 template void launch_fwd_mfma<float>(const SuperDev *, int, const ColDev *, const float *const *, const float *, int, const void *, void *, void *, int, hipStream_t);
 
-void launch_sum_parts(int L, int S, const void *parts, void *y, hipStream_t st) {
+void launch_sum_parts(int L, int S, int esz, const void *parts, void *y, hipStream_t st) {
   if (S <= 1) return;
   const size_t n = (size_t)L * L;
   hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0, st,
-                     (const cx<double> *)parts, S, n, (cx<double> *)y);
+                     (const cx<double> *)parts, S, fwd_chunk(L, S, esz), n, (cx<double> *)y);
 }
 
 int mfma_selftest() {

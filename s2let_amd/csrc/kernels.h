@@ -129,7 +129,9 @@ void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *
 constexpr int kMaxSplits = 16;
 int fwd_splits_default(int L);
 int fwd_splits_max(int L);
-void launch_sum_parts(int L, int splits, const void *parts, void *y, hipStream_t st);  // y = sum of the partials
+// y = sum of the partials (esz: bytes of the forward contraction's real type, which sets its m' chunks)
+void launch_sum_parts(int L, int splits, int esz, const void *parts, void *y, hipStream_t st);
+int fwd_chunk(int L, int splits, int esz);  // m' per split (a multiple of the staging chunk)
 int mfma_selftest();
 
 // Grouped launches: one launch runs the jobs of several column groups (each job

@@ -241,7 +241,7 @@ void contract_fwd_all(Plan &P, const Scratch &X, const ColDev *cols, const void 
   }
   if (C.splits(P.p.L, (int)sizeof(T)) > 1) {
     Stage s_("contract_fwd_sum", st);
-    launch_sum_parts(P.p.L, C.splits(P.p.L, (int)sizeof(T)), X.parts, flm, st);
+    launch_sum_parts(P.p.L, C.splits(P.p.L, (int)sizeof(T)), (int)sizeof(T), X.parts, flm, st);
   }
 }
 
@@ -430,7 +430,7 @@ void sht_forward_t(ShtPlan &S, const void *f, void *flm, hipStream_t st) {
   { Stage s_("rows", st); rows_one<double, T, T>(L, f, S.spec.p, 0, 1, -1, kTwoPi / M, S.ss, st); }
   { Stage s_("beta_fwd", st); theta_one<T>(L, col, 2, S.spec.p, S.work.p, S.ss, st); }
   { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, C.splits(L, (int)sizeof(T)), st); }
-  if (C.splits(L, (int)sizeof(T)) > 1) { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, C.splits(L, (int)sizeof(T)), S.parts.p, flm, st); }
+  if (C.splits(L, (int)sizeof(T)) > 1) { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, C.splits(L, (int)sizeof(T)), (int)sizeof(T), S.parts.p, flm, st); }
 }
 
 template <typename T>
@@ -441,7 +441,7 @@ void sht_inverse_adjoint_t(ShtPlan &S, const void *f, void *flm, hipStream_t st)
   { Stage s_("rows", st); rows_one<double, T, T>(L, f, S.spec.p, 0, 1, -1, 1.0, S.ss, st); }
   { Stage s_("beta_inv_adj", st); theta_one<T>(L, col, 1, S.spec.p, S.work.p, S.ss, st); }
   { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, C.splits(L, (int)sizeof(T)), st); }
-  if (C.splits(L, (int)sizeof(T)) > 1) { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, C.splits(L, (int)sizeof(T)), S.parts.p, flm, st); }
+  if (C.splits(L, (int)sizeof(T)) > 1) { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, C.splits(L, (int)sizeof(T)), (int)sizeof(T), S.parts.p, flm, st); }
 }
 
 template <typename T>
