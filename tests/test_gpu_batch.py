@@ -102,3 +102,43 @@ def test_autotune_keeps_results(ops):
     torch.cuda.synchronize()
     assert torch.equal(w0, w1) and torch.equal(s0, s1)
     assert rel(r1.cpu().numpy(), r0.cpu().numpy()) < 1e-13
+
+
+def test_batch_c3_mixed_vs_fp64_and_oracle(ops, oracle):
+    """c3 as a batch in both modes: every map's mixed wavelet coefficients within 1e-6
+    of the FP64 batch, and 8 maps drawn from the batch (both spins, every N, L up to
+    256 so that the oracle takes seconds) within 1e-12 of the CPU oracle."""
+    import torch
+    shapes, flms = draw(3, 256, [64, 128, 256, 512], [1, 4, 8], [0, 2])
+    print("batch c3 mixed vs fp64 vs oracle", flush=True)
+    ps64, x, fw64, fs64 = run_batch(ops, shapes, flms, 0, graph=True)
+    _, _, fw32, fs32 = run_batch(ops, shapes, flms, 1, graph=True)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for k in range(len(ps64)):
+        a64 = torch.cat([fw64[k], fs64[k]]).cpu().numpy()
+        a32 = torch.cat([fw32[k], fs32[k]]).cpu().numpy()
+        worst = max(worst, rel(a32, a64))
+    print("c3 worst mixed analysis rel err vs FP64 %.3e" % worst, flush=True)
+    assert worst < 1e-6
+    picked, seen = [], set()
+    for lmax in (128, 256):  # every (N, spin) present, at the smallest band-limits
+        for k, (L, N, s) in enumerate(shapes):
+            if L <= lmax and (N, s) not in seen:
+                seen.add((N, s))
+                picked.append(k)
+    for k, (L, N, s) in enumerate(shapes):
+        if len(picked) >= 8:
+            break
+        if L <= 128 and k not in picked:
+            picked.append(k)
+    assert len(picked) >= 8
+    for k in picked:
+        L, N, s = shapes[k]
+        q = oracle.params(L, N, 2.0, 1, s)
+        ow, os_ = oracle.analysis_lm2wav(flms[k], q)
+        nw = len(ow)
+        e = rel(np.concatenate([fw64[k].cpu().numpy()[:nw], fs64[k].cpu().numpy()]), np.concatenate([ow, os_]))
+        print("map", k, (L, N, s), "FP64 batch vs oracle %.3e" % e, flush=True)
+        assert e < 1e-12
+
