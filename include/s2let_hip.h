@@ -376,17 +376,25 @@ int s2let_hip_mw_map2alm(void *flm, const void *f, int L, int spin, int mode, vo
 int s2let_hip_mw_alm2map_adjoint(void *flm, const void *f, int L, int spin, int mode, void *stream);
 int s2let_hip_mw_map2alm_adjoint(void *f, const void *flm, int L, int spin, int mode, void *stream);
 
-/* Batched maps of uneven shape in one set of launches: op is one of
- * S2LET_HIP_OP_*; map k uses params[k] and the k-th pointers of each array. */
+/* Batched maps of uneven shape: op is one of S2LET_HIP_OP_*; map k uses
+ * params[k] and the k-th pointers of each array (out_b / in_b may be NULL for
+ * ops with one output / input).  The maps that share a parameter set (a
+ * bucket) run together: every stage of the transform is one launch over all
+ * of the bucket's maps, with the bucket's scratch sized for them and the
+ * per-map byte offsets of the caller's arrays uploaded to the device (reused
+ * while the pointers stay the same).  Each map's result is bit for bit the
+ * one the single-map entry point gives.  On a stream under capture the maps
+ * run one by one (use s2let_hip_batch_graph to capture batches). */
 int s2let_hip_batch(int op, int nmaps, void *const *out_a, void *const *out_b, const void *const *in_a,
                     const void *const *in_b, const s2let_parameters_t *params, int mode, void *stream);
 
 /* The same batch replayed as one HIP graph.  On the first call with a given
- * (op, mode, params, pointers) set, the per-map launches are captured on
- * parallel streams (one branch per distinct parameter set, the maps of a
- * branch in order) and instantiated; later calls with the same set replay the
- * graph on `stream`.  Arguments and results as s2let_hip_batch.  Graphs are
- * cached until s2let_hip_release. */
+ * (op, mode, params, pointers) set, the buckets' batched launches are
+ * captured on parallel streams (one branch per bucket) and instantiated;
+ * later calls with the same set replay the graph on `stream`.  Arguments and
+ * results as s2let_hip_batch.  At most 64 graphs are cached (least recently
+ * used evicted first); a graph is dropped when the tables or scratch it reads
+ * are reallocated, and all are dropped by s2let_hip_release. */
 int s2let_hip_batch_graph(int op, int nmaps, void *const *out_a, void *const *out_b, const void *const *in_a,
                           const void *const *in_b, const s2let_parameters_t *params, int mode, void *stream);
 

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <sstream>
 
@@ -91,14 +92,105 @@ struct HostIO {
 
 // ---- batch as a HIP graph
 
-void run_op(Plan &P, const Scratch &X, int op, void *oa, void *ob, const void *ia, const void *ib, hipStream_t st) {
+void run_op(Plan &P, const Scratch &X, int op, void *oa, void *ob, const void *ia, const void *ib, hipStream_t st,
+            const BatchIO &B = BatchIO()) {
   switch (op) {
-    case S2LET_HIP_OP_ANALYSIS_LM2WAV: run_analysis(P, X, ia, oa, ob, st); break;
-    case S2LET_HIP_OP_SYNTHESIS_WAV2LM: run_synthesis(P, X, ia, ib, oa, st); break;
-    case S2LET_HIP_OP_ANALYSIS_ADJOINT_WAV2LM: run_analysis_adjoint(P, X, ia, ib, oa, st); break;
-    case S2LET_HIP_OP_SYNTHESIS_ADJOINT_LM2WAV: run_synthesis_adjoint(P, X, ia, oa, ob, st); break;
+    case S2LET_HIP_OP_ANALYSIS_LM2WAV: run_analysis(P, X, ia, oa, ob, st, B); break;
+    case S2LET_HIP_OP_SYNTHESIS_WAV2LM: run_synthesis(P, X, ia, ib, oa, st, B); break;
+    case S2LET_HIP_OP_ANALYSIS_ADJOINT_WAV2LM: run_analysis_adjoint(P, X, ia, ib, oa, st, B); break;
+    case S2LET_HIP_OP_SYNTHESIS_ADJOINT_LM2WAV: run_synthesis_adjoint(P, X, ia, oa, ob, st, B); break;
     default: set_error(-1, "unknown op"); throw ArgFailure();
   }
+}
+
+// ---- batches: the maps that share a plan (a bucket) run as one batch, every stage one
+// launch over all of them (grouped launches along gridDim.y; the contractions deal
+// (tile, map) items so that the maps of a tile share its Delta records through L2).
+// The batch keeps the plan's launch choices, so each map's result is the one the
+// single-map entry points give, bit for bit.
+struct BatchPtrs {
+  int op;
+  void *const *out_a;
+  void *const *out_b;
+  const void *const *in_a;
+  const void *const *in_b;
+  void *oa(int k) const { return out_a[k]; }
+  void *ob(int k) const { return out_b ? out_b[k] : nullptr; }
+  const void *ia(int k) const { return in_a[k]; }
+  const void *ib(int k) const { return in_b ? in_b[k] : nullptr; }
+};
+struct Bucket {
+  Plan *P;
+  std::string key;
+  std::vector<int> maps;  // in the caller's order
+};
+
+std::vector<Bucket> make_buckets(Context &C, int nmaps, const s2let_parameters_t *params, int mode) {
+  std::vector<Bucket> bs;
+  std::map<std::string, size_t> at;
+  for (int k = 0; k < nmaps; ++k) {
+    require_mw(params + k, "s2let_hip_batch");
+    const std::string pk = plan_key(params + k, mode);
+    auto it = at.find(pk);
+    if (it == at.end()) {
+      at[pk] = bs.size();
+      bs.push_back(Bucket{&C.plan(params + k, mode, C.stream()), pk, {k}});
+    } else {
+      bs[it->second].maps.push_back(k);
+    }
+  }
+  return bs;
+}
+
+// byte offsets of each map's f_lm, f_wav and f_scal from the bucket's first map's (3 n entries)
+std::vector<long long> bucket_offsets(const Bucket &b, const BatchPtrs &q) {
+  const bool lm_in = q.op == S2LET_HIP_OP_ANALYSIS_LM2WAV || q.op == S2LET_HIP_OP_SYNTHESIS_ADJOINT_LM2WAV;
+  auto lm = [&](int k) { return lm_in ? (const char *)q.ia(k) : (const char *)q.oa(k); };
+  auto wav = [&](int k) { return lm_in ? (const char *)q.oa(k) : (const char *)q.ia(k); };
+  auto scal = [&](int k) { return lm_in ? (const char *)q.ob(k) : (const char *)q.ib(k); };
+  const size_t n = b.maps.size();
+  const int k0 = b.maps[0];
+  std::vector<long long> off(3 * n);
+  for (size_t i = 0; i < n; ++i) {
+    const int k = b.maps[i];
+    off[i] = (long long)(lm(k) - lm(k0));
+    off[n + i] = (long long)(wav(k) - wav(k0));
+    off[2 * n + i] = (long long)(scal(k) - scal(k0));
+  }
+  return off;
+}
+
+// scratch of a bucket (allocated before any launch or capture)
+Scratch bucket_scratch(Context &C, const Bucket &b, BatchIO &B) {
+  if (b.maps.size() == 1) return C.scratch(*b.P);
+  return C.batch_scratch(*b.P, (int)b.maps.size(), B);
+}
+
+// launches one bucket on st; dtab: the device copy of bucket_offsets (several maps)
+void run_bucket(Context &C, const Bucket &b, const BatchPtrs &q, const long long *dtab, hipStream_t st) {
+  BatchIO B;
+  const Scratch X = bucket_scratch(C, b, B);
+  const int k0 = b.maps[0], n = (int)b.maps.size();
+  if (n > 1) {
+    B.d_lm = dtab;
+    B.d_wav = dtab + n;
+    B.d_scal = dtab + 2 * n;
+  }
+  run_op(*b.P, X, q.op, q.oa(k0), q.ob(k0), q.ia(k0), q.ib(k0), st, B);
+}
+
+// the plan's device copy of a bucket's offsets, uploaded on st unless it already holds
+// them; st must have acquired the plan (earlier launches that read the copy come first)
+const long long *upload_offsets(Plan &P, std::vector<long long> &&off, hipStream_t st) {
+  const size_t bytes = off.size() * sizeof(long long);
+  if (P.b_dtab.p && P.b_dtab.bytes >= bytes && off == P.dtab_host) return P.b_dtab.as<long long>();
+  if (P.dtab_copied.used) S2H_CHECK_HIP(hipEventSynchronize(P.dtab_copied.done));  // done reading dtab_host
+  P.dtab_host.clear();
+  P.b_dtab.alloc(bytes);
+  P.dtab_host = std::move(off);
+  S2H_CHECK_HIP(hipMemcpyAsync(P.b_dtab.p, P.dtab_host.data(), bytes, hipMemcpyHostToDevice, st));
+  sync_release(P.dtab_copied, st);
+  return P.b_dtab.as<long long>();
 }
 
 // Captured batches.  A graph holds the device pointers of everything its launches
@@ -109,7 +201,8 @@ void run_op(Plan &P, const Scratch &X, int op, void *oa, void *ob, const void *i
 struct GraphEntry {
   hipGraphExec_t exec = nullptr;
   uint64_t last_use = 0;
-  std::vector<Plan *> plans;  // distinct plans whose scratch the graph uses
+  std::vector<Plan *> plans;      // distinct plans whose scratch the graph uses
+  std::unique_ptr<DevBuf> dtab;   // per-map offsets of its buckets of several maps
 };
 struct GraphCache {
   static constexpr size_t kMaxGraphs = 64;
@@ -459,43 +552,49 @@ int s2let_hip_batch_graph(int op, int nmaps, void *const *out_a, void *const *ou
     // caller's stream (which may be the null stream, where capture is not allowed)
     hipStream_t user = (hipStream_t)stream;
     // plans first (building one may reallocate tables and so move the generation), and
-    // their scratch, both outside any capture
-    std::vector<Plan *> plans(nmaps);
-    std::vector<Scratch> xs(nmaps);
-    for (int k = 0; k < nmaps; ++k) {
-      require_mw(params + k, "s2let_hip_batch_graph");
-      plans[k] = &C.plan(params + k, mode, C.stream());
+    // the scratch of every bucket, all outside any capture
+    const BatchPtrs q{op, out_a, out_b, in_a, in_b};
+    std::vector<Bucket> bs = make_buckets(C, nmaps, params, mode);
+    for (const Bucket &b : bs) {
+      BatchIO B;
+      (void)bucket_scratch(C, b, B);
     }
-    for (int k = 0; k < nmaps; ++k) xs[k] = C.scratch(*plans[k]);
     GraphCache &G = graphs();
     G.validate(C.generation());
     std::ostringstream key;
     key << op << ';' << mode << ';' << nmaps;
-    std::map<std::string, std::vector<int>> branch;  // plan -> maps, in order
-    std::vector<std::string> order;
-    for (int k = 0; k < nmaps; ++k) {
-      const std::string pk = plan_key(params + k, mode);
-      key << ';' << pk << '@' << out_a[k] << ',' << (out_b ? out_b[k] : nullptr) << ',' << in_a[k] << ','
-          << (in_b ? in_b[k] : nullptr);
-      if (!branch.count(pk)) order.push_back(pk);
-      branch[pk].push_back(k);
-    }
+    for (int k = 0; k < nmaps; ++k)
+      key << ';' << plan_key(params + k, mode) << '@' << q.oa(k) << ',' << q.ob(k) << ',' << q.ia(k) << ','
+          << q.ib(k);
     hipStream_t origin = G.stream(0);
     auto it = G.exec.find(key.str());
     if (it == G.exec.end()) {
       G.evict_to(GraphCache::kMaxGraphs - 1);
+      GraphEntry e;
+      // the offsets of every bucket of several maps, on the device before the capture
+      std::vector<long long> all;
+      std::vector<size_t> at(bs.size(), 0);
+      for (size_t b = 0; b < bs.size(); ++b) {
+        if (bs[b].maps.size() < 2) continue;
+        at[b] = all.size();
+        const std::vector<long long> off = bucket_offsets(bs[b], q);
+        all.insert(all.end(), off.begin(), off.end());
+      }
+      if (!all.empty()) {
+        e.dtab.reset(new DevBuf());
+        e.dtab->alloc(all.size() * sizeof(long long));
+        S2H_CHECK_HIP(hipMemcpy(e.dtab->p, all.data(), all.size() * sizeof(long long), hipMemcpyHostToDevice));
+      }
       const bool was = timing_pause();
       hipGraph_t graph = nullptr;
       try {
         S2H_CHECK_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeRelaxed));
         hipEvent_t fork = G.event(2);
         S2H_CHECK_HIP(hipEventRecord(fork, origin));
-        for (size_t b = 0; b < order.size(); ++b) {
+        for (size_t b = 0; b < bs.size(); ++b) {  // one branch per bucket
           hipStream_t sb = G.stream(b + 1);
           S2H_CHECK_HIP(hipStreamWaitEvent(sb, fork, 0));
-          for (int k : branch[order[b]])
-            run_op(*plans[k], xs[k], op, out_a[k], out_b ? out_b[k] : nullptr, in_a[k], in_b ? in_b[k] : nullptr,
-                   sb);
+          run_bucket(C, bs[b], q, e.dtab ? e.dtab->as<long long>() + at[b] : nullptr, sb);
           hipEvent_t join = G.event(b + 3);
           S2H_CHECK_HIP(hipEventRecord(join, sb));
           S2H_CHECK_HIP(hipStreamWaitEvent(origin, join, 0));
@@ -509,12 +608,10 @@ int s2let_hip_batch_graph(int op, int nmaps, void *const *out_a, void *const *ou
         throw;
       }
       timing_resume(was);
-      GraphEntry e;
       const hipError_t err = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       S2H_CHECK_HIP(err);
-      for (Plan *q : plans)
-        if (std::find(e.plans.begin(), e.plans.end(), q) == e.plans.end()) e.plans.push_back(q);
+      for (const Bucket &b : bs) e.plans.push_back(b.P);
       it = G.exec.emplace(key.str(), std::move(e)).first;
     }
     it->second.last_use = ++G.clock;
@@ -637,19 +734,39 @@ int s2let_hip_plan_info(const s2let_parameters_t *p, int mode, char *buf, int le
 
 int s2let_hip_batch(int op, int nmaps, void *const *out_a, void *const *out_b, const void *const *in_a,
                     const void *const *in_b, const s2let_parameters_t *params, int mode, void *stream) {
-  for (int k = 0; k < nmaps; ++k) {
-    const s2let_parameters_t *p = params + k;
-    int rc = 0;
-    switch (op) {
-      case S2LET_HIP_OP_ANALYSIS_LM2WAV: rc = s2let_hip_analysis_lm2wav(out_a[k], out_b[k], in_a[k], p, mode, stream); break;
-      case S2LET_HIP_OP_SYNTHESIS_WAV2LM: rc = s2let_hip_synthesis_wav2lm(out_a[k], in_a[k], in_b[k], p, mode, stream); break;
-      case S2LET_HIP_OP_ANALYSIS_ADJOINT_WAV2LM: rc = s2let_hip_analysis_adjoint_wav2lm(out_a[k], in_a[k], in_b[k], p, mode, stream); break;
-      case S2LET_HIP_OP_SYNTHESIS_ADJOINT_LM2WAV: rc = s2let_hip_synthesis_adjoint_lm2wav(out_a[k], out_b[k], in_a[k], p, mode, stream); break;
-      default: set_error(-1, "unknown op"); return -1;
+  return guarded([&] {
+    if (op < 0 || op > 3) {
+      set_error(-1, "unknown op");
+      throw ArgFailure();
     }
-    if (rc) return rc;
-  }
-  return 0;
+    Context &C = Context::get();
+    std::lock_guard<std::mutex> lk(C.mu);
+    hipStream_t st = (hipStream_t)stream;
+    const BatchPtrs q{op, out_a, out_b, in_a, in_b};
+    std::vector<Bucket> bs = make_buckets(C, nmaps, params, mode);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    S2H_CHECK_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) {
+      // captured by the caller: map by map on each plan's own scratch (the offsets upload
+      // of a bucket reads host memory that a later call rewrites)
+      for (const Bucket &b : bs) {
+        const Scratch X = C.scratch(*b.P);
+        for (int k : b.maps) run_op(*b.P, X, op, q.oa(k), q.ob(k), q.ia(k), q.ib(k), st);
+      }
+      S2H_CHECK_HIP(hipGetLastError());
+      return;
+    }
+    for (const Bucket &b : bs) {  // scratch before any launch (growing it frees the old)
+      BatchIO B;
+      (void)bucket_scratch(C, b, B);
+    }
+    for (const Bucket &b : bs) {
+      PlanUse use_(b.P->sync, st);
+      const long long *dtab = b.maps.size() > 1 ? upload_offsets(*b.P, bucket_offsets(b, q), st) : nullptr;
+      run_bucket(C, b, q, dtab, st);
+      S2H_CHECK_HIP(hipGetLastError());
+    }
+  });
 }
 
 // ---------------------------------------------------------------- host API (drop-in)

@@ -141,7 +141,10 @@ size_t xtile_elems(int Lx) { return 2 * (size_t)((Lx + 15) / 16) * ((Lx + 15) / 
 
 // (x is the caller's complex128 f_lm; the FP32 mode rounds it here)
 template <typename T>
-__global__ void k_xtile(const cx<double> *__restrict__ x, int Lx, cx<T> *__restrict__ xt) {
+__global__ void k_xtile(const cx<double> *__restrict__ x0, int Lx, cx<T> *__restrict__ xt0, MapSet ms) {
+  const int mk_ = blockIdx.y;
+  const cx<double> *__restrict__ x = (const cx<double> *)map_ptr(x0, mk_, ms.scr_in, ms.dlt_in);
+  cx<T> *__restrict__ xt = (cx<T> *)map_ptr(xt0, mk_, ms.scr_out, ms.dlt_out);
   const int nx = (Lx + 15) / 16;
   const size_t half = (size_t)nx * nx * 256;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < half; e += (size_t)gridDim.x * blockDim.x) {
@@ -169,16 +172,26 @@ __global__ void k_xtile(const cx<double> *__restrict__ x, int Lx, cx<T> *__restr
 // 1e-6 of FP64 and the 64 FP64 registers go)
 constexpr int kInvF32MaxL = 512;
 template <typename T, int MP, bool F64>
-__global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt, int nx, const T *__restrict__ dt,
-                                              cx<T> *__restrict__ Fbase, InvJobs<T> jobs) {
+__global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt0, int nx, const T *__restrict__ dt,
+                                              cx<T> *__restrict__ Fbase0, InvJobs<T> jobs, MapSet ms) {
   typedef typename Mfma<T>::acc acc_t;
+  // (block, map) of this block: maps fastest, dealt across the XCDs so that the maps of
+  // one tile read its Delta and weight records from one L2
+  int blk = blockIdx.x, mk_ = 0;
+  if (ms.n > 1) {
+    const int w = xcd_item(blockIdx.y * gridDim.x + blockIdx.x, jobs.nblk * ms.n);
+    blk = w / ms.n;
+    mk_ = w % ms.n;
+  }
+  const cx<T> *__restrict__ xt = (const cx<T> *)map_ptr(xt0, mk_, ms.scr_in, ms.dlt_in);
+  cx<T> *__restrict__ Fbase = (cx<T> *)map_ptr(Fbase0, mk_, ms.scr_out, ms.dlt_out);
   int jb = 0;
-  while (jb + 1 < jobs.n && (int)blockIdx.x >= jobs.j[jb + 1].blk0) ++jb;
+  while (jb + 1 < jobs.n && blk >= jobs.j[jb + 1].blk0) ++jb;
   const InvJob<T> &J = jobs.j[jb];
   const T *__restrict__ Ei = J.Ei;
   const ColDev *__restrict__ cols = J.cols;
   const int nc = J.nc, lmin = J.lmin, L = J.L;
-  const int b = blockIdx.x - J.blk0, bx = b % J.gx, by = b / J.gx;
+  const int b = blk - J.blk0, bx = b % J.gx, by = b / J.gx;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int r16 = lane & 15, k4 = lane >> 4;
   const int mp0 = by * MP;
@@ -356,20 +369,34 @@ __host__ __device__ inline bool fwd_split_covers(int z, int chunk, int l) { retu
 template <typename T>
 __global__ __launch_bounds__(512) void k_fwd3(const SuperDev *__restrict__ sgs, int nsg, const ColDev *__restrict__ cols,
                                               const T *const *__restrict__ Efs, const T *__restrict__ dt, int L,
-                                              const cx<T> *__restrict__ Jbase, cx<double> *__restrict__ ypart,
-                                              int mpchunk) {
+                                              const cx<T> *__restrict__ Jbase0, cx<double> *__restrict__ ypart0,
+                                              int mpchunk, MapSet ms) {
   constexpr int MC = FwdTile<T>::MC, PD = FwdTile<T>::PD;
   __shared__ T sJ[MC][16][16][4];  // [m'][column][|m|][variant]
   typedef typename Mfma<T>::acc acc_t;
-  cx<double> *__restrict__ y = ypart + (size_t)blockIdx.z * L * L;
-  const int mps = blockIdx.z * mpchunk;
+  // (|m| block, l-block group, split, map): the grid is nb x ngroups x (splits * maps);
+  // with several maps, maps fastest, dealt across the XCDs so that the maps of one tile
+  // read its Delta and weight records from one L2
+  int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z, mk_ = 0;
+  if (ms.n > 1) {
+    const int T_ = gridDim.x * gridDim.y * gridDim.z;
+    const int w = xcd_item((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x, T_);
+    mk_ = w % ms.n;
+    const int t = w / ms.n;
+    bx = t % gridDim.x;
+    by = (t / gridDim.x) % gridDim.y;
+    bz = t / (gridDim.x * gridDim.y);
+  }
+  const cx<T> *__restrict__ Jbase = (const cx<T> *)map_ptr(Jbase0, mk_, ms.scr_in, ms.dlt_in);
+  cx<double> *__restrict__ y = (cx<double> *)map_ptr(ypart0, mk_, ms.scr_out, ms.dlt_out) + (size_t)bz * L * L;
+  const int mps = bz * mpchunk;
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   const int r16 = lane & 15, k4 = lane >> 4;
-  const int mbi = blockIdx.x, mb = 16 * mbi;
+  const int mbi = bx, mb = 16 * mbi;
   const int nb = (L + 15) / 16;
   const int lb_lo = max(mbi, mps / 16);
-  if (lb_lo + 8 * (int)blockIdx.y >= nb || mps >= L) return;  // uniform: no l-block of this block
-  const int lbw = lb_lo + 8 * blockIdx.y + wave;
+  if (lb_lo + 8 * by >= nb || mps >= L) return;  // uniform: no l-block of this block
+  const int lbw = lb_lo + 8 * by + wave;
   const bool active = lbw < nb;
   const int l0 = 16 * lbw;
   double acc[4][4];
@@ -498,8 +525,10 @@ __global__ __launch_bounds__(512) void k_fwd3(const SuperDev *__restrict__ sgs, 
   }
 }
 
-__global__ void k_sum_parts(const cx<double> *__restrict__ part, int nsplit, int chunk, size_t n,
-                            cx<double> *__restrict__ y) {
+__global__ void k_sum_parts(const cx<double> *__restrict__ part0, int nsplit, int chunk, size_t n,
+                            cx<double> *__restrict__ y0, MapSet ms) {
+  const cx<double> *__restrict__ part = (const cx<double> *)map_ptr(part0, blockIdx.y, ms.scr_in, ms.dlt_in);
+  cx<double> *__restrict__ y = (cx<double> *)map_ptr(y0, blockIdx.y, ms.scr_out, ms.dlt_out);
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     int l = (int)sqrtf((float)i);  // i = l^2 + l + m
     while ((size_t)(l + 1) * (l + 1) <= i) ++l;
@@ -537,30 +566,33 @@ template void build_etab<double>(const double *, const ColDev *, int, const doub
 template void build_etab<float>(const double *, const ColDev *, int, const double *, int, float *, float *, hipStream_t);
 
 template <typename T>
-void launch_xtile(const void *x, int Lx, void *xtile, hipStream_t st) {
+void launch_xtile(const void *x, int Lx, void *xtile, const MapSet &ms, hipStream_t st) {
   const size_t half = xtile_elems(Lx) / 2;
   const unsigned g = (unsigned)std::min<size_t>((half + 255) / 256, 65535 * 4);
-  hipLaunchKernelGGL((k_xtile<T>), dim3(g), dim3(256), 0, st, (const cx<double> *)x, Lx, (cx<T> *)xtile);
+  hipLaunchKernelGGL((k_xtile<T>), dim3(g, ms.n), dim3(256), 0, st, (const cx<double> *)x, Lx, (cx<T> *)xtile, ms);
 }
-template void launch_xtile<double>(const void *, int, void *, hipStream_t);
-template void launch_xtile<float>(const void *, int, void *, hipStream_t);
+template void launch_xtile<double>(const void *, int, void *, const MapSet &, hipStream_t);
+template void launch_xtile<float>(const void *, int, void *, const MapSet &, hipStream_t);
 
 template <typename T>
-void launch_inv_mfma(const void *xtile, int Lx, const T *dt, void *Fbase, const InvJobs<T> &jobs, hipStream_t st) {
+void launch_inv_mfma(const void *xtile, int Lx, const T *dt, void *Fbase, const InvJobs<T> &jobs, const MapSet &ms,
+                     hipStream_t st) {
   if (jobs.n == 0 || jobs.nblk == 0) return;
   int Lmax = 0;
   for (int k = 0; k < jobs.n; ++k) Lmax = std::max(Lmax, jobs.j[k].L);
   const int nx = (Lx + 15) / 16;
+  const dim3 grid(jobs.nblk, ms.n);
   if (sizeof(T) == 4 && Lmax <= kInvF32MaxL)
-    hipLaunchKernelGGL((k_inv3<T, kInvMP, false>), dim3(jobs.nblk), dim3(256), 0, st, (const cx<T> *)xtile, nx, dt,
-                       (cx<T> *)Fbase, jobs);
+    hipLaunchKernelGGL((k_inv3<T, kInvMP, false>), grid, dim3(256), 0, st, (const cx<T> *)xtile, nx, dt,
+                       (cx<T> *)Fbase, jobs, ms);
   else
-    hipLaunchKernelGGL((k_inv3<T, kInvMP, true>), dim3(jobs.nblk), dim3(256), 0, st, (const cx<T> *)xtile, nx, dt,
-                       (cx<T> *)Fbase, jobs);
+    hipLaunchKernelGGL((k_inv3<T, kInvMP, true>), grid, dim3(256), 0, st, (const cx<T> *)xtile, nx, dt,
+                       (cx<T> *)Fbase, jobs, ms);
 }
 template void launch_inv_mfma<double>(const void *, int, const double *, void *, const InvJobs<double> &,
-                                      hipStream_t);
-template void launch_inv_mfma<float>(const void *, int, const float *, void *, const InvJobs<float> &, hipStream_t);
+                                      const MapSet &, hipStream_t);
+template void launch_inv_mfma<float>(const void *, int, const float *, void *, const InvJobs<float> &,
+                                     const MapSet &, hipStream_t);
 
 int fwd_splits_default(int L) { return std::max(1, std::min(8, L / 64)); }
 // 16 splits up to L = 1024 (shorter m' ranges per block: the longest waves, those of the
@@ -573,24 +605,36 @@ int fwd_chunk(int L, int S, int esz) {
   return (chunk + MC - 1) / MC * MC;
 }
 
+// ms: input side the J spectra (scratch); output side the caller's flm (dlt_out) when
+// there is one split, else the split partials (scratch stride scr_out)
 template <typename T>
 void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *const *Efs, const T *dt, int L,
-                     const void *Jbase, void *y, void *parts, int S, hipStream_t st) {
+                     const void *Jbase, void *y, void *parts, int S, const MapSet &ms, hipStream_t st) {
   const int nb = (L + 15) / 16;
   const int chunk = fwd_chunk(L, S, (int)sizeof(T));
-  dim3 grid(nb, (nb + FwdTile<T>::WAVES - 1) / FwdTile<T>::WAVES, S);
-  cx<double> *out = (S == 1) ? (cx<double> *)y : (cx<double> *)parts;
+  dim3 grid(nb, (nb + FwdTile<T>::WAVES - 1) / FwdTile<T>::WAVES, S * ms.n);
+  MapSet mo = ms;
+  cx<double> *out;
+  if (S == 1) {
+    out = (cx<double> *)y;
+  } else {
+    out = (cx<double> *)parts;
+    mo.dlt_out = nullptr;
+  }
   hipLaunchKernelGGL((k_fwd3<T>), grid, dim3(64 * FwdTile<T>::WAVES), 0, st, sgs, nsg, cols, Efs, dt, L,
-                     (const cx<T> *)Jbase, out, chunk);
+                     (const cx<T> *)Jbase, out, chunk, mo);
 }
-template void launch_fwd_mfma<double>(const SuperDev *, int, const ColDev *, const double *const *, const double *, int, const void *, void *, void *, int, hipStream_t);
-template void launch_fwd_mfma<float>(const SuperDev *, int, const ColDev *, const float *const *, const float *, int, const void *, void *, void *, int, hipStream_t);
+template void launch_fwd_mfma<double>(const SuperDev *, int, const ColDev *, const double *const *, const double *, int,
+                                      const void *, void *, void *, int, const MapSet &, hipStream_t);
+template void launch_fwd_mfma<float>(const SuperDev *, int, const ColDev *, const float *const *, const float *, int,
+                                     const void *, void *, void *, int, const MapSet &, hipStream_t);
 
-void launch_sum_parts(int L, int S, int esz, const void *parts, void *y, hipStream_t st) {
+// ms: input side the split partials (scratch stride scr_in), output side the caller's flm
+void launch_sum_parts(int L, int S, int esz, const void *parts, void *y, const MapSet &ms, hipStream_t st) {
   if (S <= 1) return;
   const size_t n = (size_t)L * L;
-  hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0, st,
-                     (const cx<double> *)parts, S, fwd_chunk(L, S, esz), n, (cx<double> *)y);
+  hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192), ms.n), dim3(256), 0, st,
+                     (const cx<double> *)parts, S, fwd_chunk(L, S, esz), n, (cx<double> *)y, ms);
 }
 
 int mfma_selftest() {

@@ -17,6 +17,32 @@ struct ColDev {
   long long out_off;  // complex offset of this column's spectrum plane in the scratch buffers
 };
 
+// Maps of a batch that share one plan.  Grouped launches run map k at blockIdx.y = k
+// (the contractions deal (block, map) items across the XCDs instead, maps fastest, so
+// that the maps of one tile share the Delta records through L2).  Map k's scratch lies
+// k * scr bytes past map 0's, its caller buffers dlt[k] bytes from map 0's (device
+// arrays).  A single map: the defaults.
+struct MapSet {
+  int n = 1;
+  long long scr_in = 0, scr_out = 0;
+  const long long *dlt_in = nullptr, *dlt_out = nullptr;
+};
+// map k's copy of a map-0 pointer p: through the caller offsets when given, else the
+// scratch stride
+__device__ inline const char *map_ptr(const void *p, int k, long long scr, const long long *dlt) {
+  return (const char *)p + (k == 0 ? 0 : (dlt ? dlt[k] : (long long)k * scr));
+}
+__device__ inline char *map_ptr(void *p, int k, long long scr, const long long *dlt) {
+  return (char *)p + (k == 0 ? 0 : (dlt ? dlt[k] : (long long)k * scr));
+}
+// Blocks are dealt to the 8 XCDs round-robin (linear block b to XCD b % 8); work item
+// w of a launch of nb blocks is taken so that each XCD runs a contiguous range of items
+// (consecutive items share its L2).  A bijection of [0, nb).
+__device__ inline int xcd_item(int lin, int nb) {
+  const int xx = lin & 7, q8 = nb >> 3, r8 = nb & 7;
+  return xx * q8 + min(xx, r8) + (lin >> 3);
+}
+
 // Consecutive columns sharing one band-limit (several wavelet scales may share
 // L_j), at most 16: one MFMA column tile.
 struct SuperDev {
@@ -118,19 +144,20 @@ constexpr int kInvMP = 2;  // m' per block of the inverse contraction
 // harmonic coefficients x (Lx^2) in the inverse contraction's operand layout (xtile_elems(Lx) entries)
 size_t xtile_elems(int Lx);
 template <typename T>
-void launch_xtile(const void *x, int Lx, void *xtile, hipStream_t st);
+void launch_xtile(const void *x, int Lx, void *xtile, const MapSet &ms, hipStream_t st);
 template <typename T>
-void launch_inv_mfma(const void *xtile, int Lx, const T *dt, void *Fbase, const InvJobs<T> &jobs, hipStream_t st);
+void launch_inv_mfma(const void *xtile, int Lx, const T *dt, void *Fbase, const InvJobs<T> &jobs, const MapSet &ms,
+                     hipStream_t st);
 template <typename T>
 void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *const *Efs, const T *dt, int L,
-                     const void *Jbase, void *y, void *parts, int splits, hipStream_t st);
+                     const void *Jbase, void *y, void *parts, int splits, const MapSet &ms, hipStream_t st);
 // m' splits of the forward contraction (partials: splits * L^2 complex128, at most
 // fwd_splits_max(L) <= kMaxSplits)
 constexpr int kMaxSplits = 16;
 int fwd_splits_default(int L);
 int fwd_splits_max(int L);
 // y = sum of the partials (esz: bytes of the forward contraction's real type, which sets its m' chunks)
-void launch_sum_parts(int L, int splits, int esz, const void *parts, void *y, hipStream_t st);
+void launch_sum_parts(int L, int splits, int esz, const void *parts, void *y, const MapSet &ms, hipStream_t st);
 int fwd_chunk(int L, int splits, int esz);  // m' per split (a multiple of the staging chunk)
 int mfma_selftest();
 
@@ -186,15 +213,15 @@ int theta_rows(int Lmax);  // output rows per block of the launch, for the large
 template <typename T>
 int theta_blocks(const ThetaArgs<T> &a, int rows);
 template <typename T>
-void launch_theta(const ThetaJobs<T> &jobs, hipStream_t st);
+void launch_theta(const ThetaJobs<T> &jobs, const MapSet &ms, hipStream_t st);
 size_t bf3_matrix_elems(int L, int ss);
 void build_theta_bf3(const double *th64, int L, int ss, unsigned short *out, hipStream_t st);
 bool theta_bf_eligible(int Lmax);
-void launch_theta_bf(const ThetaJobs<float> &jobs, hipStream_t st);
+void launch_theta_bf(const ThetaJobs<float> &jobs, const MapSet &ms, hipStream_t st);
 bool theta_bf_args_ok(const ThetaJobs<float> &jobs);  // twisted input rows: row 0 or none
 int mfma_bf_selftest();
 template <typename TI, typename TO, typename T>
-void launch_rows(const RowJobs<T> &jobs, hipStream_t st);
+void launch_rows(const RowJobs<T> &jobs, const MapSet &ms, hipStream_t st);
 constexpr int kRowsItems = 8;  // first-product items a wave keeps in registers
 int rows_items_per_wave(int M1, int M2, int R1, int lpb);
 size_t rows_lds_bytes(int M1, int M2, int CSA, int CSC, int lpb, size_t esz);
@@ -205,7 +232,7 @@ inline int mix_blocks(size_t npts) {
   return (int)std::max<size_t>(1, std::min<size_t>(kMixMaxBlocks, (npts + 511) / 512));
 }
 template <typename TI, typename TO, typename T>
-void launch_mix(const MixJobs &jobs, hipStream_t st);
+void launch_mix(const MixJobs &jobs, const MapSet &ms, hipStream_t st);
 
 void launch_lm2lmn(void *wlmn, const void *flm, const void *psi_j, int Lj, int Nj, int spin, int mode, hipStream_t st);
 void launch_lmn2lm(void *flm, const void *wlmn, const void *psi_j, int Lj, int Nj, int spin, int mode, hipStream_t st);

@@ -85,13 +85,13 @@ __global__ void k_add_cx(cx<double> *__restrict__ a, const cx<double> *__restric
 // per job: out_k[pt] = sum_j W[k][j] in_j[pt] (complex, nout x nin matrix W, nin <= NMAX),
 // mix_blocks(npts) blocks per job
 template <typename TI, typename TO, typename T, int NMAX>
-__global__ __launch_bounds__(256) void k_mix(MixJobs jobs) {
+__global__ __launch_bounds__(256) void k_mix(MixJobs jobs, MapSet ms) {
   __shared__ cx<T> sW[32 * 32];
   int jb = 0;
   while (jb + 1 < jobs.n && (int)blockIdx.x >= jobs.j[jb + 1].blk0) ++jb;
   const MixJob &J = jobs.j[jb];
-  const cx<TI> *__restrict__ in = static_cast<const cx<TI> *>(J.in);
-  cx<TO> *__restrict__ out = static_cast<cx<TO> *>(J.out);
+  const cx<TI> *__restrict__ in = (const cx<TI> *)map_ptr(J.in, blockIdx.y, ms.scr_in, ms.dlt_in);
+  cx<TO> *__restrict__ out = (cx<TO> *)map_ptr(J.out, blockIdx.y, ms.scr_out, ms.dlt_out);
   const cx<double> *__restrict__ W = static_cast<const cx<double> *>(J.W);
   const int nin = J.nin, nout = J.nout;
   for (int i = threadIdx.x; i < nin * nout; i += blockDim.x) sW[i] = mk<T>((T)W[i].x, (T)W[i].y);
@@ -185,17 +185,17 @@ void launch_cx2re(const void *a, double *b, size_t n, hipStream_t st) {
 }
 
 template <typename TI, typename TO, typename T>
-void launch_mix(const MixJobs &jobs, hipStream_t st) {
+void launch_mix(const MixJobs &jobs, const MapSet &ms, hipStream_t st) {
   if (jobs.n == 0 || jobs.nblk == 0) return;
   int nmax = 0;
   for (int k = 0; k < jobs.n; ++k) nmax = std::max(nmax, jobs.j[k].nin);
-  if (nmax <= 8) hipLaunchKernelGGL((k_mix<TI, TO, T, 8>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
-  else if (nmax <= 16) hipLaunchKernelGGL((k_mix<TI, TO, T, 16>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
-  else hipLaunchKernelGGL((k_mix<TI, TO, T, 32>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
+  if (nmax <= 8) hipLaunchKernelGGL((k_mix<TI, TO, T, 8>), dim3(jobs.nblk, ms.n), dim3(256), 0, st, jobs, ms);
+  else if (nmax <= 16) hipLaunchKernelGGL((k_mix<TI, TO, T, 16>), dim3(jobs.nblk, ms.n), dim3(256), 0, st, jobs, ms);
+  else hipLaunchKernelGGL((k_mix<TI, TO, T, 32>), dim3(jobs.nblk, ms.n), dim3(256), 0, st, jobs, ms);
 }
-template void launch_mix<double, double, double>(const MixJobs &, hipStream_t);
-template void launch_mix<float, double, float>(const MixJobs &, hipStream_t);
-template void launch_mix<double, float, float>(const MixJobs &, hipStream_t);
-template void launch_mix<float, float, float>(const MixJobs &, hipStream_t);
+template void launch_mix<double, double, double>(const MixJobs &, const MapSet &, hipStream_t);
+template void launch_mix<float, double, float>(const MixJobs &, const MapSet &, hipStream_t);
+template void launch_mix<double, float, float>(const MixJobs &, const MapSet &, hipStream_t);
+template void launch_mix<float, float, float>(const MixJobs &, const MapSet &, hipStream_t);
 
 }  // namespace s2h

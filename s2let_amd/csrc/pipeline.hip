@@ -202,16 +202,25 @@ inline char *at64(const void *p, size_t elems) { return (char *)p + elems * size
 const double kTwoPi = 6.28318530717958647692;
 
 template <typename T>
-void contract_inv_all(Plan &P, const Scratch &X, const void *x, const ColDev *cols, void *Fbase, hipStream_t st) {
+void contract_inv_all(Plan &P, const Scratch &X, const void *x, const ColDev *cols, void *Fbase, hipStream_t st,
+                      const BatchIO &B) {
   Context &C = Context::get();
   InvJobs<T> J{};
   {
     Stage s_("xtile", st);
-    launch_xtile<T>(x, P.p.L, X.xtile, st);
+    MapSet ms;
+    ms.n = B.n;
+    ms.scr_out = B.xtile;
+    ms.dlt_in = B.d_lm;
+    launch_xtile<T>(x, P.p.L, X.xtile, ms, st);
   }
+  MapSet mi;
+  mi.n = B.n;
+  mi.scr_in = B.xtile;
+  mi.scr_out = B.spec;
   auto flush = [&]() {
     Stage s_("contract_inv", st);
-    launch_inv_mfma<T>(X.xtile, P.p.L, Sel<T>::delta(C), Fbase, J, st);
+    launch_inv_mfma<T>(X.xtile, P.p.L, Sel<T>::delta(C), Fbase, J, mi, st);
     J.n = 0;
     J.nblk = 0;
   };
@@ -232,23 +241,38 @@ void contract_inv_all(Plan &P, const Scratch &X, const void *x, const ColDev *co
 }
 
 template <typename T>
-void contract_fwd_all(Plan &P, const Scratch &X, const ColDev *cols, const void *Jbase, void *flm, hipStream_t st) {
+void contract_fwd_all(Plan &P, const Scratch &X, const ColDev *cols, const void *Jbase, void *flm, hipStream_t st,
+                      const BatchIO &B) {
   Context &C = Context::get();
+  const int S = B.splits > 0 ? B.splits : C.splits(P.p.L, (int)sizeof(T));
   {
     Stage s_("contract_fwd", st);
+    MapSet ms;
+    ms.n = B.n;
+    ms.scr_in = B.spec;
+    ms.scr_out = B.parts;
+    ms.dlt_out = B.d_lm;
     launch_fwd_mfma<T>(P.d_supers.as<SuperDev>(), (int)P.supers.size(), cols, P.d_efs.as<const T *>(),
-                       Sel<T>::delta(C), P.p.L, Jbase, flm, X.parts, C.splits(P.p.L, (int)sizeof(T)), st);
+                       Sel<T>::delta(C), P.p.L, Jbase, flm, X.parts, S, ms, st);
   }
-  if (C.splits(P.p.L, (int)sizeof(T)) > 1) {
+  if (S > 1) {
     Stage s_("contract_fwd_sum", st);
-    launch_sum_parts(P.p.L, C.splits(P.p.L, (int)sizeof(T)), (int)sizeof(T), X.parts, flm, st);
+    MapSet ms;
+    ms.n = B.n;
+    ms.scr_in = B.parts;
+    ms.dlt_out = B.d_lm;
+    launch_sum_parts(P.p.L, S, (int)sizeof(T), X.parts, flm, ms, st);
   }
 }
 
 // ---- grouped launches over the plan's column groups
 // theta stage of every group: in/out = plan buffers at each group's spectra offset
 template <typename T>
-void theta_groups(Plan &P, const ColDev *cols, int kind, void *in, void *out, const char *name, hipStream_t st) {
+void theta_groups(Plan &P, const ColDev *cols, int kind, void *in, void *out, const char *name, hipStream_t st,
+                  const BatchIO &B) {
+  MapSet ms;
+  ms.n = B.n;
+  ms.scr_in = ms.scr_out = B.spec;
   // job lists of at most kMaxJobs groups, largest band-limit first (k-th job: group
   // ng - 1 - k); the rows per block of a launch follow its largest band-limit
   const size_t ng = P.groups.size();
@@ -268,7 +292,7 @@ void theta_groups(Plan &P, const ColDev *cols, int kind, void *in, void *out, co
       J.nblk += theta_blocks<T>(j.a, rows);
     }
     Stage s_(name, st);
-    launch_theta<T>(J, st);
+    launch_theta<T>(J, ms, st);
   }
 }
 
@@ -277,11 +301,16 @@ void theta_groups(Plan &P, const ColDev *cols, int kind, void *in, void *out, co
 // wavelet groups (plan buffers; all lines of a group are contiguous, M each)
 template <typename T>
 void rows_all(Plan &P, void *in, void *out, const void *scal_in, void *scal_out, int in_c, int out_c, int sign,
-              bool scaled, hipStream_t st) {
+              bool scaled, hipStream_t st, const BatchIO &B) {
   RowJobs<T> J{};
+  MapSet ms;  // scratch both sides; the scaling function's caller side through f_scal's offsets
+  ms.n = B.n;
+  ms.scr_in = ms.scr_out = B.spec;
+  ms.dlt_in = scal_in ? B.d_scal : nullptr;
+  ms.dlt_out = scal_out ? B.d_scal : nullptr;
   auto flush = [&]() {
     Stage s_("rows", st);
-    launch_rows<T, T, T>(J, st);
+    launch_rows<T, T, T>(J, ms, st);
     J.n = 0;
     J.nblk = 0;
   };
@@ -306,11 +335,20 @@ void rows_all(Plan &P, void *in, void *out, const void *scal_in, void *scal_out,
 // 3 forward adjoint (the mixing matrices of the plan)
 template <typename TI, typename TO, typename T>
 void mix_groups(Plan &P, int which, bool to_wav, const void *src_wav, void *dst_wav, void *cols, void *cols_out,
-                hipStream_t st) {
+                hipStream_t st, const BatchIO &B) {
   MixJobs J{};
+  MapSet ms;  // the column spectra in scratch, f_wav through its per-map offsets
+  ms.n = B.n;
+  if (to_wav) {
+    ms.scr_in = B.spec;
+    ms.dlt_out = B.d_wav;
+  } else {
+    ms.dlt_in = B.d_wav;
+    ms.scr_out = B.spec;
+  }
   auto flush = [&]() {
     Stage s_("gamma", st);
-    launch_mix<TI, TO, T>(J, st);
+    launch_mix<TI, TO, T>(J, ms, st);
     J.n = 0;
     J.nblk = 0;
   };
@@ -340,41 +378,45 @@ void mix_groups(Plan &P, int which, bool to_wav, const void *src_wav, void *dst_
 }
 
 template <typename T>
-void analysis_t(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
+void analysis_t(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st,
+                const BatchIO &B) {
   const void *x = flm;  // complex128; k_xtile rounds it in the FP32 mode
   const ColDev *cols = P.d_cols_ana.as<ColDev>();
-  contract_inv_all<T>(P, X, x, cols, X.spec, st);
-  theta_groups<T>(P, cols, 0, X.spec, X.work, "beta_inv", st);
-  rows_all<T>(P, X.work, X.spec, nullptr, f_scal, 1, 0, +1, false, st);
-  mix_groups<T, double, T>(P, 0, true, nullptr, f_wav, X.spec, X.spec, st);
+  contract_inv_all<T>(P, X, x, cols, X.spec, st, B);
+  theta_groups<T>(P, cols, 0, X.spec, X.work, "beta_inv", st, B);
+  rows_all<T>(P, X.work, X.spec, nullptr, f_scal, 1, 0, +1, false, st, B);
+  mix_groups<T, double, T>(P, 0, true, nullptr, f_wav, X.spec, X.spec, st, B);
 }
 
 template <typename T>
-void synthesis_t(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
+void synthesis_t(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm, hipStream_t st,
+                 const BatchIO &B) {
   const ColDev *cols = P.d_cols_syn.as<ColDev>();
-  mix_groups<double, T, T>(P, 2, false, f_wav, nullptr, X.work, X.work, st);
-  rows_all<T>(P, X.work, X.spec, f_scal, nullptr, 0, 1, -1, true, st);
-  theta_groups<T>(P, cols, 2, X.spec, X.work, "beta_fwd", st);
-  contract_fwd_all<T>(P, X, cols, X.work, flm, st);
+  mix_groups<double, T, T>(P, 2, false, f_wav, nullptr, X.work, X.work, st, B);
+  rows_all<T>(P, X.work, X.spec, f_scal, nullptr, 0, 1, -1, true, st, B);
+  theta_groups<T>(P, cols, 2, X.spec, X.work, "beta_fwd", st, B);
+  contract_fwd_all<T>(P, X, cols, X.work, flm, st, B);
 }
 
 template <typename T>
-void analysis_adjoint_t(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
+void analysis_adjoint_t(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm, hipStream_t st,
+                        const BatchIO &B) {
   const ColDev *cols = P.d_cols_ana.as<ColDev>();
-  mix_groups<double, T, T>(P, 1, false, f_wav, nullptr, X.work, X.work, st);
-  rows_all<T>(P, X.work, X.spec, f_scal, nullptr, 0, 1, -1, false, st);
-  theta_groups<T>(P, cols, 1, X.spec, X.work, "beta_inv_adj", st);
-  contract_fwd_all<T>(P, X, cols, X.work, flm, st);
+  mix_groups<double, T, T>(P, 1, false, f_wav, nullptr, X.work, X.work, st, B);
+  rows_all<T>(P, X.work, X.spec, f_scal, nullptr, 0, 1, -1, false, st, B);
+  theta_groups<T>(P, cols, 1, X.spec, X.work, "beta_inv_adj", st, B);
+  contract_fwd_all<T>(P, X, cols, X.work, flm, st, B);
 }
 
 template <typename T>
-void synthesis_adjoint_t(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
+void synthesis_adjoint_t(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st,
+                         const BatchIO &B) {
   const void *x = flm;  // complex128; k_xtile rounds it in the FP32 mode
   const ColDev *cols = P.d_cols_syn.as<ColDev>();
-  contract_inv_all<T>(P, X, x, cols, X.spec, st);
-  theta_groups<T>(P, cols, 3, X.spec, X.work, "beta_fwd_adj", st);
-  rows_all<T>(P, X.work, X.spec, nullptr, f_scal, 1, 0, +1, true, st);
-  mix_groups<T, double, T>(P, 3, true, nullptr, f_wav, X.spec, X.spec, st);
+  contract_inv_all<T>(P, X, x, cols, X.spec, st, B);
+  theta_groups<T>(P, cols, 3, X.spec, X.work, "beta_fwd_adj", st, B);
+  rows_all<T>(P, X.work, X.spec, nullptr, f_scal, 1, 0, +1, true, st, B);
+  mix_groups<T, double, T>(P, 3, true, nullptr, f_wav, X.spec, X.spec, st, B);
 }
 
 // ---- single spin-s transform at L (one column)
@@ -387,7 +429,7 @@ void theta_one(int L, const ColDev *col, int kind, const void *in, void *out, in
   J.j[0].out = out;
   J.j[0].blk0 = 0;
   J.nblk = theta_blocks<T>(J.j[0].a, theta_rows<T>(L));
-  launch_theta<T>(J, st);
+  launch_theta<T>(J, MapSet(), st);
 }
 
 template <typename TI, typename TO, typename T>
@@ -399,7 +441,7 @@ void rows_one(int L, const void *in, void *out, int in_c, int out_c, int sign, d
   J.j[0].out = out;
   J.j[0].blk0 = 0;
   J.nblk = (int)((J.j[0].a.nlines + J.j[0].a.lpb - 1) / J.j[0].a.lpb);
-  launch_rows<TI, TO, T>(J, st);
+  launch_rows<TI, TO, T>(J, MapSet(), st);
 }
 
 
@@ -415,8 +457,8 @@ void sht_inverse_t(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
     J.j[0] = InvJob<T>{(const T *)S.ei, col, 1, std::abs(S.spin), L, (L + 63) / 64, 0};
     J.nblk = J.j[0].gx * ((L + kInvMP - 1) / kInvMP);
     Stage s_("contract_inv", st);
-    launch_xtile<T>(x, L, S.xtile.p, st);
-    launch_inv_mfma<T>(S.xtile.p, L, Sel<T>::delta(C), S.spec.p, J, st);
+    launch_xtile<T>(x, L, S.xtile.p, MapSet(), st);
+    launch_inv_mfma<T>(S.xtile.p, L, Sel<T>::delta(C), S.spec.p, J, MapSet(), st);
   }
   { Stage s_("beta_inv", st); theta_one<T>(L, col, 0, S.spec.p, S.work.p, S.ss, st); }
   { Stage s_("rows", st); rows_one<T, double, T>(L, S.work.p, f, 1, 0, +1, 1.0, S.ss, st); }
@@ -429,8 +471,8 @@ void sht_forward_t(ShtPlan &S, const void *f, void *flm, hipStream_t st) {
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("rows", st); rows_one<double, T, T>(L, f, S.spec.p, 0, 1, -1, kTwoPi / M, S.ss, st); }
   { Stage s_("beta_fwd", st); theta_one<T>(L, col, 2, S.spec.p, S.work.p, S.ss, st); }
-  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, C.splits(L, (int)sizeof(T)), st); }
-  if (C.splits(L, (int)sizeof(T)) > 1) { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, C.splits(L, (int)sizeof(T)), (int)sizeof(T), S.parts.p, flm, st); }
+  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, C.splits(L, (int)sizeof(T)), MapSet(), st); }
+  if (C.splits(L, (int)sizeof(T)) > 1) { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, C.splits(L, (int)sizeof(T)), (int)sizeof(T), S.parts.p, flm, MapSet(), st); }
 }
 
 template <typename T>
@@ -440,8 +482,8 @@ void sht_inverse_adjoint_t(ShtPlan &S, const void *f, void *flm, hipStream_t st)
   const ColDev *col = S.d_col.as<ColDev>();
   { Stage s_("rows", st); rows_one<double, T, T>(L, f, S.spec.p, 0, 1, -1, 1.0, S.ss, st); }
   { Stage s_("beta_inv_adj", st); theta_one<T>(L, col, 1, S.spec.p, S.work.p, S.ss, st); }
-  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, C.splits(L, (int)sizeof(T)), st); }
-  if (C.splits(L, (int)sizeof(T)) > 1) { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, C.splits(L, (int)sizeof(T)), (int)sizeof(T), S.parts.p, flm, st); }
+  { Stage s_("contract_fwd", st); launch_fwd_mfma<T>(S.d_super.as<SuperDev>(), 1, col, S.d_efs.as<const T *>(), Sel<T>::delta(C), L, S.work.p, flm, S.parts.p, C.splits(L, (int)sizeof(T)), MapSet(), st); }
+  if (C.splits(L, (int)sizeof(T)) > 1) { Stage s_("contract_fwd_sum", st); launch_sum_parts(L, C.splits(L, (int)sizeof(T)), (int)sizeof(T), S.parts.p, flm, MapSet(), st); }
 }
 
 template <typename T>
@@ -456,8 +498,8 @@ void sht_forward_adjoint_t(ShtPlan &S, const void *flm, void *f, hipStream_t st)
     J.j[0] = InvJob<T>{(const T *)S.ei, col, 1, std::abs(S.spin), L, (L + 63) / 64, 0};
     J.nblk = J.j[0].gx * ((L + kInvMP - 1) / kInvMP);
     Stage s_("contract_inv", st);
-    launch_xtile<T>(x, L, S.xtile.p, st);
-    launch_inv_mfma<T>(S.xtile.p, L, Sel<T>::delta(C), S.spec.p, J, st);
+    launch_xtile<T>(x, L, S.xtile.p, MapSet(), st);
+    launch_inv_mfma<T>(S.xtile.p, L, Sel<T>::delta(C), S.spec.p, J, MapSet(), st);
   }
   { Stage s_("beta_fwd_adj", st); theta_one<T>(L, col, 3, S.spec.p, S.work.p, S.ss, st); }
   { Stage s_("rows", st); rows_one<T, double, T>(L, S.work.p, f, 1, 0, +1, kTwoPi / M, S.ss, st); }
@@ -473,18 +515,21 @@ void sht_forward_adjoint_t(ShtPlan &S, const void *flm, void *f, hipStream_t st)
       fn<double>(__VA_ARGS__);      \
   } while (0)
 
-void run_analysis(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
-  S2H_DISPATCH(P.mode, analysis_t, P, X, flm, f_wav, f_scal, st);
+void run_analysis(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st,
+                  const BatchIO &B) {
+  S2H_DISPATCH(P.mode, analysis_t, P, X, flm, f_wav, f_scal, st, B);
 }
-void run_synthesis(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm, hipStream_t st) {
-  S2H_DISPATCH(P.mode, synthesis_t, P, X, f_wav, f_scal, flm, st);
+void run_synthesis(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm, hipStream_t st,
+                   const BatchIO &B) {
+  S2H_DISPATCH(P.mode, synthesis_t, P, X, f_wav, f_scal, flm, st, B);
 }
 void run_analysis_adjoint(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm,
-                          hipStream_t st) {
-  S2H_DISPATCH(P.mode, analysis_adjoint_t, P, X, f_wav, f_scal, flm, st);
+                          hipStream_t st, const BatchIO &B) {
+  S2H_DISPATCH(P.mode, analysis_adjoint_t, P, X, f_wav, f_scal, flm, st, B);
 }
-void run_synthesis_adjoint(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st) {
-  S2H_DISPATCH(P.mode, synthesis_adjoint_t, P, X, flm, f_wav, f_scal, st);
+void run_synthesis_adjoint(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st,
+                           const BatchIO &B) {
+  S2H_DISPATCH(P.mode, synthesis_adjoint_t, P, X, flm, f_wav, f_scal, st, B);
 }
 void run_sht_inverse(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
   S2H_DISPATCH(S.mode, sht_inverse_t, S, flm, f, st);

@@ -64,6 +64,18 @@ struct Scratch {
   void *spec = nullptr, *work = nullptr, *parts = nullptr, *xtile = nullptr;
 };
 
+// A batch of maps on one plan: map k's scratch lies at k times the strides past map
+// 0's, its caller buffers at per-map byte offsets from map 0's (device arrays with
+// entry 0 = 0).  The defaults describe a single map.
+struct BatchIO {
+  int n = 1;
+  long long spec = 0, xtile = 0, parts = 0;  // scratch strides (bytes)
+  const long long *d_lm = nullptr;           // f_lm (input or output)
+  const long long *d_wav = nullptr;          // f_wav
+  const long long *d_scal = nullptr;         // f_scal
+  int splits = 0;                            // forward-contraction m' splits (0: the plan's)
+};
+
 // Orders calls that share a plan's scratch across streams: every call on a stream
 // other than the previous one waits for the previous call's end event.
 struct PlanSync {
@@ -111,6 +123,9 @@ struct Plan {
   DevBuf spec, work;  // scratch (T = double or float per mode)
   DevBuf parts;       // forward-contraction partial sums over m' splits
   DevBuf xtile;       // input harmonics in the inverse contraction's operand layout
+  DevBuf b_spec, b_work, b_parts, b_xtile, b_dtab;  // batch scratch (maps of one plan) and caller offsets
+  std::vector<long long> dtab_host;  // what b_dtab holds once its last upload has run
+  PlanSync dtab_copied;               // recorded after that upload (dtab_host is its source)
   bool manual = false;
   uint64_t last_use = 0;  // plan-cache clock (manual plans are evicted least recently used first)
   PlanSync sync;
@@ -163,6 +178,9 @@ class Context {
   Plan *find_plan(const s2let_parameters_t *p, int mode);  // cached plan or null (builds nothing)
   Plan &plan_manual(const ManualTiling &m, int mode, hipStream_t st);
   Scratch scratch(Plan &pl);  // the plan's own scratch, allocated on first use
+  // scratch of a batch of n maps on this plan (grown as needed; growth moves the
+  // generation), with the strides in B
+  Scratch batch_scratch(Plan &pl, int n, BatchIO &B);
   static size_t scratch_bytes(const Plan &pl);  // bytes of one Scratch set of this plan (256-aligned parts)
   // Incremented whenever a table that launches read through a pointer (Delta tables,
   // their tiled copies) is reallocated, and when plans are freed: anything that keeps
@@ -205,11 +223,15 @@ void timing_enable(bool on);
 bool timing_pause();  // stops recording without clearing; returns the previous state
 void timing_resume(bool was_on);
 std::string timing_report();
-void run_analysis(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st);
-void run_synthesis(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm, hipStream_t st);
+// The transforms of one map, or of a batch B of maps on plan P (pointers are map 0's).
+void run_analysis(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st,
+                  const BatchIO &B = BatchIO());
+void run_synthesis(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm, hipStream_t st,
+                   const BatchIO &B = BatchIO());
 void run_analysis_adjoint(Plan &P, const Scratch &X, const void *f_wav, const void *f_scal, void *flm,
-                          hipStream_t st);
-void run_synthesis_adjoint(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st);
+                          hipStream_t st, const BatchIO &B = BatchIO());
+void run_synthesis_adjoint(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st,
+                           const BatchIO &B = BatchIO());
 std::string plan_key(const s2let_parameters_t *p, int mode);
 void run_sht_inverse(ShtPlan &S, const void *flm, void *f, hipStream_t st);
 void run_sht_forward(ShtPlan &S, const void *f, void *flm, hipStream_t st);

@@ -546,6 +546,33 @@ Scratch Context::scratch(Plan &pl) {
   return x;
 }
 
+Scratch Context::batch_scratch(Plan &pl, int n, BatchIO &B) {
+  const size_t esz = pl.mode == S2LET_HIP_MIXED ? sizeof(cx<float>) : sizeof(cx<double>);
+  const int S = splits(pl.p.L, (int)(esz / 2));
+  const size_t spec = align256(pl.spec_elems * esz), xt = align256(pl.xtile_elems_ * esz);
+  const size_t parts = align256((size_t)S * pl.p.L * pl.p.L * sizeof(cx<double>));
+  auto grow = [&](DevBuf &b, size_t bytes) {
+    if (b.p && b.bytes >= bytes) return;
+    ++gen_;  // captured batches hold the old address
+    b.alloc(bytes);
+  };
+  grow(pl.b_spec, n * spec);
+  grow(pl.b_work, n * spec);
+  grow(pl.b_xtile, n * xt);
+  grow(pl.b_parts, n * parts);
+  B.n = n;
+  B.spec = (long long)spec;
+  B.xtile = (long long)xt;
+  B.parts = (long long)parts;
+  B.splits = S;
+  Scratch x;
+  x.spec = pl.b_spec.p;
+  x.work = pl.b_work.p;
+  x.xtile = pl.b_xtile.p;
+  x.parts = pl.b_parts.p;
+  return x;
+}
+
 void Context::finish_plan(Plan &pl, hipStream_t st) {
   const int L = pl.p.L;
   const int mode = pl.mode;

@@ -150,16 +150,19 @@ __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const 
 }
 
 template <typename TI, typename TO, typename T, int IMAX>
-__global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs) {
+__global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
   extern __shared__ char smem_raw[];
   typedef typename Mfma<T>::acc acc_t;
   int jb = 0;
   while (jb + 1 < jobs.n && (int)blockIdx.x >= jobs.j[jb + 1].blk0) ++jb;
   const RowArgs<T> &a = jobs.j[jb].a;
-  const cx<TI> *__restrict__ in = static_cast<const cx<TI> *>(jobs.j[jb].in);
-  cx<TO> *__restrict__ out = static_cast<cx<TO> *>(jobs.j[jb].out);
+  // map blockIdx.y: a side flagged in io64 is the caller's (f_scal), the others scratch
+  const int io = jobs.j[jb].io64;
+  const cx<TI> *__restrict__ in =
+      (const cx<TI> *)map_ptr(jobs.j[jb].in, blockIdx.y, ms.scr_in, (io & 1) ? ms.dlt_in : nullptr);
+  cx<TO> *__restrict__ out = (cx<TO> *)map_ptr(jobs.j[jb].out, blockIdx.y, ms.scr_out, (io & 2) ? ms.dlt_out : nullptr);
   // a complex128 side of a job of an FP32 launch (uniform over the block)
-  const bool in64 = sizeof(TI) == 4 && (jobs.j[jb].io64 & 1), out64 = sizeof(TO) == 4 && (jobs.j[jb].io64 & 2);
+  const bool in64 = sizeof(TI) == 4 && (io & 1), out64 = sizeof(TO) == 4 && (io & 2);
   const Dft2<T> &d = a.d;
   const int M = d.M, M1 = d.M1, M2 = d.M2, half = a.L - 1, Mh = M - half;
   const float inv_rpp = 1.0f / (float)a.rows_pp;
@@ -385,7 +388,7 @@ size_t rows_lds_bytes(int M1, int M2, int CSA, int CSC, int lpb, size_t esz) {
 }
 
 template <typename TI, typename TO, typename T>
-void launch_rows(const RowJobs<T> &jobs, hipStream_t st) {
+void launch_rows(const RowJobs<T> &jobs, const MapSet &ms, hipStream_t st) {
   if (jobs.n == 0 || jobs.nblk == 0) return;
   size_t shm = 0;
   int items = 0;
@@ -394,12 +397,12 @@ void launch_rows(const RowJobs<T> &jobs, hipStream_t st) {
     shm = std::max(shm, rows_lds_bytes(a.d.M1, a.d.M2, a.d.CSA, a.d.CSC, a.lpb, sizeof(T)));
     items = std::max(items, rows_items_per_wave(a.d.M1, a.d.M2, a.d.R1, a.lpb));
   }
-  if (items <= 4) hipLaunchKernelGGL((k_rows4<TI, TO, T, 4>), dim3(jobs.nblk), dim3(256), shm, st, jobs);
-  else hipLaunchKernelGGL((k_rows4<TI, TO, T, kRowsItems>), dim3(jobs.nblk), dim3(256), shm, st, jobs);
+  if (items <= 4) hipLaunchKernelGGL((k_rows4<TI, TO, T, 4>), dim3(jobs.nblk, ms.n), dim3(256), shm, st, jobs, ms);
+  else hipLaunchKernelGGL((k_rows4<TI, TO, T, kRowsItems>), dim3(jobs.nblk, ms.n), dim3(256), shm, st, jobs, ms);
 }
-template void launch_rows<double, double, double>(const RowJobs<double> &, hipStream_t);
-template void launch_rows<float, float, float>(const RowJobs<float> &, hipStream_t);
-template void launch_rows<float, double, float>(const RowJobs<float> &, hipStream_t);
-template void launch_rows<double, float, float>(const RowJobs<float> &, hipStream_t);
+template void launch_rows<double, double, double>(const RowJobs<double> &, const MapSet &, hipStream_t);
+template void launch_rows<float, float, float>(const RowJobs<float> &, const MapSet &, hipStream_t);
+template void launch_rows<float, double, float>(const RowJobs<float> &, const MapSet &, hipStream_t);
+template void launch_rows<double, float, float>(const RowJobs<float> &, const MapSet &, hipStream_t);
 
 }  // namespace s2h

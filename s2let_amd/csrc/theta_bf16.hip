@@ -91,7 +91,7 @@ __device__ inline void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
   l = (__bf16)(r - (float)m);
 }
 
-__global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs) {
+__global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs, MapSet ms) {
   constexpr int RT = kBfRT, SJ = kBfSJ;
   // [buf][class][part][re/im][j][k]; class 1 is offset by 32 entries (64 bytes) so that
   // the two classes of neighbouring lanes' 16-byte writes fall in different banks
@@ -104,8 +104,8 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs) {
   int jb = 0;
   while (jb + 1 < jobs.n && w >= jobs.j[jb + 1].blk0) ++jb;
   const ThetaArgs<float> &a = jobs.j[jb].a;
-  const cx<float> *__restrict__ in = static_cast<const cx<float> *>(jobs.j[jb].in);
-  cx<float> *__restrict__ out = static_cast<cx<float> *>(jobs.j[jb].out);
+  const cx<float> *__restrict__ in = (const cx<float> *)map_ptr(jobs.j[jb].in, blockIdx.y, ms.scr_in, ms.dlt_in);
+  cx<float> *__restrict__ out = (cx<float> *)map_ptr(jobs.j[jb].out, blockIdx.y, ms.scr_out, ms.dlt_out);
   w -= jobs.j[jb].blk0;
   const int L = a.L, M = 2 * L - 1, R = a.R, K = a.K;
   const int nti = (R + 32 * RT - 1) / (32 * RT), ntm = (M + 63) / 64;
@@ -253,9 +253,9 @@ bool theta_bf_args_ok(const ThetaJobs<float> &jobs) {
   return true;
 }
 
-void launch_theta_bf(const ThetaJobs<float> &jobs, hipStream_t st) {
+void launch_theta_bf(const ThetaJobs<float> &jobs, const MapSet &ms, hipStream_t st) {
   if (jobs.n == 0 || jobs.nblk == 0) return;
-  hipLaunchKernelGGL(k_theta_bf, dim3(jobs.nblk), dim3(256), 0, st, jobs);
+  hipLaunchKernelGGL(k_theta_bf, dim3(jobs.nblk, ms.n), dim3(256), 0, st, jobs, ms);
 }
 
 // ------------------------------------------------------------------ selftest

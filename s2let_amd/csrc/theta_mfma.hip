@@ -211,7 +211,7 @@ struct ThetaTile {
 
 // RT row tiles of 16 per wave: a block covers 32 RT rows x 64 consecutive m.
 template <typename T, bool F64, int RT>
-__global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
+__global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs, MapSet ms) {
   constexpr int BK = ThetaTile<T>::BK, SJ = ThetaTile<T>::SJ;
   __shared__ T sB[2][2][2][BK][SJ];  // [buffer][class][re/im][k][j]
   typedef typename Mfma<T>::acc acc_t;
@@ -224,8 +224,8 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs) {
   int jb = 0;
   while (jb + 1 < jobs.n && w >= jobs.j[jb + 1].blk0) ++jb;
   const ThetaArgs<T> &a = jobs.j[jb].a;
-  const cx<T> *__restrict__ in = static_cast<const cx<T> *>(jobs.j[jb].in);
-  cx<T> *__restrict__ out = static_cast<cx<T> *>(jobs.j[jb].out);
+  const cx<T> *__restrict__ in = (const cx<T> *)map_ptr(jobs.j[jb].in, blockIdx.y, ms.scr_in, ms.dlt_in);
+  cx<T> *__restrict__ out = (cx<T> *)map_ptr(jobs.j[jb].out, blockIdx.y, ms.scr_out, ms.dlt_out);
   w -= jobs.j[jb].blk0;
   const int L = a.L, M = 2 * L - 1, R = a.R, K = a.K;
   const int nti = (R + 32 * RT - 1) / (32 * RT), ntm = (M + 63) / 64;
@@ -372,7 +372,7 @@ template int theta_rows<float>(int);
 template int theta_blocks<float>(const ThetaArgs<float> &, int);
 
 template <typename T>
-void launch_theta(const ThetaJobs<T> &jobs, hipStream_t st) {
+void launch_theta(const ThetaJobs<T> &jobs, const MapSet &ms, hipStream_t st) {
   if (jobs.n == 0 || jobs.nblk == 0) return;
   int Lmax = 0;
   bool bf = true;
@@ -388,16 +388,16 @@ void launch_theta(const ThetaJobs<T> &jobs, hipStream_t st) {
       return !(e && e[0] == '0');
     }();
     if (bf && bf_on && theta_bf_eligible(Lmax) && theta_rows<T>(Lmax) == 128 && theta_bf_args_ok(jobs)) {
-      launch_theta_bf(jobs, st);
+      launch_theta_bf(jobs, ms, st);
       return;
     }
   }
   if (theta_rows<T>(Lmax) == 128)
-    hipLaunchKernelGGL((k_theta<T, false, 4>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
+    hipLaunchKernelGGL((k_theta<T, false, 4>), dim3(jobs.nblk, ms.n), dim3(256), 0, st, jobs, ms);
   else
-    hipLaunchKernelGGL((k_theta<T, true, 2>), dim3(jobs.nblk), dim3(256), 0, st, jobs);
+    hipLaunchKernelGGL((k_theta<T, true, 2>), dim3(jobs.nblk, ms.n), dim3(256), 0, st, jobs, ms);
 }
-template void launch_theta<double>(const ThetaJobs<double> &, hipStream_t);
-template void launch_theta<float>(const ThetaJobs<float> &, hipStream_t);
+template void launch_theta<double>(const ThetaJobs<double> &, const MapSet &, hipStream_t);
+template void launch_theta<float>(const ThetaJobs<float> &, const MapSet &, hipStream_t);
 
 }  // namespace s2h

@@ -142,3 +142,59 @@ def test_batch_c3_mixed_vs_fp64_and_oracle(ops, oracle):
         print("map", k, (L, N, s), "FP64 batch vs oracle %.3e" % e, flush=True)
         assert e < 1e-12
 
+
+
+# plans repeated and interleaved, so that buckets of 1, 2 and 4 maps run together
+EAGER_SHAPES = [(32, 4, 0), (64, 8, 2), (32, 4, 0), (16, 1, 0), (64, 8, 2), (32, 4, 0), (48, 3, 2), (32, 4, 0)]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_eager_all_ops_equal_single(ops, mode):
+    """s2let_hip_batch runs the maps of one plan as one batch (grouped launches); every
+    op must give exactly the single-map results, on a second call with new output
+    buffers (the offsets re-uploaded) and on a third with the same buffers."""
+    import torch
+    rng = np.random.default_rng(17)
+    shapes = EAGER_SHAPES
+    print("batch eager all ops", mode, shapes, flush=True)
+    ps = [ops.parameters(L, N, 2.0, 1, s) for L, N, s in shapes]
+    flm = [torch.from_numpy(powerlaw(rng, L, s)).cuda() for L, _, s in shapes]
+    wav = [torch.from_numpy(rng.standard_normal(ops.n_wav(p)) + 1j * rng.standard_normal(ops.n_wav(p))).cuda()
+           for p in ps]
+    scal = [torch.from_numpy(rng.standard_normal(ops.n_scal(p)) + 1j * rng.standard_normal(ops.n_scal(p))).cuda()
+            for p in ps]
+    n = len(ps)
+
+    def zl(k):
+        return torch.zeros(ps[k].L ** 2, dtype=torch.complex128, device="cuda")
+
+    def zw(k):
+        return torch.zeros(ops.n_wav(ps[k]), dtype=torch.complex128, device="cuda")
+
+    def zs(k):
+        return torch.zeros(ops.n_scal(ps[k]), dtype=torch.complex128, device="cuda")
+
+    ref = {
+        0: [ops.analysis(flm[k], ps[k], mode) for k in range(n)],
+        1: [ops.synthesis(wav[k], scal[k], ps[k], mode) for k in range(n)],
+        2: [ops.analysis_adjoint(wav[k], scal[k], ps[k], mode) for k in range(n)],
+        3: [ops.synthesis_adjoint(flm[k], ps[k], mode) for k in range(n)],
+    }
+    keep = {}
+    for call in range(3):
+        if call < 2:  # fresh outputs (new addresses, new offsets), then the same again
+            keep = {0: ([zw(k) for k in range(n)], [zs(k) for k in range(n)]), 1: [zl(k) for k in range(n)],
+                    2: [zl(k) for k in range(n)], 3: ([zw(k) for k in range(n)], [zs(k) for k in range(n)])}
+        else:
+            for t in keep[0][0] + keep[0][1] + keep[1] + keep[2] + keep[3][0] + keep[3][1]:
+                t.zero_()
+        ops.batch(0, keep[0][0], keep[0][1], flm, None, ps, mode, graph=False)
+        ops.batch(1, keep[1], None, wav, scal, ps, mode, graph=False)
+        ops.batch(2, keep[2], None, wav, scal, ps, mode, graph=False)
+        ops.batch(3, keep[3][0], keep[3][1], flm, None, ps, mode, graph=False)
+        torch.cuda.synchronize()
+        for k in range(n):
+            assert torch.equal(keep[0][0][k], ref[0][k][0]) and torch.equal(keep[0][1][k], ref[0][k][1]), (call, k)
+            assert torch.equal(keep[1][k], ref[1][k]), (call, k)
+            assert torch.equal(keep[2][k], ref[2][k]), (call, k)
+            assert torch.equal(keep[3][0][k], ref[3][k][0]) and torch.equal(keep[3][1][k], ref[3][k][1]), (call, k)
