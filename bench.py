@@ -183,7 +183,7 @@ def _timeit(fn):
     return time.perf_counter() - t0
 
 
-def c3_batch(s2, ops, torch, steps, warmup):
+def c3_batch(s2, ops, torch, steps, warmup, modes=("fp64", "mixed")):
     """The uneven batch: 256 maps, L in {64,128,256,512}, N in {1,4,8}, spin in {0,2},
     B=2, J_min=1, f_lm with C_l = (l+1)^-2, drawn with numpy default_rng(3); one step =
     the analysis batch then the synthesis batch, each one cached HIP graph."""
@@ -203,6 +203,8 @@ def c3_batch(s2, ops, torch, steps, warmup):
     for p in ps:
         F_total += algorithmic_work(s2, p)[0]
     for mode, name in ((s2.FP64, "fp64"), (s2.MIXED, "mixed")):
+        if name not in modes:
+            continue
         fw = [torch.zeros(s2.n_wav(p), dtype=torch.complex128, device="cuda") for p in ps]
         fs = [torch.zeros(s2.n_scal(p), dtype=torch.complex128, device="cuda") for p in ps]
         rec = [torch.zeros_like(t) for t in x]
@@ -217,6 +219,59 @@ def c3_batch(s2, ops, torch, steps, warmup):
                      "max_reconstruction_rel_err": worst}
         del fw, fs, rec
     out["flop_per_batch"] = F_total
+    pmc = pmc_c3(os.path.join(ROOT, "profiles", "pmc_traffic_c3.json"))
+    for name in modes:
+        if name in pmc:
+            out[name]["hbm_bytes_per_map"] = pmc[name]
+    return out
+
+
+def pmc_c3(path):
+    """HBM bytes per map of the 256-map batch per mode, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of `bench.py --c3-only`, or {} if absent."""
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_map", {})
+    except (OSError, ValueError):
+        return {}
+
+
+def real_half_spectrum(s2, ops, torch, p, flm_h, steps, warmup):
+    """A real signal at the headline shape: analysis + synthesis through the real
+    entry points (half-spectrum plan, n >= 0 directions) against the complex entry
+    points on the same (Hermitian) input, per mode: maps/s, speed-up, and the
+    reconstruction error of the real path."""
+    L = p.L
+    h = flm_h.copy()
+    for el in range(L):  # Hermitian part: f_l,-m = (-1)^m conj(f_lm)
+        m = np.arange(1, el + 1)
+        pos, neg = el * el + el + m, el * el + el - m
+        a, b = h[pos].copy(), h[neg].copy()
+        h[pos] = 0.5 * (a + (-1.0) ** m * np.conj(b))
+        h[neg] = (-1.0) ** m * np.conj(h[pos])
+        h[el * el + el] = h[el * el + el].real
+    x = torch.from_numpy(h).cuda()
+    out = {}
+    for mode, name in ((s2.FP64, "fp64"), (s2.MIXED, "mixed")):
+        fw, fs = ops.analysis_real(x, p, mode)
+        rec = ops.synthesis_real(fw, fs, p, mode)
+        cw, cs = ops.analysis(x, p, mode)
+        crec = ops.synthesis(cw, cs, p, mode)
+
+        def real_step():
+            ops.analysis_real(x, p, mode, out=(fw, fs))
+            ops.synthesis_real(fw, fs, p, mode, out=rec)
+
+        def cx_step():
+            ops.analysis(x, p, mode, out=(cw, cs))
+            ops.synthesis(cw, cs, p, mode, out=crec)
+
+        tr = timed(real_step, steps, warmup, torch)
+        tc = timed(cx_step, steps, warmup, torch)
+        err = float(torch.linalg.norm(rec - x) / torch.linalg.norm(x))
+        out[name] = {"value": steps / tr, "unit": "maps/s", "complex_maps_per_s": steps / tc, "speedup": tc / tr,
+                     "reconstruction_rel_err": err}
+        del fw, fs, rec, cw, cs, crec
     return out
 
 
@@ -227,6 +282,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the 256-map uneven batch")
+    ap.add_argument("--c3-only", choices=("fp64", "mixed", "both"), default=None,
+                    help="run only the 256-map batch (in the given modes, --steps batches after --warmup)")
     ap.add_argument("--cpu-reps", type=int, default=1, help="CPU baseline maps (median)")
     ap.add_argument("--L", type=int, default=CFG["L"])
     ap.add_argument("--N", type=int, default=CFG["N"])
@@ -250,6 +307,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+
+    if args.c3_only:
+        modes = ("fp64", "mixed") if args.c3_only == "both" else (args.c3_only,)
+        print(json.dumps({"c3_batch": c3_batch(s2, ops, torch, args.steps, args.warmup, modes)}))
+        return
 
     cfg = dict(CFG, L=args.L, N=args.N, J_min=args.J_min)
     p = s2.parameters(cfg["L"], cfg["N"], cfg["B"], cfg["J_min"], cfg["spin"], cfg["upsample"])
@@ -375,6 +437,7 @@ def main():
         "kernels_fp64": kernels64,
         "plan_mixed": {"splits": info32["splits"], "spec_elems": info32["spec_elems"]},
     }
+    out["real_signal"] = real_half_spectrum(s2, ops, torch, p, flm_h, max(2, args.steps // 2), 1)
     if not args.no_c3:
         out["c3_batch"] = c3_batch(s2, ops, torch, max(2, args.steps // 4), 1)
     if not args.no_cpu:

@@ -261,44 +261,58 @@ GraphCache &graphs() {
 // wavelet obeys Psi_{l,-n} = (-1)^n conj(Psi_{ln}), so wavelet coefficients of
 // a real signal are real, and the reference's n >= 0 storage with the implied
 // f_{l,m,-n} = (-1)^{m+n} conj(f_{l,-m,n}) is the complex transform on that
-// subspace.  The real variants promote real inputs to complex, run the complex
-// kernels and keep real parts (on a non-Hermitian flm: the transform of its
-// Hermitian part).
+// subspace.  For spin 0 they run on a real plan (plan.real), which holds only the
+// n >= 0 direction columns (half the contraction, theta and row work of the complex
+// transform): column -n of a real signal is the conjugate of column n, so the
+// sample-side outputs are the real parts of the half sums with n > 0 counted twice,
+// and the harmonic-side outputs are Y + H(Y) (H(Y)_lm = (-1)^m conj(Y_l,-m)) of the
+// half sum Y with n = 0 and the scaling function at half weight.  A non-Hermitian
+// flm gives the transform of its Hermitian part.  Other spins promote real inputs to
+// complex, run the complex kernels and keep real parts.
 struct RealWork {
   Plan &P;
-  size_t nwav, nscal, npx;
-  explicit RealWork(Plan &pl) : P(pl) {
+  size_t nlm, nwav, nscal, npx;
+  RealWork(Plan &pl, bool host) : P(pl) {
     const int L = pl.p.L;
+    nlm = (size_t)L * L;
     nwav = pl.n_wav_used;
     const int Ls = pl.groups[0].L;  // scaling-function band-limit
     nscal = map_elems(Ls, pl.ss);
     npx = map_elems(L, pl.ss);
-    if (pl.r_wav.bytes < nwav * CB) pl.r_wav.alloc(nwav * CB);
+    if (!pl.real && pl.r_wav.bytes < nwav * CB) pl.r_wav.alloc(nwav * CB);
     if (pl.r_scal.bytes < nscal * CB) pl.r_scal.alloc(nscal * CB);
+    if (!host) return;
+    // host-pointer and pixel-space variants: staging of maps, harmonics and real samples
     if (pl.r_map.bytes < npx * CB) pl.r_map.alloc(npx * CB);
     const size_t nr = std::max(nwav, npx);
     if (pl.r_real.bytes < nr * sizeof(double)) pl.r_real.alloc(nr * sizeof(double));
-    if (pl.r_real2.bytes < std::max(nscal, (size_t)L * L) * CB) pl.r_real2.alloc(std::max(nscal, (size_t)L * L) * CB);
+    if (pl.r_real2.bytes < std::max(nscal, nlm) * CB) pl.r_real2.alloc(std::max(nscal, nlm) * CB);
+    pl.h_flm.alloc(nlm * CB);
+    pl.h_tmp.alloc(nlm * CB);
   }
 };
 
-// device-pointer real transforms on plan P (f_wav, f_scal, maps real; flm complex128)
+// real transforms on plan P (f_wav, f_scal, maps real; flm complex128; device pointers)
 void real_analysis(Plan &P, const Scratch &X, RealWork &w, const void *flm, double *f_wav, double *f_scal, hipStream_t st) {
+  if (P.real) return run_real_analysis(P, X, flm, f_wav, f_scal, P.r_scal.p, w.nscal, st);
   run_analysis(P, X, flm, P.r_wav.p, P.r_scal.p, st);
   launch_cx2re(P.r_wav.p, f_wav, w.nwav, st);
   launch_cx2re(P.r_scal.p, f_scal, w.nscal, st);
 }
 void real_synthesis(Plan &P, const Scratch &X, RealWork &w, const double *f_wav, const double *f_scal, void *flm, hipStream_t st) {
+  if (P.real) return run_real_synthesis(P, X, f_wav, f_scal, flm, P.r_scal.p, w.nscal, st);
   launch_re2cx(f_wav, P.r_wav.p, w.nwav, st);
   launch_re2cx(f_scal, P.r_scal.p, w.nscal, st);
   run_synthesis(P, X, P.r_wav.p, P.r_scal.p, flm, st);
 }
 void real_analysis_adjoint(Plan &P, const Scratch &X, RealWork &w, const double *f_wav, const double *f_scal, void *flm, hipStream_t st) {
+  if (P.real) return run_real_analysis_adjoint(P, X, f_wav, f_scal, flm, P.r_scal.p, w.nscal, st);
   launch_re2cx(f_wav, P.r_wav.p, w.nwav, st);
   launch_re2cx(f_scal, P.r_scal.p, w.nscal, st);
   run_analysis_adjoint(P, X, P.r_wav.p, P.r_scal.p, flm, st);
 }
 void real_synthesis_adjoint(Plan &P, const Scratch &X, RealWork &w, const void *flm, double *f_wav, double *f_scal, hipStream_t st) {
+  if (P.real) return run_real_synthesis_adjoint(P, X, flm, f_wav, f_scal, P.r_scal.p, w.nscal, st);
   run_synthesis_adjoint(P, X, flm, P.r_wav.p, P.r_scal.p, st);
   launch_cx2re(P.r_wav.p, f_wav, w.nwav, st);
   launch_cx2re(P.r_scal.p, f_scal, w.nscal, st);
@@ -352,6 +366,16 @@ void s2let_hip_release(void) {
 }
 
 // ---------------------------------------------------------------- device API
+// real signals: the half-spectrum plan for spin 0
+#define S2H_DEV_REAL_PLAN                                   \
+  Context &C = Context::get();                              \
+  std::lock_guard<std::mutex> lk(C.mu);                     \
+  hipStream_t st = (hipStream_t)stream;                     \
+  Plan &P = C.plan(p, mode, st, p->spin == 0);              \
+  const Scratch X = C.scratch(P);                           \
+  PlanUse use_(P.sync, st);                                 \
+  RealWork w(P, false);
+
 #define S2H_DEV_PLAN                              \
   Context &C = Context::get();                    \
   std::lock_guard<std::mutex> lk(C.mu);           \
@@ -469,8 +493,7 @@ extern "C" {
 int s2let_hip_analysis_lm2wav_real(double *f_wav, double *f_scal, const void *flm, const s2let_parameters_t *p,
                                    int mode, void *stream) {
   return guarded([&] {
-    S2H_DEV_PLAN
-    RealWork w(P);
+    S2H_DEV_REAL_PLAN
     real_analysis(P, X, w, flm, f_wav, f_scal, st);
     S2H_CHECK_HIP(hipGetLastError());
   });
@@ -478,8 +501,7 @@ int s2let_hip_analysis_lm2wav_real(double *f_wav, double *f_scal, const void *fl
 int s2let_hip_synthesis_wav2lm_real(void *flm, const double *f_wav, const double *f_scal, const s2let_parameters_t *p,
                                     int mode, void *stream) {
   return guarded([&] {
-    S2H_DEV_PLAN
-    RealWork w(P);
+    S2H_DEV_REAL_PLAN
     real_synthesis(P, X, w, f_wav, f_scal, flm, st);
     S2H_CHECK_HIP(hipGetLastError());
   });
@@ -487,8 +509,7 @@ int s2let_hip_synthesis_wav2lm_real(void *flm, const double *f_wav, const double
 int s2let_hip_analysis_adjoint_wav2lm_real(void *flm, const double *f_wav, const double *f_scal,
                                            const s2let_parameters_t *p, int mode, void *stream) {
   return guarded([&] {
-    S2H_DEV_PLAN
-    RealWork w(P);
+    S2H_DEV_REAL_PLAN
     real_analysis_adjoint(P, X, w, f_wav, f_scal, flm, st);
     S2H_CHECK_HIP(hipGetLastError());
   });
@@ -496,8 +517,7 @@ int s2let_hip_analysis_adjoint_wav2lm_real(void *flm, const double *f_wav, const
 int s2let_hip_synthesis_adjoint_lm2wav_real(double *f_wav, double *f_scal, const void *flm,
                                             const s2let_parameters_t *p, int mode, void *stream) {
   return guarded([&] {
-    S2H_DEV_PLAN
-    RealWork w(P);
+    S2H_DEV_REAL_PLAN
     real_synthesis_adjoint(P, X, w, flm, f_wav, f_scal, st);
     S2H_CHECK_HIP(hipGetLastError());
   });
@@ -778,6 +798,16 @@ int s2let_hip_batch(int op, int nmaps, void *const *out_a, void *const *out_b, c
   const Scratch X = C.scratch(P);       \
   PlanUse use_(P.sync, st);             \
   HostIO io(P);
+
+// real signals: the half-spectrum plan for spin 0 (FP64)
+#define S2H_HOST_REAL_PLAN                                  \
+  Context &C = Context::get();                              \
+  std::lock_guard<std::mutex> lk(C.mu);                     \
+  hipStream_t st = C.stream();                              \
+  Plan &P = C.plan(p, S2LET_HIP_FP64, st, p->spin == 0);    \
+  const Scratch X = C.scratch(P);                           \
+  PlanUse use_(P.sync, st);                                 \
+  RealWork w(P, true);
 
 void s2let_analysis_lm2wav(std::complex<double> *f_wav, std::complex<double> *f_scal, const std::complex<double> *flm,
                            const s2let_parameters_t *p) {
@@ -1140,9 +1170,8 @@ void s2let_analysis_lm2wav_real(double *f_wav, double *f_scal, const std::comple
                                 const s2let_parameters_t *p) {
   require_mw(p, __func__);
   host_call(__func__, [&] {
-    S2H_HOST_PLAN
-    RealWork w(P);
-    h2d(P.h_flm.p, flm, io.nlm * CB, st);
+    S2H_HOST_REAL_PLAN
+    h2d(P.h_flm.p, flm, w.nlm * CB, st);
     real_analysis(P, X, w, P.h_flm.p, P.r_real.as<double>(), P.r_real2.as<double>(), st);
     d2h(f_wav, P.r_real.p, w.nwav * DB, st);
     d2h(f_scal, P.r_real2.p, w.nscal * DB, st);
@@ -1154,12 +1183,11 @@ void s2let_synthesis_wav2lm_real(std::complex<double> *flm, const double *f_wav,
                                  const s2let_parameters_t *p) {
   require_mw(p, __func__);
   host_call(__func__, [&] {
-    S2H_HOST_PLAN
-    RealWork w(P);
+    S2H_HOST_REAL_PLAN
     h2d(P.r_real.p, f_wav, w.nwav * DB, st);
     h2d(P.r_real2.p, f_scal, w.nscal * DB, st);
     real_synthesis(P, X, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_flm.p, st);
-    d2h(flm, P.h_flm.p, io.nlm * CB, st);
+    d2h(flm, P.h_flm.p, w.nlm * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
   });
 }
@@ -1167,8 +1195,7 @@ void s2let_synthesis_wav2lm_real(std::complex<double> *flm, const double *f_wav,
 void s2let_analysis_px2wav_real(double *f_wav, double *f_scal, const double *f, const s2let_parameters_t *p) {
   require_mw(p, __func__);
   host_call(__func__, [&] {
-    S2H_HOST_PLAN
-    RealWork w(P);
+    S2H_HOST_REAL_PLAN
     ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st, P.ss);  // real maps: spin 0
     PlanUse suse_(S.sync, st);
     h2d(P.r_real.p, f, w.npx * DB, st);
@@ -1184,8 +1211,7 @@ void s2let_analysis_px2wav_real(double *f_wav, double *f_scal, const double *f, 
 void s2let_synthesis_wav2px_real(double *f, const double *f_wav, const double *f_scal, const s2let_parameters_t *p) {
   require_mw(p, __func__);
   host_call(__func__, [&] {
-    S2H_HOST_PLAN
-    RealWork w(P);
+    S2H_HOST_REAL_PLAN
     ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st, P.ss);
     PlanUse suse_(S.sync, st);
     h2d(P.r_real.p, f_wav, w.nwav * DB, st);
@@ -1202,12 +1228,11 @@ void s2let_analysis_adjoint_wav2lm_real(std::complex<double> *flm, const double 
                                         const s2let_parameters_t *p) {
   require_mw(p, __func__);
   host_call(__func__, [&] {
-    S2H_HOST_PLAN
-    RealWork w(P);
+    S2H_HOST_REAL_PLAN
     h2d(P.r_real.p, f_wav, w.nwav * DB, st);
     h2d(P.r_real2.p, f_scal, w.nscal * DB, st);
     real_analysis_adjoint(P, X, w, P.r_real.as<double>(), P.r_real2.as<double>(), P.h_flm.p, st);
-    d2h(flm, P.h_flm.p, io.nlm * CB, st);
+    d2h(flm, P.h_flm.p, w.nlm * CB, st);
     S2H_CHECK_HIP(hipStreamSynchronize(st));
   });
 }
@@ -1216,8 +1241,7 @@ void s2let_analysis_adjoint_wav2px_real(double *f, const double *f_wav, const do
                                         const s2let_parameters_t *p) {
   require_mw(p, __func__);
   host_call(__func__, [&] {
-    S2H_HOST_PLAN
-    RealWork w(P);
+    S2H_HOST_REAL_PLAN
     ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st, P.ss);
     PlanUse suse_(S.sync, st);
     h2d(P.r_real.p, f_wav, w.nwav * DB, st);
@@ -1234,9 +1258,8 @@ void s2let_synthesis_adjoint_lm2wav_real(double *f_wav, double *f_scal, const st
                                          const s2let_parameters_t *p) {
   require_mw(p, __func__);
   host_call(__func__, [&] {
-    S2H_HOST_PLAN
-    RealWork w(P);
-    h2d(P.h_flm.p, flm, io.nlm * CB, st);
+    S2H_HOST_REAL_PLAN
+    h2d(P.h_flm.p, flm, w.nlm * CB, st);
     real_synthesis_adjoint(P, X, w, P.h_flm.p, P.r_real.as<double>(), P.r_real2.as<double>(), st);
     d2h(f_wav, P.r_real.p, w.nwav * DB, st);
     d2h(f_scal, P.r_real2.p, w.nscal * DB, st);
@@ -1248,8 +1271,7 @@ void s2let_synthesis_adjoint_px2wav_real(double *f_wav, double *f_scal, const do
                                          const s2let_parameters_t *p) {
   require_mw(p, __func__);
   host_call(__func__, [&] {
-    S2H_HOST_PLAN
-    RealWork w(P);
+    S2H_HOST_REAL_PLAN
     ShtPlan &S = C.sht(p->L, 0, S2LET_HIP_FP64, st, P.ss);
     PlanUse suse_(S.sync, st);
     h2d(P.r_real.p, f, w.npx * DB, st);

@@ -116,6 +116,10 @@ void launch_add_cx(void *a, const void *b, size_t n, hipStream_t st);  // comple
 void launch_hardthreshold(double *g, size_t n, double thr, hipStream_t st);
 void launch_re2cx(const double *a, void *b, size_t n, hipStream_t st);  // complex128 b = a + 0i
 void launch_cx2re(const void *a, double *b, size_t n, hipStream_t st);  // b = Re(complex128 a)
+void launch_re2cx_scaled(const double *a, void *b, size_t n, double s, hipStream_t st);  // b = s a + 0i
+// complex128 y (L^2, index l^2 + l + m) <- y + H(y), H(y)_lm = (-1)^m conj(y_l,-m): the
+// harmonics of a real signal from the half-spectrum sum Y
+void launch_herm_sym(void *y, int L, hipStream_t st);
 
 // matrix-core contractions (contract_mfma.hip).
 // Delta tiles (16 x 16 tiles of Delta^l_{m'm}, one 256-entry record per m'):
@@ -232,7 +236,7 @@ inline int mix_blocks(size_t npts) {
   return (int)std::max<size_t>(1, std::min<size_t>(kMixMaxBlocks, (npts + 511) / 512));
 }
 template <typename TI, typename TO, typename T>
-void launch_mix(const MixJobs &jobs, const MapSet &ms, hipStream_t st);
+void launch_mix(const MixJobs &jobs, const MapSet &ms, hipStream_t st, int re = 0);  // re: see k_mix
 
 void launch_lm2lmn(void *wlmn, const void *flm, const void *psi_j, int Lj, int Nj, int spin, int mode, hipStream_t st);
 void launch_lmn2lm(void *flm, const void *wlmn, const void *psi_j, int Lj, int Nj, int spin, int mode, hipStream_t st);

@@ -62,6 +62,28 @@ __global__ void k_re2cx(const double *__restrict__ a, cx<double> *__restrict__ b
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     b[i] = mk<double>(a[i], 0.0);
 }
+// b = s a (real to complex)
+__global__ void k_re2cx_scaled(const double *__restrict__ a, cx<double> *__restrict__ b, size_t n, double s) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = mk<double>(s * a[i], 0.0);
+}
+// y <- y + H(y), H(y)_lm = (-1)^m conj(y_l,-m): one thread per pair (l, m >= 0), pair index
+// q = l (l + 1) / 2 + m, both entries of the pair read before either is written
+__global__ void k_herm_sym(cx<double> *__restrict__ y, int L) {
+  const size_t npair = (size_t)L * (L + 1) / 2;
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < npair; q += (size_t)gridDim.x * blockDim.x) {
+    int l = (int)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+    while ((size_t)l * (l + 1) / 2 > q) --l;
+    while ((size_t)(l + 1) * (l + 2) / 2 <= q) ++l;
+    const int m = (int)(q - (size_t)l * (l + 1) / 2);
+    const size_t ip = (size_t)l * l + l + m, im = (size_t)l * l + l - m;
+    const cx<double> a = y[ip], b = y[im];
+    const double sg = (m & 1) ? -1.0 : 1.0;
+    y[ip] = mk<double>(a.x + sg * b.x, a.y - sg * b.y);
+    if (m > 0) y[im] = mk<double>(b.x + sg * a.x, b.y - sg * a.y);
+  }
+}
+
 __global__ void k_cx2re(const cx<double> *__restrict__ a, double *__restrict__ b, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     b[i] = a[i].x;
@@ -83,15 +105,17 @@ __global__ void k_add_cx(cx<double> *__restrict__ a, const cx<double> *__restric
 
 // ------------------------------------------------------------------ gamma mixing
 // per job: out_k[pt] = sum_j W[k][j] in_j[pt] (complex, nout x nin matrix W, nin <= NMAX),
-// mix_blocks(npts) blocks per job
-template <typename TI, typename TO, typename T, int NMAX>
+// mix_blocks(npts) blocks per job.  RE = 1: the output is real (the sum's real part,
+// TO per sample); RE = 2: the input is real (TI per sample).
+template <typename TI, typename TO, typename T, int NMAX, int RE>
 __global__ __launch_bounds__(256) void k_mix(MixJobs jobs, MapSet ms) {
   __shared__ cx<T> sW[32 * 32];
   int jb = 0;
   while (jb + 1 < jobs.n && (int)blockIdx.x >= jobs.j[jb + 1].blk0) ++jb;
   const MixJob &J = jobs.j[jb];
-  const cx<TI> *__restrict__ in = (const cx<TI> *)map_ptr(J.in, blockIdx.y, ms.scr_in, ms.dlt_in);
-  cx<TO> *__restrict__ out = (cx<TO> *)map_ptr(J.out, blockIdx.y, ms.scr_out, ms.dlt_out);
+  const char *__restrict__ in = map_ptr(J.in, blockIdx.y, ms.scr_in, ms.dlt_in);
+  char *__restrict__ out = map_ptr(J.out, blockIdx.y, ms.scr_out, ms.dlt_out);
+  constexpr size_t isz = RE == 2 ? sizeof(TI) : sizeof(cx<TI>), osz = RE == 1 ? sizeof(TO) : sizeof(cx<TO>);
   const cx<double> *__restrict__ W = static_cast<const cx<double> *>(J.W);
   const int nin = J.nin, nout = J.nout;
   for (int i = threadIdx.x; i < nin * nout; i += blockDim.x) sW[i] = mk<T>((T)W[i].x, (T)W[i].y);
@@ -113,14 +137,14 @@ __global__ __launch_bounds__(256) void k_mix(MixJobs jobs, MapSet ms) {
     for (int h = 0; h < 2; ++h)
 #pragma unroll
       for (int j = 0; j < NMAX; ++j) {
-        const cx<TI> *src = in + min(j, nin - 1) * J.in_stride + pts[h];
-        TI ux, uy;
+        const TI *src = (const TI *)(in + (min(j, nin - 1) * J.in_stride + pts[h]) * isz);
+        TI ux, uy = 0;
         if constexpr (sizeof(TI) == 8) {
-          ux = __builtin_nontemporal_load(&src->x);
-          uy = __builtin_nontemporal_load(&src->y);
+          ux = __builtin_nontemporal_load(src);
+          if constexpr (RE != 2) uy = __builtin_nontemporal_load(src + 1);
         } else {
-          ux = src->x;
-          uy = src->y;
+          ux = src[0];
+          if constexpr (RE != 2) uy = src[1];
         }
         const T z = (j < nin) ? (T)1 : (T)0;
         v[h][j] = mk<T>((T)ux * z, (T)uy * z);
@@ -141,12 +165,15 @@ __global__ __launch_bounds__(256) void k_mix(MixJobs jobs, MapSet ms) {
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         if (h == 1 && !ok1) break;
-        cx<TO> *dst = out + k * J.out_stride + pts[h];
-        if constexpr (sizeof(TO) == 8 && sizeof(T) == 4) {
-          __builtin_nontemporal_store((TO)ax[h], &dst->x);
-          __builtin_nontemporal_store((TO)ay[h], &dst->y);
+        TO *dst = (TO *)(out + (k * J.out_stride + pts[h]) * osz);
+        if constexpr (RE == 1) {
+          if constexpr (sizeof(TO) == 8 && sizeof(T) == 4) __builtin_nontemporal_store((TO)ax[h], dst);
+          else *dst = (TO)ax[h];
+        } else if constexpr (sizeof(TO) == 8 && sizeof(T) == 4) {
+          __builtin_nontemporal_store((TO)ax[h], dst);
+          __builtin_nontemporal_store((TO)ay[h], dst + 1);
         } else {
-          *dst = mk<TO>((TO)ax[h], (TO)ay[h]);
+          *(cx<TO> *)dst = mk<TO>((TO)ax[h], (TO)ay[h]);
         }
       }
     }
@@ -183,19 +210,34 @@ void launch_re2cx(const double *a, void *b, size_t n, hipStream_t st) {
 void launch_cx2re(const void *a, double *b, size_t n, hipStream_t st) {
   if (n) hipLaunchKernelGGL(k_cx2re, dim3(grid1d(n)), dim3(256), 0, st, (const cx<double> *)a, b, n);
 }
+void launch_re2cx_scaled(const double *a, void *b, size_t n, double s, hipStream_t st) {
+  if (n) hipLaunchKernelGGL(k_re2cx_scaled, dim3(grid1d(n)), dim3(256), 0, st, a, (cx<double> *)b, n, s);
+}
+void launch_herm_sym(void *y, int L, hipStream_t st) {
+  const size_t np = (size_t)L * (L + 1) / 2;
+  if (np) hipLaunchKernelGGL(k_herm_sym, dim3(grid1d(np)), dim3(256), 0, st, (cx<double> *)y, L);
+}
 
-template <typename TI, typename TO, typename T>
-void launch_mix(const MixJobs &jobs, const MapSet &ms, hipStream_t st) {
-  if (jobs.n == 0 || jobs.nblk == 0) return;
+template <typename TI, typename TO, typename T, int RE>
+static void launch_mix_re(const MixJobs &jobs, const MapSet &ms, hipStream_t st) {
   int nmax = 0;
   for (int k = 0; k < jobs.n; ++k) nmax = std::max(nmax, jobs.j[k].nin);
-  if (nmax <= 8) hipLaunchKernelGGL((k_mix<TI, TO, T, 8>), dim3(jobs.nblk, ms.n), dim3(256), 0, st, jobs, ms);
-  else if (nmax <= 16) hipLaunchKernelGGL((k_mix<TI, TO, T, 16>), dim3(jobs.nblk, ms.n), dim3(256), 0, st, jobs, ms);
-  else hipLaunchKernelGGL((k_mix<TI, TO, T, 32>), dim3(jobs.nblk, ms.n), dim3(256), 0, st, jobs, ms);
+  const dim3 grid(jobs.nblk, ms.n);
+  if (nmax <= 8) hipLaunchKernelGGL((k_mix<TI, TO, T, 8, RE>), grid, dim3(256), 0, st, jobs, ms);
+  else if (nmax <= 16) hipLaunchKernelGGL((k_mix<TI, TO, T, 16, RE>), grid, dim3(256), 0, st, jobs, ms);
+  else hipLaunchKernelGGL((k_mix<TI, TO, T, 32, RE>), grid, dim3(256), 0, st, jobs, ms);
 }
-template void launch_mix<double, double, double>(const MixJobs &, const MapSet &, hipStream_t);
-template void launch_mix<float, double, float>(const MixJobs &, const MapSet &, hipStream_t);
-template void launch_mix<double, float, float>(const MixJobs &, const MapSet &, hipStream_t);
-template void launch_mix<float, float, float>(const MixJobs &, const MapSet &, hipStream_t);
+
+template <typename TI, typename TO, typename T>
+void launch_mix(const MixJobs &jobs, const MapSet &ms, hipStream_t st, int re) {
+  if (jobs.n == 0 || jobs.nblk == 0) return;
+  if (re == 1 && sizeof(TO) == 8) launch_mix_re<TI, TO, T, 1>(jobs, ms, st);  // real f_wav out
+  else if (re == 2 && sizeof(TI) == 8) launch_mix_re<TI, TO, T, 2>(jobs, ms, st);  // real f_wav in
+  else launch_mix_re<TI, TO, T, 0>(jobs, ms, st);
+}
+template void launch_mix<double, double, double>(const MixJobs &, const MapSet &, hipStream_t, int);
+template void launch_mix<float, double, float>(const MixJobs &, const MapSet &, hipStream_t, int);
+template void launch_mix<double, float, float>(const MixJobs &, const MapSet &, hipStream_t, int);
+template void launch_mix<float, float, float>(const MixJobs &, const MapSet &, hipStream_t, int);
 
 }  // namespace s2h

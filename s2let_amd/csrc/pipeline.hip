@@ -335,7 +335,9 @@ void rows_all(Plan &P, void *in, void *out, const void *scal_in, void *scal_out,
 // 3 forward adjoint (the mixing matrices of the plan)
 template <typename TI, typename TO, typename T>
 void mix_groups(Plan &P, int which, bool to_wav, const void *src_wav, void *dst_wav, void *cols, void *cols_out,
-                hipStream_t st, const BatchIO &B) {
+                hipStream_t st, const BatchIO &B, int re = 0) {
+  // f_wav entries: complex128, or double when the samples are real (re = 1 out, 2 in)
+  const size_t wsz = re ? sizeof(double) : 2 * sizeof(double);
   MixJobs J{};
   MapSet ms;  // the column spectra in scratch, f_wav through its per-map offsets
   ms.n = B.n;
@@ -348,7 +350,7 @@ void mix_groups(Plan &P, int which, bool to_wav, const void *src_wav, void *dst_
   }
   auto flush = [&]() {
     Stage s_("gamma", st);
-    launch_mix<TI, TO, T>(J, ms, st);
+    launch_mix<TI, TO, T>(J, ms, st, re);
     J.n = 0;
     J.nblk = 0;
   };
@@ -361,10 +363,10 @@ void mix_groups(Plan &P, int which, bool to_wav, const void *src_wav, void *dst_
     if (to_wav) {
       j.in = at<T>(cols, g.spec_off);
       j.nin = g.ncol;
-      j.out = at64(dst_wav, g.out_off);
+      j.out = (char *)dst_wav + g.out_off * wsz;
       j.nout = g.ngamma;
     } else {
-      j.in = at64(src_wav, g.out_off);
+      j.in = (const char *)src_wav + g.out_off * wsz;
       j.nin = g.ngamma;
       j.out = at<T>(cols_out, g.spec_off);
       j.nout = g.ncol;
@@ -417,6 +419,61 @@ void synthesis_adjoint_t(Plan &P, const Scratch &X, const void *flm, void *f_wav
   theta_groups<T>(P, cols, 3, X.spec, X.work, "beta_fwd_adj", st, B);
   rows_all<T>(P, X.work, X.spec, nullptr, f_scal, 1, 0, +1, true, st, B);
   mix_groups<T, double, T>(P, 3, true, nullptr, f_wav, X.spec, X.spec, st, B);
+}
+
+// ---- real signals on a real plan (columns n >= 0; spin 0).  f_wav and f_scal are real;
+// scal_tmp: complex128 staging of the scaling-function samples (nscal entries).
+// Sample-side outputs are the real parts of the half-spectrum sums (the mixing counts
+// n > 0 twice); harmonic-side outputs are Y + H(Y) with Y the half-spectrum sum (the
+// mixing and the scaling function at half weight for n = 0).
+template <typename T>
+void real_analysis_t(Plan &P, const Scratch &X, const void *flm, double *f_wav, double *f_scal, void *scal_tmp,
+                     size_t nscal, hipStream_t st) {
+  const BatchIO B;
+  const ColDev *cols = P.d_cols_ana.as<ColDev>();
+  contract_inv_all<T>(P, X, flm, cols, X.spec, st, B);
+  theta_groups<T>(P, cols, 0, X.spec, X.work, "beta_inv", st, B);
+  rows_all<T>(P, X.work, X.spec, nullptr, scal_tmp, 1, 0, +1, false, st, B);
+  launch_cx2re(scal_tmp, f_scal, nscal, st);
+  mix_groups<T, double, T>(P, 0, true, nullptr, f_wav, X.spec, X.spec, st, B, 1);
+}
+
+template <typename T>
+void real_synthesis_t(Plan &P, const Scratch &X, const double *f_wav, const double *f_scal, void *flm,
+                      void *scal_tmp, size_t nscal, hipStream_t st) {
+  const BatchIO B;
+  const ColDev *cols = P.d_cols_syn.as<ColDev>();
+  launch_re2cx_scaled(f_scal, scal_tmp, nscal, 0.5, st);
+  mix_groups<double, T, T>(P, 2, false, f_wav, nullptr, X.work, X.work, st, B, 2);
+  rows_all<T>(P, X.work, X.spec, scal_tmp, nullptr, 0, 1, -1, true, st, B);
+  theta_groups<T>(P, cols, 2, X.spec, X.work, "beta_fwd", st, B);
+  contract_fwd_all<T>(P, X, cols, X.work, flm, st, B);
+  launch_herm_sym(flm, P.p.L, st);
+}
+
+template <typename T>
+void real_analysis_adjoint_t(Plan &P, const Scratch &X, const double *f_wav, const double *f_scal, void *flm,
+                             void *scal_tmp, size_t nscal, hipStream_t st) {
+  const BatchIO B;
+  const ColDev *cols = P.d_cols_ana.as<ColDev>();
+  launch_re2cx_scaled(f_scal, scal_tmp, nscal, 0.5, st);
+  mix_groups<double, T, T>(P, 1, false, f_wav, nullptr, X.work, X.work, st, B, 2);
+  rows_all<T>(P, X.work, X.spec, scal_tmp, nullptr, 0, 1, -1, false, st, B);
+  theta_groups<T>(P, cols, 1, X.spec, X.work, "beta_inv_adj", st, B);
+  contract_fwd_all<T>(P, X, cols, X.work, flm, st, B);
+  launch_herm_sym(flm, P.p.L, st);
+}
+
+template <typename T>
+void real_synthesis_adjoint_t(Plan &P, const Scratch &X, const void *flm, double *f_wav, double *f_scal,
+                              void *scal_tmp, size_t nscal, hipStream_t st) {
+  const BatchIO B;
+  const ColDev *cols = P.d_cols_syn.as<ColDev>();
+  contract_inv_all<T>(P, X, flm, cols, X.spec, st, B);
+  theta_groups<T>(P, cols, 3, X.spec, X.work, "beta_fwd_adj", st, B);
+  rows_all<T>(P, X.work, X.spec, nullptr, scal_tmp, 1, 0, +1, true, st, B);
+  launch_cx2re(scal_tmp, f_scal, nscal, st);
+  mix_groups<T, double, T>(P, 3, true, nullptr, f_wav, X.spec, X.spec, st, B, 1);
 }
 
 // ---- single spin-s transform at L (one column)
@@ -530,6 +587,22 @@ void run_analysis_adjoint(Plan &P, const Scratch &X, const void *f_wav, const vo
 void run_synthesis_adjoint(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st,
                            const BatchIO &B) {
   S2H_DISPATCH(P.mode, synthesis_adjoint_t, P, X, flm, f_wav, f_scal, st, B);
+}
+void run_real_analysis(Plan &P, const Scratch &X, const void *flm, double *f_wav, double *f_scal, void *scal_tmp,
+                       size_t nscal, hipStream_t st) {
+  S2H_DISPATCH(P.mode, real_analysis_t, P, X, flm, f_wav, f_scal, scal_tmp, nscal, st);
+}
+void run_real_synthesis(Plan &P, const Scratch &X, const double *f_wav, const double *f_scal, void *flm,
+                        void *scal_tmp, size_t nscal, hipStream_t st) {
+  S2H_DISPATCH(P.mode, real_synthesis_t, P, X, f_wav, f_scal, flm, scal_tmp, nscal, st);
+}
+void run_real_analysis_adjoint(Plan &P, const Scratch &X, const double *f_wav, const double *f_scal, void *flm,
+                               void *scal_tmp, size_t nscal, hipStream_t st) {
+  S2H_DISPATCH(P.mode, real_analysis_adjoint_t, P, X, f_wav, f_scal, flm, scal_tmp, nscal, st);
+}
+void run_real_synthesis_adjoint(Plan &P, const Scratch &X, const void *flm, double *f_wav, double *f_scal,
+                                void *scal_tmp, size_t nscal, hipStream_t st) {
+  S2H_DISPATCH(P.mode, real_synthesis_adjoint_t, P, X, flm, f_wav, f_scal, scal_tmp, nscal, st);
 }
 void run_sht_inverse(ShtPlan &S, const void *flm, void *f, hipStream_t st) {
   S2H_DISPATCH(S.mode, sht_inverse_t, S, flm, f, st);

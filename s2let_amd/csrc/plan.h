@@ -127,6 +127,7 @@ struct Plan {
   std::vector<long long> dtab_host;  // what b_dtab holds once its last upload has run
   PlanSync dtab_copied;               // recorded after that upload (dtab_host is its source)
   bool manual = false;
+  bool real = false;  // half spectrum of a real signal: columns n >= 0 only (spin 0)
   uint64_t last_use = 0;  // plan-cache clock (manual plans are evicted least recently used first)
   PlanSync sync;
   size_t xtile_elems_ = 0, parts_elems = 0;  // sizes of the scratch parts (entries)
@@ -174,7 +175,7 @@ class Context {
   const float *tile32() const { return tile32_.as<float>(); }
   const DftTables &dft(int n);
   const LTables &ltab(int L, int ss = 0);
-  Plan &plan(const s2let_parameters_t *p, int mode, hipStream_t st);
+  Plan &plan(const s2let_parameters_t *p, int mode, hipStream_t st, bool real = false);
   Plan *find_plan(const s2let_parameters_t *p, int mode);  // cached plan or null (builds nothing)
   Plan &plan_manual(const ManualTiling &m, int mode, hipStream_t st);
   Scratch scratch(Plan &pl);  // the plan's own scratch, allocated on first use
@@ -232,7 +233,17 @@ void run_analysis_adjoint(Plan &P, const Scratch &X, const void *f_wav, const vo
                           hipStream_t st, const BatchIO &B = BatchIO());
 void run_synthesis_adjoint(Plan &P, const Scratch &X, const void *flm, void *f_wav, void *f_scal, hipStream_t st,
                            const BatchIO &B = BatchIO());
-std::string plan_key(const s2let_parameters_t *p, int mode);
+std::string plan_key(const s2let_parameters_t *p, int mode, bool real = false);
+// Real signals on a real plan (P.real): f_wav, f_scal real; scal_tmp complex128 staging of
+// nscal scaling-function samples.
+void run_real_analysis(Plan &P, const Scratch &X, const void *flm, double *f_wav, double *f_scal, void *scal_tmp,
+                       size_t nscal, hipStream_t st);
+void run_real_synthesis(Plan &P, const Scratch &X, const double *f_wav, const double *f_scal, void *flm,
+                        void *scal_tmp, size_t nscal, hipStream_t st);
+void run_real_analysis_adjoint(Plan &P, const Scratch &X, const double *f_wav, const double *f_scal, void *flm,
+                               void *scal_tmp, size_t nscal, hipStream_t st);
+void run_real_synthesis_adjoint(Plan &P, const Scratch &X, const void *flm, double *f_wav, double *f_scal,
+                                void *scal_tmp, size_t nscal, hipStream_t st);
 void run_sht_inverse(ShtPlan &S, const void *flm, void *f, hipStream_t st);
 void run_sht_forward(ShtPlan &S, const void *f, void *flm, hipStream_t st);
 void run_sht_inverse_adjoint(ShtPlan &S, const void *f, void *flm, hipStream_t st);

@@ -301,11 +301,11 @@ static inline void ipow(int k, double &re, double &im) {
   im = (k == 1) ? 1 : (k == 3 ? -1 : 0);
 }
 
-std::string plan_key(const s2let_parameters_t *p, int mode) {
+std::string plan_key(const s2let_parameters_t *p, int mode, bool real) {
   std::ostringstream o;
   o.precision(17);
   o << p->L << ',' << p->N << ',' << p->B << ',' << p->J_min << ',' << p->spin << ',' << p->original_spin << ','
-    << p->upsample << ',' << p->sampling_scheme << ',' << kernel_type() << ',' << mode;
+    << p->upsample << ',' << p->sampling_scheme << ',' << kernel_type() << ',' << mode << (real ? ",real" : "");
   return o.str();
 }
 
@@ -363,8 +363,8 @@ Plan *Context::find_plan(const s2let_parameters_t *p, int mode) {
   return it->second.get();
 }
 
-Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
-  const std::string key = plan_key(p, mode);
+Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st, bool real) {
+  const std::string key = plan_key(p, mode, real);
   auto it = plans_.find(key);
   if (it != plans_.end()) {
     it->second->last_use = ++clock_;
@@ -382,6 +382,7 @@ Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
   Plan &pl = *P;
   pl.p = *p;
   pl.mode = mode;
+  pl.real = real;
   pl.ss = p->sampling_scheme == S2LET_SAMPLING_MW_SS;
   pl.J = j_max(p);
   const int L = p->L, J = pl.J;
@@ -397,7 +398,7 @@ Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st) {
     scale_shape(p, j, &Lj, &Nj);
     Group g = cb.open(Lj, Nj, j);
     const int L0 = el0(j, p);
-    for (int n = -Nj + 1; n < Nj; n += 2) {
+    for (int n = real ? (Nj + 1) % 2 : -Nj + 1; n < Nj; n += 2) {  // a real signal: n >= 0
       // Psi^j_ln = nu * r(l): analysis uses conj(Psi) (phase i^n conj(nu)), synthesis Psi (i^n nu)
       std::vector<double> b(L, 0.0);
       const int lmin = std::max(std::max(aspin, std::abs(n)), L0);
@@ -601,13 +602,20 @@ void Context::finish_plan(Plan &pl, hipStream_t st) {
   for (const Group &g : pl.groups) ltab(g.L, pl.ss);
   // gamma mixing matrices per scale: [inv (G x nc), inv_adj (nc x G), fwd (nc x G), fwd_adj (G x nc)]
   std::vector<cx<double>> gm;
+  // Real plans (columns n >= 0 of a real signal, whose column -n is the conjugate of
+  // column n): the sample-side outputs (matrices 0 and 3) take the real part of the sum
+  // over n >= 0 with the n > 0 columns counted twice; the harmonic-side outputs
+  // (matrices 1 and 2) become Y with f_lm = Y + H(Y), H(Y)_lm = (-1)^m conj(Y_l,-m), so
+  // the n = 0 column enters Y at half weight.
   for (const Group &g : pl.groups) {
     pl.gamma_off.push_back(gm.size());
     if (g.ngamma == 0) continue;
     const int G = g.ngamma, nc = g.ncol;
     auto E = [&](int c, int gi, int s) {
-      const long double a = s * 2.0L * 3.14159265358979323846264338327950288L * pl.cols_ana[g.col0 + c].n * gi / G;
-      return std::complex<long double>(cosl(a), sinl(a));
+      const int n = pl.cols_ana[g.col0 + c].n;
+      const long double a = s * 2.0L * 3.14159265358979323846264338327950288L * n * gi / G;
+      const long double w = !pl.real ? 1.0L : (s > 0 ? (n > 0 ? 2.0L : 1.0L) : (n == 0 ? 0.5L : 1.0L));
+      return std::complex<long double>(w * cosl(a), w * sinl(a));
     };
     for (int gi = 0; gi < G; ++gi)
       for (int c = 0; c < nc; ++c) {

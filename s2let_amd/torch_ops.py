@@ -14,7 +14,8 @@ import torch
 
 from . import FP64, MIXED, Parameters, check, lib, n_scal, n_wav, parameters  # noqa: F401
 
-__all__ = ["analysis", "synthesis", "analysis_adjoint", "synthesis_adjoint", "mw_alm2map", "mw_map2alm",
+__all__ = ["analysis", "synthesis", "analysis_adjoint", "synthesis_adjoint", "analysis_real", "synthesis_real",
+           "mw_alm2map", "mw_map2alm",
            "batch", "autotune", "WaveletAnalysis", "wavelet_analysis", "FP64", "MIXED", "parameters"]
 
 
@@ -61,6 +62,31 @@ def synthesis_adjoint(flm, p: Parameters, mode=FP64, out=None):
     check(lib().s2let_hip_synthesis_adjoint_lm2wav(_dev(fw), _dev(fs), _dev(flm), ctypes.byref(p), int(mode),
                                                    _stream()))
     return fw, fs
+
+
+def _devr(t):
+    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise ValueError("expected a contiguous float64 tensor on the GPU")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def analysis_real(flm, p: Parameters, mode=FP64, out=None):
+    """s2let_analysis_lm2wav_real on device (flm Hermitian complex128): returns real (f_wav, f_scal)."""
+    if out is None:
+        out = (torch.zeros(int(n_wav(p)), dtype=torch.float64, device=flm.device),
+               torch.zeros(int(n_scal(p)), dtype=torch.float64, device=flm.device))
+    fw, fs = out
+    check(lib().s2let_hip_analysis_lm2wav_real(_devr(fw), _devr(fs), _dev(flm), ctypes.byref(p), int(mode),
+                                               _stream()))
+    return fw, fs
+
+
+def synthesis_real(f_wav, f_scal, p: Parameters, mode=FP64, out=None):
+    """s2let_synthesis_wav2lm_real on device (real f_wav, f_scal): returns the Hermitian flm."""
+    flm = out if out is not None else torch.zeros(p.L * p.L, dtype=torch.complex128, device=f_wav.device)
+    check(lib().s2let_hip_synthesis_wav2lm_real(_dev(flm), _devr(f_wav), _devr(f_scal), ctypes.byref(p), int(mode),
+                                                _stream()))
+    return flm
 
 
 def batch(op, outs_a, outs_b, ins_a, ins_b, params, mode=FP64, graph=True):

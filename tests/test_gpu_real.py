@@ -47,7 +47,10 @@ def s2():
 
 
 REF_STATES = [(16, 4, 2, 0, 1), (16, 4, 2, 2, 1), (16, 4, 2, 2, 0)]  # test/c/test_adjoint.c:476-480
-CASES = REF_STATES + [(64, 4, 0, 0, 0), (33, 3, 1, 0, 0), (48, 8, 1, -2, 0)]
+# spin 0 runs on the half-spectrum plan (n >= 0 columns): N = 1 (the n = 0 column
+# only), N = 2 (n = 1 only), odd N (n = 0, 2, ...), upsampled and multiresolution
+CASES = REF_STATES + [(64, 4, 0, 0, 0), (33, 3, 1, 0, 0), (48, 8, 1, -2, 0), (32, 1, 0, 0, 0), (32, 2, 1, 0, 1),
+                      (40, 5, 2, 0, 0)]
 
 
 @pytest.mark.parametrize("L,N,J_min,spin,up", CASES)
@@ -164,3 +167,53 @@ def test_device_real_entry_points(s2):
         bw, bs = s2.synthesis_adjoint_lm2wav_real(flm, p)
         _dev(p, s2, torch, "s2let_hip_synthesis_adjoint_lm2wav_real", [fw, fs], [dflm], mode)
         assert rel(fw.cpu().numpy(), bw) < tol and rel(fs.cpu().numpy(), bs) < tol
+
+
+@pytest.mark.parametrize("L,N,J_min,up", [(24, 4, 1, 0), (17, 3, 0, 1)])
+def test_real_mwss_match_oracle(s2, oracle, L, N, J_min, up):
+    """MWSS sampling on the half-spectrum plan: all four real transforms against the
+    complex oracle on the promoted input (real parts kept)."""
+    print("mwss real", L, N, J_min, up, flush=True)
+    p = s2.parameters(L, N, 2.0, J_min, 0, up, sampling_scheme=1)
+    q = oracle.params(L, N, 2.0, J_min, 0, up, sampling=1)
+    rng = np.random.default_rng(L + N)
+    flm = hermitian_flm(rng, L)
+    fw, fs = s2.analysis_lm2wav_real(flm, p)
+    ow, os_ = oracle.analysis_lm2wav(flm, q)
+    assert rel(fw, ow.real) < TOL and rel(fs, os_.real) < TOL
+    yw, ys = rng.standard_normal(s2.n_wav(p)), rng.standard_normal(s2.n_scal(p))
+    assert rel(s2.synthesis_wav2lm_real(yw, ys, p), oracle.synthesis_wav2lm(yw, ys, q)) < TOL
+    assert rel(s2.analysis_adjoint_wav2lm_real(yw, ys, p), oracle.analysis_adjoint_wav2lm(yw, ys, q)) < TOL
+    bw, bs = s2.synthesis_adjoint_lm2wav_real(flm, p)
+    cw, cs = oracle.synthesis_adjoint_lm2wav(flm, q)
+    assert rel(bw, cw.real) < TOL and rel(bs, cs.real) < TOL
+
+
+def test_half_spectrum_matches_complex_path_c2_shape(s2):
+    """L = 512, N = 8, B = 2, J_min = 2, spin 0, a real signal: the half-spectrum
+    device transforms against the complex device transforms on the promoted input,
+    FP64 within 1e-12 and mixed within 1e-6 (of the FP64 complex result); synthesis
+    of the analysis reproduces flm (FP64 1e-12)."""
+    import torch
+    from s2let_amd import torch_ops as ops
+    L, N = 512, 8
+    p = s2.parameters(L, N, 2.0, 2, 0)
+    rng = np.random.default_rng(9)
+    ell = np.repeat(np.arange(L), 2 * np.arange(L) + 1)
+    flm = hermitian_flm(rng, L) / (ell + 1.0)
+    dflm = torch.from_numpy(flm).cuda()
+    nw, ns = s2.n_wav(p), s2.n_scal(p)
+    cw, cs = ops.analysis(dflm, p, ops.FP64)
+    c_rec = ops.synthesis(cw, cs, p, ops.FP64)
+    ref_a = torch.cat([cw, cs]).real.cpu().numpy()
+    for mode, tol in ((s2.FP64, 1e-12), (s2.MIXED, 1e-6)):
+        print("half spectrum c2 shape, mode", mode, flush=True)
+        fw = torch.zeros(nw, dtype=torch.float64, device="cuda")
+        fs = torch.zeros(ns, dtype=torch.float64, device="cuda")
+        _dev(p, s2, torch, "s2let_hip_analysis_lm2wav_real", [fw, fs], [dflm], mode)
+        assert rel(torch.cat([fw, fs]).cpu().numpy(), ref_a) < tol
+        out = torch.zeros(L * L, dtype=torch.complex128, device="cuda")
+        _dev(p, s2, torch, "s2let_hip_synthesis_wav2lm_real", [out], [fw, fs], mode)
+        assert rel(out.cpu().numpy(), c_rec.cpu().numpy()) < tol
+        if mode == s2.FP64:
+            assert rel(out.cpu().numpy(), flm) < 1e-12
