@@ -82,11 +82,19 @@ struct ThetaArgs {
 // (k & 3) K/4 + (k >> 2) (four consecutive K steps of a lane in one 16-byte read),
 // strides K1 + 4 and K2 + 4.  perm[q] is the LDS offset, within a line, of the
 // real part of input sample q.  tw[k] = e^{-2 pi i k / M}.
+constexpr int kRowsDirectM1 = 32;  // largest M1 whose first product reads the lines directly
+
 template <typename T>
 struct Dft2 {
   int M, M1, M2;
   int R1, K1, R2, K2, CSA, CSC;
   const T *F1f, *F1b, *F2f, *F2b;
+  // M1 <= kRowsDirectM1: the first product's matrices with the reduction reordered for
+  // operands read straight from the lines (K step s of lane (r16, k4): sample
+  // n1 = 4 (s / 2) + k4, real part for even s, imaginary for odd), fragments
+  // [mt][s / 4][lane][s % 4] over nks1 K steps (a multiple of 4); else null
+  const T *F1fd, *F1bd;
+  int nks1;
   const cx<T> *tw;
   const int *perm;
 };
@@ -228,7 +236,7 @@ template <typename TI, typename TO, typename T>
 void launch_rows(const RowJobs<T> &jobs, const MapSet &ms, hipStream_t st);
 constexpr int kRowsItems = 8;  // first-product items a wave keeps in registers
 int rows_items_per_wave(int M1, int M2, int R1, int lpb);
-size_t rows_lds_bytes(int M1, int M2, int CSA, int CSC, int lpb, size_t esz);
+size_t rows_lds_bytes(int M1, int M2, int CSA, int CSC, int lpb, size_t esz, int R1, int nks1);
 // blocks of a mixing job over npts sample points: one pass of two points per thread
 // (256 threads; enough waves in flight to keep HBM busy), at most kMixMaxBlocks
 constexpr int kMixMaxBlocks = 4096;

@@ -179,6 +179,23 @@ const DftTables &Context::dft(int n) {
         for (int ln = 0; ln < 64; ++ln)
           out[base + ((size_t)mt * (K / 4) + s) * 64 + ln] = G[(size_t)(4 * s + ln / 16) * R + 16 * mt + ln % 16];
   };
+  // direct-operand order of the first product (m1 <= kRowsDirectM1): K step s of lane
+  // (r16, k4) is sample n1 = 4 (s / 2) + k4, part s % 2; fragments [mt][s / 4][lane][s % 4]
+  const bool direct = m1 > 1 && m1 <= kRowsDirectM1;
+  const int hs = (m1 + 3) / 4, nks1 = round_up(2 * hs, 4);
+  std::vector<double> g64d;
+  std::vector<float> g32d;
+  auto frag_direct = [&](const auto &G, auto &out) {
+    const size_t base = out.size();
+    out.resize(base + (size_t)(R1 / 16) * nks1 * 64);
+    for (int mt = 0; mt < R1 / 16; ++mt)
+      for (int s = 0; s < nks1; ++s)
+        for (int ln = 0; ln < 64; ++ln) {
+          const int n1 = 4 * (s / 2) + ln / 16, c = s % 2;
+          const size_t o = base + (((size_t)mt * (nks1 / 4) + s / 4) * 64 + ln) * 4 + s % 4;
+          out[o] = (n1 < m1) ? G[(size_t)(2 * n1 + c) * R1 + 16 * mt + ln % 16] : 0;
+        }
+  };
   for (int which = 0; which < 4; ++which) {  // F1f, F1b, F2f, F2b
     const int P = which < 2 ? m1 : m2, R = which < 2 ? R1 : R2, K = which < 2 ? K1 : K2;
     const int sign = (which & 1) ? +1 : -1;
@@ -186,6 +203,10 @@ const DftTables &Context::dft(int n) {
     dft_embed<float>(P, R, K, sign, true, part32);
     frag(part64, R, K, g64);
     frag(part32, R, K, g32);
+    if (direct && which < 2) {
+      frag_direct(part64, g64d);
+      frag_direct(part32, g32d);
+    }
   }
   std::vector<cx<double>> w(n);
   std::vector<cx<float>> wf(n);
@@ -207,8 +228,11 @@ const DftTables &Context::dft(int n) {
   upload(t->tw32, wf, st);
   upload(t->perm, perm, st);
   upload(t->perm32, perm32, st);
+  upload(t->g64d, g64d, st);
+  upload(t->g32d, g32d, st);
   S2H_CHECK_HIP(hipStreamSynchronize(st));
-  auto fill = [&](auto &d, auto *g, auto *tw, int csa, int csc, const int *pm) {
+  const size_t sd = (size_t)(R1 / 16) * nks1 * 64;
+  auto fill = [&](auto &d, auto *g, auto *tw, int csa, int csc, const int *pm, auto *gd) {
     d.M = n;
     d.M1 = m1;
     d.M2 = m2;
@@ -224,9 +248,12 @@ const DftTables &Context::dft(int n) {
     d.F2b = g + 2 * s1 + s2;
     d.tw = tw;
     d.perm = pm;
+    d.F1fd = direct ? gd : nullptr;
+    d.F1bd = direct ? gd + sd : nullptr;
+    d.nks1 = direct ? nks1 : 0;
   };
-  fill(t->d64, t->g64.as<double>(), t->tw64.as<cx<double>>(), CSA, CSC, t->perm.as<int>());
-  fill(t->d32, t->g32.as<float>(), t->tw32.as<cx<float>>(), CSA32, CSC32, t->perm32.as<int>());
+  fill(t->d64, t->g64.as<double>(), t->tw64.as<cx<double>>(), CSA, CSC, t->perm.as<int>(), t->g64d.as<double>());
+  fill(t->d32, t->g32.as<float>(), t->tw32.as<cx<float>>(), CSA32, CSC32, t->perm32.as<int>(), t->g32d.as<float>());
   auto &ref = *t;
   dft_[n] = std::move(t);
   return ref;

@@ -8,12 +8,18 @@
 //   X[k1 + M1 k2] = sum_{n2} w2^{n2 k2} Y[k1][n2]                  (second product, w2 = w^{M1})
 // where both sums are real matrix products: the complex DFT matrices embedded
 // as real matrices (rows (k, re/im), columns (n, re/im)) times the lines of a
-// block side by side as columns.  The block's lines are staged in LDS with the
-// reduction index fastest (one column per (line, n2) for the first product,
-// per (line, k1) for the second; the second overwrites the first in place), so
-// that the inner loop is an LDS read and a global read of the DFT-matrix
-// fragment, both at immediate offsets, per MFMA.  Padding rows and columns
-// are zero in the tables and in LDS.
+// block side by side as columns.  For M1 <= kRowsDirectM1 the first product reads
+// its operand straight from the lines into registers (lane (r16, k4) of a column
+// tile holds the samples n1 = 4 j + k4 of column (line, n2); each complex sample
+// feeds two K steps, real and imaginary part, and the matrix's reduction order
+// matches), and its twiddled result is written to LDS in the second product's
+// layout (one column per (line, k1), reduction index fastest).  Otherwise the lines
+// are first staged in LDS in the first product's layout (one column per
+// (line, n2)), which the second overwrites in place.  The inner loops are an LDS
+// read and a global read of the DFT-matrix fragment at immediate offsets per
+// MFMA.  Padding rows and columns are zero in the tables and in LDS (the direct
+// path clears the second product's region with 16-byte stores first: a per-entry
+// clearing loop of its padding rows measured 10% of the kernel).
 #include "kernels.h"
 #include "mfma.h"
 
@@ -39,6 +45,7 @@ __device__ inline void divmod_small(int n, int d, float inv, int &q, int &r) {
 // row tile (load = false); FP64 and longer reductions stream them four K steps at a
 // time, the next four in flight.
 constexpr int kRowsKR = 20;
+constexpr int kRowsDirectNMT = 4;  // first-product row tiles of the direct path: R1 <= 64 for M1 <= 32
 template <typename T>
 __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const T *S, int cs, int t0, bool two,
                                 int lane, T (&av)[kRowsKR], bool load, typename Mfma<T>::acc &a0,
@@ -56,21 +63,28 @@ __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const 
 #pragma unroll
         for (int s = 0; s < kRowsKR; ++s) av[s] = ga[min(s, nks - 1) * 64];  // s >= nks: not used
       }
+      // both column tiles without a branch (a missing second tile repeats the first; its
+      // products are not used), and the next group's B reads issued before this group's
+      // products so that the LDS latency hides behind them
+      const float *c1 = two ? b1 : b0;
+      float4 x0 = *reinterpret_cast<const float4 *>(b0), x1 = *reinterpret_cast<const float4 *>(c1);
 #pragma unroll
       for (int s = 0; s < kRowsKR; s += 4) {
         if (s >= nks) break;
-        const float4 x0 = *reinterpret_cast<const float4 *>(b0 + s);
+        const int sn = min(s + 4, nks - 4);  // the last group's look-ahead repeats it
+        const float4 y0 = *reinterpret_cast<const float4 *>(b0 + sn);
+        const float4 y1 = *reinterpret_cast<const float4 *>(c1 + sn);
+        __builtin_amdgcn_sched_barrier(0);  // keep the look-ahead reads ahead of the products
         a0 = Mfma<T>::mma(av[s], x0.x, a0);
+        a1 = Mfma<T>::mma(av[s], x1.x, a1);
         a0 = Mfma<T>::mma(av[s + 1], x0.y, a0);
+        a1 = Mfma<T>::mma(av[s + 1], x1.y, a1);
         a0 = Mfma<T>::mma(av[s + 2], x0.z, a0);
+        a1 = Mfma<T>::mma(av[s + 2], x1.z, a1);
         a0 = Mfma<T>::mma(av[s + 3], x0.w, a0);
-        if (two) {
-          const float4 x1 = *reinterpret_cast<const float4 *>(b1 + s);
-          a1 = Mfma<T>::mma(av[s], x1.x, a1);
-          a1 = Mfma<T>::mma(av[s + 1], x1.y, a1);
-          a1 = Mfma<T>::mma(av[s + 2], x1.z, a1);
-          a1 = Mfma<T>::mma(av[s + 3], x1.w, a1);
-        }
+        a1 = Mfma<T>::mma(av[s + 3], x1.w, a1);
+        x0 = y0;
+        x1 = y1;
       }
       return;
     }
@@ -151,7 +165,7 @@ __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const 
 
 template <typename TI, typename TO, typename T, int IMAX>
 __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
-  extern __shared__ char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
   typedef typename Mfma<T>::acc acc_t;
   int jb = 0;
   while (jb + 1 < jobs.n && (int)blockIdx.x >= jobs.j[jb + 1].blk0) ++jb;
@@ -183,8 +197,12 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
   constexpr bool KM = sizeof(T) == 4;
   auto kpos = [](int k, int K) { return KM ? (k & 3) * (K >> 2) + (k >> 2) : k; };
 
+  // two-level with M1 <= kRowsDirectM1: the first product reads its operand straight
+  // from the lines (no staging pass)
+  const bool direct = two_level && d.F1fd != nullptr;
+
   // ---- stage the lines: zero the padding, then scatter the samples
-  {
+  if (!direct) {
     const int cw = two_level ? M2 : 1, cs = two_level ? d.CSA : d.CSC, K = two_level ? d.K1 : d.K2;
     const int P2 = 2 * (two_level ? M1 : M2);  // used reduction rows
     const int ncol = nl * cw, ncolp = (ncol + 15) & ~15;
@@ -248,10 +266,109 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
         }
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
 
-  if (two_level) {
+  if (direct) {
+    // zero the second product's region (its padding rows and columns stay zero; the
+    // twiddle pass below writes the rest), 16 bytes per store, then one barrier
+    {
+      const int ncolcp = (nl * M1 + 15) & ~15;
+      const int n16 = ncolcp * d.CSC * (int)sizeof(T) / 16;  // ncolcp is a multiple of 16
+      float4 *S4 = reinterpret_cast<float4 *>(S);
+      for (int i = tid; i < n16; i += 256) S4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __syncthreads();
+    const T *__restrict__ F = fwd ? d.F1fd : d.F1bd;
+    const int ng = d.nks1 / 4, nmt = d.R1 / 16;  // K-step groups of four, row tiles
+    const int ncol = nl * M2, nt = (ncol + 15) / 16, ntp = (nt + 1) / 2;
+    constexpr int HS = kRowsDirectM1 / 4;  // samples per lane and column tile
+    for (int tp = wave; tp < ntp; tp += 4) {
+      // lane (r16, k4) of column tile u holds column (line, n2) = 16 (2 tp + u) + r16 and
+      // its samples n1 = 4 j + k4 (K steps 2 j and 2 j + 1: real and imaginary part);
+      // every load is issued before the first is used
+      cx<T> xb[2][HS];
+      int cl[2], cn2[2];
+      // branch-free: clamped indexes (a sample that is not there reads entry 0 and is zeroed)
+      size_t base[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        divmod_small(min(16 * (2 * tp + u) + r16, ncol - 1), M2, 1.0f / (float)M2, cl[u], cn2[u]);
+        base[u] = line_at((int)line0 + cl[u], a.in_len);
+      }
+      auto sample = [&](int u, int j, int &si) {
+        const int n1 = 4 * j + k4, q = M2 * n1 + cn2[u];
+        // centred side: bin q <- m = q (q <= L - 1) or q - M (q >= M - (L - 1)); MWSS bin L is zero
+        const bool lo = q <= half, hi = q >= Mh;
+        const bool ok = 16 * (2 * tp + u) + r16 < ncol && n1 < M1 && (!a.in_centred || lo || hi);
+        si = ok ? (a.in_centred ? (lo ? q + half : q - Mh) : q) : 0;
+        return ok;
+      };
+      auto fetch = [&](auto ptr) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int j = 0; j < HS; ++j) {
+            int si;
+            const bool ok = sample(u, j, si);
+            const auto w = ptr[base[u] + si];
+            xb[u][j] = (ok && 4 * j < M1) ? mk<T>((T)w.x, (T)w.y) : mk<T>((T)0, (T)0);
+          }
+      };
+      if (in64) fetch(reinterpret_cast<const cx<double> *>(in));  // uniform: the caller's complex128 samples
+      else fetch(in);
+      acc_t r[kRowsDirectNMT][2];
+      typedef T T4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+      for (int mt = 0; mt < kRowsDirectNMT; ++mt) {
+        if (mt >= nmt) break;
+        const T *ga = F + ((size_t)mt * ng * 64 + lane) * 4;
+        acc_t c0 = Mfma<T>::zero(), c1 = Mfma<T>::zero();
+        // A fragments four K steps per load, the next group's in flight
+        T4 an = *reinterpret_cast<const T4 *>(ga);
+#pragma unroll
+        for (int g = 0; g < HS / 2; ++g) {
+          if (g >= ng) break;
+          const T4 ac = an;
+          an = *reinterpret_cast<const T4 *>(ga + (size_t)min(g + 1, ng - 1) * 256);
+          __builtin_amdgcn_sched_barrier(0);
+          c0 = Mfma<T>::mma(ac[0], xb[0][2 * g].x, c0);
+          c1 = Mfma<T>::mma(ac[0], xb[1][2 * g].x, c1);
+          c0 = Mfma<T>::mma(ac[1], xb[0][2 * g].y, c0);
+          c1 = Mfma<T>::mma(ac[1], xb[1][2 * g].y, c1);
+          c0 = Mfma<T>::mma(ac[2], xb[0][2 * g + 1].x, c0);
+          c1 = Mfma<T>::mma(ac[2], xb[1][2 * g + 1].x, c1);
+          c0 = Mfma<T>::mma(ac[3], xb[0][2 * g + 1].y, c0);
+          c1 = Mfma<T>::mma(ac[3], xb[1][2 * g + 1].y, c1);
+        }
+        r[mt][0] = c0;
+        r[mt][1] = c1;
+      }
+      // twiddle and write the second product's columns (line, k1), rows 2 n2 + re/im
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int col = 16 * (2 * tp + u) + r16;
+        if (col >= ncol) continue;
+        const int line = cl[u], n2 = cn2[u];
+#pragma unroll
+        for (int mt = 0; mt < kRowsDirectNMT; ++mt) {
+          if (mt >= nmt) break;
+#pragma unroll
+          for (int p = 0; p < 2; ++p) {
+            const int k1 = 8 * mt + 2 * k4 + p;
+            if (k1 >= M1) continue;
+            cx<T> w = d.tw[n2 * k1];  // e^{-2 pi i n2 k1 / M}
+            if (!fwd) w.y = -w.y;
+            const cx<T> z = cmul(mk<T>(r[mt][u][2 * p], r[mt][u][2 * p + 1]), w);
+            T *yo = S + (line * M1 + k1) * d.CSC + kpos(2 * n2, d.K2);
+            yo[0] = z.x;
+            yo[KM ? d.K2 / 4 : 1] = z.y;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  } else if (two_level) {
     const T *__restrict__ F = fwd ? d.F1f : d.F1b;
     const int nks = d.K1 / 4, nmt = d.R1 / 16;
     const float inmt = 1.0f / (float)nmt;
@@ -349,21 +466,18 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int col = 16 * (2 * tp + u) + r16;
-        if (col >= ncol) continue;
         int line, k1;
-        divmod_small(col, M1, 1.0f / (float)M1, line, k1);
+        divmod_small(min(col, ncol - 1), M1, 1.0f / (float)M1, line, k1);
         const acc_t &r = u ? r1 : r0;
         cx<TO> *dst = out + line_at((int)line0 + line, a.out_len);
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
           const int k2 = 8 * mt + 2 * k4 + p;
-          if (k2 >= M2) continue;
           int o = k1 + M1 * k2;
-          if (a.out_centred) {  // bin o -> centred m; the MWSS Nyquist bin is dropped
-            if (o <= half) o += half;
-            else if (o >= Mh) o -= Mh;
-            else continue;
-          }
+          // bin o -> centred m; the MWSS Nyquist bin is dropped
+          const bool lo = o <= half, hi = o >= Mh;
+          if (a.out_centred) o = lo ? o + half : o - Mh;
+          if (!(col < ncol && k2 < M2 && (!a.out_centred || lo || hi))) continue;
           if (out64)
             reinterpret_cast<cx<double> *>(out)[(dst - out) + o] = mk<double>((double)(r[2 * p] * scale), (double)(r[2 * p + 1] * scale));
           else
@@ -381,7 +495,9 @@ int rows_items_per_wave(int M1, int M2, int R1, int lpb) {
   return ((R1 / 16) * ntp + 3) / 4;
 }
 
-size_t rows_lds_bytes(int M1, int M2, int CSA, int CSC, int lpb, size_t esz) {
+size_t rows_lds_bytes(int M1, int M2, int CSA, int CSC, int lpb, size_t esz, int R1, int nks1) {
+  if (nks1 > 0)  // direct first product: the second product's columns only
+    return (size_t)((lpb * M1 + 15) & ~15) * CSC * esz;
   const size_t a = (M1 > 1) ? (size_t)((lpb * M2 + 15) & ~15) * CSA : 0;
   const size_t c = (size_t)((lpb * M1 + 15) & ~15) * CSC;
   return std::max(a, c) * esz;
@@ -394,7 +510,7 @@ void launch_rows(const RowJobs<T> &jobs, const MapSet &ms, hipStream_t st) {
   int items = 0;
   for (int k = 0; k < jobs.n; ++k) {
     const RowArgs<T> &a = jobs.j[k].a;
-    shm = std::max(shm, rows_lds_bytes(a.d.M1, a.d.M2, a.d.CSA, a.d.CSC, a.lpb, sizeof(T)));
+    shm = std::max(shm, rows_lds_bytes(a.d.M1, a.d.M2, a.d.CSA, a.d.CSC, a.lpb, sizeof(T), a.d.R1, a.d.nks1));
     items = std::max(items, rows_items_per_wave(a.d.M1, a.d.M2, a.d.R1, a.lpb));
   }
   if (items <= 4) hipLaunchKernelGGL((k_rows4<TI, TO, T, 4>), dim3(jobs.nblk, ms.n), dim3(256), shm, st, jobs, ms);
