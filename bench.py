@@ -199,6 +199,14 @@ def c3_batch(s2, ops, torch, steps, warmup, modes=("fp64", "mixed")):
     ps = [s2.parameters(L, N, 2.0, 1, sp) for L, N, sp in shapes]
     x = [torch.from_numpy(f).cuda() for f in flms]
     out = {"maps": len(ps)}
+    # launch shapes per band-limit bucket (outside the timed region): one autotune per
+    # distinct parameter set and mode
+    distinct = {(L, N, sp): p for (L, N, sp), p in zip(shapes, ps)}
+    out["buckets"] = len(distinct)
+    for mode in (s2.FP64, s2.MIXED):
+        if ("fp64" if mode == s2.FP64 else "mixed") in modes:
+            for p in distinct.values():
+                ops.autotune(p, mode)
     F_total = 0
     for p in ps:
         F_total += algorithmic_work(s2, p)[0]
@@ -275,6 +283,48 @@ def real_half_spectrum(s2, ops, torch, p, flm_h, steps, warmup):
     return out
 
 
+def c5_single(s2, ops, torch, steps, warmup):
+    """L = 2048, N = 8, B = 2, J_min = 3, spin 0, multiresolution, f_lm with C_l = (l+1)^-2
+    (numpy default_rng(5)): analysis + synthesis maps/s in both modes, the FP64 round trip
+    and the mixed error against FP64, and the stage times."""
+    L, N, J_min = 2048, 8, 3
+    p = s2.parameters(L, N, 2.0, J_min, 0)
+    rng = np.random.default_rng(5)
+    ell = np.repeat(np.arange(L), 2 * np.arange(L) + 1)
+    flm = (rng.standard_normal(L * L) + 1j * rng.standard_normal(L * L)) / np.sqrt(2) / (ell + 1.0)
+    x = torch.from_numpy(flm).cuda()
+    out = {"L": L, "N": N, "J_min": J_min}
+    res = {}
+    for mode, name in ((s2.FP64, "fp64"), (s2.MIXED, "mixed")):
+        t0 = time.perf_counter()
+        s2.prepare(p, mode)
+        ops.autotune(p, mode)
+        prep = time.perf_counter() - t0
+        fw, fs = ops.analysis(x, p, mode)
+        rec = ops.synthesis(fw, fs, p, mode)
+
+        def step():
+            ops.analysis(x, p, mode, out=(fw, fs))
+            ops.synthesis(fw, fs, p, mode, out=rec)
+
+        dt = timed(step, steps, warmup, torch)
+        s2.timing(True)
+        step()
+        torch.cuda.synchronize()
+        st = s2.timing_report()
+        s2.timing(False)
+        res[name] = (torch.cat([fw, fs]).cpu().numpy(), rec.cpu().numpy())
+        out[name] = {"value": steps / dt, "unit": "maps/s", "ms_per_step": 1e3 * dt / steps,
+                     "plan_and_autotune_s": prep, "stages_ms": {k: v[1] for k, v in st.items()}}
+        del fw, fs, rec
+    a64, r64 = res["fp64"]
+    a32, r32 = res["mixed"]
+    out["fp64"]["reconstruction_rel_err"] = float(np.linalg.norm(r64 - flm) / np.linalg.norm(flm))
+    out["mixed"]["analysis_rel_err_vs_fp64"] = float(np.linalg.norm(a32 - a64) / np.linalg.norm(a64))
+    out["mixed"]["reconstruction_rel_err"] = float(np.linalg.norm(r32 - flm) / np.linalg.norm(flm))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,6 +332,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the 256-map uneven batch")
+    ap.add_argument("--c5", action="store_true", help="also the L = 2048 single map (a few minutes of table builds)")
     ap.add_argument("--c3-only", choices=("fp64", "mixed", "both"), default=None,
                     help="run only the 256-map batch (in the given modes, --steps batches after --warmup)")
     ap.add_argument("--cpu-reps", type=int, default=1, help="CPU baseline maps (median)")
@@ -440,6 +491,8 @@ def main():
     out["real_signal"] = real_half_spectrum(s2, ops, torch, p, flm_h, max(2, args.steps // 2), 1)
     if not args.no_c3:
         out["c3_batch"] = c3_batch(s2, ops, torch, max(2, args.steps // 4), 1)
+    if args.c5:
+        out["c5_L2048"] = c5_single(s2, ops, torch, 3, 1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, flm_h, args.cpu_reps)
     print(json.dumps(out))

@@ -65,11 +65,11 @@ __host__ __device__ inline cx<T> cscale(cx<T> a, T s) {
 // i^k * a
 template <typename T>
 __host__ __device__ inline cx<T> ipow_mul(int k, cx<T> a) {
+  // i^k a by selects (no branches): k & 3 = 0: (x, y), 1: (-y, x), 2: (-x, -y), 3: (y, -x)
   k &= 3;
-  if (k == 0) return a;
-  if (k == 1) return mk<T>(-a.y, a.x);
-  if (k == 2) return mk<T>(-a.x, -a.y);
-  return mk<T>(a.y, -a.x);
+  const bool sw = k & 1, nx = k == 1 || k == 2, ny = k >= 2;
+  const T x = sw ? a.y : a.x, y = sw ? a.x : a.y;
+  return mk<T>(nx ? -x : x, ny ? -y : y);
 }
 
 }  // namespace s2h

@@ -229,7 +229,7 @@ void launch_theta(const ThetaJobs<T> &jobs, const MapSet &ms, hipStream_t st);
 size_t bf3_matrix_elems(int L, int ss);
 void build_theta_bf3(const double *th64, int L, int ss, unsigned short *out, hipStream_t st);
 bool theta_bf_eligible(int Lmax);
-void launch_theta_bf(const ThetaJobs<float> &jobs, const MapSet &ms, hipStream_t st);
+void launch_theta_bf(const ThetaJobs<float> &jobs, const MapSet &ms, int rows, hipStream_t st);
 bool theta_bf_args_ok(const ThetaJobs<float> &jobs);  // twisted input rows: row 0 or none
 int mfma_bf_selftest();
 template <typename TI, typename TO, typename T>

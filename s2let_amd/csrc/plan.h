@@ -37,7 +37,7 @@ struct LTables {  // per band-limit L and sampling (MW / MWSS)
   int L;
   int ss;
   DevBuf th64, th32;  // theta-stage matrices, 8 x L x L (see build_theta_tables)
-  DevBuf thbf;        // their bf16 three-part tiles (theta_bf16.hip), for L <= kThetaBfMaxL
+  DevBuf thbf;        // their bf16 three-part tiles (theta_bf16.hip)
   const DftTables *dM;
 };
 

@@ -1,5 +1,8 @@
 // Theta (beta) stage in the FP32 mode on the bf16 matrix cores
-// (v_mfma_f32_16x16x32_bf16), for band-limits up to kThetaBfMaxL.
+// (v_mfma_f32_16x16x32_bf16).  Up to L = kThetaF32MaxL the products accumulate in FP32
+// over the whole reduction (128 rows per block); above it in FP32 over runs of
+// kThetaRun chunks whose sums are added in FP64 (64 rows per block), as the FP32
+// kernel does.
 //
 // The products are those of k_theta (theta_mfma.hip): per column and parity
 // class a real matrix A (R x K) times the complex samples.  Every operand is
@@ -72,17 +75,17 @@ void build_theta_bf3(const double *th64, int L, int ss, unsigned short *out, hip
 }
 
 // ------------------------------------------------------------------ GEMM
-// Block tile as in k_theta<float, false, 4>: 128 rows x 64 consecutive mi of one
-// column; waves 0,1 take rows 0-63 / 64-127 of the block's first class, waves
-// 2,3 the other class.  Per wave: 4 row tiles x {re, im} x 2 column tiles of 16.
+// Block tile as in k_theta<float, F64S, RT>: 32 RT rows x 64 consecutive mi of one
+// column; waves 0,1 take the first / second half of the rows of the block's first
+// class, waves 2,3 the other class.  Per wave: RT row tiles x {re, im} x 2 column
+// tiles of 16.
 // The samples of a K chunk of 32 are split into bf16 parts while staged in LDS
 // (double-buffered, one barrier per chunk; row length 40 entries, which keeps the
 // 16-byte reads and writes free of bank conflicts); the A fragments of a row tile
 // for the next chunk are loaded as soon as the tile's products of the current
 // chunk have been issued.
-constexpr int kThetaBfMaxL = 512;
-constexpr int kBfRT = 4;
 constexpr int kBfSJ = 40;
+constexpr int kBfRun = 4;  // FP64 run sums: chunks of 32 per FP32 run
 
 __device__ inline void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
   h = (__bf16)x;
@@ -91,8 +94,9 @@ __device__ inline void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
   l = (__bf16)(r - (float)m);
 }
 
+template <int RT, bool F64S>
 __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs, MapSet ms) {
-  constexpr int RT = kBfRT, SJ = kBfSJ;
+  constexpr int SJ = kBfSJ;
   // [buf][class][part][re/im][j][k]; class 1 is offset by 32 entries (64 bytes) so that
   // the two classes of neighbouring lanes' 16-byte writes fall in different banks
   constexpr int kCls = 3 * 2 * 32 * SJ + 32, kBuf = 2 * kCls;
@@ -134,6 +138,15 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs, MapSet 
   for (int r = 0; r < RT; ++r)
 #pragma unroll
     for (int qq = 0; qq < 4; ++qq) acc[r][qq] = f4{0.f, 0.f, 0.f, 0.f};
+  double hs[F64S ? RT : 1][4][4];
+  if constexpr (F64S) {
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) hs[r][qq][v] = 0.0;
+  }
 
   // staging: thread (mi = mi0 + tid % 64, kq = tid / 64) moves k = k0 + 8 kq + e, e < 8;
   // unconditional loads from clamped indexes: past the last sample row the table's
@@ -210,6 +223,18 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs, MapSet 
       load_a(r, kc + 1);
       __builtin_amdgcn_sched_barrier(0);
     }
+    if constexpr (F64S) {  // end of an FP32 run: into the FP64 sums
+      if ((kc + 1) % kBfRun == 0 || kc + 1 == nkc) {
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+          for (int qq = 0; qq < 4; ++qq) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) hs[r][qq][v] += (double)acc[r][qq][v];
+            acc[r][qq] = f4{0.f, 0.f, 0.f, 0.f};
+          }
+      }
+    }
   };
   load_b(b0, 0);
   load_b(b1, 32);
@@ -222,7 +247,7 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs, MapSet 
   }
   if (kc < nkc) chunk(kc, b0);
 
-  const cx<double> ph = mk<double>(col.ph_re, col.ph_im);
+  const cx<double> php = mk<double>(col.ph_re, a.conj_ph ? -col.ph_im : col.ph_im);
   const int zc = a.z_e[cls], orow = a.out_row[cls], oe = a.out_e[cls];
   cx<float> *__restrict__ outc = out + (size_t)c * a.plane;
 #pragma unroll
@@ -235,16 +260,18 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs, MapSet 
       for (int v = 0; v < 4; ++v) {
         const int i = i0w + 16 * r + 4 * kg + v;
         if (i >= R || mo >= M) continue;
-        cx<double> val = mk<double>((double)acc[r][qq][v], (double)acc[r][qq + 2][v]);
-        if (i == orow) val = ipow_mul(oe, val);
-        val = ipow_mul(zc, val);
-        val = a.conj_ph ? ipow_mul(m, cmulc(val, ph)) : ipow_mul(-m, cmul(ph, val));
+        cx<double> val;
+        if constexpr (F64S) val = mk<double>(hs[r][qq][v], hs[r][qq + 2][v]);
+        else val = mk<double>((double)acc[r][qq][v], (double)acc[r][qq + 2][v]);
+        // i^(oe [row orow] + zc + (conj_ph ? m : -m)) times (conj_ph ? conj(ph) : ph): one
+        // exponent, one product
+        val = cmul(ipow_mul((i == orow ? oe : 0) + zc + (a.conj_ph ? m : -m), php), val);
         outc[(size_t)i * M + mo] = mk<float>((float)val.x, (float)val.y);
       }
     }
 }
 
-bool theta_bf_eligible(int Lmax) { return Lmax <= kThetaBfMaxL; }
+bool theta_bf_eligible(int) { return true; }
 
 bool theta_bf_args_ok(const ThetaJobs<float> &jobs) {
   for (int k = 0; k < jobs.n; ++k)
@@ -253,9 +280,11 @@ bool theta_bf_args_ok(const ThetaJobs<float> &jobs) {
   return true;
 }
 
-void launch_theta_bf(const ThetaJobs<float> &jobs, const MapSet &ms, hipStream_t st) {
+// rows: the launch's rows per block (theta_rows): 128 (FP32 sums) or 64 (FP64 run sums)
+void launch_theta_bf(const ThetaJobs<float> &jobs, const MapSet &ms, int rows, hipStream_t st) {
   if (jobs.n == 0 || jobs.nblk == 0) return;
-  hipLaunchKernelGGL(k_theta_bf, dim3(jobs.nblk, ms.n), dim3(256), 0, st, jobs, ms);
+  if (rows == 128) hipLaunchKernelGGL((k_theta_bf<4, false>), dim3(jobs.nblk, ms.n), dim3(256), 0, st, jobs, ms);
+  else hipLaunchKernelGGL((k_theta_bf<2, true>), dim3(jobs.nblk, ms.n), dim3(256), 0, st, jobs, ms);
 }
 
 // ------------------------------------------------------------------ selftest

@@ -325,7 +325,7 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs, MapSet ms) {
     }
   }
 
-  const cx<double> ph = mk<double>(col.ph_re, col.ph_im);
+  const cx<double> php = mk<double>(col.ph_re, a.conj_ph ? -col.ph_im : col.ph_im);
   const int zc = a.z_e[cls], orow = a.out_row[cls], oe = a.out_e[cls];
   cx<T> *__restrict__ outc = out + (size_t)c * a.plane;
 #pragma unroll
@@ -347,9 +347,9 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs, MapSet ms) {
           im = (double)acc[r][qq + 2][v];
         }
         cx<double> val = mk<double>(re, im);
-        if (i == orow) val = ipow_mul(oe, val);
-        val = ipow_mul(zc, val);
-        val = a.conj_ph ? ipow_mul(m, cmulc(val, ph)) : ipow_mul(-m, cmul(ph, val));
+        // i^(oe [row orow] + zc + (conj_ph ? m : -m)) times (conj_ph ? conj(ph) : ph): one
+        // exponent, one product
+        val = cmul(ipow_mul((i == orow ? oe : 0) + zc + (a.conj_ph ? m : -m), php), val);
         outc[(size_t)i * M + mi] = mk<T>((T)val.x, (T)val.y);
       }
     }
@@ -387,8 +387,8 @@ void launch_theta(const ThetaJobs<T> &jobs, const MapSet &ms, hipStream_t st) {
       const char *e = getenv("S2LET_THETA_BF");
       return !(e && e[0] == '0');
     }();
-    if (bf && bf_on && theta_bf_eligible(Lmax) && theta_rows<T>(Lmax) == 128 && theta_bf_args_ok(jobs)) {
-      launch_theta_bf(jobs, ms, st);
+    if (bf && bf_on && theta_bf_eligible(Lmax) && theta_bf_args_ok(jobs)) {
+      launch_theta_bf(jobs, ms, theta_rows<T>(Lmax), st);
       return;
     }
   }
