@@ -24,6 +24,8 @@
 //
 // In FP32 the inverse products accumulate in FP32 over 32 ell and the block sums
 // are added in FP64; the forward H (16 columns) is converted to FP64 per m'.
+#include <type_traits>
+
 #include "kernels.h"
 #include "mfma.h"
 
@@ -171,7 +173,8 @@ __global__ void k_xtile(const cx<double> *__restrict__ x0, int Lx, cx<T> *__rest
 // accumulate in FP32 (band-limits <= kInvF32MaxL, where the transform stays within
 // 1e-6 of FP64 and the 64 FP64 registers go)
 constexpr int kInvF32MaxL = 512;
-template <typename T, int MP, bool F64>
+// HM: a real signal's m >= 0 half (the -m products and outputs are skipped)
+template <typename T, int MP, bool F64, bool HM>
 __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt0, int nx, const T *__restrict__ dt,
                                               cx<T> *__restrict__ Fbase0, InvJobs<T> jobs, MapSet ms) {
   typedef typename Mfma<T>::acc acc_t;
@@ -240,22 +243,26 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt0, int
     const cx<T> *xq = xt + (((size_t)lb * nx + mbi) * 64 + lane) * 4;
     o.p01 = ld4(&xq[0].x);
     o.p23 = ld4(&xq[2].x);
-    o.q01 = ld4(&xq[xhalf].x);
-    o.q23 = ld4(&xq[xhalf + 2].x);
+    if constexpr (!HM) {
+      o.q01 = ld4(&xq[xhalf].x);
+      o.q23 = ld4(&xq[xhalf + 2].x);
+    }
   };
   auto compute = [&](int lb, const Ops &o) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const V4<T> &pp = s < 2 ? o.p01 : o.p23, &qq = s < 2 ? o.q01 : o.q23;
       const T xpr = pp.v[2 * (s & 1)], xpi = pp.v[2 * (s & 1) + 1];
-      const T xmr = qq.v[2 * (s & 1)], xmi = qq.v[2 * (s & 1) + 1];
 #pragma unroll
       for (int j = 0; j < MP; ++j) {
         const T dp = o.d[j].v[s] * o.z[j];
         acc[j][0] = Mfma<T>::mma(o.e[j].v[s], dp * xpr, acc[j][0]);
         acc[j][1] = Mfma<T>::mma(o.e[j].v[s], dp * xpi, acc[j][1]);
-        acc[j][2] = Mfma<T>::mma(o.e[j].v[s], dp * xmr, acc[j][2]);
-        acc[j][3] = Mfma<T>::mma(o.e[j].v[s], dp * xmi, acc[j][3]);
+        if constexpr (!HM) {
+          const T xmr = qq.v[2 * (s & 1)], xmi = qq.v[2 * (s & 1) + 1];
+          acc[j][2] = Mfma<T>::mma(o.e[j].v[s], dp * xmr, acc[j][2]);
+          acc[j][3] = Mfma<T>::mma(o.e[j].v[s], dp * xmi, acc[j][3]);
+        }
       }
     }
     if constexpr (sizeof(T) == 4 && F64) {
@@ -313,8 +320,10 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt0, int
       const cx<T> *xq = xt + (((size_t)lb * nx + mbi) * 64 + lane) * 4;
       o.p01 = ld4(&xq[0].x);
       o.p23 = ld4(&xq[2].x);
-      o.q01 = ld4(&xq[xhalf].x);
-      o.q23 = ld4(&xq[xhalf + 2].x);
+      if constexpr (!HM) {
+        o.q01 = ld4(&xq[xhalf].x);
+        o.q23 = ld4(&xq[xhalf + 2].x);
+      }
       compute(lb, o);
     }
   }
@@ -340,7 +349,7 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt0, int
       }
       cx<T> *row = F + (size_t)mp * M;
       row[m] = mk<T>((T)rp, (T)ip);
-      if (m > 0) row[-m] = mk<T>((T)rm, (T)im);
+      if (!HM && m > 0) row[-m] = mk<T>((T)rm, (T)im);
     }
   }
 }
@@ -366,7 +375,7 @@ struct FwdTile {
 // last m' of l's l-block
 __host__ __device__ inline bool fwd_split_covers(int z, int chunk, int l) { return z * chunk <= 16 * (l / 16) + 15; }
 
-template <typename T>
+template <typename T, bool HM>
 __global__ __launch_bounds__(512) void k_fwd3(const SuperDev *__restrict__ sgs, int nsg, const ColDev *__restrict__ cols,
                                               const T *const *__restrict__ Efs, const T *__restrict__ dt, int L,
                                               const cx<T> *__restrict__ Jbase0, cx<double> *__restrict__ ypart0,
@@ -416,7 +425,7 @@ __global__ __launch_bounds__(512) void k_fwd3(const SuperDev *__restrict__ sgs, 
     const int mpe = min(Lg - 1, mps + mpchunk - 1);
     // this thread's element offset at m' = 0 (an invalid slot reads column 0, m = 0, and
     // is zeroed): every load is unconditional, so the wait counts stay exact
-    const bool cok = cc < nc && mm < Lg;
+    const bool cok = cc < nc && mm < Lg && !(HM && pm);  // HM: the m >= 0 half only
     const long long coff = cok ? gc[cc].out_off + (Lg - 1) + (pm ? -mm : mm) : gc[0].out_off + (Lg - 1);
     cx<T> jr[MC];
     auto fetch = [&](int mp0) {
@@ -479,7 +488,7 @@ __global__ __launch_bounds__(512) void k_fwd3(const SuperDev *__restrict__ sgs, 
         for (int s = 0; s < 4; ++s) {
           const T *bj = &sJ[mc][4 * s + k4][r16][0];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) h[u] = Mfma<T>::mma(ev.v[s], bj[u], h[u]);
+          for (int u = 0; u < (HM ? 2 : 4); ++u) h[u] = Mfma<T>::mma(ev.v[s], bj[u], h[u]);
         }
         if (mc + PD < MC) {
           const size_t o = (size_t)min(mc + PD, n - 1) * 256;
@@ -493,13 +502,17 @@ __global__ __launch_bounds__(512) void k_fwd3(const SuperDev *__restrict__ sgs, 
           if constexpr (part) {
             pacc[0][v] += d * h[0][v];
             pacc[1][v] += d * h[1][v];
-            pacc[2][v] += dn * h[2][v];
-            pacc[3][v] += dn * h[3][v];
+            if constexpr (!HM) {
+              pacc[2][v] += dn * h[2][v];
+              pacc[3][v] += dn * h[3][v];
+            }
           } else {
             acc[0][v] += d * h[0][v];
             acc[1][v] += d * h[1][v];
-            acc[2][v] += dn * h[2][v];
-            acc[3][v] += dn * h[3][v];
+            if constexpr (!HM) {
+              acc[2][v] += dn * h[2][v];
+              acc[3][v] += dn * h[3][v];
+            }
           }
         }
       }
@@ -521,7 +534,7 @@ __global__ __launch_bounds__(512) void k_fwd3(const SuperDev *__restrict__ sgs, 
     if (l >= L || mo > l) continue;
     cx<double> *yl = y + (size_t)l * l + l;
     yl[mo] = mk<double>(acc[0][v], acc[1][v]);
-    if (mo > 0) yl[-mo] = mk<double>(acc[2][v], acc[3][v]);
+    if (!HM && mo > 0) yl[-mo] = mk<double>(acc[2][v], acc[3][v]);
   }
 }
 
@@ -576,23 +589,29 @@ template void launch_xtile<float>(const void *, int, void *, const MapSet &, hip
 
 template <typename T>
 void launch_inv_mfma(const void *xtile, int Lx, const T *dt, void *Fbase, const InvJobs<T> &jobs, const MapSet &ms,
-                     hipStream_t st) {
+                     hipStream_t st, bool half_m) {
   if (jobs.n == 0 || jobs.nblk == 0) return;
   int Lmax = 0;
   for (int k = 0; k < jobs.n; ++k) Lmax = std::max(Lmax, jobs.j[k].L);
   const int nx = (Lx + 15) / 16;
   const dim3 grid(jobs.nblk, ms.n);
-  if (sizeof(T) == 4 && Lmax <= kInvF32MaxL)
-    hipLaunchKernelGGL((k_inv3<T, kInvMP, false>), grid, dim3(256), 0, st, (const cx<T> *)xtile, nx, dt,
-                       (cx<T> *)Fbase, jobs, ms);
-  else
-    hipLaunchKernelGGL((k_inv3<T, kInvMP, true>), grid, dim3(256), 0, st, (const cx<T> *)xtile, nx, dt,
-                       (cx<T> *)Fbase, jobs, ms);
+  auto go = [&](auto f64, auto hm) {
+    hipLaunchKernelGGL((k_inv3<T, kInvMP, decltype(f64)::value, decltype(hm)::value>), grid, dim3(256), 0, st,
+                       (const cx<T> *)xtile, nx, dt, (cx<T> *)Fbase, jobs, ms);
+  };
+  const bool f32sums = sizeof(T) == 4 && Lmax <= kInvF32MaxL;
+  if (f32sums) {
+    if (half_m) go(std::false_type{}, std::true_type{});
+    else go(std::false_type{}, std::false_type{});
+  } else {
+    if (half_m) go(std::true_type{}, std::true_type{});
+    else go(std::true_type{}, std::false_type{});
+  }
 }
 template void launch_inv_mfma<double>(const void *, int, const double *, void *, const InvJobs<double> &,
-                                      const MapSet &, hipStream_t);
+                                      const MapSet &, hipStream_t, bool);
 template void launch_inv_mfma<float>(const void *, int, const float *, void *, const InvJobs<float> &,
-                                     const MapSet &, hipStream_t);
+                                     const MapSet &, hipStream_t, bool);
 
 int fwd_splits_default(int L) { return std::max(1, std::min(8, L / 64)); }
 // 16 splits up to L = 1024 (shorter m' ranges per block: the longest waves, those of the
@@ -609,7 +628,7 @@ int fwd_chunk(int L, int S, int esz) {
 // there is one split, else the split partials (scratch stride scr_out)
 template <typename T>
 void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *const *Efs, const T *dt, int L,
-                     const void *Jbase, void *y, void *parts, int S, const MapSet &ms, hipStream_t st) {
+                     const void *Jbase, void *y, void *parts, int S, const MapSet &ms, hipStream_t st, bool half_m) {
   const int nb = (L + 15) / 16;
   const int chunk = fwd_chunk(L, S, (int)sizeof(T));
   dim3 grid(nb, (nb + FwdTile<T>::WAVES - 1) / FwdTile<T>::WAVES, S * ms.n);
@@ -621,13 +640,17 @@ void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *
     out = (cx<double> *)parts;
     mo.dlt_out = nullptr;
   }
-  hipLaunchKernelGGL((k_fwd3<T>), grid, dim3(64 * FwdTile<T>::WAVES), 0, st, sgs, nsg, cols, Efs, dt, L,
-                     (const cx<T> *)Jbase, out, chunk, mo);
+  if (half_m)
+    hipLaunchKernelGGL((k_fwd3<T, true>), grid, dim3(64 * FwdTile<T>::WAVES), 0, st, sgs, nsg, cols, Efs, dt, L,
+                       (const cx<T> *)Jbase, out, chunk, mo);
+  else
+    hipLaunchKernelGGL((k_fwd3<T, false>), grid, dim3(64 * FwdTile<T>::WAVES), 0, st, sgs, nsg, cols, Efs, dt, L,
+                       (const cx<T> *)Jbase, out, chunk, mo);
 }
 template void launch_fwd_mfma<double>(const SuperDev *, int, const ColDev *, const double *const *, const double *, int,
-                                      const void *, void *, void *, int, const MapSet &, hipStream_t);
+                                      const void *, void *, void *, int, const MapSet &, hipStream_t, bool);
 template void launch_fwd_mfma<float>(const SuperDev *, int, const ColDev *, const float *const *, const float *, int,
-                                     const void *, void *, void *, int, const MapSet &, hipStream_t);
+                                     const void *, void *, void *, int, const MapSet &, hipStream_t, bool);
 
 // ms: input side the split partials (scratch stride scr_in), output side the caller's flm
 void launch_sum_parts(int L, int S, int esz, const void *parts, void *y, const MapSet &ms, hipStream_t st) {

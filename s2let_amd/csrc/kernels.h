@@ -70,6 +70,7 @@ struct ThetaArgs {
   const T *AT[2];
   const unsigned short *AB[2];  // bf16 three-part tables (FP32 mode, theta_bf16.hip) or null
   int z_e[2], in_row[2], in_e[2], out_row[2], out_e[2];
+  int mi_lo;  // first m index (m + L - 1) of the blocks: 0, or L - 1 for the m >= 0 half
 };
 
 // Two-level DFT of length M = M1 M2 on the matrix cores (rows_mfma.hip): real
@@ -115,6 +116,11 @@ struct RowArgs {
   int in_centred, out_centred, sign;
   T scale;
   Dft2<T> d;
+  // a real signal's half spectra (paired): line (plane c, row r) is column n >= 0 and
+  // plane pair - c (relative to the job's planes, may be negative) its column -n, whose
+  // spectrum at m >= 0 is the conjugate of column n's at -m.  The centred side reads
+  // (input) or writes (output, except into its own plane) m < 0 through that plane.
+  int paired, pair;
 };
 
 void launch_risbo_half(const double *in, double *out, int n, int rmax, int symrow, int lp, const double *sq,
@@ -124,10 +130,9 @@ void launch_add_cx(void *a, const void *b, size_t n, hipStream_t st);  // comple
 void launch_hardthreshold(double *g, size_t n, double thr, hipStream_t st);
 void launch_re2cx(const double *a, void *b, size_t n, hipStream_t st);  // complex128 b = a + 0i
 void launch_cx2re(const void *a, double *b, size_t n, hipStream_t st);  // b = Re(complex128 a)
-void launch_re2cx_scaled(const double *a, void *b, size_t n, double s, hipStream_t st);  // b = s a + 0i
-// complex128 y (L^2, index l^2 + l + m) <- y + H(y), H(y)_lm = (-1)^m conj(y_l,-m): the
-// harmonics of a real signal from the half-spectrum sum Y
-void launch_herm_sym(void *y, int L, hipStream_t st);
+// complex128 y (L^2, index l^2 + l + m): y_l,-m = (-1)^m conj(y_lm) for m > 0 (the harmonics
+// of a real signal from their m >= 0 half)
+void launch_herm_fill(void *y, int L, hipStream_t st);
 
 // matrix-core contractions (contract_mfma.hip).
 // Delta tiles (16 x 16 tiles of Delta^l_{m'm}, one 256-entry record per m'):
@@ -159,10 +164,11 @@ template <typename T>
 void launch_xtile(const void *x, int Lx, void *xtile, const MapSet &ms, hipStream_t st);
 template <typename T>
 void launch_inv_mfma(const void *xtile, int Lx, const T *dt, void *Fbase, const InvJobs<T> &jobs, const MapSet &ms,
-                     hipStream_t st);
+                     hipStream_t st, bool half_m = false);  // half_m: the m >= 0 half of a real signal
 template <typename T>
 void launch_fwd_mfma(const SuperDev *sgs, int nsg, const ColDev *cols, const T *const *Efs, const T *dt, int L,
-                     const void *Jbase, void *y, void *parts, int splits, const MapSet &ms, hipStream_t st);
+                     const void *Jbase, void *y, void *parts, int splits, const MapSet &ms, hipStream_t st,
+                     bool half_m = false);
 // m' splits of the forward contraction (partials: splits * L^2 complex128, at most
 // fwd_splits_max(L) <= kMaxSplits)
 constexpr int kMaxSplits = 16;

@@ -62,25 +62,17 @@ __global__ void k_re2cx(const double *__restrict__ a, cx<double> *__restrict__ b
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     b[i] = mk<double>(a[i], 0.0);
 }
-// b = s a (real to complex)
-__global__ void k_re2cx_scaled(const double *__restrict__ a, cx<double> *__restrict__ b, size_t n, double s) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    b[i] = mk<double>(s * a[i], 0.0);
-}
-// y <- y + H(y), H(y)_lm = (-1)^m conj(y_l,-m): one thread per pair (l, m >= 0), pair index
-// q = l (l + 1) / 2 + m, both entries of the pair read before either is written
-__global__ void k_herm_sym(cx<double> *__restrict__ y, int L) {
-  const size_t npair = (size_t)L * (L + 1) / 2;
-  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < npair; q += (size_t)gridDim.x * blockDim.x) {
-    int l = (int)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
-    while ((size_t)l * (l + 1) / 2 > q) --l;
-    while ((size_t)(l + 1) * (l + 2) / 2 <= q) ++l;
-    const int m = (int)(q - (size_t)l * (l + 1) / 2);
-    const size_t ip = (size_t)l * l + l + m, im = (size_t)l * l + l - m;
-    const cx<double> a = y[ip], b = y[im];
-    const double sg = (m & 1) ? -1.0 : 1.0;
-    y[ip] = mk<double>(a.x + sg * b.x, a.y - sg * b.y);
-    if (m > 0) y[im] = mk<double>(b.x + sg * a.x, b.y - sg * a.y);
+// y_l,-m = (-1)^m conj(y_lm), m > 0: one thread per (l, m > 0), index i = l^2 + l + m
+__global__ void k_herm_fill(cx<double> *__restrict__ y, int L) {
+  const size_t n = (size_t)L * L;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    int l = (int)sqrt((double)i);
+    while ((size_t)l * l > i) --l;
+    while ((size_t)(l + 1) * (l + 1) <= i) ++l;
+    const int m = (int)(i - (size_t)l * l) - l;
+    if (m <= 0) continue;
+    const cx<double> a = y[i];
+    y[i - 2 * (size_t)m] = (m & 1) ? mk<double>(-a.x, a.y) : mk<double>(a.x, -a.y);
   }
 }
 
@@ -210,12 +202,9 @@ void launch_re2cx(const double *a, void *b, size_t n, hipStream_t st) {
 void launch_cx2re(const void *a, double *b, size_t n, hipStream_t st) {
   if (n) hipLaunchKernelGGL(k_cx2re, dim3(grid1d(n)), dim3(256), 0, st, (const cx<double> *)a, b, n);
 }
-void launch_re2cx_scaled(const double *a, void *b, size_t n, double s, hipStream_t st) {
-  if (n) hipLaunchKernelGGL(k_re2cx_scaled, dim3(grid1d(n)), dim3(256), 0, st, a, (cx<double> *)b, n, s);
-}
-void launch_herm_sym(void *y, int L, hipStream_t st) {
-  const size_t np = (size_t)L * (L + 1) / 2;
-  if (np) hipLaunchKernelGGL(k_herm_sym, dim3(grid1d(np)), dim3(256), 0, st, (cx<double> *)y, L);
+void launch_herm_fill(void *y, int L, hipStream_t st) {
+  const size_t n = (size_t)L * L;
+  if (n) hipLaunchKernelGGL(k_herm_fill, dim3(grid1d(n)), dim3(256), 0, st, (cx<double> *)y, L);
 }
 
 template <typename TI, typename TO, typename T, int RE>

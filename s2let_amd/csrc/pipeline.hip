@@ -168,6 +168,8 @@ RowArgs<T> row_args(int L, size_t nlines, int in_c, int out_c, int sign, double 
   }
   a.in_centred = in_c;
   a.out_centred = out_c;
+  a.paired = 0;
+  a.pair = 0;
   a.sign = sign;
   a.scale = (T)scale;
   a.d = Sel<T>::d(*lt.dM);
@@ -220,7 +222,7 @@ void contract_inv_all(Plan &P, const Scratch &X, const void *x, const ColDev *co
   mi.scr_out = B.spec;
   auto flush = [&]() {
     Stage s_("contract_inv", st);
-    launch_inv_mfma<T>(X.xtile, P.p.L, Sel<T>::delta(C), Fbase, J, mi, st);
+    launch_inv_mfma<T>(X.xtile, P.p.L, Sel<T>::delta(C), Fbase, J, mi, st, P.real);
     J.n = 0;
     J.nblk = 0;
   };
@@ -253,7 +255,7 @@ void contract_fwd_all(Plan &P, const Scratch &X, const ColDev *cols, const void 
     ms.scr_out = B.parts;
     ms.dlt_out = B.d_lm;
     launch_fwd_mfma<T>(P.d_supers.as<SuperDev>(), (int)P.supers.size(), cols, P.d_efs.as<const T *>(),
-                       Sel<T>::delta(C), P.p.L, Jbase, flm, X.parts, S, ms, st);
+                       Sel<T>::delta(C), P.p.L, Jbase, flm, X.parts, S, ms, st, P.real);
   }
   if (S > 1) {
     Stage s_("contract_fwd_sum", st);
@@ -286,6 +288,7 @@ void theta_groups(Plan &P, const ColDev *cols, int kind, void *in, void *out, co
       const Group &g = P.groups[ng - 1 - k];
       ThetaJob<T> &j = J.j[J.n++];
       j.a = theta_args<T>(g.L, cols + g.col0, g.ncol, kind, P.ss);
+      if (P.real) j.a.mi_lo = g.L - 1;  // the m >= 0 half
       j.in = at<T>(in, g.spec_off);
       j.out = at<T>(out, g.spec_off);
       j.blk0 = J.nblk;
@@ -317,11 +320,18 @@ void rows_all(Plan &P, void *in, void *out, const void *scal_in, void *scal_out,
   for (const Group &g : P.groups) {  // small band-limits (long single-level DFT blocks) first
     const bool scal = g.ngamma == 0;
     const int M = 2 * g.L - 1 + P.ss;  // samples per line
+    // real plans: the lines of the columns n >= 0 (from pos0 on), each paired with its
+    // column -n (relative plane ncol - 1 - 2 pos0 - c); the scaling function with itself
+    const int c0 = P.real ? g.pos0 : 0, nc = scal ? 1 : g.ncol - c0;
+    const size_t so = g.spec_off + (size_t)c0 * map_elems(g.L, P.ss);
     RowJob<T> &j = J.j[J.n++];
-    j.a = row_args<T>(g.L, (size_t)(scal ? 1 : g.ncol) * (g.L + P.ss), in_c, out_c, sign, scaled ? kTwoPi / M : 1.0,
-                      P.ss);
-    j.in = scal ? (scal_in ? scal_in : (const void *)at<T>(in, g.spec_off)) : at<T>(in, g.spec_off);
-    j.out = scal ? (scal_out ? scal_out : (void *)at<T>(out, g.spec_off)) : at<T>(out, g.spec_off);
+    j.a = row_args<T>(g.L, (size_t)nc * (g.L + P.ss), in_c, out_c, sign, scaled ? kTwoPi / M : 1.0, P.ss);
+    if (P.real) {
+      j.a.paired = 1;
+      j.a.pair = scal ? 0 : g.ncol - 1 - 2 * c0;
+    }
+    j.in = scal ? (scal_in ? scal_in : (const void *)at<T>(in, g.spec_off)) : at<T>(in, so);
+    j.out = scal ? (scal_out ? scal_out : (void *)at<T>(out, g.spec_off)) : at<T>(out, so);
     j.io64 = scal ? ((scal_in ? 1 : 0) | (scal_out ? 2 : 0)) : 0;
     j.blk0 = J.nblk;
     J.nblk += (int)((j.a.nlines + j.a.lpb - 1) / j.a.lpb);
@@ -358,18 +368,21 @@ void mix_groups(Plan &P, int which, bool to_wav, const void *src_wav, void *dst_
     const Group &g = P.groups[gi];
     if (g.ngamma == 0) continue;
     const size_t LM = map_elems(g.L, P.ss);
+    // real plans mix the columns n >= 0 only (their matrices are sized for them)
+    const int c0 = P.real ? g.pos0 : 0, nc = g.ncol - c0;
+    const size_t so = g.spec_off + (size_t)c0 * LM;
     MixJob &j = J.j[J.n++];
-    j.W = P.d_gamma.as<cx<double>>() + P.gamma_off[gi] + (size_t)which * g.ngamma * g.ncol;
+    j.W = P.d_gamma.as<cx<double>>() + P.gamma_off[gi] + (size_t)which * g.ngamma * nc;
     if (to_wav) {
-      j.in = at<T>(cols, g.spec_off);
-      j.nin = g.ncol;
+      j.in = at<T>(cols, so);
+      j.nin = nc;
       j.out = (char *)dst_wav + g.out_off * wsz;
       j.nout = g.ngamma;
     } else {
       j.in = (const char *)src_wav + g.out_off * wsz;
       j.nin = g.ngamma;
-      j.out = at<T>(cols_out, g.spec_off);
-      j.nout = g.ncol;
+      j.out = at<T>(cols_out, so);
+      j.nout = nc;
     }
     j.in_stride = j.out_stride = j.npts = LM;
     j.blk0 = J.nblk;
@@ -421,11 +434,13 @@ void synthesis_adjoint_t(Plan &P, const Scratch &X, const void *flm, void *f_wav
   mix_groups<T, double, T>(P, 3, true, nullptr, f_wav, X.spec, X.spec, st, B);
 }
 
-// ---- real signals on a real plan (columns n >= 0; spin 0).  f_wav and f_scal are real;
-// scal_tmp: complex128 staging of the scaling-function samples (nscal entries).
-// Sample-side outputs are the real parts of the half-spectrum sums (the mixing counts
-// n > 0 twice); harmonic-side outputs are Y + H(Y) with Y the half-spectrum sum (the
-// mixing and the scaling function at half weight for n = 0).
+// ---- real signals on a real plan (P.real; spin 0).  A real signal's spectrum of column
+// -n is the conjugate of column n's at -m (after the theta stage and in the row DFTs'
+// spectra), and f_l,-m = (-1)^m conj(f_lm): the contractions and theta stages run on the
+// m >= 0 half of every column, the row DFTs on the columns n >= 0 (m < 0 read from or
+// written to the partner column, conjugated), the gamma mixing on the columns n >= 0
+// (matrices of the real plan), and the harmonics come out for m >= 0 and are mirrored.
+// f_wav and f_scal are real; scal_tmp: complex128 staging of the scaling-function samples.
 template <typename T>
 void real_analysis_t(Plan &P, const Scratch &X, const void *flm, double *f_wav, double *f_scal, void *scal_tmp,
                      size_t nscal, hipStream_t st) {
@@ -443,12 +458,12 @@ void real_synthesis_t(Plan &P, const Scratch &X, const double *f_wav, const doub
                       void *scal_tmp, size_t nscal, hipStream_t st) {
   const BatchIO B;
   const ColDev *cols = P.d_cols_syn.as<ColDev>();
-  launch_re2cx_scaled(f_scal, scal_tmp, nscal, 0.5, st);
+  launch_re2cx(f_scal, scal_tmp, nscal, st);
   mix_groups<double, T, T>(P, 2, false, f_wav, nullptr, X.work, X.work, st, B, 2);
   rows_all<T>(P, X.work, X.spec, scal_tmp, nullptr, 0, 1, -1, true, st, B);
   theta_groups<T>(P, cols, 2, X.spec, X.work, "beta_fwd", st, B);
   contract_fwd_all<T>(P, X, cols, X.work, flm, st, B);
-  launch_herm_sym(flm, P.p.L, st);
+  launch_herm_fill(flm, P.p.L, st);
 }
 
 template <typename T>
@@ -456,12 +471,12 @@ void real_analysis_adjoint_t(Plan &P, const Scratch &X, const double *f_wav, con
                              void *scal_tmp, size_t nscal, hipStream_t st) {
   const BatchIO B;
   const ColDev *cols = P.d_cols_ana.as<ColDev>();
-  launch_re2cx_scaled(f_scal, scal_tmp, nscal, 0.5, st);
+  launch_re2cx(f_scal, scal_tmp, nscal, st);
   mix_groups<double, T, T>(P, 1, false, f_wav, nullptr, X.work, X.work, st, B, 2);
   rows_all<T>(P, X.work, X.spec, scal_tmp, nullptr, 0, 1, -1, false, st, B);
   theta_groups<T>(P, cols, 1, X.spec, X.work, "beta_inv_adj", st, B);
   contract_fwd_all<T>(P, X, cols, X.work, flm, st, B);
-  launch_herm_sym(flm, P.p.L, st);
+  launch_herm_fill(flm, P.p.L, st);
 }
 
 template <typename T>

@@ -54,6 +54,8 @@ struct Group {
   size_t spec_off;  // complex offset of this group's spectra in the scratch buffers
   size_t out_off;   // complex offset of this group's samples in f_wav (scales)
   int scale;        // j, or -1 for the scaling function / plain SHT
+  int pos0 = 0;     // first column with n >= 0 (columns are in increasing n; column -n of
+                    // column c is ncol - 1 - c)
 };
 
 // Scratch of one transform call: the two spectra buffers (spec_elems complex entries each,
@@ -128,7 +130,7 @@ struct Plan {
   std::vector<long long> dtab_host;  // what b_dtab holds once its last upload has run
   PlanSync dtab_copied;               // recorded after that upload (dtab_host is its source)
   bool manual = false;
-  bool real = false;  // half spectrum of a real signal: columns n >= 0 only (spin 0)
+  bool real = false;  // a real signal (spin 0): every stage on half the spectrum (see real_*_t)
   uint64_t last_use = 0;  // plan-cache clock (manual plans are evicted least recently used first)
   PlanSync sync;
   size_t xtile_elems_ = 0, parts_elems = 0;  // sizes of the scratch parts (entries)

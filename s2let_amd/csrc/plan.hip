@@ -425,7 +425,8 @@ Plan &Context::plan(const s2let_parameters_t *p, int mode, hipStream_t st, bool 
     scale_shape(p, j, &Lj, &Nj);
     Group g = cb.open(Lj, Nj, j);
     const int L0 = el0(j, p);
-    for (int n = real ? (Nj + 1) % 2 : -Nj + 1; n < Nj; n += 2) {  // a real signal: n >= 0
+    g.pos0 = Nj / 2;  // n = -Nj + 1 + 2 c >= 0 from c = Nj / 2 on
+    for (int n = -Nj + 1; n < Nj; n += 2) {
       // Psi^j_ln = nu * r(l): analysis uses conj(Psi) (phase i^n conj(nu)), synthesis Psi (i^n nu)
       std::vector<double> b(L, 0.0);
       const int lmin = std::max(std::max(aspin, std::abs(n)), L0);
@@ -629,19 +630,18 @@ void Context::finish_plan(Plan &pl, hipStream_t st) {
   for (const Group &g : pl.groups) ltab(g.L, pl.ss);
   // gamma mixing matrices per scale: [inv (G x nc), inv_adj (nc x G), fwd (nc x G), fwd_adj (G x nc)]
   std::vector<cx<double>> gm;
-  // Real plans (columns n >= 0 of a real signal, whose column -n is the conjugate of
-  // column n): the sample-side outputs (matrices 0 and 3) take the real part of the sum
-  // over n >= 0 with the n > 0 columns counted twice; the harmonic-side outputs
-  // (matrices 1 and 2) become Y with f_lm = Y + H(Y), H(Y)_lm = (-1)^m conj(Y_l,-m), so
-  // the n = 0 column enters Y at half weight.
+  // Real plans: the mixing touches only the columns n >= 0 (c >= pos0); column -n, the
+  // conjugate of column n where the mixing meets it, is implied.  The sample-side outputs
+  // (matrices 0 and 3) are the real parts of the n >= 0 sums with n > 0 counted twice; the
+  // harmonic-side outputs (matrices 1 and 2) are the n >= 0 columns themselves.
   for (const Group &g : pl.groups) {
     pl.gamma_off.push_back(gm.size());
     if (g.ngamma == 0) continue;
-    const int G = g.ngamma, nc = g.ncol;
+    const int G = g.ngamma, c0 = pl.real ? g.pos0 : 0, nc = g.ncol - c0;
     auto E = [&](int c, int gi, int s) {
-      const int n = pl.cols_ana[g.col0 + c].n;
+      const int n = pl.cols_ana[g.col0 + c0 + c].n;
       const long double a = s * 2.0L * 3.14159265358979323846264338327950288L * n * gi / G;
-      const long double w = !pl.real ? 1.0L : (s > 0 ? (n > 0 ? 2.0L : 1.0L) : (n == 0 ? 0.5L : 1.0L));
+      const long double w = (pl.real && s > 0 && n > 0) ? 2.0L : 1.0L;
       return std::complex<long double>(w * cosl(a), w * sinl(a));
     };
     for (int gi = 0; gi < G; ++gi)

@@ -163,7 +163,9 @@ __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const 
   else stream(std::false_type{});
 }
 
-template <typename TI, typename TO, typename T, int IMAX>
+// PAIR: the launch's jobs are a real signal's half spectra (RowArgs::paired); a compile-time
+// switch, so that the complex launches carry none of its code
+template <typename TI, typename TO, typename T, int IMAX, bool PAIR>
 __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
   extern __shared__ __attribute__((aligned(16))) char smem_raw[];
   typedef typename Mfma<T>::acc acc_t;
@@ -185,6 +187,15 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
     int c, r;
     divmod_small(ln, a.rows_pp, inv_rpp, c, r);
     return (size_t)c * a.plane + (size_t)r * len;
+  };
+  // real signals' half spectra: the same row of the partner plane a.pair - c (column -n)
+  // and whether that is the line's own plane
+  const bool pair_in = PAIR && a.in_centred, pair_out = PAIR && a.out_centred;
+  auto pair_at = [&](int ln, int len, bool &self) -> long long {
+    int c, r;
+    divmod_small(ln, a.rows_pp, inv_rpp, c, r);
+    self = a.pair - c == c;
+    return (long long)(a.pair - c) * (long long)a.plane + (long long)r * len;
   };
   T *S = reinterpret_cast<T *>(smem_raw);
   const size_t line0 = (size_t)(blockIdx.x - jobs.j[jb].blk0) * a.lpb;
@@ -215,22 +226,31 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
     // first and is not written)
     constexpr int SU = 4;
     for (int k0 = 0; k0 < nl; k0 += 2) {
-      size_t base[2];
+      long long base[2], pbase[2];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) base[h] = line_at((int)line0 + min(k0 + h, nl - 1), a.in_len);
+      for (int h = 0; h < 2; ++h) {
+        const int ln = (int)line0 + min(k0 + h, nl - 1);
+        bool self;
+        base[h] = (long long)line_at(ln, a.in_len);
+        pbase[h] = pair_in ? pair_at(ln, a.in_len, self) : 0;
+      }
       for (int q0 = 0; q0 < M; q0 += SU * 256) {
         int sidx[SU], pos[SU];
-        bool ok[SU];
+        bool ok[SU], pm[SU];
 #pragma unroll
         for (int u = 0; u < SU; ++u) {
-          // DFT bin q <- centred m = q (q <= L-1) or q - M (q >= M - (L-1)); MWSS bin L is zero
+          // DFT bin q <- centred m = q (q <= L-1) or q - M (q >= M - (L-1)); MWSS bin L is zero;
+          // with a partner plane, m < 0 is its conjugate at -m
           const int q = q0 + 256 * u + tid;
           int si = q;
           bool okq = q < M;
+          pm[u] = false;
           if (a.in_centred) {
             if (q <= half) si = q + half;
-            else if (q >= Mh) si = q - Mh;
-            else okq = false;
+            else if (q >= Mh) {
+              si = pair_in ? M - q + half : q - Mh;
+              pm[u] = pair_in;
+            } else okq = false;
           }
           ok[u] = okq;
           sidx[u] = okq ? si : 0;
@@ -243,14 +263,17 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
           for (int h = 0; h < 2; ++h)
 #pragma unroll
             for (int u = 0; u < SU; ++u) {
-              const cx<double> w = in2[base[h] + sidx[u]];
-              v[h][u] = mk<TI>((TI)w.x, (TI)w.y);
+              const cx<double> w = in2[(pm[u] ? pbase[h] : base[h]) + sidx[u]];
+              v[h][u] = mk<TI>((TI)w.x, (TI)(pm[u] ? -w.y : w.y));
             }
         } else {
 #pragma unroll
           for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int u = 0; u < SU; ++u) v[h][u] = in[base[h] + sidx[u]];
+            for (int u = 0; u < SU; ++u) {
+              const cx<TI> w = in[(pm[u] ? pbase[h] : base[h]) + sidx[u]];
+              v[h][u] = mk<TI>(w.x, pm[u] ? -w.y : w.y);
+            }
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -290,18 +313,24 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
       cx<T> xb[2][HS];
       int cl[2], cn2[2];
       // branch-free: clamped indexes (a sample that is not there reads entry 0 and is zeroed)
-      size_t base[2];
+      long long base[2], pbase[2];
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         divmod_small(min(16 * (2 * tp + u) + r16, ncol - 1), M2, 1.0f / (float)M2, cl[u], cn2[u]);
-        base[u] = line_at((int)line0 + cl[u], a.in_len);
+        bool self;
+        base[u] = (long long)line_at((int)line0 + cl[u], a.in_len);
+        pbase[u] = pair_in ? pair_at((int)line0 + cl[u], a.in_len, self) : 0;
       }
-      auto sample = [&](int u, int j, int &si) {
+      // entry offset of sample j of tile u (the partner plane's conjugate for m < 0 with a
+      // partner: pm)
+      auto sample = [&](int u, int j, long long &off, bool &pm) {
         const int n1 = 4 * j + k4, q = M2 * n1 + cn2[u];
         // centred side: bin q <- m = q (q <= L - 1) or q - M (q >= M - (L - 1)); MWSS bin L is zero
         const bool lo = q <= half, hi = q >= Mh;
         const bool ok = 16 * (2 * tp + u) + r16 < ncol && n1 < M1 && (!a.in_centred || lo || hi);
-        si = ok ? (a.in_centred ? (lo ? q + half : q - Mh) : q) : 0;
+        pm = ok && pair_in && !lo;
+        const int si = ok ? (a.in_centred ? (lo ? q + half : (pair_in ? M - q + half : q - Mh)) : q) : 0;
+        off = (pm ? pbase[u] : base[u]) + si;
         return ok;
       };
       auto fetch = [&](auto ptr) {
@@ -309,10 +338,11 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
         for (int u = 0; u < 2; ++u)
 #pragma unroll
           for (int j = 0; j < HS; ++j) {
-            int si;
-            const bool ok = sample(u, j, si);
-            const auto w = ptr[base[u] + si];
-            xb[u][j] = (ok && 4 * j < M1) ? mk<T>((T)w.x, (T)w.y) : mk<T>((T)0, (T)0);
+            long long off;
+            bool pm;
+            const bool ok = sample(u, j, off, pm);
+            const auto w = ptr[off];
+            xb[u][j] = (ok && 4 * j < M1) ? mk<T>((T)w.x, (T)(pm ? -w.y : w.y)) : mk<T>((T)0, (T)0);
           }
       };
       if (in64) fetch(reinterpret_cast<const cx<double> *>(in));  // uniform: the caller's complex128 samples
@@ -469,19 +499,31 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
         int line, k1;
         divmod_small(min(col, ncol - 1), M1, 1.0f / (float)M1, line, k1);
         const acc_t &r = u ? r1 : r0;
-        cx<TO> *dst = out + line_at((int)line0 + line, a.out_len);
+        const long long lo_off = (long long)line_at((int)line0 + line, a.out_len);
+        bool self = false;
+        const long long po_off = pair_out ? pair_at((int)line0 + line, a.out_len, self) : 0;
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
           const int k2 = 8 * mt + 2 * k4 + p;
           int o = k1 + M1 * k2;
-          // bin o -> centred m; the MWSS Nyquist bin is dropped
+          // bin o -> centred m; the MWSS Nyquist bin is dropped; with a partner plane m < 0
+          // goes there conjugated at -m (not into the line's own plane)
           const bool lo = o <= half, hi = o >= Mh;
-          if (a.out_centred) o = lo ? o + half : o - Mh;
-          if (!(col < ncol && k2 < M2 && (!a.out_centred || lo || hi))) continue;
+          const bool pw = pair_out && !lo;
+          if (a.out_centred) o = lo ? o + half : (pair_out ? M - o + half : o - Mh);
+          if (!(col < ncol && k2 < M2 && (!a.out_centred || lo || hi) && !(pw && self))) continue;
+          const long long e = (pw ? po_off : lo_off) + o;
+          const T re = r[2 * p] * scale, im = pw ? -(r[2 * p + 1] * scale) : r[2 * p + 1] * scale;
           if (out64)
-            reinterpret_cast<cx<double> *>(out)[(dst - out) + o] = mk<double>((double)(r[2 * p] * scale), (double)(r[2 * p + 1] * scale));
+            reinterpret_cast<cx<double> *>(out)[e] = mk<double>((double)re, (double)im);
           else
-            dst[o] = mk<TO>((TO)(r[2 * p] * scale), (TO)(r[2 * p + 1] * scale));
+            out[e] = mk<TO>((TO)re, (TO)im);
+          if (pair_out && o == half && lo && !self) {  // m = 0: the partner's too, conjugated
+            if (out64)
+              reinterpret_cast<cx<double> *>(out)[po_off + half] = mk<double>((double)re, -(double)im);
+            else
+              out[po_off + half] = mk<TO>((TO)re, (TO)(-im));
+          }
         }
       }
     }
@@ -513,8 +555,15 @@ void launch_rows(const RowJobs<T> &jobs, const MapSet &ms, hipStream_t st) {
     shm = std::max(shm, rows_lds_bytes(a.d.M1, a.d.M2, a.d.CSA, a.d.CSC, a.lpb, sizeof(T), a.d.R1, a.d.nks1));
     items = std::max(items, rows_items_per_wave(a.d.M1, a.d.M2, a.d.R1, a.lpb));
   }
-  if (items <= 4) hipLaunchKernelGGL((k_rows4<TI, TO, T, 4>), dim3(jobs.nblk, ms.n), dim3(256), shm, st, jobs, ms);
-  else hipLaunchKernelGGL((k_rows4<TI, TO, T, kRowsItems>), dim3(jobs.nblk, ms.n), dim3(256), shm, st, jobs, ms);
+  const bool pair = jobs.j[0].a.paired;  // all jobs of a launch or none
+  const dim3 grid(jobs.nblk, ms.n);
+  if (pair) {
+    if (items <= 4) hipLaunchKernelGGL((k_rows4<TI, TO, T, 4, true>), grid, dim3(256), shm, st, jobs, ms);
+    else hipLaunchKernelGGL((k_rows4<TI, TO, T, kRowsItems, true>), grid, dim3(256), shm, st, jobs, ms);
+  } else {
+    if (items <= 4) hipLaunchKernelGGL((k_rows4<TI, TO, T, 4, false>), grid, dim3(256), shm, st, jobs, ms);
+    else hipLaunchKernelGGL((k_rows4<TI, TO, T, kRowsItems, false>), grid, dim3(256), shm, st, jobs, ms);
+  }
 }
 template void launch_rows<double, double, double>(const RowJobs<double> &, const MapSet &, hipStream_t);
 template void launch_rows<float, float, float>(const RowJobs<float> &, const MapSet &, hipStream_t);

@@ -112,9 +112,9 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs, MapSet 
   cx<float> *__restrict__ out = (cx<float> *)map_ptr(jobs.j[jb].out, blockIdx.y, ms.scr_out, ms.dlt_out);
   w -= jobs.j[jb].blk0;
   const int L = a.L, M = 2 * L - 1, R = a.R, K = a.K;
-  const int nti = (R + 32 * RT - 1) / (32 * RT), ntm = (M + 63) / 64;
+  const int nti = (R + 32 * RT - 1) / (32 * RT), ntm = (M - a.mi_lo + 63) / 64;
   const int ti = w % nti, rest = w / nti, tm = rest % ntm, c = rest / ntm;
-  const int i0 = ti * 32 * RT, mi0 = tm * 64;
+  const int i0 = ti * 32 * RT, mi0 = a.mi_lo + tm * 64;
   // wave-uniform indexes in scalar registers (scalar address arithmetic)
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int cls_lo = (mi0 - (L - 1)) & 1;

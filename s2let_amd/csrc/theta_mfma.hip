@@ -228,9 +228,9 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs, MapSet ms) {
   cx<T> *__restrict__ out = (cx<T> *)map_ptr(jobs.j[jb].out, blockIdx.y, ms.scr_out, ms.dlt_out);
   w -= jobs.j[jb].blk0;
   const int L = a.L, M = 2 * L - 1, R = a.R, K = a.K;
-  const int nti = (R + 32 * RT - 1) / (32 * RT), ntm = (M + 63) / 64;
+  const int nti = (R + 32 * RT - 1) / (32 * RT), ntm = (M - a.mi_lo + 63) / 64;
   const int ti = w % nti, rest = w / nti, tm = rest % ntm, c = rest / ntm;
-  const int i0 = ti * 32 * RT, mi0 = tm * 64;
+  const int i0 = ti * 32 * RT, mi0 = a.mi_lo + tm * 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   const int cls_lo = (mi0 - (L - 1)) & 1;  // parity of m at mi0 (class of ee = 0 is cls_lo ^ (n & 1))
   const ColDev col = a.cols[c];
@@ -364,7 +364,7 @@ int theta_rows(int Lmax) {
 template <typename T>
 int theta_blocks(const ThetaArgs<T> &a, int rows) {
   const int M = 2 * a.L - 1;
-  return ((a.R + rows - 1) / rows) * ((M + 63) / 64) * a.ncol;
+  return ((a.R + rows - 1) / rows) * ((M - a.mi_lo + 63) / 64) * a.ncol;
 }
 template int theta_blocks<double>(const ThetaArgs<double> &, int);
 template int theta_rows<double>(int);
