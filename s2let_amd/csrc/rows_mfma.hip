@@ -347,6 +347,22 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
       };
       if (in64) fetch(reinterpret_cast<const cx<double> *>(in));  // uniform: the caller's complex128 samples
       else fetch(in);
+      // twiddles w^{n2 k1} of this tile pair's entries (k1 = 8 mt + 2 k4 + p; clamped index
+      // past M1: not written): FP32 issues all of them here, ahead of the products, so that
+      // their latency hides behind them; FP64 (registers) loads them per column tile below
+      const cx<T> *__restrict__ tw = d.tw;
+      constexpr bool kTwAhead = sizeof(T) == 4;
+      cx<T> wv[kTwAhead ? 2 : 1][kRowsDirectNMT][2];
+      auto tw_fetch = [&](int u, cx<T>(&wo)[kRowsDirectNMT][2]) {
+#pragma unroll
+        for (int mt = 0; mt < kRowsDirectNMT; ++mt)
+#pragma unroll
+          for (int p = 0; p < 2; ++p) wo[mt][p] = tw[cn2[u] * min(8 * mt + 2 * k4 + p, M1 - 1)];
+      };
+      if constexpr (kTwAhead) {
+        tw_fetch(0, wv[0]);
+        tw_fetch(1, wv[1]);
+      }
       acc_t r[kRowsDirectNMT][2];
       typedef T T4 __attribute__((ext_vector_type(4)));
 #pragma unroll
@@ -375,11 +391,22 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
         r[mt][1] = c1;
       }
       // twiddle and write the second product's columns (line, k1), rows 2 n2 + re/im
+      const int CSC = d.CSC, K2 = d.K2, imo = KM ? K2 / 4 : 1;
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int col = 16 * (2 * tp + u) + r16;
         if (col >= ncol) continue;
         const int line = cl[u], n2 = cn2[u];
+        cx<T> wl[kRowsDirectNMT][2];
+        if constexpr (kTwAhead) {
+#pragma unroll
+          for (int mt = 0; mt < kRowsDirectNMT; ++mt)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) wl[mt][p] = wv[u][mt][p];
+        } else {
+          tw_fetch(u, wl);
+        }
+        T *y0 = S + (line * M1 + 2 * k4) * CSC + kpos(2 * n2, K2);  // k1 = 2 k4 at mt = 0, p = 0
 #pragma unroll
         for (int mt = 0; mt < kRowsDirectNMT; ++mt) {
           if (mt >= nmt) break;
@@ -387,12 +414,12 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
           for (int p = 0; p < 2; ++p) {
             const int k1 = 8 * mt + 2 * k4 + p;
             if (k1 >= M1) continue;
-            cx<T> w = d.tw[n2 * k1];  // e^{-2 pi i n2 k1 / M}
+            cx<T> w = wl[mt][p];  // e^{-2 pi i n2 k1 / M}
             if (!fwd) w.y = -w.y;
             const cx<T> z = cmul(mk<T>(r[mt][u][2 * p], r[mt][u][2 * p + 1]), w);
-            T *yo = S + (line * M1 + k1) * d.CSC + kpos(2 * n2, d.K2);
+            T *yo = y0 + (8 * mt + p) * CSC;
             yo[0] = z.x;
-            yo[KM ? d.K2 / 4 : 1] = z.y;
+            yo[imo] = z.y;
           }
         }
       }
