@@ -99,11 +99,13 @@ def real_field_alm(rng, L, lmin=2):
     return a
 
 
-def test_c4_spin2_gradient(ops):
+def test_c4_spin2_gradient(ops, oracle):
     """c4: L=256, N=4, B=2, J_min=2, spin 2 (E/B fields, seed 4; target seed 40).
     Loss sum_j ||W_j(f) - T_j||^2 (+ the scaling part); its autograd gradient
-    (hand-written adjoint) against a central finite difference (FP64, step 1e-6)
-    along two random directions."""
+    (hand-written adjoint) against the reference adjoint routine
+    (s2let_analysis_adjoint.c:304-369, the oracle's restatement) at this shape,
+    and against a central finite difference (FP64, step 1e-6) along two random
+    directions."""
     import torch
     L, N = 256, 4
     p = ops.parameters(L, N, 2.0, 2, 2)
@@ -118,6 +120,13 @@ def test_c4_spin2_gradient(ops):
     loss = (w - tw).abs().pow(2).sum() + (s - ts).abs().pow(2).sum()
     loss.backward()
     g = x.grad.detach()
+    gw = (2 * (w.detach() - tw)).cpu().numpy()
+    gs = (2 * (s.detach() - ts)).cpu().numpy()
+    print("c4 oracle adjoint", flush=True)
+    ref = oracle.analysis_adjoint_wav2lm(gw, gs, oracle.params(L, N, 2.0, 2, 2))
+    e_adj = np.linalg.norm(g.cpu().numpy() - ref) / np.linalg.norm(ref)
+    print("c4 gradient vs oracle adjoint %.3e" % e_adj, flush=True)
+    assert e_adj < 1e-12
 
     def lossf(v):
         w2, s2 = ops.analysis(v, p)
