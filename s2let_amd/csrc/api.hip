@@ -4,10 +4,13 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <condition_variable>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <sstream>
+#include <thread>
+#include <vector>
 
 #include "plan.h"
 
@@ -63,11 +66,146 @@ void host_call(const char *fn, F &&f) {
 
 size_t nwav_used(const Plan &P) { return P.n_wav_used; }
 
+// ---- host-pointer transfers.  The reference's entry points take pageable host arrays;
+// large ones (f_wav: 293 MB at L = 512, N = 8) move in chunks through pinned staging
+// buffers: threads copy between the caller's array and a chunk while the DMA engine
+// moves the previous chunk.  Small ones go through hipMemcpyAsync directly.
+
+// Copies a range with the calling thread and kCopyWorkers persistent workers.
+class CopyPool {
+ public:
+  static CopyPool &get() {
+    static CopyPool *p = new CopyPool();  // lives until process exit (its threads too)
+    return *p;
+  }
+  void copy(void *dst, const void *src, size_t n) {
+    const int parts = (int)workers_.size() + 1;
+    if (n < ((size_t)4 << 20) || parts == 1) {
+      memcpy(dst, src, n);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      dst_ = (char *)dst;
+      src_ = (const char *)src;
+      n_ = n;
+      parts_ = parts;
+      pending_ = parts - 1;
+      ++gen_;
+    }
+    cv_.notify_all();
+    piece(0);
+    std::unique_lock<std::mutex> g(mu_);
+    done_.wait(g, [&] { return pending_ == 0; });
+  }
+
+ private:
+  CopyPool() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nw = (int)std::min<unsigned>(kCopyWorkers, hw > 1 ? hw - 1 : 0);
+    for (int i = 0; i < nw; ++i) workers_.emplace_back([this, i] { run(i + 1); });
+  }
+  static constexpr unsigned kCopyWorkers = 7;
+  void piece(int k) {  // 64-byte aligned pieces
+    const size_t per = ((n_ + parts_ - 1) / parts_ + 63) & ~(size_t)63;
+    const size_t b = std::min(n_, per * k), e = std::min(n_, b + per);
+    if (e > b) memcpy(dst_ + b, src_ + b, e - b);
+  }
+  void run(int k) {
+    unsigned seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_.wait(g, [&] { return gen_ != seen; });
+        seen = gen_;
+      }
+      piece(k);
+      std::lock_guard<std::mutex> g(mu_);
+      if (--pending_ == 0) done_.notify_one();
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  char *dst_ = nullptr;
+  const char *src_ = nullptr;
+  size_t n_ = 0;
+  int parts_ = 1, pending_ = 0;
+  unsigned gen_ = 0;
+};
+
+// Pinned staging chunks, each with the event of the last DMA that used it.  Used under
+// the context lock (every host-pointer entry point holds it).
+struct Staging {
+  static constexpr size_t kChunk = (size_t)32 << 20;
+  static constexpr int kBufs = 3;
+  static constexpr size_t kMin = (size_t)8 << 20;  // smaller transfers go directly
+  void *buf[kBufs] = {};
+  hipEvent_t ev[kBufs] = {};
+  bool used[kBufs] = {};
+  int next = 0;
+  static Staging &get() {
+    static Staging *s = new Staging();  // lives until process exit
+    return *s;
+  }
+  int slot() {  // a free chunk (its last DMA done)
+    const int k = next;
+    next = (next + 1) % kBufs;
+    if (!buf[k]) {
+      S2H_CHECK_HIP(hipHostMalloc(&buf[k], kChunk, hipHostMallocDefault));
+      S2H_CHECK_HIP(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+    }
+    if (used[k]) S2H_CHECK_HIP(hipEventSynchronize(ev[k]));
+    return k;
+  }
+  void mark(int k, hipStream_t st) {
+    S2H_CHECK_HIP(hipEventRecord(ev[k], st));
+    used[k] = true;
+  }
+};
+
 void h2d(void *d, const void *h, size_t bytes, hipStream_t st) {
-  if (bytes) S2H_CHECK_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st));
+  if (!bytes) return;
+  if (bytes < Staging::kMin) {
+    S2H_CHECK_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st));
+    return;
+  }
+  Staging &S = Staging::get();
+  for (size_t off = 0; off < bytes; off += Staging::kChunk) {
+    const size_t n = std::min(Staging::kChunk, bytes - off);
+    const int k = S.slot();
+    CopyPool::get().copy(S.buf[k], (const char *)h + off, n);
+    S2H_CHECK_HIP(hipMemcpyAsync((char *)d + off, S.buf[k], n, hipMemcpyHostToDevice, st));
+    S.mark(k, st);
+  }
 }
+// returns once the data is in the caller's array
 void d2h(void *h, const void *d, size_t bytes, hipStream_t st) {
-  if (bytes) S2H_CHECK_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st));
+  if (!bytes) return;
+  if (bytes < Staging::kMin) {
+    S2H_CHECK_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st));
+    S2H_CHECK_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  Staging &S = Staging::get();
+  int pk = -1;
+  size_t poff = 0, pn = 0;
+  auto drain = [&] {  // the previous chunk, once its DMA is done, into the caller's array
+    if (pk < 0) return;
+    S2H_CHECK_HIP(hipEventSynchronize(S.ev[pk]));
+    CopyPool::get().copy((char *)h + poff, S.buf[pk], pn);
+  };
+  for (size_t off = 0; off < bytes; off += Staging::kChunk) {
+    const size_t n = std::min(Staging::kChunk, bytes - off);
+    const int k = S.slot();
+    S2H_CHECK_HIP(hipMemcpyAsync(S.buf[k], (const char *)d + off, n, hipMemcpyDeviceToHost, st));
+    S.mark(k, st);
+    drain();
+    pk = k;
+    poff = off;
+    pn = n;
+  }
+  drain();
 }
 
 constexpr size_t CB = sizeof(std::complex<double>);
