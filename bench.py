@@ -183,10 +183,12 @@ def _timeit(fn):
     return time.perf_counter() - t0
 
 
-def c3_batch(s2, ops, torch, steps, warmup, modes=("fp64", "mixed")):
+def c3_batch(s2, ops, torch, steps, warmup, modes=("fp64", "mixed"), before_timed=None):
     """The uneven batch: 256 maps, L in {64,128,256,512}, N in {1,4,8}, spin in {0,2},
     B=2, J_min=1, f_lm with C_l = (l+1)^-2, drawn with numpy default_rng(3); one step =
-    the analysis batch then the synthesis batch, each one cached HIP graph."""
+    the analysis batch then the synthesis batch, each one cached HIP graph.
+    before_timed() runs right before each mode's warm-up and timed batches (after the
+    autotuner's trial launches), e.g. to mark where a counter profile starts."""
     rng = np.random.default_rng(3)
     shapes = [(int(rng.choice([64, 128, 256, 512])), int(rng.choice([1, 4, 8])), int(rng.choice([0, 2])))
               for _ in range(256)]
@@ -221,6 +223,8 @@ def c3_batch(s2, ops, torch, steps, warmup, modes=("fp64", "mixed")):
             ops.batch(0, fw, fs, x, None, ps, mode, graph=True)
             ops.batch(1, rec, None, fw, fs, ps, mode, graph=True)
 
+        if before_timed is not None:
+            before_timed()
         dt = timed(step, steps, warmup, torch)
         worst = max(float((rec[k] - x[k]).abs().max() / x[k].abs().max()) for k in range(len(ps)))
         out[name] = {"value": steps * len(ps) / dt, "unit": "maps/s", "ms_per_batch": 1e3 * dt / steps,
@@ -340,7 +344,8 @@ def main():
     ap.add_argument("--N", type=int, default=CFG["N"])
     ap.add_argument("--J_min", type=int, default=CFG["J_min"])
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
-                    help="first copy MIB MiB device-to-device 3 times (reference kernel for PMC byte counters)")
+                    help="copy MIB MiB device-to-device 3 times (reference kernel for PMC byte counters): first "
+                         "thing, or with --c3-only right before each mode's warm-up and timed batches")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -351,7 +356,7 @@ def main():
     import s2let_amd as s2
     from s2let_amd import torch_ops as ops
 
-    if args.copy_calib > 0:
+    def copy_calib():
         src = torch.ones(args.copy_calib << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
@@ -361,8 +366,11 @@ def main():
 
     if args.c3_only:
         modes = ("fp64", "mixed") if args.c3_only == "both" else (args.c3_only,)
-        print(json.dumps({"c3_batch": c3_batch(s2, ops, torch, args.steps, args.warmup, modes)}))
+        mark = copy_calib if args.copy_calib > 0 else None
+        print(json.dumps({"c3_batch": c3_batch(s2, ops, torch, args.steps, args.warmup, modes, mark)}))
         return
+    if args.copy_calib > 0:
+        copy_calib()
 
     cfg = dict(CFG, L=args.L, N=args.N, J_min=args.J_min)
     p = s2.parameters(cfg["L"], cfg["N"], cfg["B"], cfg["J_min"], cfg["spin"], cfg["upsample"])
