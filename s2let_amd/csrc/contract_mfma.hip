@@ -368,7 +368,9 @@ template <typename T>
 struct FwdTile {
   static constexpr int MC = sizeof(T) == 8 ? 4 : 8;  // m' per staged chunk
   static constexpr int PD = sizeof(T) == 8 ? 1 : 2;  // m' records in flight (FP64: one, for registers)
+  static_assert(MC % PD == 0, "the record ring runs across chunk ends");
   static constexpr int WAVES = 8;
+  static constexpr int CPT = 512 / (64 * WAVES);  // staged columns per thread (16 columns x 32 lanes)
 };
 
 // split z (m' from z * chunk) holds partial sums for row l iff it starts at or below the
@@ -376,12 +378,12 @@ struct FwdTile {
 __host__ __device__ inline bool fwd_split_covers(int z, int chunk, int l) { return z * chunk <= 16 * (l / 16) + 15; }
 
 template <typename T, bool HM>
-__global__ __launch_bounds__(512) void k_fwd3(const SuperDev *__restrict__ sgs, int nsg, const ColDev *__restrict__ cols,
+__global__ __launch_bounds__(64 * FwdTile<T>::WAVES) __attribute__((amdgpu_waves_per_eu(4))) void k_fwd3(const SuperDev *__restrict__ sgs, int nsg, const ColDev *__restrict__ cols,
                                               const T *const *__restrict__ Efs, const T *__restrict__ dt, int L,
                                               const cx<T> *__restrict__ Jbase0, cx<double> *__restrict__ ypart0,
                                               int mpchunk, MapSet ms) {
-  constexpr int MC = FwdTile<T>::MC, PD = FwdTile<T>::PD;
-  __shared__ T sJ[MC][16][16][4];  // [m'][column][|m|][variant]
+  constexpr int MC = FwdTile<T>::MC, PD = FwdTile<T>::PD, WV = FwdTile<T>::WAVES, CPT = FwdTile<T>::CPT;
+  __shared__ T sJ[2][MC][16][16][4];  // [buffer][m'][column][|m|][variant]
   typedef typename Mfma<T>::acc acc_t;
   // (|m| block, l-block group, split, map): the grid is nb x ngroups x (splits * maps);
   // with several maps, maps fastest, dealt across the XCDs so that the maps of one tile
@@ -404,8 +406,8 @@ __global__ __launch_bounds__(512) void k_fwd3(const SuperDev *__restrict__ sgs, 
   const int mbi = bx, mb = 16 * mbi;
   const int nb = (L + 15) / 16;
   const int lb_lo = max(mbi, mps / 16);
-  if (lb_lo + 8 * by >= nb || mps >= L) return;  // uniform: no l-block of this block
-  const int lbw = lb_lo + 8 * by + wave;
+  if (lb_lo + WV * by >= nb || mps >= L) return;  // uniform: no l-block of this block
+  const int lbw = lb_lo + WV * by + wave;
   const bool active = lbw < nb;
   const int l0 = 16 * lbw;
   double acc[4][4];
@@ -413,60 +415,90 @@ __global__ __launch_bounds__(512) void k_fwd3(const SuperDev *__restrict__ sgs, 
   for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-  // J staging: thread tid moves |m| = mb + tid % 16, sign (tid / 16) % 2, column tid / 32,
-  // for the MC m' of a chunk
-  const int ml = tid & 15, pm = (tid >> 4) & 1, cc = tid >> 5, mm = mb + ml;
+  // J staging: thread tid moves |m| = mb + tid % 16, sign (tid / 16) % 2, columns
+  // tid / 32 + k 64 WAVES / 32 (k < CPT), for the MC m' of a chunk
+  const int ml = tid & 15, pm = (tid >> 4) & 1, cc0 = tid >> 5, mm = mb + ml;
   for (int g = 0; g < nsg; ++g) {
     const SuperDev S = sgs[g];
     const int Lg = S.L, Mg = 2 * Lg - 1, nc = S.ncol;
-    if (mb >= Lg) continue;  // uniform over the block
+    if (mb >= Lg || mps >= Lg) continue;  // uniform over the block
     const ColDev *gc = cols + S.col0;
     const T *__restrict__ Ef = Efs[g];
     const int mpe = min(Lg - 1, mps + mpchunk - 1);
     // this thread's element offset at m' = 0 (an invalid slot reads column 0, m = 0, and
     // is zeroed): every load is unconditional, so the wait counts stay exact
-    const bool cok = cc < nc && mm < Lg && !(HM && pm);  // HM: the m >= 0 half only
-    const long long coff = cok ? gc[cc].out_off + (Lg - 1) + (pm ? -mm : mm) : gc[0].out_off + (Lg - 1);
-    cx<T> jr[MC];
+    bool cok[CPT];
+    long long coff[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int cc = cc0 + k * (2 * WV);
+      cok[k] = cc < nc && mm < Lg && !(HM && pm);  // HM: the m >= 0 half only
+      coff[k] = cok[k] ? gc[cc].out_off + (Lg - 1) + (pm ? -mm : mm) : gc[0].out_off + (Lg - 1);
+    }
+    // raw loads; the invalid slots are zeroed where they are stored (a select right after
+    // the loads would make the loop wait for them, and for every record load behind them)
+    cx<T> jr[CPT][MC];
     auto fetch = [&](int mp0) {
 #pragma unroll
-      for (int u = 0; u < MC; ++u) {
-        const int mpp = mp0 + u;
-        const cx<T> v = Jbase[coff + (long long)min(mpp, Lg - 1) * Mg];
-        jr[u] = (cok && mpp < Lg) ? v : mk<T>(0, 0);
-      }
+      for (int k = 0; k < CPT; ++k)
+#pragma unroll
+        for (int u = 0; u < MC; ++u) jr[k][u] = Jbase[coff[k] + (long long)min(mp0 + u, Lg - 1) * Mg];
+    };
+    auto stage = [&](int b, int mp0) {  // jr holds the J of m' = mp0 ...
+#pragma unroll
+      for (int k = 0; k < CPT; ++k)
+#pragma unroll
+        for (int u = 0; u < MC; ++u) {
+          const bool ok = cok[k] && mp0 + u < Lg;
+          sJ[b][u][cc0 + k * (2 * WV)][ml][2 * pm] = ok ? jr[k][u].x : (T)0;
+          sJ[b][u][cc0 + k * (2 * WV)][ml][2 * pm + 1] = ok ? jr[k][u].y : (T)0;
+        }
     };
     const bool run_g = active && l0 < Lg;
-    const int nmp = 16 * (lbw + 1);
+    // this wave's records (run_g): m' from mps to mend (m' <= l0 + 15 and within the
+    // split), record m' of its Delta tile at dq + 256 m', of its weight table at eq + 256 m'
+    // (mps <= mend: l0 >= 16 (mps / 16)).  Every path through the chunk loop issues the
+    // same loads, so that the wait counts stay exact (a path with fewer loads in flight
+    // makes every wait after the join count as if it had them all)
+    const int mend = min(l0 + 15, mpe);
+    const T *dq = dt + dtile_base(lbw) + (size_t)mbi * (16 * (lbw + 1)) * 256 + 4 * lane;
+    const T *eq = Ef + (size_t)lbw * Lg * 256 + 4 * lane;
     fetch(mps);
-    for (int mp0 = mps; mp0 <= mpe; mp0 += MC) {
-      __syncthreads();
+    __syncthreads();  // every wave is done with the previous group's last chunk
+    stage(0, mps);
+    fetch(mps + MC);
+    __builtin_amdgcn_sched_barrier(0);  // J before the records here too, as in the loop
+    // the Delta / weight records of the next PD m' are in flight, across chunk ends (ring
+    // of registers; past mend the loads repeat it)
+    V4<T> dr[PD], er[PD];
+    if (run_g) {
 #pragma unroll
-      for (int u = 0; u < MC; ++u) {
-        sJ[u][cc][ml][2 * pm] = jr[u].x;
-        sJ[u][cc][ml][2 * pm + 1] = jr[u].y;
+      for (int k = 0; k < PD; ++k) {
+        const size_t o = (size_t)min(mps + k, mend) * 256;
+        dr[k] = ld4(dq + o);
+        er[k] = ld4(eq + o);
       }
+    }
+    // J double-buffered: chunk i reads sJ[i % 2]; after the one barrier of chunk i (every
+    // wave done with chunk i - 1, which read the other buffer) the J of chunk i + 1, fetched
+    // during chunk i - 1, goes to the other buffer, and the J of chunk i + 2 is fetched.
+    // Past the group's last chunk these are clamped loads that no chunk reads.
+    // A wave without records takes a loop of its own with the same barriers (run_g is
+    // fixed for the group), so the loop with the records has no path without their loads.
+    if (!run_g) {
+      for (int mp0 = mps, b = 0; mp0 <= mpe; mp0 += MC, b ^= 1) {
+        __syncthreads();
+        stage(b ^ 1, mp0 + MC);
+        fetch(mp0 + 2 * MC);
+      }
+      continue;
+    }
+    int b = 0;
+    for (int mp0 = mps; mp0 <= mpe; mp0 += MC, b ^= 1) {
       __syncthreads();
-      // m' <= l0 + 15 and within the split
-      const int n = min(MC, min(l0 + 16, mpe + 1) - mp0);
-      const bool run = run_g && n > 0;
-      const T *dq = dt + dtile_base(lbw) + ((size_t)mbi * nmp + mp0) * 256 + 4 * lane;
-      const T *eq = Ef + ((size_t)lbw * Lg + mp0) * 256 + 4 * lane;
-      // the Delta / weight records of the next PD m' are in flight (ring of registers;
-      // past the chunk's last m' the loads repeat it)
-      V4<T> dr[PD], er[PD];
-      if (run) {
-#pragma unroll
-        for (int k = 0; k < PD; ++k) {
-          const size_t o = (size_t)min(k, n - 1) * 256;
-          dr[k] = ld4(dq + o);
-          er[k] = ld4(eq + o);
-        }
-      }
-      // the next chunk's J is issued behind the first records, so that waiting for those
-      // does not wait for it (loads complete in order); the last chunk's fetch is never stored
-      fetch(mp0 + MC);
-      if (!run) continue;
+      stage(b ^ 1, mp0 + MC);
+      fetch(mp0 + 2 * MC);
+      const int n = min(MC, mend + 1 - mp0);  // this wave's records in the chunk
       const int par = (l0 + k4 + mp0) & 1;
       // FP32: the chunk's Delta-weighted sums in FP32, added to the FP64 sums once per chunk
       // (the chunks are the same MC-aligned m' ranges for every split); FP64: straight into
@@ -479,21 +511,21 @@ __global__ __launch_bounds__(512) void k_fwd3(const SuperDev *__restrict__ sgs, 
         for (int v = 0; v < 4; ++v) pacc[u][v] = (T)0;
 #pragma unroll
       for (int mc = 0; mc < MC; ++mc) {
-        if (mc >= n) break;
         const V4<T> dv = dr[mc % PD], ev = er[mc % PD];
+        {
+          const size_t o = (size_t)min(mp0 + mc + PD, mend) * 256;  // MC is a multiple of PD
+          dr[mc % PD] = ld4(dq + o);
+          er[mc % PD] = ld4(eq + o);
+        }
+        if (mc >= n) continue;  // uniform; no loads inside
         acc_t h[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) h[u] = Mfma<T>::zero();
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-          const T *bj = &sJ[mc][4 * s + k4][r16][0];
+          const T *bj = &sJ[b][mc][4 * s + k4][r16][0];
 #pragma unroll
           for (int u = 0; u < (HM ? 2 : 4); ++u) h[u] = Mfma<T>::mma(ev.v[s], bj[u], h[u]);
-        }
-        if (mc + PD < MC) {
-          const size_t o = (size_t)min(mc + PD, n - 1) * 256;
-          dr[mc % PD] = ld4(dq + o);
-          er[mc % PD] = ld4(eq + o);
         }
         const bool odd = (par + mc) & 1;
 #pragma unroll
