@@ -85,10 +85,14 @@ struct ThetaArgs {
 // real part of input sample q.  tw[k] = e^{-2 pi i k / M}.
 constexpr int kRowsDirectM1 = 32;  // largest M1 whose first product reads the lines directly
 
+// ext2 (M2 = 8 q + 1): the second product's matrices cover k2, n2 < M2 - 1 (R2 rows,
+// nks2 K steps); the kernel adds the n2 = M2 - 1 term to those outputs and computes the
+// output k2 = M2 - 1 on the vector units.  K2 stays the LDS column length (>= 2 M2).
 template <typename T>
 struct Dft2 {
   int M, M1, M2;
   int R1, K1, R2, K2, CSA, CSC;
+  int nks2, ext2;
   const T *F1f, *F1b, *F2f, *F2b;
   // M1 <= kRowsDirectM1: the first product's matrices with the reduction reordered for
   // operands read straight from the lines (K step s of lane (r16, k4): sample
@@ -242,7 +246,7 @@ template <typename TI, typename TO, typename T>
 void launch_rows(const RowJobs<T> &jobs, const MapSet &ms, hipStream_t st);
 constexpr int kRowsItems = 8;  // first-product items a wave keeps in registers
 int rows_items_per_wave(int M1, int M2, int R1, int lpb);
-size_t rows_lds_bytes(int M1, int M2, int CSA, int CSC, int lpb, size_t esz, int R1, int nks1);
+size_t rows_lds_bytes(int M1, int M2, int CSA, int CSC, int lpb, size_t esz, int R1, int nks1, int ext2);
 // blocks of a mixing job over npts sample points: one pass of two points per thread
 // (256 threads; enough waves in flight to keep HBM busy), at most kMixMaxBlocks
 constexpr int kMixMaxBlocks = 4096;

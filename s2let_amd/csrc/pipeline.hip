@@ -175,7 +175,7 @@ RowArgs<T> row_args(int L, size_t nlines, int in_c, int out_c, int sign, double 
   a.d = Sel<T>::d(*lt.dM);
   const Dft2<T> &d = a.d;
   auto fits = [&](int lpb, size_t cap) {
-    return rows_lds_bytes(d.M1, d.M2, d.CSA, d.CSC, lpb, sizeof(T), d.R1, d.nks1) <= cap &&
+    return rows_lds_bytes(d.M1, d.M2, d.CSA, d.CSC, lpb, sizeof(T), d.R1, d.nks1, d.ext2) <= cap &&
            rows_items_per_wave(d.M1, d.M2, d.R1, lpb) <= kRowsItems;
   };
   // a quarter of the LDS per block (several blocks per CU overlap their staging

@@ -126,12 +126,13 @@ void Context::ensure_delta(int L, bool need_f32, hipStream_t st) {
   }
 }
 
-// Real embedding of the P-point DFT matrix (exponent sign), R x K padded, stored
-// transposed [k][r]; row r = 16 mt + rl holds output (k, re/im) with
-// k = 8 mt + 2 k4 + (v >> 1), re/im = v & 1, where (k4, v) is the accumulator
-// element of a lane that holds row rl (FP32: rl = 4 k4 + v, FP64: rl = k4 + 4 v).
+// Real embedding of the len-point DFT matrix (exponent sign) restricted to the
+// indexes k, n < P (P <= len), R x K padded, stored transposed [k][r]; row
+// r = 16 mt + rl holds output (k, re/im) with k = 8 mt + 2 k4 + (v >> 1), re/im = v & 1,
+// where (k4, v) is the accumulator element of a lane that holds row rl (FP32:
+// rl = 4 k4 + v, FP64: rl = k4 + 4 v).
 template <typename T>
-static void dft_embed(int P, int R, int K, int sign, bool f32layout, std::vector<T> &G) {
+static void dft_embed(int len, int P, int R, int K, int sign, bool f32layout, std::vector<T> &G) {
   const long double pi = 3.14159265358979323846264338327950288L;
   G.assign((size_t)K * R, (T)0);
   for (int r = 0; r < R; ++r) {
@@ -140,7 +141,7 @@ static void dft_embed(int P, int R, int K, int sign, bool f32layout, std::vector
     const int k = 8 * mt + 2 * k4 + (v >> 1), ri = v & 1;
     if (k >= P) continue;
     for (int n = 0; n < P; ++n) {
-      const long double ang = sign * 2.0L * pi * (long double)(((long long)n * k) % P) / (long double)P;
+      const long double ang = sign * 2.0L * pi * (long double)(((long long)n * k) % len) / (long double)len;
       const long double c = cosl(ang), sn = sinl(ang);
       G[(size_t)(2 * n) * R + r] = (T)(ri ? sn : c);
       G[(size_t)(2 * n + 1) * R + r] = (T)(ri ? c : -sn);
@@ -162,12 +163,18 @@ const DftTables &Context::dft(int n) {
     throw ArgFailure();
   }
   auto t = std::make_unique<DftTables>();
-  const int R1 = round_up(2 * m1, 16), K1 = round_up(2 * m1, 16), R2 = round_up(2 * m2, 16), K2 = round_up(2 * m2, 16);
+  // M2 = 8 q + 1 (every power-of-two band-limit): the second product runs indexes
+  // k2, n2 < M2 - 1 on the matrix cores (rows and reduction multiples of 16, no
+  // padding) and the last index on the vector units (Dft2::ext2)
+  const bool ext2 = m1 > 1 && m2 >= 9 && m2 % 8 == 1;
+  const int P2 = ext2 ? m2 - 1 : m2;
+  const int R1 = round_up(2 * m1, 16), K1 = round_up(2 * m1, 16), R2 = round_up(2 * P2, 16), K2 = round_up(2 * m2, 16);
+  const int K2m = round_up(2 * P2, 16);  // reduction of the second product's matrix
   // FP64: columns with the reduction index fastest, odd strides (consecutive columns in
   // consecutive LDS banks); FP32: K-step-major columns (entry k at (k & 3) K/4 + (k >> 2):
   // a lane's four consecutive K steps in one 16-byte read), strides K + 4
   const int CSA = K1 + 5, CSC = K2 + 5, CSA32 = K1 + 4, CSC32 = K2 + 4;
-  const size_t s1 = (size_t)R1 * K1, s2 = (size_t)R2 * K2;
+  const size_t s1 = (size_t)R1 * K1, s2 = (size_t)R2 * K2m;
   std::vector<double> g64, part64;
   std::vector<float> g32, part32;
   // [k][r] embedding -> A-fragment order [mt][s][lane]
@@ -197,10 +204,10 @@ const DftTables &Context::dft(int n) {
         }
   };
   for (int which = 0; which < 4; ++which) {  // F1f, F1b, F2f, F2b
-    const int P = which < 2 ? m1 : m2, R = which < 2 ? R1 : R2, K = which < 2 ? K1 : K2;
+    const int len = which < 2 ? m1 : m2, P = which < 2 ? m1 : P2, R = which < 2 ? R1 : R2, K = which < 2 ? K1 : K2m;
     const int sign = (which & 1) ? +1 : -1;
-    dft_embed<double>(P, R, K, sign, false, part64);
-    dft_embed<float>(P, R, K, sign, true, part32);
+    dft_embed<double>(len, P, R, K, sign, false, part64);
+    dft_embed<float>(len, P, R, K, sign, true, part32);
     frag(part64, R, K, g64);
     frag(part32, R, K, g32);
     if (direct && which < 2) {
@@ -240,6 +247,8 @@ const DftTables &Context::dft(int n) {
     d.K1 = K1;
     d.R2 = R2;
     d.K2 = K2;
+    d.nks2 = K2m / 4;
+    d.ext2 = ext2 ? 1 : 0;
     d.CSA = csa;
     d.CSC = csc;
     d.F1f = g;

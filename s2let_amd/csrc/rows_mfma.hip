@@ -39,15 +39,16 @@ __device__ inline void divmod_small(int n, int d, float inv, int &q, int &r) {
   }
 }
 
-// one (row tile, up to two column tiles) item of a product: acc[u] += F_mt * S[tile u].
-// In FP32, up to kRowsKR K-steps the item's A fragments are all loaded first (one
-// latency per item) into av, which the caller keeps from item to item of the same
+// one (row tile, up to two column tiles) item of a product: acc[u] += F_mt * S[tile u]
+// over nks K steps; seg is the FP32 columns' segment length (K/4 of the LDS column,
+// >= nks).  In FP32, up to kRowsKR K-steps the item's A fragments are all loaded first
+// (one latency per item) into av, which the caller keeps from item to item of the same
 // row tile (load = false); FP64 and longer reductions stream them four K steps at a
 // time, the next four in flight.
 constexpr int kRowsKR = 20;
 constexpr int kRowsDirectNMT = 4;  // first-product row tiles of the direct path: R1 <= 64 for M1 <= 32
 template <typename T>
-__device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const T *S, int cs, int t0, bool two,
+__device__ inline void dft_item(const T *__restrict__ F, int nks, int seg, int mt, const T *S, int cs, int t0, bool two,
                                 int lane, T (&av)[kRowsKR], bool load, typename Mfma<T>::acc &a0,
                                 typename Mfma<T>::acc &a1) {
   const int r16 = lane & 15, k4 = lane >> 4;
@@ -56,7 +57,7 @@ __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const 
   a1 = Mfma<T>::zero();
   if constexpr (sizeof(T) == 4) {
     // K-step-major columns: the lane's K steps s at b[s], four per 16-byte read
-    const float *b0 = S + (16 * t0 + r16) * cs + k4 * nks;
+    const float *b0 = S + (16 * t0 + r16) * cs + k4 * seg;
     const float *b1 = b0 + 16 * cs;
     if (nks <= kRowsKR) {
       if (load) {
@@ -163,10 +164,25 @@ __device__ inline void dft_item(const T *__restrict__ F, int nks, int mt, const 
   else stream(std::false_type{});
 }
 
+// ext2 twiddles per parity class: ceil((M2 + 1) / 2) rounded up to a multiple of 4 (the
+// last-output pass reads them four at a time)
+__host__ __device__ inline int rows_tv_half(int M2) { return ((M2 + 1) / 2 + 3) & ~3; }
+
+// LDS entries of a block's columns: the second product's (direct first product) or the
+// larger of the two products' regions
+__host__ __device__ inline size_t rows_cols_entries(int M1, int M2, int CSA, int CSC, int lpb, bool direct) {
+  const size_t c = (size_t)((lpb * M1 + 15) & ~15) * CSC;
+  if (direct) return c;
+  const size_t a = (M1 > 1) ? (size_t)((lpb * M2 + 15) & ~15) * CSA : 0;
+  return a > c ? a : c;
+}
+
 // PAIR: the launch's jobs are a real signal's half spectra (RowArgs::paired); a compile-time
 // switch, so that the complex launches carry none of its code
 template <typename TI, typename TO, typename T, int IMAX, bool PAIR>
-__global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
+// FP32: at most 168 registers, three waves per SIMD (FP64 would spill)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) void k_rows4(
+    RowJobs<T> jobs, MapSet ms) {
   extern __shared__ __attribute__((aligned(16))) char smem_raw[];
   typedef typename Mfma<T>::acc acc_t;
   int jb = 0;
@@ -211,6 +227,21 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
   // two-level with M1 <= kRowsDirectM1: the first product reads its operand straight
   // from the lines (no staging pass)
   const bool direct = two_level && d.F1fd != nullptr;
+
+  // ext2: w2^{-e} = tw[M1 (M2 - e)] (forward sign; conjugated for the backward one),
+  // e < M2, after the columns in LDS, split by parity: tv[h JT + j] = w2^{-(2 j + h)},
+  // zero past e = M2 - 1 (written before the first barrier, read after it)
+  const bool ext = d.ext2 != 0;
+  const int JT = rows_tv_half(M2);
+  cx<T> *tv = reinterpret_cast<cx<T> *>(S + rows_cols_entries(M1, M2, d.CSA, d.CSC, a.lpb, direct));
+  if (ext) {
+    for (int q = tid; q < 2 * JT; q += 256) {
+      const int h = q >= JT, e = 2 * (q - h * JT) + h;
+      cx<T> w = d.tw[M1 * (e < M2 && e ? M2 - e : 0)];
+      if (!fwd) w.y = -w.y;
+      tv[q] = e < M2 ? w : mk<T>((T)0, (T)0);
+    }
+  }
 
   // ---- stage the lines: zero the padding, then scatter the samples
   if (!direct) {
@@ -347,22 +378,16 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
       };
       if (in64) fetch(reinterpret_cast<const cx<double> *>(in));  // uniform: the caller's complex128 samples
       else fetch(in);
-      // twiddles w^{n2 k1} of this tile pair's entries (k1 = 8 mt + 2 k4 + p; clamped index
-      // past M1: not written): FP32 issues all of them here, ahead of the products, so that
-      // their latency hides behind them; FP64 (registers) loads them per column tile below
+      // twiddles w^{n2 k1} of a column tile's entries (k1 = 8 mt + 2 k4 + p; clamped index
+      // past M1: not written), loaded per column tile after the products (loaded ahead of
+      // them, the 32 registers they hold cost the FP32 kernel its third wave per SIMD)
       const cx<T> *__restrict__ tw = d.tw;
-      constexpr bool kTwAhead = sizeof(T) == 4;
-      cx<T> wv[kTwAhead ? 2 : 1][kRowsDirectNMT][2];
       auto tw_fetch = [&](int u, cx<T>(&wo)[kRowsDirectNMT][2]) {
 #pragma unroll
         for (int mt = 0; mt < kRowsDirectNMT; ++mt)
 #pragma unroll
           for (int p = 0; p < 2; ++p) wo[mt][p] = tw[cn2[u] * min(8 * mt + 2 * k4 + p, M1 - 1)];
       };
-      if constexpr (kTwAhead) {
-        tw_fetch(0, wv[0]);
-        tw_fetch(1, wv[1]);
-      }
       acc_t r[kRowsDirectNMT][2];
       typedef T T4 __attribute__((ext_vector_type(4)));
 #pragma unroll
@@ -398,14 +423,7 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
         if (col >= ncol) continue;
         const int line = cl[u], n2 = cn2[u];
         cx<T> wl[kRowsDirectNMT][2];
-        if constexpr (kTwAhead) {
-#pragma unroll
-          for (int mt = 0; mt < kRowsDirectNMT; ++mt)
-#pragma unroll
-            for (int p = 0; p < 2; ++p) wl[mt][p] = wv[u][mt][p];
-        } else {
-          tw_fetch(u, wl);
-        }
+        tw_fetch(u, wl);
         T *y0 = S + (line * M1 + 2 * k4) * CSC + kpos(2 * n2, K2);  // k1 = 2 k4 at mt = 0, p = 0
 #pragma unroll
         for (int mt = 0; mt < kRowsDirectNMT; ++mt) {
@@ -439,7 +457,7 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
       if (item >= nitems) break;
       int tp, mt;
       divmod_small(item, nmt, inmt, tp, mt);
-      dft_item<T>(F, nks, mt, S, d.CSA, 2 * tp, 2 * tp + 1 < nt, lane, av, mt != cur, r0[it], r1[it]);
+      dft_item<T>(F, nks, nks, mt, S, d.CSA, 2 * tp, 2 * tp + 1 < nt, lane, av, mt != cur, r0[it], r1[it]);
       cur = mt;
     }
     __syncthreads();  // every column of the first product has been read
@@ -503,9 +521,36 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
 
   {  // second product and output
     const T *__restrict__ F = fwd ? d.F2f : d.F2b;
-    const int nks = d.K2 / 4, nmt = d.R2 / 16;
+    const int nks = d.nks2, seg = d.K2 / 4, nmt = d.R2 / 16;
     const int ncol = nl * M1, nt = (ncol + 15) / 16, ntp = (nt + 1) / 2, nitems = nmt * ntp;
     const T scale = a.scale;
+    // ext2: the matrix covers k2, n2 < P = M2 - 1; the n2 = P term of each output
+    // (w2^{P k2} = w2^{-k2}) and the output k2 = P come from the vector units
+    const int P = M2 - 1, K2 = d.K2, CSC = d.CSC;
+    // output bin o (0 <= o < M) of block line `line`, scaled value (re, im): bin -> centred
+    // m; the MWSS Nyquist bin is dropped; with a partner plane m < 0 goes there conjugated
+    // at -m (not into the line's own plane), and m = 0 into both
+    auto put = [&](int line, int o, T re, T im) {
+      const long long lo_off = (long long)line_at((int)line0 + line, a.out_len);
+      bool self = false;
+      const long long po_off = pair_out ? pair_at((int)line0 + line, a.out_len, self) : 0;
+      const bool lo = o <= half, hi = o >= Mh;
+      const bool pw = pair_out && !lo;
+      if (a.out_centred) o = lo ? o + half : (pair_out ? M - o + half : o - Mh);
+      if (!((!a.out_centred || lo || hi) && !(pw && self))) return;
+      const long long e = (pw ? po_off : lo_off) + o;
+      if (pw) im = -im;
+      if (out64)
+        reinterpret_cast<cx<double> *>(out)[e] = mk<double>((double)re, (double)im);
+      else
+        out[e] = mk<TO>((TO)re, (TO)im);
+      if (pair_out && o == half && lo && !self) {  // m = 0: the partner's too, conjugated
+        if (out64)
+          reinterpret_cast<cx<double> *>(out)[po_off + half] = mk<double>((double)re, -(double)im);
+        else
+          out[po_off + half] = mk<TO>((TO)re, (TO)(-im));
+      }
+    };
     // FP32: contiguous items per wave, row tile major, so that the A fragments kept in
     // registers change once or twice; FP64 (streamed fragments): items round-robin
     constexpr bool contig = sizeof(T) == 4;
@@ -517,40 +562,73 @@ __global__ __launch_bounds__(256) void k_rows4(RowJobs<T> jobs, MapSet ms) {
       int mt, tp;
       if (contig) divmod_small(item, ntp, 1.0f / (float)ntp, mt, tp);
       else divmod_small(item, nmt, 1.0f / (float)nmt, tp, mt);
+      // ext2: w2^{-k2} of the lane's outputs and Y[k1][P] of its columns, read before the
+      // products so that their latency hides behind them
+      cx<T> wp[2], yp[2];
+#pragma unroll
+      for (int p = 0; p < 2; ++p) wp[p] = ext ? tv[p * JT + min(4 * mt + k4, JT - 1)] : mk<T>((T)0, (T)0);
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int colc = min(16 * (2 * tp + u) + r16, ncol - 1);
+        yp[u] = ext ? mk<T>(S[colc * CSC + kpos(2 * P, K2)], S[colc * CSC + kpos(2 * P + 1, K2)]) : mk<T>((T)0, (T)0);
+      }
       acc_t r0, r1;
-      dft_item<T>(F, nks, mt, S, d.CSC, 2 * tp, 2 * tp + 1 < nt, lane, av, mt != cur, r0, r1);
+      dft_item<T>(F, nks, seg, mt, S, CSC, 2 * tp, 2 * tp + 1 < nt, lane, av, mt != cur, r0, r1);
       cur = mt;
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
-        const int col = 16 * (2 * tp + u) + r16;
+        const int col = 16 * (2 * tp + u) + r16, colc = min(col, ncol - 1);
         int line, k1;
-        divmod_small(min(col, ncol - 1), M1, 1.0f / (float)M1, line, k1);
+        divmod_small(colc, M1, 1.0f / (float)M1, line, k1);
         const acc_t &r = u ? r1 : r0;
-        const long long lo_off = (long long)line_at((int)line0 + line, a.out_len);
-        bool self = false;
-        const long long po_off = pair_out ? pair_at((int)line0 + line, a.out_len, self) : 0;
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
           const int k2 = 8 * mt + 2 * k4 + p;
-          int o = k1 + M1 * k2;
-          // bin o -> centred m; the MWSS Nyquist bin is dropped; with a partner plane m < 0
-          // goes there conjugated at -m (not into the line's own plane)
-          const bool lo = o <= half, hi = o >= Mh;
-          const bool pw = pair_out && !lo;
-          if (a.out_centred) o = lo ? o + half : (pair_out ? M - o + half : o - Mh);
-          if (!(col < ncol && k2 < M2 && (!a.out_centred || lo || hi) && !(pw && self))) continue;
-          const long long e = (pw ? po_off : lo_off) + o;
-          const T re = r[2 * p] * scale, im = pw ? -(r[2 * p + 1] * scale) : r[2 * p + 1] * scale;
-          if (out64)
-            reinterpret_cast<cx<double> *>(out)[e] = mk<double>((double)re, (double)im);
-          else
-            out[e] = mk<TO>((TO)re, (TO)im);
-          if (pair_out && o == half && lo && !self) {  // m = 0: the partner's too, conjugated
-            if (out64)
-              reinterpret_cast<cx<double> *>(out)[po_off + half] = mk<double>((double)re, -(double)im);
-            else
-              out[po_off + half] = mk<TO>((TO)re, (TO)(-im));
+          if (!(col < ncol && k2 < (ext ? P : M2))) continue;
+          cx<T> z = mk<T>(r[2 * p], r[2 * p + 1]);
+          if (ext) z = cadd(z, cmul(wp[p], yp[u]));
+          put(line, k1 + M1 * k2, z.x * scale, z.y * scale);
+        }
+      }
+    }
+    if (ext) {
+      // output k2 = P of every column: sum_{n2 <= P} w2^{-n2} Y[k1][n2], the even n2 and the
+      // odd n2 summed by neighbouring threads (both in or both out of the loop)
+      for (int c2 = tid; c2 < 2 * ncol; c2 += 256) {
+        const int col = c2 >> 1, h = c2 & 1;
+        const T *y = S + col * CSC;
+        const cx<T> *w = tv + h * JT;
+        T sr = 0, si = 0;
+        if constexpr (KM) {
+          // n2 = 2 j + h: real parts at 2 h seg + j, imaginary at (2 h + 1) seg + j, four j per
+          // 16-byte read (past the last n2 the columns hold zeros and the twiddles are zero)
+          const float *yr = y + 2 * h * seg, *yi = yr + seg;
+          for (int j0 = 0; 2 * j0 + h <= P; j0 += 4) {
+            const float4 r4 = *reinterpret_cast<const float4 *>(yr + j0);
+            const float4 i4 = *reinterpret_cast<const float4 *>(yi + j0);
+            const float rv[4] = {r4.x, r4.y, r4.z, r4.w}, iv[4] = {i4.x, i4.y, i4.z, i4.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const cx<T> wq = w[j0 + q];
+              sr = fma(wq.x, rv[q], fma(-wq.y, iv[q], sr));
+              si = fma(wq.x, iv[q], fma(wq.y, rv[q], si));
+            }
           }
+        } else {
+#pragma unroll 4
+          for (int j = 0; 2 * j + h <= P; ++j) {
+            const cx<T> wq = w[j];
+            const T yr = y[2 * (2 * j + h)], yi = y[2 * (2 * j + h) + 1];
+            sr = fma(wq.x, yr, fma(-wq.y, yi, sr));
+            si = fma(wq.x, yi, fma(wq.y, yr, si));
+          }
+        }
+        sr += __shfl_xor(sr, 1);
+        si += __shfl_xor(si, 1);
+        if (h == 0) {
+          int line, k1;
+          divmod_small(col, M1, 1.0f / (float)M1, line, k1);
+          put(line, k1 + M1 * P, sr * scale, si * scale);
         }
       }
     }
@@ -564,12 +642,8 @@ int rows_items_per_wave(int M1, int M2, int R1, int lpb) {
   return ((R1 / 16) * ntp + 3) / 4;
 }
 
-size_t rows_lds_bytes(int M1, int M2, int CSA, int CSC, int lpb, size_t esz, int R1, int nks1) {
-  if (nks1 > 0)  // direct first product: the second product's columns only
-    return (size_t)((lpb * M1 + 15) & ~15) * CSC * esz;
-  const size_t a = (M1 > 1) ? (size_t)((lpb * M2 + 15) & ~15) * CSA : 0;
-  const size_t c = (size_t)((lpb * M1 + 15) & ~15) * CSC;
-  return std::max(a, c) * esz;
+size_t rows_lds_bytes(int M1, int M2, int CSA, int CSC, int lpb, size_t esz, int R1, int nks1, int ext2) {
+  return (rows_cols_entries(M1, M2, CSA, CSC, lpb, nks1 > 0) + (ext2 ? 4 * rows_tv_half(M2) : 0)) * esz;
 }
 
 template <typename TI, typename TO, typename T>
@@ -579,7 +653,7 @@ void launch_rows(const RowJobs<T> &jobs, const MapSet &ms, hipStream_t st) {
   int items = 0;
   for (int k = 0; k < jobs.n; ++k) {
     const RowArgs<T> &a = jobs.j[k].a;
-    shm = std::max(shm, rows_lds_bytes(a.d.M1, a.d.M2, a.d.CSA, a.d.CSC, a.lpb, sizeof(T), a.d.R1, a.d.nks1));
+    shm = std::max(shm, rows_lds_bytes(a.d.M1, a.d.M2, a.d.CSA, a.d.CSC, a.lpb, sizeof(T), a.d.R1, a.d.nks1, a.d.ext2));
     items = std::max(items, rows_items_per_wave(a.d.M1, a.d.M2, a.d.R1, a.lpb));
   }
   const bool pair = jobs.j[0].a.paired;  // all jobs of a launch or none
