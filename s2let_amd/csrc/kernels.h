@@ -100,6 +100,12 @@ struct Dft2 {
   // [mt][s / 4][lane][s % 4] over nks1 K steps (a multiple of 4); else null
   const T *F1fd, *F1bd;
   int nks1;
+  // FP32 mode, M1 <= kRowsDirectM1: the same matrices as bf16 three-part A fragments of
+  // v_mfma_f32_16x16x32_bf16, [mt][c][part][lane][e] = part of G[16 mt + lane % 16][K index
+  // (n1 = 16 c + 4 (lane / 16) + e / 2, real / imaginary e % 2)], nc1 = ceil(M1 / 16) chunks
+  // of 32 reduction terms; else null
+  const unsigned short *F1fb, *F1bb;
+  int nc1;
   const cx<T> *tw;
   const int *perm;
 };

@@ -30,4 +30,33 @@ struct Mfma<float> {
   static __device__ inline acc mma(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 };
 
+// v_mfma_f32_16x16x32_bf16: lane i holds A[i % 16][8 (i / 16) + e] and
+// B[8 (i / 16) + e][i % 16], e < 8; C as for v_mfma_f32_16x16x4f32.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline bf16x8 ldbf8(const unsigned short *p) {
+  return *(const __attribute__((address_space(1))) bf16x8 *)p;
+}
+
+__device__ inline f4 mma_bf(bf16x8 a, bf16x8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+// x = h + m + l exactly for FP32 x (8 + 8 + 8 significant bits)
+__device__ inline void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
+  h = (__bf16)x;
+  const float r = x - (float)h;
+  m = (__bf16)r;
+  l = (__bf16)(r - (float)m);
+}
+
+// A B ~ A_h B_h + A_h B_m + A_m B_h + A_h B_l + A_l B_h + A_m B_m with FP32 accumulation
+// (the omitted products are below 2^-24 |A| |B|), smallest terms first
+__device__ inline f4 mma_bf3(const bf16x8 (&a)[3], const bf16x8 (&b)[3], f4 t) {
+  t = mma_bf(a[1], b[1], t);
+  t = mma_bf(a[0], b[2], t);
+  t = mma_bf(a[2], b[0], t);
+  t = mma_bf(a[0], b[1], t);
+  t = mma_bf(a[1], b[0], t);
+  return mma_bf(a[0], b[0], t);
+}
+
 }  // namespace s2h

@@ -31,6 +31,7 @@ struct DftTables {  // two-level DFT of one length M (see Dft2)
   Dft2<float> d32;
   DevBuf g64, g32, tw64, tw32, perm, perm32;
   DevBuf g64d, g32d;  // first-product matrices in the direct-operand order (M1 <= kRowsDirectM1)
+  DevBuf gbf;         // their bf16 three-part split for the FP32 mode (Dft2<float>::F1fb / F1bb)
 };
 
 struct LTables {  // per band-limit L and sampling (MW / MWSS)

@@ -203,6 +203,43 @@ const DftTables &Context::dft(int n) {
           out[o] = (n1 < m1) ? G[(size_t)(2 * n1 + c) * R1 + 16 * mt + ln % 16] : 0;
         }
   };
+  // bf16 three-part A fragments of the direct first product (FP32 mode), from the FP64
+  // entries: x = h + m + l with h = bf16(x), m = bf16(x - h), l = bf16(x - h - m) (RNE)
+  const int nc1 = (m1 + 15) / 16;
+  std::vector<unsigned short> gbf;
+  auto bf16_bits = [](double v) -> unsigned short {
+    const float f = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+  };
+  auto bf16_val = [](unsigned short b) -> double {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return (double)f;
+  };
+  auto frag_bf = [&](const std::vector<double> &G) {  // G: [k][r] embedding, FP32 row layout
+    const size_t base = gbf.size(), pe = (size_t)64 * 8;
+    gbf.resize(base + (size_t)(R1 / 16) * nc1 * 3 * pe);
+    for (int mt = 0; mt < R1 / 16; ++mt)
+      for (int c = 0; c < nc1; ++c)
+        for (int ln = 0; ln < 64; ++ln)
+          for (int e = 0; e < 8; ++e) {
+            const int n1 = 16 * c + 4 * (ln / 16) + e / 2, ri = e % 2;
+            const double x = n1 < m1 ? G[(size_t)(2 * n1 + ri) * R1 + 16 * mt + ln % 16] : 0.0;
+            const unsigned short h = bf16_bits(x);
+            const double r1 = x - bf16_val(h);
+            const unsigned short m = bf16_bits(r1);
+            const unsigned short l = bf16_bits(r1 - bf16_val(m));
+            const size_t o = base + ((size_t)(mt * nc1 + c) * 3) * pe + (size_t)ln * 8 + e;
+            gbf[o] = h;
+            gbf[o + pe] = m;
+            gbf[o + 2 * pe] = l;
+          }
+  };
+  std::vector<double> part64f;
   for (int which = 0; which < 4; ++which) {  // F1f, F1b, F2f, F2b
     const int len = which < 2 ? m1 : m2, P = which < 2 ? m1 : P2, R = which < 2 ? R1 : R2, K = which < 2 ? K1 : K2m;
     const int sign = (which & 1) ? +1 : -1;
@@ -213,6 +250,8 @@ const DftTables &Context::dft(int n) {
     if (direct && which < 2) {
       frag_direct(part64, g64d);
       frag_direct(part32, g32d);
+      dft_embed<double>(len, P, R, K, sign, true, part64f);
+      frag_bf(part64f);
     }
   }
   std::vector<cx<double>> w(n);
@@ -237,6 +276,7 @@ const DftTables &Context::dft(int n) {
   upload(t->perm32, perm32, st);
   upload(t->g64d, g64d, st);
   upload(t->g32d, g32d, st);
+  upload(t->gbf, gbf, st);
   S2H_CHECK_HIP(hipStreamSynchronize(st));
   const size_t sd = (size_t)(R1 / 16) * nks1 * 64;
   auto fill = [&](auto &d, auto *g, auto *tw, int csa, int csc, const int *pm, auto *gd) {
@@ -259,10 +299,18 @@ const DftTables &Context::dft(int n) {
     d.perm = pm;
     d.F1fd = direct ? gd : nullptr;
     d.F1bd = direct ? gd + sd : nullptr;
+    d.F1fb = d.F1bb = nullptr;
+    d.nc1 = 0;
     d.nks1 = direct ? nks1 : 0;
   };
   fill(t->d64, t->g64.as<double>(), t->tw64.as<cx<double>>(), CSA, CSC, t->perm.as<int>(), t->g64d.as<double>());
   fill(t->d32, t->g32.as<float>(), t->tw32.as<cx<float>>(), CSA32, CSC32, t->perm32.as<int>(), t->g32d.as<float>());
+  if (direct) {
+    const size_t sb = (size_t)(R1 / 16) * nc1 * 3 * 64 * 8;
+    t->d32.F1fb = t->gbf.as<unsigned short>();
+    t->d32.F1bb = t->gbf.as<unsigned short>() + sb;
+    t->d32.nc1 = nc1;
+  }
   auto &ref = *t;
   dft_[n] = std::move(t);
   return ref;

@@ -354,8 +354,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
       }
       // entry offset of sample j of tile u (the partner plane's conjugate for m < 0 with a
       // partner: pm)
+      // FP32 (bf16 products): sample j is n1 = 16 (j / 4) + 4 k4 + j % 4, the order of the
+      // B fragments of v_mfma_f32_16x16x32_bf16; FP64: n1 = 4 j + k4
       auto sample = [&](int u, int j, long long &off, bool &pm) {
-        const int n1 = 4 * j + k4, q = M2 * n1 + cn2[u];
+        const int n1 = KM ? 16 * (j >> 2) + 4 * k4 + (j & 3) : 4 * j + k4, q = M2 * n1 + cn2[u];
         // centred side: bin q <- m = q (q <= L - 1) or q - M (q >= M - (L - 1)); MWSS bin L is zero
         const bool lo = q <= half, hi = q >= Mh;
         const bool ok = 16 * (2 * tp + u) + r16 < ncol && n1 < M1 && (!a.in_centred || lo || hi);
@@ -373,7 +375,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
             bool pm;
             const bool ok = sample(u, j, off, pm);
             const auto w = ptr[off];
-            xb[u][j] = (ok && 4 * j < M1) ? mk<T>((T)w.x, (T)(pm ? -w.y : w.y)) : mk<T>((T)0, (T)0);
+            xb[u][j] = (ok && (KM || 4 * j < M1)) ? mk<T>((T)w.x, (T)(pm ? -w.y : w.y)) : mk<T>((T)0, (T)0);
           }
       };
       if (in64) fetch(reinterpret_cast<const cx<double> *>(in));  // uniform: the caller's complex128 samples
@@ -390,6 +392,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
       };
       acc_t r[kRowsDirectNMT][2];
       typedef T T4 __attribute__((ext_vector_type(4)));
+      if constexpr (KM) {
+        // FP32: bf16 three-part products (mma_bf3), FP32 accumulation; element e of chunk c
+        // is the real (e even) or imaginary part of sample 4 c + e / 2.  A row tile's A
+        // fragments are loaded together (a look-ahead of the next tile's would spill).
+        const unsigned short *__restrict__ Fb = fwd ? d.F1fb : d.F1bb;
+        const int nc = d.nc1;
+        constexpr int NC = kRowsDirectM1 / 16;
+        bf16x8 bp[2][NC][3];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+              const cx<T> v = xb[u][4 * c + (e >> 1)];
+              __bf16 h, m, l;
+              split3((e & 1) ? v.y : v.x, h, m, l);
+              bp[u][c][0][e] = h;
+              bp[u][c][1][e] = m;
+              bp[u][c][2][e] = l;
+            }
+#pragma unroll
+        for (int mt = 0; mt < kRowsDirectNMT; ++mt) {
+          if (mt >= nmt) break;
+          bf16x8 ac[NC][3];
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+              ac[c][q] = ldbf8(Fb + ((size_t)(mt * nc + min(c, nc - 1)) * 3 + q) * 512 + 8 * lane);
+          f4 c0 = Mfma<T>::zero(), c1 = Mfma<T>::zero();
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            if (c >= nc) break;
+            c0 = mma_bf3(ac[c], bp[0][c], c0);
+            c1 = mma_bf3(ac[c], bp[1][c], c1);
+          }
+          r[mt][0] = c0;
+          r[mt][1] = c1;
+        }
+      } else {
 #pragma unroll
       for (int mt = 0; mt < kRowsDirectNMT; ++mt) {
         if (mt >= nmt) break;
@@ -414,6 +457,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         }
         r[mt][0] = c0;
         r[mt][1] = c1;
+      }
       }
       // twiddle and write the second product's columns (line, k1), rows 2 n2 + re/im
       const int CSC = d.CSC, K2 = d.K2, imo = KM ? K2 / 4 : 1;

@@ -28,13 +28,6 @@
 
 namespace s2h {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
-__device__ inline bf16x8 ldbf8(const unsigned short *p) {
-  return *(const __attribute__((address_space(1))) bf16x8 *)p;
-}
-
-__device__ inline f4 mma_bf(bf16x8 a, bf16x8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
 __host__ __device__ inline size_t bf3_elems(int L, int ss) {
   const size_t n = (size_t)L + ss;
@@ -87,12 +80,6 @@ void build_theta_bf3(const double *th64, int L, int ss, unsigned short *out, hip
 constexpr int kBfSJ = 40;
 constexpr int kBfRun = 4;  // FP64 run sums: chunks of 32 per FP32 run
 
-__device__ inline void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
-  h = (__bf16)x;
-  const float r = x - (float)h;
-  m = (__bf16)r;
-  l = (__bf16)(r - (float)m);
-}
 
 template <int RT, bool F64S>
 __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs, MapSet ms) {
