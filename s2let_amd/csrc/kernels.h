@@ -106,6 +106,14 @@ struct Dft2 {
   // of 32 reduction terms; else null
   const unsigned short *F1fb, *F1bb;
   int nc1;
+  // with them, the second product too: A fragments [mt][c][part][lane][e] of the rows
+  // (k2, re/im) for every k2 < M2 (FP32 row order) against the reduction index
+  // (n2 = 16 c + 4 (lane / 16) + e / 2, re/im e % 2) over the n2 < M2 (M2 - 1 with ext2),
+  // nc2 chunks of 32; the second product's LDS columns then hold, per chunk c and lane
+  // group kg, the three bf16 parts of its 8 reduction entries (16 bytes each), followed
+  // (ext2) by Y[M2 - 1] in FP32, column stride CSC floats
+  const unsigned short *F2fb, *F2bb;
+  int nc2;
   const cx<T> *tw;
   const int *perm;
 };

@@ -32,6 +32,7 @@ struct DftTables {  // two-level DFT of one length M (see Dft2)
   DevBuf g64, g32, tw64, tw32, perm, perm32;
   DevBuf g64d, g32d;  // first-product matrices in the direct-operand order (M1 <= kRowsDirectM1)
   DevBuf gbf;         // their bf16 three-part split for the FP32 mode (Dft2<float>::F1fb / F1bb)
+  DevBuf gbf2;        // the second product's, same split (Dft2<float>::F2fb / F2bb)
 };
 
 struct LTables {  // per band-limit L and sampling (MW / MWSS)

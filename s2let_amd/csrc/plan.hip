@@ -239,6 +239,40 @@ const DftTables &Context::dft(int n) {
             gbf[o + 2 * pe] = l;
           }
   };
+  // second product (direct first product only): rows (k2, re/im) for k2 < m2, reduction
+  // over n2 < PK (P2), nc2 chunks of 32
+  const int nc2 = (2 * P2 + 31) / 32, R2b = round_up(2 * m2, 16);
+  std::vector<unsigned short> gbf2;
+  auto frag_bf2 = [&](int sign) {
+    const long double pi = 3.14159265358979323846264338327950288L;
+    const size_t base = gbf2.size(), pe = (size_t)64 * 8;
+    gbf2.resize(base + (size_t)(R2b / 16) * nc2 * 3 * pe);
+    for (int mt = 0; mt < R2b / 16; ++mt)
+      for (int c = 0; c < nc2; ++c)
+        for (int ln = 0; ln < 64; ++ln)
+          for (int e = 0; e < 8; ++e) {
+            const int rl = ln % 16, k2 = 8 * mt + 2 * (rl >> 2) + ((rl & 3) >> 1), ro = rl & 1;
+            const int n2 = 16 * c + 4 * (ln / 16) + e / 2, ri = e % 2;
+            double x = 0.0;
+            if (k2 < m2 && n2 < P2) {
+              const long double ang = sign * 2.0L * pi * (long double)(((long long)n2 * k2) % m2) / (long double)m2;
+              const double cs = (double)cosl(ang), sn = (double)sinl(ang);
+              x = ri ? (ro ? cs : -sn) : (ro ? sn : cs);
+            }
+            const unsigned short h = bf16_bits(x);
+            const double r1 = x - bf16_val(h);
+            const unsigned short m = bf16_bits(r1);
+            const unsigned short l = bf16_bits(r1 - bf16_val(m));
+            const size_t o = base + ((size_t)(mt * nc2 + c) * 3) * pe + (size_t)ln * 8 + e;
+            gbf2[o] = h;
+            gbf2[o + pe] = m;
+            gbf2[o + 2 * pe] = l;
+          }
+  };
+  if (direct) {
+    frag_bf2(-1);
+    frag_bf2(+1);
+  }
   std::vector<double> part64f;
   for (int which = 0; which < 4; ++which) {  // F1f, F1b, F2f, F2b
     const int len = which < 2 ? m1 : m2, P = which < 2 ? m1 : P2, R = which < 2 ? R1 : R2, K = which < 2 ? K1 : K2m;
@@ -277,6 +311,7 @@ const DftTables &Context::dft(int n) {
   upload(t->g64d, g64d, st);
   upload(t->g32d, g32d, st);
   upload(t->gbf, gbf, st);
+  upload(t->gbf2, gbf2, st);
   S2H_CHECK_HIP(hipStreamSynchronize(st));
   const size_t sd = (size_t)(R1 / 16) * nks1 * 64;
   auto fill = [&](auto &d, auto *g, auto *tw, int csa, int csc, const int *pm, auto *gd) {
@@ -299,8 +334,8 @@ const DftTables &Context::dft(int n) {
     d.perm = pm;
     d.F1fd = direct ? gd : nullptr;
     d.F1bd = direct ? gd + sd : nullptr;
-    d.F1fb = d.F1bb = nullptr;
-    d.nc1 = 0;
+    d.F1fb = d.F1bb = d.F2fb = d.F2bb = nullptr;
+    d.nc1 = d.nc2 = 0;
     d.nks1 = direct ? nks1 : 0;
   };
   fill(t->d64, t->g64.as<double>(), t->tw64.as<cx<double>>(), CSA, CSC, t->perm.as<int>(), t->g64d.as<double>());
@@ -310,6 +345,13 @@ const DftTables &Context::dft(int n) {
     t->d32.F1fb = t->gbf.as<unsigned short>();
     t->d32.F1bb = t->gbf.as<unsigned short>() + sb;
     t->d32.nc1 = nc1;
+    const size_t sb2 = (size_t)(R2b / 16) * nc2 * 3 * 64 * 8;
+    t->d32.F2fb = t->gbf2.as<unsigned short>();
+    t->d32.F2bb = t->gbf2.as<unsigned short>() + sb2;
+    t->d32.nc2 = nc2;
+    // bf16 columns: nc2 chunks x 4 lane groups x 3 parts x 16 bytes, Y[M2 - 1] (8 bytes,
+    // ext2), 16-byte padding to an odd number of 16-byte units (conflict-free 16-byte reads)
+    t->d32.CSC = nc2 * 48 + 4;
   }
   auto &ref = *t;
   dft_[n] = std::move(t);

@@ -227,6 +227,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
   // two-level with M1 <= kRowsDirectM1: the first product reads its operand straight
   // from the lines (no staging pass)
   const bool direct = two_level && d.F1fd != nullptr;
+  // FP32 direct path: the second product on bf16 three-part products too (Dft2::F2fb)
+  const bool bfsp = KM && direct && d.F2fb != nullptr;
 
   // ext2: w2^{-e} = tw[M1 (M2 - e)] (forward sign; conjugated for the backward one),
   // e < M2, after the columns in LDS, split by parity: tv[h JT + j] = w2^{-(2 j + h)},
@@ -461,6 +463,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
       }
       // twiddle and write the second product's columns (line, k1), rows 2 n2 + re/im
       const int CSC = d.CSC, K2 = d.K2, imo = KM ? K2 / 4 : 1;
+      if (bfsp) {
+        // bf16 columns: the three parts of (re, im) of n2 < PK as one 32-bit word each at
+        // chunk n2 / 16, lane group (n2 / 4) % 4, entry pair n2 % 4; with ext2 Y[P] in FP32
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int col = 16 * (2 * tp + u) + r16;
+          if (col >= ncol) continue;
+          const int line = cl[u], n2 = cn2[u];
+          cx<T> wl[kRowsDirectNMT][2];
+          tw_fetch(u, wl);
+          const bool last = ext && n2 == M2 - 1;
+          char *y0 = reinterpret_cast<char *>(S + (line * M1 + 2 * k4) * CSC) +
+                     (last ? d.nc2 * 192 : (((n2 >> 4) * 4 + ((n2 >> 2) & 3)) * 3) * 16 + (n2 & 3) * 4);
+#pragma unroll
+          for (int mt = 0; mt < kRowsDirectNMT; ++mt) {
+            if (mt >= nmt) break;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+              const int k1 = 8 * mt + 2 * k4 + p;
+              if (k1 >= M1) continue;
+              cx<T> w = wl[mt][p];  // e^{-2 pi i n2 k1 / M}
+              if (!fwd) w.y = -w.y;
+              const cx<T> z = cmul(mk<T>(r[mt][u][2 * p], r[mt][u][2 * p + 1]), w);
+              char *yo = y0 + (size_t)(8 * mt + p) * CSC * sizeof(T);
+              if (last) {
+                *reinterpret_cast<float2 *>(yo) = make_float2(z.x, z.y);
+              } else {
+                __bf16 hx, mx, lx, hy, my, ly;
+                split3(z.x, hx, mx, lx);
+                split3(z.y, hy, my, ly);
+                auto pk = [](__bf16 lo, __bf16 hi) {
+                  return (unsigned)__builtin_bit_cast(unsigned short, lo) |
+                         ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16);
+                };
+                *reinterpret_cast<unsigned *>(yo) = pk(hx, hy);
+                *reinterpret_cast<unsigned *>(yo + 16) = pk(mx, my);
+                *reinterpret_cast<unsigned *>(yo + 32) = pk(lx, ly);
+              }
+            }
+          }
+        }
+      } else
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int col = 16 * (2 * tp + u) + r16;
@@ -595,6 +639,58 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
           out[po_off + half] = mk<TO>((TO)re, (TO)(-im));
       }
     };
+    if (bfsp) {
+      if constexpr (KM) {  // (bfsp implies FP32)
+      // bf16 three-part products over nc2 chunks of 32 reduction terms; rows (k2, re/im) for
+      // every k2 < M2 (with ext2 the n2 = P term is added below, w2^{P k2} = w2^{-k2} for
+      // k2 = P too); contiguous items per wave, row tile major
+      const unsigned short *__restrict__ Fb = fwd ? d.F2fb : d.F2bb;
+      const int nc = d.nc2, nmt2 = (2 * M2 + 15) / 16, nit2 = nmt2 * ntp;
+      const int per2 = (nit2 + 3) / 4, j0 = wave * per2, j1 = min(nit2, j0 + per2);
+      const char *Sb = reinterpret_cast<const char *>(S);
+      for (int item = j0; item < j1; ++item) {
+        int mt, tp;
+        divmod_small(item, ntp, 1.0f / (float)ntp, mt, tp);
+        const int colc[2] = {min(32 * tp + r16, ncol - 1), min(32 * tp + 16 + r16, ncol - 1)};
+        cx<T> wp[2], yp[2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) wp[p] = ext ? tv[p * JT + min(4 * mt + k4, JT - 1)] : mk<T>((T)0, (T)0);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const float2 y = *reinterpret_cast<const float2 *>(Sb + (size_t)colc[u] * CSC * sizeof(T) + nc * 192);
+          yp[u] = ext ? mk<T>(y.x, y.y) : mk<T>((T)0, (T)0);
+        }
+        f4 r0 = Mfma<T>::zero(), r1 = Mfma<T>::zero();
+        for (int c = 0; c < nc; ++c) {
+          bf16x8 av[3], b0[3], b1[3];
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            av[q] = ldbf8(Fb + ((size_t)(mt * nc + c) * 3 + q) * 512 + 8 * lane);
+            const size_t o = ((size_t)(c * 4 + k4) * 3 + q) * 16;
+            b0[q] = *reinterpret_cast<const bf16x8 *>(Sb + (size_t)colc[0] * CSC * sizeof(T) + o);
+            b1[q] = *reinterpret_cast<const bf16x8 *>(Sb + (size_t)colc[1] * CSC * sizeof(T) + o);
+          }
+          r0 = mma_bf3(av, b0, r0);
+          r1 = mma_bf3(av, b1, r1);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int col = 32 * tp + 16 * u + r16;
+          int line, k1;
+          divmod_small(colc[u], M1, 1.0f / (float)M1, line, k1);
+          const f4 &r = u ? r1 : r0;
+#pragma unroll
+          for (int p = 0; p < 2; ++p) {
+            const int k2 = 8 * mt + 2 * k4 + p;
+            if (!(col < ncol && k2 < M2)) continue;
+            cx<T> z = mk<T>(r[2 * p], r[2 * p + 1]);
+            if (ext) z = cadd(z, cmul(wp[p], yp[u]));
+            put(line, k1 + M1 * k2, z.x * scale, z.y * scale);
+          }
+        }
+      }
+      }
+    } else {
     // FP32: contiguous items per wave, row tile major, so that the A fragments kept in
     // registers change once or twice; FP64 (streamed fragments): items round-robin
     constexpr bool contig = sizeof(T) == 4;
@@ -635,7 +731,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         }
       }
     }
-    if (ext) {
+    }
+    if (ext && !bfsp) {
       // output k2 = P of every column: sum_{n2 <= P} w2^{-n2} Y[k1][n2], the even n2 and the
       // odd n2 summed by neighbouring threads (both in or both out of the loop)
       for (int c2 = tid; c2 < 2 * ncol; c2 += 256) {
