@@ -26,17 +26,13 @@
 namespace s2h {
 
 // q = n / d, r = n % d for 0 <= n < 2^22, 0 < d < 2^12 through the FP32 reciprocal
-// (one correction step makes it exact)
+// (one correction step makes it exact; branch-free)
 __device__ inline void divmod_small(int n, int d, float inv, int &q, int &r) {
   q = (int)((float)n * inv);
   r = n - q * d;
-  if (r < 0) {
-    --q;
-    r += d;
-  } else if (r >= d) {
-    ++q;
-    r -= d;
-  }
+  const int lo = r < 0, hi = r >= d;
+  q += hi - lo;
+  r += (lo - hi) * d;
 }
 
 // one (row tile, up to two column tiles) item of a product: acc[u] += F_mt * S[tile u]
@@ -197,21 +193,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
   const bool in64 = sizeof(TI) == 4 && (io & 1), out64 = sizeof(TO) == 4 && (io & 2);
   const Dft2<T> &d = a.d;
   const int M = d.M, M1 = d.M1, M2 = d.M2, half = a.L - 1, Mh = M - half;
-  const float inv_rpp = 1.0f / (float)a.rows_pp;
+  // the job's line geometry in registers (read once; reads of the job table inside the
+  // output loop each waited for every LDS read in flight)
+  const int rows_pp = a.rows_pp, pair = a.pair, in_len = a.in_len, out_len = a.out_len;
+  const size_t plane = a.plane;
+  const bool in_c = a.in_centred, out_c = a.out_centred;
+  const float inv_rpp = 1.0f / (float)rows_pp;
   // entry offset of line `ln` of this job: plane ln / rows_pp, row ln % rows_pp
   auto line_at = [&](int ln, int len) -> size_t {
     int c, r;
-    divmod_small(ln, a.rows_pp, inv_rpp, c, r);
-    return (size_t)c * a.plane + (size_t)r * len;
+    divmod_small(ln, rows_pp, inv_rpp, c, r);
+    return (size_t)c * plane + (size_t)r * len;
   };
   // real signals' half spectra: the same row of the partner plane a.pair - c (column -n)
   // and whether that is the line's own plane
-  const bool pair_in = PAIR && a.in_centred, pair_out = PAIR && a.out_centred;
+  const bool pair_in = PAIR && in_c, pair_out = PAIR && out_c;
   auto pair_at = [&](int ln, int len, bool &self) -> long long {
     int c, r;
-    divmod_small(ln, a.rows_pp, inv_rpp, c, r);
-    self = a.pair - c == c;
-    return (long long)(a.pair - c) * (long long)a.plane + (long long)r * len;
+    divmod_small(ln, rows_pp, inv_rpp, c, r);
+    self = pair - c == c;
+    return (long long)(pair - c) * (long long)plane + (long long)r * len;
   };
   T *S = reinterpret_cast<T *>(smem_raw);
   const size_t line0 = (size_t)(blockIdx.x - jobs.j[jb].blk0) * a.lpb;
@@ -264,8 +265,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
       for (int h = 0; h < 2; ++h) {
         const int ln = (int)line0 + min(k0 + h, nl - 1);
         bool self;
-        base[h] = (long long)line_at(ln, a.in_len);
-        pbase[h] = pair_in ? pair_at(ln, a.in_len, self) : 0;
+        base[h] = (long long)line_at(ln, in_len);
+        pbase[h] = pair_in ? pair_at(ln, in_len, self) : 0;
       }
       for (int q0 = 0; q0 < M; q0 += SU * 256) {
         int sidx[SU], pos[SU];
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
           int si = q;
           bool okq = q < M;
           pm[u] = false;
-          if (a.in_centred) {
+          if (in_c) {
             if (q <= half) si = q + half;
             else if (q >= Mh) {
               si = pair_in ? M - q + half : q - Mh;
@@ -351,8 +352,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
       for (int u = 0; u < 2; ++u) {
         divmod_small(min(16 * (2 * tp + u) + r16, ncol - 1), M2, 1.0f / (float)M2, cl[u], cn2[u]);
         bool self;
-        base[u] = (long long)line_at((int)line0 + cl[u], a.in_len);
-        pbase[u] = pair_in ? pair_at((int)line0 + cl[u], a.in_len, self) : 0;
+        base[u] = (long long)line_at((int)line0 + cl[u], in_len);
+        pbase[u] = pair_in ? pair_at((int)line0 + cl[u], in_len, self) : 0;
       }
       // entry offset of sample j of tile u (the partner plane's conjugate for m < 0 with a
       // partner: pm)
@@ -362,9 +363,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         const int n1 = KM ? 16 * (j >> 2) + 4 * k4 + (j & 3) : 4 * j + k4, q = M2 * n1 + cn2[u];
         // centred side: bin q <- m = q (q <= L - 1) or q - M (q >= M - (L - 1)); MWSS bin L is zero
         const bool lo = q <= half, hi = q >= Mh;
-        const bool ok = 16 * (2 * tp + u) + r16 < ncol && n1 < M1 && (!a.in_centred || lo || hi);
+        const bool ok = 16 * (2 * tp + u) + r16 < ncol && n1 < M1 && (!in_c || lo || hi);
         pm = ok && pair_in && !lo;
-        const int si = ok ? (a.in_centred ? (lo ? q + half : (pair_in ? M - q + half : q - Mh)) : q) : 0;
+        const int si = ok ? (in_c ? (lo ? q + half : (pair_in ? M - q + half : q - Mh)) : q) : 0;
         off = (pm ? pbase[u] : base[u]) + si;
         return ok;
       };
@@ -619,13 +620,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     // m; the MWSS Nyquist bin is dropped; with a partner plane m < 0 goes there conjugated
     // at -m (not into the line's own plane), and m = 0 into both
     auto put = [&](int line, int o, T re, T im) {
-      const long long lo_off = (long long)line_at((int)line0 + line, a.out_len);
+      const long long lo_off = (long long)line_at((int)line0 + line, out_len);
       bool self = false;
-      const long long po_off = pair_out ? pair_at((int)line0 + line, a.out_len, self) : 0;
+      const long long po_off = pair_out ? pair_at((int)line0 + line, out_len, self) : 0;
       const bool lo = o <= half, hi = o >= Mh;
       const bool pw = pair_out && !lo;
-      if (a.out_centred) o = lo ? o + half : (pair_out ? M - o + half : o - Mh);
-      if (!((!a.out_centred || lo || hi) && !(pw && self))) return;
+      if (out_c) o = lo ? o + half : (pair_out ? M - o + half : o - Mh);
+      if (!((!out_c || lo || hi) && !(pw && self))) return;
       const long long e = (pw ? po_off : lo_off) + o;
       if (pw) im = -im;
       if (out64)
@@ -661,17 +662,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
           yp[u] = ext ? mk<T>(y.x, y.y) : mk<T>((T)0, (T)0);
         }
         f4 r0 = Mfma<T>::zero(), r1 = Mfma<T>::zero();
-        for (int c = 0; c < nc; ++c) {
-          bf16x8 av[3], b0[3], b1[3];
+        // operands of chunk c + 1 loaded while chunk c's products run (two register sets;
+        // past the last chunk the loads repeat it, unused)
+        bf16x8 av[2][3], b0[2][3], b1[2][3];
+        const char *s0 = Sb + (size_t)colc[0] * CSC * sizeof(T), *s1 = Sb + (size_t)colc[1] * CSC * sizeof(T);
+        auto ld = [&](int c, int h) {
+          const int cc = min(c, nc - 1);
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
-            av[q] = ldbf8(Fb + ((size_t)(mt * nc + c) * 3 + q) * 512 + 8 * lane);
-            const size_t o = ((size_t)(c * 4 + k4) * 3 + q) * 16;
-            b0[q] = *reinterpret_cast<const bf16x8 *>(Sb + (size_t)colc[0] * CSC * sizeof(T) + o);
-            b1[q] = *reinterpret_cast<const bf16x8 *>(Sb + (size_t)colc[1] * CSC * sizeof(T) + o);
+            av[h][q] = ldbf8(Fb + ((size_t)(mt * nc + cc) * 3 + q) * 512 + 8 * lane);
+            const size_t o = ((size_t)(cc * 4 + k4) * 3 + q) * 16;
+            b0[h][q] = *reinterpret_cast<const bf16x8 *>(s0 + o);
+            b1[h][q] = *reinterpret_cast<const bf16x8 *>(s1 + o);
           }
-          r0 = mma_bf3(av, b0, r0);
-          r1 = mma_bf3(av, b1, r1);
+        };
+        ld(0, 0);
+        int c = 0;
+        for (; c + 1 < nc; c += 2) {
+          ld(c + 1, 1);
+          __builtin_amdgcn_sched_barrier(0);
+          r0 = mma_bf3(av[0], b0[0], r0);
+          r1 = mma_bf3(av[0], b1[0], r1);
+          ld(c + 2, 0);
+          __builtin_amdgcn_sched_barrier(0);
+          r0 = mma_bf3(av[1], b0[1], r0);
+          r1 = mma_bf3(av[1], b1[1], r1);
+        }
+        if (c < nc) {
+          r0 = mma_bf3(av[0], b0[0], r0);
+          r1 = mma_bf3(av[0], b1[0], r1);
         }
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
