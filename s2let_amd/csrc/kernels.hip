@@ -110,8 +110,13 @@ __global__ __launch_bounds__(256) void k_mix(MixJobs jobs, MapSet ms) {
   constexpr size_t isz = RE == 2 ? sizeof(TI) : sizeof(cx<TI>), osz = RE == 1 ? sizeof(TO) : sizeof(cx<TO>);
   const cx<double> *__restrict__ W = static_cast<const cx<double> *>(J.W);
   const int nin = J.nin, nout = J.nout;
-  for (int i = threadIdx.x; i < nin * nout; i += blockDim.x) sW[i] = mk<T>((T)W[i].x, (T)W[i].y);
+  // rows of NMAX entries, zero past nin: every row is read without branches (NMAX nout <= 32 x 32)
+  for (int i = threadIdx.x; i < NMAX * nout; i += blockDim.x) {
+    const int k = i / NMAX, j = i % NMAX;
+    sW[i] = j < nin ? mk<T>((T)W[k * nin + j].x, (T)W[k * nin + j].y) : mk<T>((T)0, (T)0);
+  }
   __syncthreads();
+  const size_t npts = J.npts, in_stride = J.in_stride, out_stride = J.out_stride;
   const size_t b0 = blockIdx.x - J.blk0, nbj = (jb + 1 < jobs.n ? jobs.j[jb + 1].blk0 : jobs.nblk) - J.blk0;
   // two sample points per thread and iteration (twice the loads in flight); the
   // complex128 f_wav side is read with non-temporal loads, and written with
@@ -121,15 +126,15 @@ __global__ __launch_bounds__(256) void k_mix(MixJobs jobs, MapSet ms) {
   // every load of an iteration is unconditional (clamped point and column, the
   // values of columns j >= nin scaled by zero: a select would let the compiler
   // branch around the load), so that all are in flight before the first wait
-  for (size_t p0 = b0 * blockDim.x + threadIdx.x; p0 < J.npts; p0 += 2 * step) {
-    const size_t pts[2] = {p0, min(p0 + step, J.npts - 1)};
-    const bool ok1 = p0 + step < J.npts;
+  for (size_t p0 = b0 * blockDim.x + threadIdx.x; p0 < npts; p0 += 2 * step) {
+    const size_t pts[2] = {p0, min(p0 + step, npts - 1)};
+    const bool ok1 = p0 + step < npts;
     cx<T> v[2][NMAX];
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
       for (int j = 0; j < NMAX; ++j) {
-        const TI *src = (const TI *)(in + (min(j, nin - 1) * J.in_stride + pts[h]) * isz);
+        const TI *src = (const TI *)(in + (min(j, nin - 1) * in_stride + pts[h]) * isz);
         TI ux, uy = 0;
         if constexpr (sizeof(TI) == 8) {
           ux = __builtin_nontemporal_load(src);
@@ -145,19 +150,17 @@ __global__ __launch_bounds__(256) void k_mix(MixJobs jobs, MapSet ms) {
       T ax[2] = {0, 0}, ay[2] = {0, 0};
 #pragma unroll
       for (int j = 0; j < NMAX; ++j) {
-        if (j < nin) {
-          const cx<T> w = sW[k * nin + j];
+        const cx<T> w = sW[k * NMAX + j];
 #pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            ax[h] += v[h][j].x * w.x - v[h][j].y * w.y;
-            ay[h] += v[h][j].x * w.y + v[h][j].y * w.x;
-          }
+        for (int h = 0; h < 2; ++h) {
+          ax[h] += v[h][j].x * w.x - v[h][j].y * w.y;
+          ay[h] += v[h][j].x * w.y + v[h][j].y * w.x;
         }
       }
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         if (h == 1 && !ok1) break;
-        TO *dst = (TO *)(out + (k * J.out_stride + pts[h]) * osz);
+        TO *dst = (TO *)(out + (k * out_stride + pts[h]) * osz);
         if constexpr (RE == 1) {
           if constexpr (sizeof(TO) == 8 && sizeof(T) == 4) __builtin_nontemporal_store((TO)ax[h], dst);
           else *dst = (TO)ax[h];
