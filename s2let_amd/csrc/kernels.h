@@ -18,8 +18,9 @@ struct ColDev {
 };
 
 // Maps of a batch that share one plan.  Grouped launches run map k at blockIdx.y = k
-// (the contractions deal (block, map) items across the XCDs instead, maps fastest, so
-// that the maps of one tile share the Delta records through L2).  Map k's scratch lies
+// (the contractions take (block, map) items from the linear block index instead, maps
+// fastest, so that the maps of one tile run at the same time and share its Delta and
+// weight records).  Map k's scratch lies
 // k * scr bytes past map 0's, its caller buffers dlt[k] bytes from map 0's (device
 // arrays).  A single map: the defaults.
 struct MapSet {
@@ -34,13 +35,6 @@ __device__ inline const char *map_ptr(const void *p, int k, long long scr, const
 }
 __device__ inline char *map_ptr(void *p, int k, long long scr, const long long *dlt) {
   return (char *)p + (k == 0 ? 0 : (dlt ? dlt[k] : (long long)k * scr));
-}
-// Blocks are dealt to the 8 XCDs round-robin (linear block b to XCD b % 8); work item
-// w of a launch of nb blocks is taken so that each XCD runs a contiguous range of items
-// (consecutive items share its L2).  A bijection of [0, nb).
-__device__ inline int xcd_item(int lin, int nb) {
-  const int xx = lin & 7, q8 = nb >> 3, r8 = nb & 7;
-  return xx * q8 + min(xx, r8) + (lin >> 3);
 }
 
 // Consecutive columns sharing one band-limit (several wavelet scales may share
