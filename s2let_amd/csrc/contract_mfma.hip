@@ -195,7 +195,10 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt0, int
   const T *__restrict__ Ei = J.Ei;
   const ColDev *__restrict__ cols = J.cols;
   const int nc = J.nc, lmin = J.lmin, L = J.L;
-  const int b = blk - J.blk0, bx = b % J.gx, by = b / J.gx;
+  // m' groups fastest: neighbouring blocks read neighbouring Delta and weight records and
+  // the same harmonics tiles (m groups fastest measured 16 % slower)
+  const int gy = (J.L + MP - 1) / MP;
+  const int b = blk - J.blk0, by = b % gy, bx = b / gy;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int r16 = lane & 15, k4 = lane >> 4;
   const int mp0 = by * MP;

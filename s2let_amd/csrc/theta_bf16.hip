@@ -89,8 +89,7 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs, MapSet 
   constexpr int kCls = 3 * 2 * 32 * SJ + 32, kBuf = 2 * kCls;
   __shared__ __attribute__((aligned(16))) __bf16 sB[2 * kBuf];
   auto sbo = [&](int buf, int cl, int p, int ri, int j) { return buf * kBuf + cl * kCls + ((p * 2 + ri) * 32 + j) * SJ; };
-  // plain block order: the row tiles of one m-tile on neighbouring blocks (spread over
-  // the XCDs); giving each XCD a contiguous range instead measured 5 % slower
+  // plain block order (giving each XCD a contiguous range of items measured 5 % slower)
   int w = blockIdx.x;
   int jb = 0;
   while (jb + 1 < jobs.n && w >= jobs.j[jb + 1].blk0) ++jb;
@@ -100,7 +99,8 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs, MapSet 
   w -= jobs.j[jb].blk0;
   const int L = a.L, M = 2 * L - 1, R = a.R, K = a.K;
   const int nti = (R + 32 * RT - 1) / (32 * RT), ntm = (M - a.mi_lo + 63) / 64;
-  const int ti = w % nti, rest = w / nti, tm = rest % ntm, c = rest / ntm;
+  // m tiles fastest (row tiles fastest measured 3.5 % slower)
+  const int tm = w % ntm, rest = w / ntm, ti = rest % nti, c = rest / nti;
   const int i0 = ti * 32 * RT, mi0 = a.mi_lo + tm * 64;
   // wave-uniform indexes in scalar registers (scalar address arithmetic)
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
