@@ -20,6 +20,16 @@
 // MFMA.  Padding rows and columns are zero in the tables and in LDS (the direct
 // path clears the second product's region with 16-byte stores first: a per-entry
 // clearing loop of its padding rows measured 10% of the kernel).
+//
+// FP32 data with the direct first product: both products run on bf16 three-part
+// products (mma_bf3, v_mfma_f32_16x16x32_bf16, FP32 accumulation): the lane of the
+// first product holds samples n1 = 16 c + 4 k4 + j of chunk c, and the twiddle pass
+// writes each Y entry's three bf16 parts into LDS in the B-fragment order of the
+// second product (Dft2::F1fb, F2fb).
+// M2 = 8 q + 1 (ext2): the second product's matrix covers n2, k2 < M2 - 1; the term
+// n2 = M2 - 1 is added as w2^{-k2} Y[k1][M2 - 1] (w2^{(M2 - 1) k2} = w2^{-k2}) from an
+// LDS twiddle vector, and the output k2 = M2 - 1 is a dot product on the vector units
+// (FP32 / FP64 products) or one more row tile (bf16 products).
 #include "kernels.h"
 #include "mfma.h"
 
