@@ -48,6 +48,48 @@ __device__ inline void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
   l = (__bf16)(r - (float)m);
 }
 
+// Vector loads through a buffer resource: a wave-uniform base (scalar registers), the
+// lane's 32-bit byte offset and a wave-uniform byte offset (a scalar register, read from
+// the first active lane), so no 64-bit address arithmetic per load on the vector units.  Raw buffer, no range limit
+// (every offset used stays inside the allocation the base points into).
+typedef int i2v __attribute__((ext_vector_type(2)));
+typedef int i4v __attribute__((ext_vector_type(4)));
+// base must be wave-uniform: it is read from the first active lane (a uniform value the
+// compiler holds in vector registers, e.g. after a 64-bit multiply, would otherwise be
+// read lane by lane in a loop)
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void *base) {
+  const unsigned long long u = (unsigned long long)base;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  void *b = (void *)(((unsigned long long)hi << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(b, (short)0, -1, 0x00020000);
+}
+template <typename V>
+__device__ inline V buf_ld8(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  static_assert(sizeof(V) == 8, "8-byte load");
+  return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, __builtin_amdgcn_readfirstlane((int)soff), 0));
+}
+template <typename V>
+__device__ inline V buf_ld16(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  static_assert(sizeof(V) == 16, "16-byte load");
+  return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, __builtin_amdgcn_readfirstlane((int)soff), 0));
+}
+
+// split3 of two values at once, each part as a packed pair (x0's part in the low half):
+// one v_cvt_pk_bf16_f32 per part, the parts' FP32 values from the packed bits
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+__device__ inline unsigned pk_bf16(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_convertvector((f2v){a, b}, bf16x2));
+}
+__device__ inline void split3x2(float x0, float x1, unsigned &h, unsigned &m, unsigned &l) {
+  h = pk_bf16(x0, x1);
+  const float r0 = x0 - __uint_as_float(h << 16), r1 = x1 - __uint_as_float(h & 0xffff0000u);
+  m = pk_bf16(r0, r1);
+  l = pk_bf16(r0 - __uint_as_float(m << 16), r1 - __uint_as_float(m & 0xffff0000u));
+}
+
 // A B ~ A_h B_h + A_h B_m + A_m B_h + A_h B_l + A_l B_h + A_m B_m with FP32 accumulation
 // (the omitted products are below 2^-24 |A| |B|), smallest terms first
 __device__ inline f4 mma_bf3(const bf16x8 (&a)[3], const bf16x8 (&b)[3], f4 t) {

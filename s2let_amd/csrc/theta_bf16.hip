@@ -116,9 +116,12 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs, MapSet 
   const size_t ps = (size_t)nit * nkc * 512;
   // A fragments of row tile r: part p of chunk kc at ab[r] + p ps + kc 512 (tiles past
   // the last one repeat it; their rows are not written)
-  const unsigned short *ab[RT];
+  // (buffer loads: the table base and the tile offsets are wave-uniform)
+  const __amdgpu_buffer_rsrc_t arsc = buf_rsrc(a.AB[cls]);
+  unsigned aoff[RT];
 #pragma unroll
-  for (int r = 0; r < RT; ++r) ab[r] = a.AB[cls] + (size_t)min(i0w / 16 + r, nit - 1) * nkc * 512 + 8 * lane;
+  for (int r = 0; r < RT; ++r) aoff[r] = __builtin_amdgcn_readfirstlane(min(i0w / 16 + r, nit - 1) * nkc * 512);
+  const unsigned lane16 = 16 * lane;
 
   f4 acc[RT][4];
 #pragma unroll
@@ -140,48 +143,50 @@ __global__ __launch_bounds__(256) void k_theta_bf(ThetaJobs<float> jobs, MapSet 
   // columns are zero, and columns mi >= M are not written
   const int ml = tid & 63, kq = wave, mi = mi0 + ml;
   const int scl = (ml & 1) ^ par0, sj = ml >> 1;
-  const cx<float> *src = inc + min(mi, M - 1);
   // the samples of the next two chunks are in flight (two register slots; the K loop
-  // is unrolled by two so that the slot of a chunk is known at compile time)
+  // is unrolled by two so that the slot of a chunk is known at compile time); buffer
+  // loads with the row offset wave-uniform (kq is the wave)
+  const __amdgpu_buffer_rsrc_t brsc = buf_rsrc(inc);
+  const unsigned mic8 = 8u * (unsigned)min(mi, M - 1);
   cx<float> b0[8], b1[8];
   auto load_b = [&](cx<float>(&br)[8], int k0) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) br[e] = src[(size_t)min(k0 + 8 * kq + e, K - 1) * M];
+    for (int e = 0; e < 8; ++e) br[e] = buf_ld8<cx<float>>(brsc, mic8, 8u * (unsigned)(min(k0 + 8 * kq + e, K - 1) * M));
   };
   bf16x8 ar[RT][3];
   auto load_a = [&](int r, int kc) {
     const int kk = min(kc, nkc - 1);
 #pragma unroll
-    for (int p = 0; p < 3; ++p) ar[r][p] = ldbf8(ab[r] + p * ps + (size_t)kk * 512);
+    for (int p = 0; p < 3; ++p) ar[r][p] = buf_ld16<bf16x8>(arsc, lane16, 2u * (aoff[r] + (unsigned)(p * ps) + (unsigned)kk * 512u));
   };
   auto chunk = [&](int kc, cx<float>(&br)[8]) {
     const int k0 = 32 * kc, buf = kc & 1;
     {
-      bf16x8 pr[3], pi[3];
+      // the twisted input row is row 0 or none (launch_theta_bf checks): only wave 0
+      // of the first chunk holds it, as its e = 0 entry
+      if (k0 == 0 && kq == 0) {
+        const bool tw = (scl ? in_row1 : in_row0) == 0;
+        const cx<float> t = ipow_mul(scl ? in_e1 : in_e0, br[0]);
+        br[0] = tw ? t : br[0];
+      }
+      // the parts of entries 2 q and 2 q + 1 as one packed word q
+      u4 pr[3], pi[3];
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        cx<float> v = br[e];
-        // the twisted input row is row 0 or none (launch_theta_bf checks): only
-        // wave 0 of the first chunk holds it, as its e = 0 entry
-        if (e == 0 && k0 == 0 && kq == 0) {
-          const bool tw = (scl ? in_row1 : in_row0) == 0;
-          const cx<float> t = ipow_mul(scl ? in_e1 : in_e0, v);
-          v = tw ? t : v;
-        }
-        __bf16 h, m, l;
-        split3(v.x, h, m, l);
-        pr[0][e] = h;
-        pr[1][e] = m;
-        pr[2][e] = l;
-        split3(v.y, h, m, l);
-        pi[0][e] = h;
-        pi[1][e] = m;
-        pi[2][e] = l;
+      for (int q = 0; q < 4; ++q) {
+        unsigned h, m, l;
+        split3x2(br[2 * q].x, br[2 * q + 1].x, h, m, l);
+        pr[0][q] = h;
+        pr[1][q] = m;
+        pr[2][q] = l;
+        split3x2(br[2 * q].y, br[2 * q + 1].y, h, m, l);
+        pi[0][q] = h;
+        pi[1][q] = m;
+        pi[2][q] = l;
       }
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
-        *reinterpret_cast<bf16x8 *>(&sB[sbo(buf, scl, p, 0, sj) + 8 * kq]) = pr[p];
-        *reinterpret_cast<bf16x8 *>(&sB[sbo(buf, scl, p, 1, sj) + 8 * kq]) = pi[p];
+        *reinterpret_cast<u4 *>(&sB[sbo(buf, scl, p, 0, sj) + 8 * kq]) = pr[p];
+        *reinterpret_cast<u4 *>(&sB[sbo(buf, scl, p, 1, sj) + 8 * kq]) = pi[p];
       }
     }
     __syncthreads();
