@@ -65,6 +65,11 @@ __device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void *base) {
   return __builtin_amdgcn_make_buffer_rsrc(b, (short)0, -1, 0x00020000);
 }
 template <typename V>
+__device__ inline V buf_ld4(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  static_assert(sizeof(V) == 4, "4-byte load");
+  return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, __builtin_amdgcn_readfirstlane((int)soff), 0));
+}
+template <typename V>
 __device__ inline V buf_ld8(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
   static_assert(sizeof(V) == 8, "8-byte load");
   return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, __builtin_amdgcn_readfirstlane((int)soff), 0));
@@ -73,6 +78,13 @@ template <typename V>
 __device__ inline V buf_ld16(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
   static_assert(sizeof(V) == 16, "16-byte load");
   return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, __builtin_amdgcn_readfirstlane((int)soff), 0));
+}
+
+template <typename V>
+__device__ inline V buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(V) == 4) return buf_ld4<V>(r, voff, soff);
+  else if constexpr (sizeof(V) == 8) return buf_ld8<V>(r, voff, soff);
+  else return buf_ld16<V>(r, voff, soff);
 }
 
 // split3 of two values at once, each part as a packed pair (x0's part in the low half):

@@ -380,8 +380,13 @@ const LTables &Context::ltab(int L, int ss) {
   upload(dcs, cs, st);
   upload(dsn, sn, st);
   const size_t n = 8 * (size_t)L * (L + t->ss);
-  t->th64.alloc(n * sizeof(double));
-  t->th32.alloc(n * sizeof(float));
+  // zero-filled tail: k_theta reads the A fragments of a whole last K chunk (up to 31 rows
+  // past K) and row tile (up to 127 entries past R) without bounds checks
+  const size_t tail = 32 * (size_t)(L + t->ss) + 192;
+  t->th64.alloc((n + tail) * sizeof(double));
+  t->th32.alloc((n + tail) * sizeof(float));
+  S2H_CHECK_HIP(hipMemsetAsync(t->th64.as<double>() + n, 0, tail * sizeof(double), st));
+  S2H_CHECK_HIP(hipMemsetAsync(t->th32.as<float>() + n, 0, tail * sizeof(float), st));
   if (t->ss) build_theta_tables_ss(L, dcs.as<double>(), dsn.as<double>(), t->th64.as<double>(), st);
   else build_theta_tables(L, dcs.as<double>(), dsn.as<double>(), t->th64.as<double>(), st);
   launch_to_f32(t->th64.as<double>(), t->th32.as<float>(), n, st);

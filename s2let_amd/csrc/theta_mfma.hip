@@ -211,7 +211,7 @@ struct ThetaTile {
 
 // RT row tiles of 16 per wave: a block covers 32 RT rows x 64 consecutive m.
 template <typename T, bool F64, int RT>
-__global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs, MapSet ms) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && RT == 2 ? 3 : 1))) void k_theta(ThetaJobs<T> jobs, MapSet ms) {
   constexpr int BK = ThetaTile<T>::BK, SJ = ThetaTile<T>::SJ;
   __shared__ T sB[2][2][2][BK][SJ];  // [buffer][class][re/im][k][j]
   typedef typename Mfma<T>::acc acc_t;
@@ -258,47 +258,55 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs, MapSet ms) {
 
   // software pipeline over K chunks: the next chunk's samples (HBM) and A
   // fragments (L2) are loaded while the current chunk's MFMAs run; the samples
-  // go through a double-buffered LDS tile (one barrier per chunk).
+  // go through a double-buffered LDS tile (one barrier per chunk).  The A fragments
+  // alternate between two register sets (the chunk loop is unrolled by two), so no
+  // copy of a set waits for loads issued in the same chunk.
   constexpr int UPT = BK * 64 / 256;  // complex samples staged per thread per chunk
   const int nch = (K + BK - 1) / BK;
   cx<T> breg[UPT];
-  T anx[BK / 4][RT], acur[BK / 4][RT];
-  auto load_chunk = [&](int k0) {
+  T a0[BK / 4][RT], a1[BK / 4][RT];
+  // Unconditional buffer loads with wave-uniform offsets: thread (wave, lane) stages the
+  // samples k = k0 + wave + 4 u of column mi = mi0 + lane, clamped to the last row and
+  // column (rows past K are zeroed where staged; columns past M are not written); the A
+  // fragments of rows k >= K or i >= R come from the table's zero-filled tail or the next
+  // matrix and meet zero samples or unwritten rows.
+  const __amdgpu_buffer_rsrc_t arsc = buf_rsrc(AT), brsc = buf_rsrc(inc);
+  const unsigned voa = (unsigned)sizeof(T) * (unsigned)(k4 * R + r16);
+  const unsigned vob = (unsigned)sizeof(cx<T>) * (unsigned)min(mi0 + lane, M - 1);
+  auto load_b = [&](int k0) {
 #pragma unroll
-    for (int u = 0; u < UPT; ++u) {
-      const int e = tid + 256 * u, kk = e >> 6, ee = e & 63;
-      const int k = k0 + kk, mi = mi0 + ee;
-      breg[u] = (k < K && mi < M) ? inc[(size_t)k * M + mi] : mk<T>(0, 0);
-    }
-#pragma unroll
-    for (int s = 0; s < BK / 4; ++s) {
-      const int k = k0 + 4 * s + k4;
-#pragma unroll
-      for (int r = 0; r < RT; ++r) {
-        const int i = i0w + 16 * r + r16;
-        anx[s][r] = (k < K && i < R) ? AT[(size_t)k * R + i] : (T)0;
-      }
-    }
+    for (int u = 0; u < UPT; ++u)
+      breg[u] = buf_ld<cx<T>>(brsc, vob, (unsigned)sizeof(cx<T>) * (unsigned)(min(k0 + wave + 4 * u, K - 1) * M));
   };
-  load_chunk(0);
-  for (int ch = 0; ch < nch; ++ch) {
-    const int k0 = ch * BK, buf = ch & 1;
+  auto load_a = [&](T(&an)[BK / 4][RT], int k0) {
 #pragma unroll
-    for (int u = 0; u < UPT; ++u) {
-      const int e = tid + 256 * u, kk = e >> 6, ee = e & 63;
-      const int cl = (ee & 1) ^ par0;
-      cx<T> v = breg[u];
-      if (k0 + kk == (cl ? in_row1 : in_row0)) v = ipow_mul(cl ? in_e1 : in_e0, v);
-      sB[buf][cl][0][kk][ee >> 1] = v.x;
-      sB[buf][cl][1][kk][ee >> 1] = v.y;
+    for (int s = 0; s < BK / 4; ++s)
+#pragma unroll
+      for (int r = 0; r < RT; ++r)
+        an[s][r] = buf_ld<T>(arsc, voa, (unsigned)sizeof(T) * (unsigned)((k0 + 4 * s) * R + i0w + 16 * r));
+  };
+  auto chunk = [&](int ch, const T(&ac)[BK / 4][RT], T(&an)[BK / 4][RT]) {
+    const int k0 = ch * BK, buf = ch & 1;
+    // the twisted input row is row 0 or none (launch_theta checks): wave 0's u = 0 entry
+    // of the first chunk; rows past K (last chunk only) are zero
+    const int cl = (lane & 1) ^ par0;
+    if (k0 == 0 && wave == 0 && (cl ? in_row1 : in_row0) == 0) breg[0] = ipow_mul(cl ? in_e1 : in_e0, breg[0]);
+    if (k0 + BK > K) {
+#pragma unroll
+      for (int u = 0; u < UPT; ++u)
+        if (k0 + wave + 4 * u >= K) breg[u] = mk<T>(0, 0);
     }
 #pragma unroll
-    for (int s = 0; s < BK / 4; ++s) {
-#pragma unroll
-      for (int r = 0; r < RT; ++r) acur[s][r] = anx[s][r];
+    for (int u = 0; u < UPT; ++u) {
+      const int kk = wave + 4 * u;
+      sB[buf][cl][0][kk][lane >> 1] = breg[u].x;
+      sB[buf][cl][1][kk][lane >> 1] = breg[u].y;
     }
     __syncthreads();
-    if (ch + 1 < nch) load_chunk(k0 + BK);
+    if (ch + 1 < nch) {
+      load_b(k0 + BK);
+      load_a(an, k0 + BK);
+    }
 #pragma unroll
     for (int s = 0; s < BK / 4; ++s) {
       const int kk = 4 * s + k4;
@@ -308,7 +316,7 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs, MapSet ms) {
 #pragma unroll
       for (int r = 0; r < RT; ++r)
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq) acc[r][qq] = Mfma<T>::mma(acur[s][r], bv[qq], acc[r][qq]);
+        for (int qq = 0; qq < 4; ++qq) acc[r][qq] = Mfma<T>::mma(ac[s][r], bv[qq], acc[r][qq]);
     }
     if constexpr (f64sums) {
       if ((ch + 1) % kThetaRun == 0 || ch + 1 == nch)
@@ -321,7 +329,15 @@ __global__ __launch_bounds__(256) void k_theta(ThetaJobs<T> jobs, MapSet ms) {
           acc[r][qq] = Mfma<T>::zero();
         }
     }
+  };
+  load_b(0);
+  load_a(a0, 0);
+  int ch = 0;
+  for (; ch + 1 < nch; ch += 2) {
+    chunk(ch, a0, a1);
+    chunk(ch + 1, a1, a0);
   }
+  if (ch < nch) chunk(ch, a0, a1);
 
   const cx<double> php = mk<double>(col.ph_re, a.conj_ph ? -col.ph_im : col.ph_im);
   const int zc = a.z_e[cls], orow = a.out_row[cls], oe = a.out_e[cls];
@@ -375,6 +391,8 @@ void launch_theta(const ThetaJobs<T> &jobs, const MapSet &ms, hipStream_t st) {
   int Lmax = 0;
   bool bf = true;
   for (int k = 0; k < jobs.n; ++k) {
+    for (int c = 0; c < 2; ++c)
+      if (jobs.j[k].a.in_row[c] > 0) throw std::runtime_error("launch_theta: twisted input row other than row 0");
     Lmax = std::max(Lmax, jobs.j[k].a.L);
     bf = bf && jobs.j[k].a.AB[0] && jobs.j[k].a.AB[1];
   }
