@@ -56,7 +56,7 @@ constexpr int kRowsDirectNMT = 4;  // first-product row tiles of the direct path
 template <typename T>
 __device__ inline void dft_item(const T *__restrict__ F, int nks, int seg, int mt, const T *S, int cs, int t0, bool two,
                                 int lane, T (&av)[kRowsKR], bool load, typename Mfma<T>::acc &a0,
-                                typename Mfma<T>::acc &a1) {
+                                typename Mfma<T>::acc &a1, bool pre = false) {
   const int r16 = lane & 15, k4 = lane >> 4;
   const T *ga = F + (size_t)mt * nks * 64 + lane;
   a0 = Mfma<T>::zero();
@@ -153,8 +153,14 @@ __device__ inline void dft_item(const T *__restrict__ F, int nks, int seg, int m
         if constexpr (TWO) a1 = Mfma<T>::mma(w[q], b1[4 * (s0 + q)], a1);
       }
     };
+    // pre: the caller has loaded the first group into av[0..3] (dft_preload)
     T wa[4], wb[4];
-    load(0, wa);
+    if (pre) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) wa[q] = av[q];
+    } else {
+      load(0, wa);
+    }
     int s0 = 0;
     for (; s0 + 8 <= nks; s0 += 8) {  // no branch between a load and its use
       load(s0 + 4, wb);
@@ -168,6 +174,14 @@ __device__ inline void dft_item(const T *__restrict__ F, int nks, int seg, int m
   };
   if (two) stream(std::true_type{});
   else stream(std::false_type{});
+}
+
+// streamed (FP64) items: the first four K steps' A fragments of row tile mt into av[0..3]
+template <typename T>
+__device__ inline void dft_preload(const T *__restrict__ F, int nks, int mt, int lane, T (&av)[kRowsKR]) {
+  const T *ga = F + (size_t)mt * nks * 64 + lane;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) av[q] = ga[q * 64];
 }
 
 // ext2 twiddles per parity class: ceil((M2 + 1) / 2) rounded up to a multiple of 4 (the
@@ -629,10 +643,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     // output bin o (0 <= o < M) of block line `line`, scaled value (re, im): bin -> centred
     // m; the MWSS Nyquist bin is dropped; with a partner plane m < 0 goes there conjugated
     // at -m (not into the line's own plane), and m = 0 into both
-    auto put = [&](int line, int o, T re, T im) {
-      const long long lo_off = (long long)line_at((int)line0 + line, out_len);
-      bool self = false;
-      const long long po_off = pair_out ? pair_at((int)line0 + line, out_len, self) : 0;
+    // (put_at: with the line's entry offsets lo_off, po_off and self of base_of)
+    struct LineBase {
+      long long lo_off, po_off;
+      bool self;
+    };
+    auto base_of = [&](int line) {
+      LineBase b;
+      b.lo_off = (long long)line_at((int)line0 + line, out_len);
+      b.self = false;
+      b.po_off = pair_out ? pair_at((int)line0 + line, out_len, b.self) : 0;
+      return b;
+    };
+    auto put_at = [&](const LineBase &b, int o, T re, T im) {
+      const long long lo_off = b.lo_off, po_off = b.po_off;
+      const bool self = b.self;
       const bool lo = o <= half, hi = o >= Mh;
       const bool pw = pair_out && !lo;
       if (out_c) o = lo ? o + half : (pair_out ? M - o + half : o - Mh);
@@ -650,6 +675,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
           out[po_off + half] = mk<TO>((TO)re, (TO)(-im));
       }
     };
+    auto put = [&](int line, int o, T re, T im) { put_at(base_of(line), o, re, im); };
     if (bfsp) {
       if constexpr (KM) {  // (bfsp implies FP32)
       // bf16 three-part products over nc2 chunks of 32 reduction terms; rows (k2, re/im) for
@@ -659,41 +685,52 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
       const int nc = d.nc2, nmt2 = (2 * M2 + 15) / 16, nit2 = nmt2 * ntp;
       const int per2 = (nit2 + 3) / 4, j0 = wave * per2, j1 = min(nit2, j0 + per2);
       const char *Sb = reinterpret_cast<const char *>(S);
-      for (int item = j0; item < j1; ++item) {
-        int mt, tp;
+      // operands of chunk c + 1 loaded while chunk c's products run (two register sets;
+      // past the last chunk the loads repeat it, unused); the next item's first chunk is
+      // loaded before this item's outputs are stored (a load issued after the stores
+      // would wait for them: the vector memory counter retires in order)
+      bf16x8 av[2][3], b0[2][3], b1[2][3];
+      auto ld = [&](int mt, const char *s0, const char *s1, int c, int h) {
+        const int cc = min(c, nc - 1);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          av[h][q] = ldbf8(Fb + ((size_t)(mt * nc + cc) * 3 + q) * 512 + 8 * lane);
+          const size_t o = ((size_t)(cc * 4 + k4) * 3 + q) * 16;
+          b0[h][q] = *reinterpret_cast<const bf16x8 *>(s0 + o);
+          b1[h][q] = *reinterpret_cast<const bf16x8 *>(s1 + o);
+        }
+      };
+      // item geometry: row tile mt, column tile pair tp, the lanes' columns (clamped)
+      auto geo = [&](int item, int &mt, int &tp, int (&colc)[2], const char *(&sc)[2]) {
         divmod_small(item, ntp, 1.0f / (float)ntp, mt, tp);
-        const int colc[2] = {min(32 * tp + r16, ncol - 1), min(32 * tp + 16 + r16, ncol - 1)};
+        colc[0] = min(32 * tp + r16, ncol - 1);
+        colc[1] = min(32 * tp + 16 + r16, ncol - 1);
+        sc[0] = Sb + (size_t)colc[0] * CSC * sizeof(T);
+        sc[1] = Sb + (size_t)colc[1] * CSC * sizeof(T);
+      };
+      int mt = 0, tp = 0, colc[2] = {0, 0};
+      const char *sc[2] = {Sb, Sb};
+      if (j0 < j1) {
+        geo(j0, mt, tp, colc, sc);
+        ld(mt, sc[0], sc[1], 0, 0);
+      }
+      for (int item = j0; item < j1; ++item) {
         cx<T> wp[2], yp[2];
 #pragma unroll
         for (int p = 0; p < 2; ++p) wp[p] = ext ? tv[p * JT + min(4 * mt + k4, JT - 1)] : mk<T>((T)0, (T)0);
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-          const float2 y = *reinterpret_cast<const float2 *>(Sb + (size_t)colc[u] * CSC * sizeof(T) + nc * 192);
+          const float2 y = *reinterpret_cast<const float2 *>(sc[u] + nc * 192);
           yp[u] = ext ? mk<T>(y.x, y.y) : mk<T>((T)0, (T)0);
         }
         f4 r0 = Mfma<T>::zero(), r1 = Mfma<T>::zero();
-        // operands of chunk c + 1 loaded while chunk c's products run (two register sets;
-        // past the last chunk the loads repeat it, unused)
-        bf16x8 av[2][3], b0[2][3], b1[2][3];
-        const char *s0 = Sb + (size_t)colc[0] * CSC * sizeof(T), *s1 = Sb + (size_t)colc[1] * CSC * sizeof(T);
-        auto ld = [&](int c, int h) {
-          const int cc = min(c, nc - 1);
-#pragma unroll
-          for (int q = 0; q < 3; ++q) {
-            av[h][q] = ldbf8(Fb + ((size_t)(mt * nc + cc) * 3 + q) * 512 + 8 * lane);
-            const size_t o = ((size_t)(cc * 4 + k4) * 3 + q) * 16;
-            b0[h][q] = *reinterpret_cast<const bf16x8 *>(s0 + o);
-            b1[h][q] = *reinterpret_cast<const bf16x8 *>(s1 + o);
-          }
-        };
-        ld(0, 0);
         int c = 0;
         for (; c + 1 < nc; c += 2) {
-          ld(c + 1, 1);
+          ld(mt, sc[0], sc[1], c + 1, 1);
           __builtin_amdgcn_sched_barrier(0);
           r0 = mma_bf3(av[0], b0[0], r0);
           r1 = mma_bf3(av[0], b1[0], r1);
-          ld(c + 2, 0);
+          ld(mt, sc[0], sc[1], c + 2, 0);
           __builtin_amdgcn_sched_barrier(0);
           r0 = mma_bf3(av[1], b0[1], r0);
           r1 = mma_bf3(av[1], b1[1], r1);
@@ -702,19 +739,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
           r0 = mma_bf3(av[0], b0[0], r0);
           r1 = mma_bf3(av[0], b1[0], r1);
         }
+        // this item's outputs, then the next item's first chunk ahead of their stores
+        const int mo = mt, to = tp, co[2] = {colc[0], colc[1]};
+        if (item + 1 < j1) {
+          geo(item + 1, mt, tp, colc, sc);
+          ld(mt, sc[0], sc[1], 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-          const int col = 32 * tp + 16 * u + r16;
+          const int col = 32 * to + 16 * u + r16;
+          if (col >= ncol) continue;
           int line, k1;
-          divmod_small(colc[u], M1, 1.0f / (float)M1, line, k1);
+          divmod_small(co[u], M1, 1.0f / (float)M1, line, k1);
+          const LineBase lb = base_of(line);
           const f4 &r = u ? r1 : r0;
 #pragma unroll
           for (int p = 0; p < 2; ++p) {
-            const int k2 = 8 * mt + 2 * k4 + p;
-            if (!(col < ncol && k2 < M2)) continue;
+            const int k2 = 8 * mo + 2 * k4 + p;
+            if (k2 >= M2) continue;
             cx<T> z = mk<T>(r[2 * p], r[2 * p + 1]);
             if (ext) z = cadd(z, cmul(wp[p], yp[u]));
-            put(line, k1 + M1 * k2, z.x * scale, z.y * scale);
+            put_at(lb, k1 + M1 * k2, z.x * scale, z.y * scale);
           }
         }
       }
@@ -727,6 +773,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     const int i1 = contig ? min(nitems, i0 + per) : nitems, istep = contig ? 1 : 4;
     T av[kRowsKR];
     int cur = -1;
+    // FP64: each item's first A fragments are loaded before the previous item's outputs
+    // are stored (a load issued after the stores would wait for them: the vector memory
+    // counter retires in order)
+    if (!contig && i0 < i1) {
+      int tq, mq;
+      divmod_small(i0, nmt, 1.0f / (float)nmt, tq, mq);
+      dft_preload<T>(F, nks, mq, lane, av);
+    }
     for (int item = i0; item < i1; item += istep) {
       int mt, tp;
       if (contig) divmod_small(item, ntp, 1.0f / (float)ntp, mt, tp);
@@ -742,21 +796,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         yp[u] = ext ? mk<T>(S[colc * CSC + kpos(2 * P, K2)], S[colc * CSC + kpos(2 * P + 1, K2)]) : mk<T>((T)0, (T)0);
       }
       acc_t r0, r1;
-      dft_item<T>(F, nks, seg, mt, S, CSC, 2 * tp, 2 * tp + 1 < nt, lane, av, mt != cur, r0, r1);
+      dft_item<T>(F, nks, seg, mt, S, CSC, 2 * tp, 2 * tp + 1 < nt, lane, av, mt != cur, r0, r1, !contig);
       cur = mt;
+      if (!contig && item + istep < i1) {
+        int tq, mq;
+        divmod_small(item + istep, nmt, 1.0f / (float)nmt, tq, mq);
+        dft_preload<T>(F, nks, mq, lane, av);
+      }
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int col = 16 * (2 * tp + u) + r16, colc = min(col, ncol - 1);
+        if (col >= ncol) continue;
         int line, k1;
         divmod_small(colc, M1, 1.0f / (float)M1, line, k1);
+        const LineBase lb = base_of(line);
         const acc_t &r = u ? r1 : r0;
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
           const int k2 = 8 * mt + 2 * k4 + p;
-          if (!(col < ncol && k2 < (ext ? P : M2))) continue;
+          if (k2 >= (ext ? P : M2)) continue;
           cx<T> z = mk<T>(r[2 * p], r[2 * p + 1]);
           if (ext) z = cadd(z, cmul(wp[p], yp[u]));
-          put(line, k1 + M1 * k2, z.x * scale, z.y * scale);
+          put_at(lb, k1 + M1 * k2, z.x * scale, z.y * scale);
         }
       }
     }
