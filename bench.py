@@ -240,6 +240,31 @@ def c3_batch(s2, ops, torch, steps, warmup, modes=("fp64", "mixed"), before_time
     return out
 
 
+def c2_batched(s2, ops, torch, p, nmaps, steps, warmup):
+    """The headline shape as a batch: nmaps maps of L=512, N=8 (f_lm drawn like the
+    headline's, numpy seeds 2, 3, ...) through the batched entry point (one bucket: every
+    launch covers all maps; one cached HIP graph per direction).  Maps/s per mode and the
+    worst FP64 reconstruction error; informational (the headline is one map per step)."""
+    x = [torch.from_numpy(powerlaw_flm(p.L, 2 + k)).cuda() for k in range(nmaps)]
+    ps = [p] * nmaps
+    out = {"maps": nmaps}
+    for mode, name in ((s2.FP64, "fp64"), (s2.MIXED, "mixed")):
+        fw = [torch.zeros(s2.n_wav(p), dtype=torch.complex128, device="cuda") for _ in ps]
+        fs = [torch.zeros(s2.n_scal(p), dtype=torch.complex128, device="cuda") for _ in ps]
+        rec = [torch.zeros_like(t) for t in x]
+
+        def step():
+            ops.batch(0, fw, fs, x, None, ps, mode, graph=True)
+            ops.batch(1, rec, None, fw, fs, ps, mode, graph=True)
+
+        dt = timed(step, steps, warmup, torch)
+        worst = max(float((rec[k] - x[k]).abs().max() / x[k].abs().max()) for k in range(nmaps))
+        out[name] = {"value": steps * nmaps / dt, "unit": "maps/s", "ms_per_batch": 1e3 * dt / steps,
+                     "max_reconstruction_rel_err": worst}
+        del fw, fs, rec
+    return out
+
+
 def pmc_c3(path):
     """HBM bytes per map of the 256-map batch per mode, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of `bench.py --c3-only`, or {} if absent."""
@@ -339,6 +364,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the 256-map uneven batch")
     ap.add_argument("--no-real", action="store_true", help="skip the real-signal (half-spectrum) comparison")
+    ap.add_argument("--no-c2-batch", action="store_true", help="skip the 8-map batch of the headline shape")
     ap.add_argument("--c5", action="store_true", help="also the L = 2048 single map (a few minutes of table builds)")
     ap.add_argument("--c3-only", choices=("fp64", "mixed", "both"), default=None,
                     help="run only the 256-map batch (in the given modes, --steps batches after --warmup)")
@@ -503,6 +529,8 @@ def main():
     }
     if not args.no_real:
         out["real_signal"] = real_half_spectrum(s2, ops, torch, p, flm_h, max(2, args.steps // 2), 1)
+    if not args.no_c2_batch:
+        out["c2_batch8"] = c2_batched(s2, ops, torch, p, 8, max(2, args.steps // 4), 1)
     if not args.no_c3:
         out["c3_batch"] = c3_batch(s2, ops, torch, max(2, args.steps // 4), 1)
     if args.c5:
