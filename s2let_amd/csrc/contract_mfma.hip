@@ -68,6 +68,15 @@ __device__ inline V4<double> ld4(const double *p) {
   const f64x2 a = ((gf64x2 *)p)[0], b = ((gf64x2 *)p)[1];
   return V4<double>{{a.x, a.y, b.x, b.y}};
 }
+// the same through a buffer resource: byte offsets voff (the lane's) and soff (wave-uniform)
+__device__ inline V4<float> bld4(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float) {
+  const f32x4 a = buf_ld16<f32x4>(r, voff, soff);
+  return V4<float>{{a.x, a.y, a.z, a.w}};
+}
+__device__ inline V4<double> bld4(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, double) {
+  const f64x2 a = buf_ld16<f64x2>(r, voff, soff), b = buf_ld16<f64x2>(r, voff + 16, soff);
+  return V4<double>{{a.x, a.y, b.x, b.y}};
+}
 
 // ------------------------------------------------------------------ selftest
 // C = A B for A[16][4] = a_ik, B[4][16] = b_kj with small integers; out[i*16+j]
@@ -429,22 +438,26 @@ __global__ __launch_bounds__(64 * FwdTile<T>::WAVES) __attribute__((amdgpu_waves
     const int mpe = min(Lg - 1, mps + mpchunk - 1);
     // this thread's element offset at m' = 0 (an invalid slot reads column 0, m = 0, and
     // is zeroed): every load is unconditional, so the wait counts stay exact
+    // (buffer loads: byte offsets of the lane's element below 4 GB, the m' row's wave-uniform)
     bool cok[CPT];
-    long long coff[CPT];
+    unsigned coff[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int cc = cc0 + k * (2 * WV);
       cok[k] = cc < nc && mm < Lg && !(HM && pm);  // HM: the m >= 0 half only
-      coff[k] = cok[k] ? gc[cc].out_off + (Lg - 1) + (pm ? -mm : mm) : gc[0].out_off + (Lg - 1);
+      coff[k] = (unsigned)sizeof(cx<T>) *
+                (unsigned)(cok[k] ? gc[cc].out_off + (Lg - 1) + (pm ? -mm : mm) : gc[0].out_off + (Lg - 1));
     }
     // raw loads; the invalid slots are zeroed where they are stored (a select right after
     // the loads would make the loop wait for them, and for every record load behind them)
     cx<T> jr[CPT][MC];
+    const __amdgpu_buffer_rsrc_t rj = buf_rsrc(Jbase);
     auto fetch = [&](int mp0) {
 #pragma unroll
       for (int k = 0; k < CPT; ++k)
 #pragma unroll
-        for (int u = 0; u < MC; ++u) jr[k][u] = Jbase[coff[k] + (long long)min(mp0 + u, Lg - 1) * Mg];
+        for (int u = 0; u < MC; ++u)
+          jr[k][u] = buf_ld<cx<T>>(rj, coff[k], (unsigned)sizeof(cx<T>) * (unsigned)(min(mp0 + u, Lg - 1) * Mg));
     };
     auto stage = [&](int b, int mp0) {  // jr holds the J of m' = mp0 ...
 #pragma unroll
@@ -463,8 +476,9 @@ __global__ __launch_bounds__(64 * FwdTile<T>::WAVES) __attribute__((amdgpu_waves
     // same loads, so that the wait counts stay exact (a path with fewer loads in flight
     // makes every wait after the join count as if it had them all)
     const int mend = min(l0 + 15, mpe);
-    const T *dq = dt + dtile_base(lbw) + (size_t)mbi * (16 * (lbw + 1)) * 256 + 4 * lane;
-    const T *eq = Ef + (size_t)lbw * Lg * 256 + 4 * lane;
+    const __amdgpu_buffer_rsrc_t rd = buf_rsrc(dt + dtile_base(lbw) + (size_t)mbi * (16 * (lbw + 1)) * 256);
+    const __amdgpu_buffer_rsrc_t re = buf_rsrc(Ef + (size_t)lbw * Lg * 256);
+    const unsigned vrec = (unsigned)(4 * sizeof(T)) * (unsigned)lane;
     fetch(mps);
     __syncthreads();  // every wave is done with the previous group's last chunk
     stage(0, mps);
@@ -476,9 +490,9 @@ __global__ __launch_bounds__(64 * FwdTile<T>::WAVES) __attribute__((amdgpu_waves
     if (run_g) {
 #pragma unroll
       for (int k = 0; k < PD; ++k) {
-        const size_t o = (size_t)min(mps + k, mend) * 256;
-        dr[k] = ld4(dq + o);
-        er[k] = ld4(eq + o);
+        const unsigned o = (unsigned)(sizeof(T) * 256) * (unsigned)min(mps + k, mend);
+        dr[k] = bld4(rd, vrec, o, T());
+        er[k] = bld4(re, vrec, o, T());
       }
     }
     // J double-buffered: chunk i reads sJ[i % 2]; after the one barrier of chunk i (every
@@ -515,9 +529,9 @@ __global__ __launch_bounds__(64 * FwdTile<T>::WAVES) __attribute__((amdgpu_waves
       for (int mc = 0; mc < MC; ++mc) {
         const V4<T> dv = dr[mc % PD], ev = er[mc % PD];
         {
-          const size_t o = (size_t)min(mp0 + mc + PD, mend) * 256;  // MC is a multiple of PD
-          dr[mc % PD] = ld4(dq + o);
-          er[mc % PD] = ld4(eq + o);
+          const unsigned o = (unsigned)(sizeof(T) * 256) * (unsigned)min(mp0 + mc + PD, mend);  // MC is a multiple of PD
+          dr[mc % PD] = bld4(rd, vrec, o, T());
+          er[mc % PD] = bld4(re, vrec, o, T());
         }
         if (mc >= n) continue;  // uniform; no loads inside
         acc_t h[4];
