@@ -240,25 +240,32 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt0, int
     T z[MP];
     V4<T> p01, p23, q01, q23;
   };
+  // buffer loads: wave-uniform bases and offsets (scalar registers), the lane's 4-entry
+  // record at vrec; every offset stays below 4 GB (records of one l-block, one map's tiles)
+  const unsigned vrec = (unsigned)(4 * sizeof(T)) * (unsigned)lane;
+  const __amdgpu_buffer_rsrc_t rei = buf_rsrc(Ei), rx = buf_rsrc(xt);
+  const unsigned xh = (unsigned)(sizeof(cx<T>) * xhalf);
   auto load = [&](int lbr, Ops &o) {
     const int lb = min(lbr, nlb - 1), nmp = 16 * (lb + 1);
+    const __amdgpu_buffer_rsrc_t rd = buf_rsrc(dt + dtile_base(lb));
 #pragma unroll
     for (int j = 0; j < MP; ++j) {
       const int mp = mp0 + j;
       const bool ok = mp < nmp && mp < L;
       const int mq = ok ? mp : mp0;  // mp0 < nmp for every l-block from lb0 on
-      o.d[j] = ld4(dt + dtile_base(lb) + ((size_t)mbi * nmp + mq) * 256 + 4 * lane);
-      o.e[j] = ld4(Ei + (((size_t)mq * nlb + lb) * 64 + lane) * 4);
+      o.d[j] = bld4(rd, vrec, (unsigned)(256 * sizeof(T)) * (unsigned)(mbi * nmp + mq), T());
+      o.e[j] = bld4(rei, vrec, (unsigned)(256 * sizeof(T)) * (unsigned)(mq * nlb + lb), T());
       o.z[j] = ok ? (T)1 : (T)0;
     }
     // x[l, m] and (-1)^l x[l, -m] of this lane's four l (the sign (-1)^{m'} of the -m
     // products is applied at the end); E vanishes for l >= L
-    const cx<T> *xq = xt + (((size_t)lb * nx + mbi) * 64 + lane) * 4;
-    o.p01 = ld4(&xq[0].x);
-    o.p23 = ld4(&xq[2].x);
+    const unsigned xo = (unsigned)(256 * sizeof(cx<T>)) * (unsigned)(lb * nx + mbi);
+    const unsigned vx = (unsigned)(4 * sizeof(cx<T>)) * (unsigned)lane;
+    o.p01 = bld4(rx, vx, xo, T());
+    o.p23 = bld4(rx, vx + 2 * (unsigned)sizeof(cx<T>), xo, T());
     if constexpr (!HM) {
-      o.q01 = ld4(&xq[xhalf].x);
-      o.q23 = ld4(&xq[xhalf + 2].x);
+      o.q01 = bld4(rx, vx, xo + xh, T());
+      o.q23 = bld4(rx, vx + 2 * (unsigned)sizeof(cx<T>), xo + xh, T());
     }
   };
   auto compute = [&](int lb, const Ops &o) {
@@ -318,24 +325,26 @@ __global__ __launch_bounds__(256) void k_inv3(const cx<T> *__restrict__ xt0, int
     for (int lb = lb0; lb < nlb; ++lb) {
       const int nmp = 16 * (lb + 1);
       Ops o;
+      const __amdgpu_buffer_rsrc_t rd = buf_rsrc(dt + dtile_base(lb));
 #pragma unroll
       for (int j = 0; j < MP; ++j) {
         const int mp = mp0 + j;
         o.z[j] = (T)1;
         if (mp < nmp && mp < L) {
-          o.d[j] = ld4(dt + dtile_base(lb) + ((size_t)mbi * nmp + mp) * 256 + 4 * lane);
-          o.e[j] = ld4(Ei + (((size_t)mp * nlb + lb) * 64 + lane) * 4);
+          o.d[j] = bld4(rd, vrec, (unsigned)(256 * sizeof(T)) * (unsigned)(mbi * nmp + mp), T());
+          o.e[j] = bld4(rei, vrec, (unsigned)(256 * sizeof(T)) * (unsigned)(mp * nlb + lb), T());
         } else {
 #pragma unroll
           for (int s = 0; s < 4; ++s) o.d[j].v[s] = o.e[j].v[s] = (T)0;
         }
       }
-      const cx<T> *xq = xt + (((size_t)lb * nx + mbi) * 64 + lane) * 4;
-      o.p01 = ld4(&xq[0].x);
-      o.p23 = ld4(&xq[2].x);
+      const unsigned xo = (unsigned)(256 * sizeof(cx<T>)) * (unsigned)(lb * nx + mbi);
+      const unsigned vx = (unsigned)(4 * sizeof(cx<T>)) * (unsigned)lane;
+      o.p01 = bld4(rx, vx, xo, T());
+      o.p23 = bld4(rx, vx + 2 * (unsigned)sizeof(cx<T>), xo, T());
       if constexpr (!HM) {
-        o.q01 = ld4(&xq[xhalf].x);
-        o.q23 = ld4(&xq[xhalf + 2].x);
+        o.q01 = bld4(rx, vx, xo + xh, T());
+        o.q23 = bld4(rx, vx + 2 * (unsigned)sizeof(cx<T>), xo + xh, T());
       }
       compute(lb, o);
     }
@@ -438,20 +447,22 @@ __global__ __launch_bounds__(64 * FwdTile<T>::WAVES) __attribute__((amdgpu_waves
     const int mpe = min(Lg - 1, mps + mpchunk - 1);
     // this thread's element offset at m' = 0 (an invalid slot reads column 0, m = 0, and
     // is zeroed): every load is unconditional, so the wait counts stay exact
-    // (buffer loads: byte offsets of the lane's element below 4 GB, the m' row's wave-uniform)
+    // (buffer loads from the super-group's first column: the lane's byte offset, below 4 GB
+    // (launch_fwd_mfma checks), and the m' row's, wave-uniform)
     bool cok[CPT];
     unsigned coff[CPT];
+    const long long off0 = gc[0].out_off;
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int cc = cc0 + k * (2 * WV);
       cok[k] = cc < nc && mm < Lg && !(HM && pm);  // HM: the m >= 0 half only
       coff[k] = (unsigned)sizeof(cx<T>) *
-                (unsigned)(cok[k] ? gc[cc].out_off + (Lg - 1) + (pm ? -mm : mm) : gc[0].out_off + (Lg - 1));
+                (unsigned)(cok[k] ? gc[cc].out_off - off0 + (Lg - 1) + (pm ? -mm : mm) : (long long)(Lg - 1));
     }
     // raw loads; the invalid slots are zeroed where they are stored (a select right after
     // the loads would make the loop wait for them, and for every record load behind them)
     cx<T> jr[CPT][MC];
-    const __amdgpu_buffer_rsrc_t rj = buf_rsrc(Jbase);
+    const __amdgpu_buffer_rsrc_t rj = buf_rsrc(Jbase + off0);
     auto fetch = [&](int mp0) {
 #pragma unroll
       for (int k = 0; k < CPT; ++k)

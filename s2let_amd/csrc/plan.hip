@@ -730,6 +730,15 @@ void Context::finish_plan(Plan &pl, hipStream_t st) {
     }
   }
   const bool f32 = (mode == S2LET_HIP_MIXED);
+  // the forward contraction reads a super-group's spectra through one buffer resource with
+  // 32-bit byte offsets from its first column
+  for (const SuperDev &S : pl.supers) {
+    const long long span = pl.cols_ana[S.col0 + S.ncol - 1].out_off + (long long)map_elems(S.L, pl.ss) -
+                           pl.cols_ana[S.col0].out_off;
+    const size_t esz = f32 ? sizeof(cx<float>) : sizeof(cx<double>);
+    if ((unsigned long long)span * esz >= (1ull << 32))
+      throw std::runtime_error("plan: the spectra of one column group exceed 4 GB (band-limit too large)");
+  }
   ensure_delta(L, f32, st);
   for (const Group &g : pl.groups) ltab(g.L, pl.ss);
   // gamma mixing matrices per scale: [inv (G x nc), inv_adj (nc x G), fwd (nc x G), fwd_adj (G x nc)]
