@@ -426,20 +426,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         const unsigned short *__restrict__ Fb = fwd ? d.F1fb : d.F1bb;
         const int nc = d.nc1;
         constexpr int NC = kRowsDirectM1 / 16;
+        // entries 2 w and 2 w + 1 of a fragment are the real and imaginary part of one sample:
+        // split as a pair, one packed word per part
         bf16x8 bp[2][NC][3];
 #pragma unroll
         for (int u = 0; u < 2; ++u)
 #pragma unroll
-          for (int c = 0; c < NC; ++c)
+          for (int c = 0; c < NC; ++c) {
+            u4 w[3];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-              const cx<T> v = xb[u][4 * c + (e >> 1)];
-              __bf16 h, m, l;
-              split3((e & 1) ? v.y : v.x, h, m, l);
-              bp[u][c][0][e] = h;
-              bp[u][c][1][e] = m;
-              bp[u][c][2][e] = l;
+            for (int q = 0; q < 4; ++q) {
+              const cx<T> v = xb[u][4 * c + q];
+              unsigned h, m, l;
+              split3x2(v.x, v.y, h, m, l);
+              w[0][q] = h;
+              w[1][q] = m;
+              w[2][q] = l;
             }
+#pragma unroll
+            for (int p = 0; p < 3; ++p) bp[u][c][p] = __builtin_bit_cast(bf16x8, w[p]);
+          }
 #pragma unroll
         for (int mt = 0; mt < kRowsDirectNMT; ++mt) {
           if (mt >= nmt) break;
@@ -515,16 +521,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
               if (last) {
                 *reinterpret_cast<float2 *>(yo) = make_float2(z.x, z.y);
               } else {
-                __bf16 hx, mx, lx, hy, my, ly;
-                split3(z.x, hx, mx, lx);
-                split3(z.y, hy, my, ly);
-                auto pk = [](__bf16 lo, __bf16 hi) {
-                  return (unsigned)__builtin_bit_cast(unsigned short, lo) |
-                         ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16);
-                };
-                *reinterpret_cast<unsigned *>(yo) = pk(hx, hy);
-                *reinterpret_cast<unsigned *>(yo + 16) = pk(mx, my);
-                *reinterpret_cast<unsigned *>(yo + 32) = pk(lx, ly);
+                unsigned h, m, l;  // (re, im) parts as packed pairs
+                split3x2(z.x, z.y, h, m, l);
+                *reinterpret_cast<unsigned *>(yo) = h;
+                *reinterpret_cast<unsigned *>(yo + 16) = m;
+                *reinterpret_cast<unsigned *>(yo + 32) = l;
               }
             }
           }
